@@ -1,0 +1,160 @@
+"""Benchmark of the TVD advection forward step on one MI355X.
+
+Workload (the plain FP64 form at real shape): a 512x512x256 local domain
+(dx = dy = 25 m, z tanh-stretched to 5 km), fields u, v, w, th, q with the
+seeded distributions of the synthetic set-up, advection-only forward-Euler
+steps at CFL 0.3.  One step = the TVD tendencies of all five fields from the
+pre-step state plus the update field <- field + dt*s and the periodic halo of
+the new state, fused in one kernel launch (monc_tvd_forward_steps).  The same
+step in FP32 is reported beside it.
+
+Output: one JSON line.  value = interior cell-updates per second (all five
+fields) of the FP64 form; roofline = the step kernel's algorithmic bytes over
+its average duration (HIP events on the launch stream); cpu_baseline = the C
+restatement (oracle) timed on a bounded sample on this host's CPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gcell-updates/s & %HBM peak: FP32/FP64 fwd + adjoint at stated error vs FP64"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_form(state, precision: int, steps: int, warmup: int, dt: float):
+    import torch
+
+    from monc_amd import tvd
+
+    t = tvd.TvdAdvection.initialise(state, precision=precision)
+    try:
+        stream = torch.cuda.current_stream()
+        t.set_stream(stream.cuda_stream)
+        t.halo_fill()
+        for _ in range(warmup):
+            t.forward_steps(dt, 1)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(steps):
+            evs[s][0].record(stream)
+            t.forward_steps(dt, 1)  # exactly one kernel launch per step
+            evs[s][1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kms = [a.elapsed_time(b) for a, b in evs]
+        return wall, kms
+    finally:
+        t.close()
+
+
+def cpu_baseline(nz: int, seconds_hint: float = 12.0):
+    """The oracle (single-threaded C restatement) on a bounded sample of the
+    same workload: one forward step of a 160x160 column block at full depth."""
+    import numpy as np
+
+    import oracle
+    from monc_amd import setup_state as ss
+
+    n = 160
+    st = ss.make_state(nz, n, n, 25.0, 5000.0, seed=0, n_q=1, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    t0 = time.perf_counter()
+    srcs, _ = oracle.tendencies(nz, n, n, st.cx, st.cy, st.grid.metric_vectors(), dt, 1, 1, 1, st.fields)
+    _ = [oracle.forward_step(f, s, dt) for f, s in zip(st.fields, srcs)]
+    el = time.perf_counter() - t0
+    cells = n * n * nz
+    return {"value": cells / el / 1e9, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"one forward step (5 fields) of a {n}x{n}x{nz} block of the workload, "
+                      f"{el:.1f} s on 1 core"}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--nz", type=int, default=256)
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from monc_amd import setup_state as ss
+    from monc_amd import tvd
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    tvd.load_library()
+    nx, ny, nz = args.nx, args.ny, args.nz
+    log(f"bench: building {nx}x{ny}x{nz} state")
+    st = ss.make_state(nz, ny, nx, 25.0, 5000.0, seed=0, n_q=1, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    cells = nx * ny * nz
+
+    log("bench: FP64 form")
+    wall64, k64 = run_form(st, tvd.FP64, args.steps, args.warmup, dt)
+    ms_step = wall64 / args.steps * 1e3
+    value = cells * args.steps / wall64 / 1e9
+    kavg = sum(k64) / len(k64)
+    bytes_cell_64 = 80.0  # read u,v,w,th,q + write 5, 8 B each
+    achieved = bytes_cell_64 * cells / (kavg * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded u=5+N(0,1), v=N(0,1), w=N(0,0.5), th=300+0.003z+U(-0.1,0.1), "
+                "q=0.01exp(-z/2000)+U(0,1e-4))",
+        "config": {"workload": "plain FP64 TVD advection forward-Euler step, 512x512x256, dx=dy=25 m, "
+                               "z tanh-stretched to 5 km, fields u,v,w,th,q, CFL 0.3",
+                   "nx": nx, "ny": ny, "nz": nz, "fields": 5, "dt_s": dt},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "tvd::cell_kernel<double, MODE_STEP>", "kernel_ms": round(kavg, 4),
+                     "bytes_per_cell_update": bytes_cell_64},
+    }
+    if not args.no_fp32:
+        log("bench: FP32 form")
+        wall32, k32 = run_form(st, tvd.FP32, args.steps, args.warmup, dt)
+        v32 = cells * args.steps / wall32 / 1e9
+        k32avg = sum(k32) / len(k32)
+        a32 = 40.0 * cells / (k32avg * 1e-3) / 1e9
+        out["fp32"] = {"value": round(v32, 4), "unit": "Gcell-updates/s", "ms_per_step": round(wall32 / args.steps
+                                                                                                * 1e3, 4),
+                       "kernel_ms": round(k32avg, 4), "achieved_GBs": round(a32, 2),
+                       "frac": round(a32 / HBM_PEAK_GBS, 4), "speedup_vs_fp64": round(v32 / value, 3)}
+    if not args.no_cpu:
+        log("bench: CPU baseline (oracle) on a bounded sample")
+        out["cpu_baseline"] = cpu_baseline(nz)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,134 @@
+/*
+ * monc_tvd.h -- C ABI of the MI355X TVD advection library (libmonc_tvd.so).
+ *
+ * The library replaces MONC's `tvd_advection` component
+ * (components/tvdadvection/src/tvdadvection.F90) and the single-rank periodic
+ * halo fill it relies on (components/haloswapper/src/haloswapper.F90:62-71 ->
+ * model_core/src/communication/halocommunication.F90:1058-1168).  The
+ * reference component is a set of callbacks on `model_state_type`
+ * (model_core/src/components/monc_component.F90:62-118); the entry points below
+ * are the plain-pointer form of those callbacks that a Fortran shim reaches
+ * through iso_c_binding.
+ *
+ * Arrays are Fortran column-major data(k,j,i) = (z,y,x), z fastest, with the
+ * reference's depth-2 x/y halo: shape (nz, ny+4, nx+4), as allocated by
+ * components/simplesetup/src/simplesetup.F90:60-137.  Vertical vectors have
+ * length nz (index 1 unused where the reference leaves it unset).
+ *
+ * Every function returns MONC_TVD_OK (0) or a non-zero status; the message of
+ * the last failure on the calling thread is returned by monc_tvd_last_error().
+ * The reference aborts through log_log(LOG_ERROR) (model_core/src/utils/
+ * logging.F90:24,53,86); the Fortran shim turns a non-zero status into that call.
+ * One context serves one host thread; calls on it run on the context's HIP
+ * stream and the host-memory entry points synchronise before returning.
+ */
+#ifndef MONC_TVD_H
+#define MONC_TVD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MONC_TVD_OK 0
+#define MONC_TVD_ERR_ARG 1   /* invalid argument (sizes, halo, pointer, index) */
+#define MONC_TVD_ERR_HIP 2   /* a HIP runtime call failed (allocation, launch) */
+#define MONC_TVD_ERR_STATE 3 /* a field the call needs was never set or bound */
+
+/* state.F90:15 */
+#define MONC_TVD_CENTRED_STEPPING 0
+#define MONC_TVD_FORWARD_STEPPING 1
+
+/* precision of the device-resident fields */
+#define MONC_TVD_FP64 0
+#define MONC_TVD_FP32 1
+
+/* advect flags: tvdadvection.F90:349-352 (advection_flow_fields,
+ * advection_theta_field, advection_q_fields selectors resolved to tvd) */
+#define MONC_TVD_ADVECT_FLOW 1
+#define MONC_TVD_ADVECT_THETA 2
+#define MONC_TVD_ADVECT_Q 4
+
+/* field ids: u, v, w, th, then the n_q scalar fields (q(1..) then tracers,
+ * tvdadvection.F90:650-695 advects both with advect_scalar_field) */
+#define MONC_TVD_U 0
+#define MONC_TVD_V 1
+#define MONC_TVD_W 2
+#define MONC_TVD_TH 3
+#define MONC_TVD_Q(n) (4 + (n))
+
+/* field variants: the prognostic, its z-field (time level t-1, the advected
+ * field under centred stepping) and its source (tendency) field,
+ * state.F90:86-92 */
+#define MONC_TVD_PROG 0
+#define MONC_TVD_ZFIELD 1
+#define MONC_TVD_SOURCE 2
+
+typedef struct monc_tvd_ctx monc_tvd_ctx;
+
+/*
+ * Replaces tvdadvection.F90:286-481 initialisation_callback (plus the metric
+ * vectors of components/gridmanager/src/gridmanager.F90:2048-2067, 2449-2456
+ * and cx = 1/dx, cy = 1/dy of :2624-2633).  nz, ny, nx are the local interior
+ * sizes; halo must be 2 (global_config halo_depth); nz >= 4, ny, nx >= 4.
+ * Copies the FP64 metric vectors to the device and allocates nothing else:
+ * fields are allocated by monc_tvd_set_field or supplied by monc_tvd_bind_field.
+ */
+int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags, double cx, double cy,
+                    const double *rdz, const double *rdzn, const double *dz, const double *dzn,
+                    const double *tzc1, const double *tzc2, const double *tzd1, const double *tzd2,
+                    int precision, monc_tvd_ctx **out);
+
+/* Replaces tvdadvection.F90:485-524 finalisation_callback: frees the context
+ * and every buffer the library allocated (bound buffers are not freed). */
+int monc_tvd_destroy(monc_tvd_ctx *ctx);
+
+/* Host <-> device staging of one field (FP64 host array of shape
+ * (nz, ny+4, nx+4)); converts to/from FP32 for an FP32 context. */
+int monc_tvd_set_field(monc_tvd_ctx *ctx, int field, int variant, const double *host);
+int monc_tvd_get_field(monc_tvd_ctx *ctx, int field, int variant, double *host);
+
+/* Device-resident mode: use a caller-owned device buffer of the context's
+ * precision and shape for a field, or read back the library's own buffer. */
+int monc_tvd_bind_field(monc_tvd_ctx *ctx, int field, int variant, void *device_ptr);
+int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **device_ptr);
+
+/* Run subsequent work on this hipStream_t (NULL: the legacy default stream). */
+int monc_tvd_set_stream(monc_tvd_ctx *ctx, void *hip_stream);
+int monc_tvd_synchronize(monc_tvd_ctx *ctx);
+
+/* Replaces haloswapper.F90:62-71 on a single rank: periodic fill of the depth-2
+ * x/y halo and its corners (halocommunication.F90:486-498, 1058-1168) of every
+ * set prognostic and z-field (haloswapper.F90:472-520). */
+int monc_tvd_halo_fill(monc_tvd_ctx *ctx);
+
+/*
+ * Replaces tvdadvection.F90:528-599 timestep_callback, applied to every column
+ * of the local domain (timestepper.F90:65-85): adds the TVD advection
+ * tendencies to the source fields (su, sv, sw, sth, sq(:)), overwriting level
+ * nz of su, sv and the scalars (:942-959).  The ultflx timestep is 2*dtm unless
+ * the corresponding stepping flag is FORWARD (:608-609, :656-657, :704-705); the
+ * advected field is the prognostic when field_stepping is FORWARD and its
+ * z-field otherwise (:739-749).  Halos must be filled (monc_tvd_halo_fill or by
+ * the caller).  Asynchronous on the context stream.
+ */
+int monc_tvd_tendencies(monc_tvd_ctx *ctx, double dtm, int field_stepping, int momentum_stepping,
+                        int scalar_stepping);
+
+/*
+ * Device-resident forward-Euler stepping of every advected field, nsteps times:
+ * per step the tendencies of all fields from the pre-step state with FORWARD
+ * stepping (ultflx dt = dtm, advected field = prognostic), then
+ * field <- field + dtm * s as stepfields.F90:597-627 does for FORWARD_STEPPING
+ * with s(1) = 0 and nothing else in the source fields, the periodic halo of the
+ * new state written by the same kernel.  The source fields are not touched.
+ * Requires filled halos on entry.  Asynchronous on the context stream.
+ */
+int monc_tvd_forward_steps(monc_tvd_ctx *ctx, double dtm, int nsteps);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *monc_tvd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MONC_TVD_H */

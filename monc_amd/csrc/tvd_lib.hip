@@ -1,0 +1,423 @@
+// libmonc_tvd: the C ABI of include/monc_tvd.h over the gfx950 kernels of
+// tvd_kernels.hpp.  Host code owns the device buffers, the metric vectors and
+// the launch shapes; every entry point validates its arguments and reports
+// failures through a status code and a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/monc_tvd.h"
+#include "tvd_kernels.hpp"
+
+using namespace tvd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) return fail(MONC_TVD_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+enum { MET_RDZ, MET_RDZN, MET_DZ, MET_DZN, MET_TZC1, MET_TZC2, MET_TZD1, MET_TZD2, MET_N };
+constexpr int NVAR = 3;
+
+}  // namespace
+
+struct monc_tvd_ctx {
+  int nz, ny, nx, n_q, advect, precision;
+  double cx, cy;
+  size_t n_elem;     // elements per field, nz*(ny+4)*(nx+4)
+  size_t elem_size;  // 8 or 4
+  double* dmet;      // MET_N vectors of length nz, device
+  int nfields;
+  std::vector<void*> buf;      // [field*NVAR + variant]
+  std::vector<char> owned;     // library-allocated?
+  std::vector<void*> bound;    // caller pointer of a bound PROG buffer (restored after stepping)
+  std::vector<void*> spare;    // library-owned second buffer of each prognostic (stepping)
+  hipStream_t stream;
+};
+
+namespace {
+
+int check_ctx(monc_tvd_ctx* c) {
+  if (!c) return fail(MONC_TVD_ERR_ARG, "null context");
+  return MONC_TVD_OK;
+}
+
+int check_fv(monc_tvd_ctx* c, int field, int variant) {
+  if (field < 0 || field >= c->nfields) return fail(MONC_TVD_ERR_ARG, "field id %d out of range [0,%d)", field, c->nfields);
+  if (variant < 0 || variant >= NVAR) return fail(MONC_TVD_ERR_ARG, "variant %d out of range", variant);
+  return MONC_TVD_OK;
+}
+
+void*& slot(monc_tvd_ctx* c, int field, int variant) { return c->buf[(size_t)field * NVAR + variant]; }
+
+int ensure(monc_tvd_ctx* c, int field, int variant) {
+  void*& p = slot(c, field, variant);
+  if (p) return MONC_TVD_OK;
+  HIP_TRY(hipMalloc(&p, c->n_elem * c->elem_size));
+  HIP_TRY(hipMemsetAsync(p, 0, c->n_elem * c->elem_size, c->stream));
+  c->owned[(size_t)field * NVAR + variant] = 1;
+  return MONC_TVD_OK;
+}
+
+Dims dims_of(const monc_tvd_ctx* c) {
+  Dims d;
+  d.nz = c->nz;
+  d.ny = c->ny;
+  d.nx = c->nx;
+  d.sj = c->nz;
+  d.si = (int64_t)c->nz * (c->ny + 4);
+  return d;
+}
+
+VMet vmet_of(const monc_tvd_ctx* c, bool is_w) {
+  const double* m = c->dmet;
+  const int nz = c->nz;
+  VMet v;
+  if (is_w) {  // tvdadvection.F90:881-894, :898-904
+    v.rdz = m + MET_RDZN * nz;
+    v.rdzn = m + MET_RDZ * nz;
+    v.dzn = m + MET_DZ * nz;
+    v.t1 = m + MET_TZD1 * nz;
+    v.t2 = m + MET_TZD2 * nz;
+  } else {  // :739-757
+    v.rdz = m + MET_RDZ * nz;
+    v.rdzn = m + MET_RDZN * nz;
+    v.dzn = m + MET_DZN * nz;
+    v.t1 = m + MET_TZC1 * nz;
+    v.t2 = m + MET_TZC2 * nz;
+  }
+  return v;
+}
+
+template <typename T>
+LaunchArgs<T> launch_args(monc_tvd_ctx* c, double dtm) {
+  LaunchArgs<T> L;
+  L.d = dims_of(c);
+  L.u = (const T*)slot(c, MONC_TVD_U, MONC_TVD_PROG);
+  L.v = (const T*)slot(c, MONC_TVD_V, MONC_TVD_PROG);
+  L.w = (const T*)slot(c, MONC_TVD_W, MONC_TVD_PROG);
+  L.mc = vmet_of(c, false);
+  L.mw = vmet_of(c, true);
+  L.dtm = (T)dtm;
+  L.cx = (T)c->cx;
+  L.cy = (T)c->cy;
+  return L;
+}
+
+int kind_of(int field) {
+  return field == MONC_TVD_U ? KIND_U : field == MONC_TVD_V ? KIND_V : field == MONC_TVD_W ? KIND_W : KIND_SCALAR;
+}
+
+// One launch of cell_kernel over fields[first, first+count).
+template <typename T, int MODE>
+int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
+  const int nk = MODE == MODE_STEP ? c->nz : c->nz - 1;
+  dim3 block(64, 4, 1);
+  dim3 grid((nk + 63) / 64, (c->ny + 3) / 4, c->nx);
+  hipLaunchKernelGGL((cell_kernel<T, MODE>), grid, block, 0, c->stream, L, S);
+  HIP_TRY(hipGetLastError());
+  return MONC_TVD_OK;
+}
+
+template <typename T>
+int launch_halo(monc_tvd_ctx* c, T* f) {
+  Dims d = dims_of(c);
+  const int64_t nh = 4 * (int64_t)(c->nx + 4) + 4 * (int64_t)c->ny;
+  dim3 block(64, 4, 1);
+  dim3 grid((c->nz + 63) / 64, (unsigned)((nh + 3) / 4), 1);
+  hipLaunchKernelGGL((halo_kernel<T>), grid, block, 0, c->stream, f, d);
+  HIP_TRY(hipGetLastError());
+  return MONC_TVD_OK;
+}
+
+// Fields advected this call, in the reference's order u, v, w, th, q(:)
+// (tvdadvection.F90:591-594).
+std::vector<int> advected_fields(const monc_tvd_ctx* c) {
+  std::vector<int> f;
+  if (c->advect & MONC_TVD_ADVECT_FLOW) {
+    f.push_back(MONC_TVD_U);
+    f.push_back(MONC_TVD_V);
+    f.push_back(MONC_TVD_W);
+  }
+  if (c->advect & MONC_TVD_ADVECT_THETA) f.push_back(MONC_TVD_TH);
+  if (c->advect & MONC_TVD_ADVECT_Q)
+    for (int n = 0; n < c->n_q; n++) f.push_back(MONC_TVD_Q(n));
+  return f;
+}
+
+int need(monc_tvd_ctx* c, int field, int variant) {
+  if (!slot(c, field, variant))
+    return fail(MONC_TVD_ERR_STATE, "field %d variant %d is needed but was never set or bound", field, variant);
+  return MONC_TVD_OK;
+}
+
+template <typename T>
+int tendencies_impl(monc_tvd_ctx* c, double dtm, int field_stepping, int momentum_stepping, int scalar_stepping) {
+  int rc;
+  for (int fl : {MONC_TVD_U, MONC_TVD_V, MONC_TVD_W})
+    if ((rc = need(c, fl, MONC_TVD_PROG))) return rc;
+  const int adv_var = field_stepping == MONC_TVD_FORWARD_STEPPING ? MONC_TVD_PROG : MONC_TVD_ZFIELD;
+  const double dt_mom = momentum_stepping == MONC_TVD_FORWARD_STEPPING ? dtm : dtm * 2.0;
+  const double dt_sca = scalar_stepping == MONC_TVD_FORWARD_STEPPING ? dtm : dtm * 2.0;
+  std::vector<int> fields = advected_fields(c);
+  for (int fl : fields) {
+    if ((rc = need(c, fl, adv_var)) || (rc = need(c, fl, MONC_TVD_SOURCE))) return rc;
+  }
+  const LaunchArgs<T> L = launch_args<T>(c, dtm);
+  for (size_t first = 0; first < fields.size(); first += MAX_FIELDS) {
+    FieldSet<T> S;
+    S.nf = (int)std::min(fields.size() - first, (size_t)MAX_FIELDS);
+    for (int n = 0; n < S.nf; n++) {
+      const int fl = fields[first + n];
+      S.kind[n] = kind_of(fl);
+      S.F[n] = (const T*)slot(c, fl, adv_var);
+      S.f[n] = (const T*)slot(c, fl, MONC_TVD_PROG);
+      S.out[n] = (T*)slot(c, fl, MONC_TVD_SOURCE);
+      S.dt[n] = (T)(fl <= MONC_TVD_W ? dt_mom : dt_sca);
+    }
+    if ((rc = launch_set<T, MODE_ACCUM>(c, L, S))) return rc;
+  }
+  return MONC_TVD_OK;
+}
+
+template <typename T>
+int forward_steps_impl(monc_tvd_ctx* c, double dtm, int nsteps) {
+  int rc;
+  for (int fl : {MONC_TVD_U, MONC_TVD_V, MONC_TVD_W})
+    if ((rc = need(c, fl, MONC_TVD_PROG))) return rc;
+  std::vector<int> fields = advected_fields(c);
+  for (int fl : fields) {
+    if ((rc = need(c, fl, MONC_TVD_PROG))) return rc;
+    if (!c->spare[fl]) HIP_TRY(hipMalloc(&c->spare[fl], c->n_elem * c->elem_size));
+  }
+  // the flows are read by every field's update, so no field may be updated in
+  // place: each step writes the spare buffers, then the roles swap.
+  for (int s = 0; s < nsteps; s++) {
+    const LaunchArgs<T> L = launch_args<T>(c, dtm);
+    for (size_t first = 0; first < fields.size(); first += MAX_FIELDS) {
+      FieldSet<T> S;
+      S.nf = (int)std::min(fields.size() - first, (size_t)MAX_FIELDS);
+      for (int n = 0; n < S.nf; n++) {
+        const int fl = fields[first + n];
+        S.kind[n] = kind_of(fl);
+        S.F[n] = (const T*)slot(c, fl, MONC_TVD_PROG);
+        S.f[n] = S.F[n];
+        S.out[n] = (T*)c->spare[fl];
+        S.dt[n] = (T)dtm;
+      }
+      if ((rc = launch_set<T, MODE_STEP>(c, L, S))) return rc;
+    }
+    for (int fl : fields) std::swap(slot(c, fl, MONC_TVD_PROG), c->spare[fl]);
+  }
+  // a caller-bound prognostic keeps holding the state after the call
+  for (int fl : fields) {
+    void* b = c->bound[fl];
+    if (b && slot(c, fl, MONC_TVD_PROG) != b) {
+      HIP_TRY(hipMemcpyAsync(b, slot(c, fl, MONC_TVD_PROG), c->n_elem * c->elem_size, hipMemcpyDeviceToDevice,
+                             c->stream));
+      std::swap(slot(c, fl, MONC_TVD_PROG), c->spare[fl]);
+    }
+  }
+  return MONC_TVD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* monc_tvd_last_error(void) { return g_err.c_str(); }
+
+int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags, double cx, double cy,
+                    const double* rdz, const double* rdzn, const double* dz, const double* dzn, const double* tzc1,
+                    const double* tzc2, const double* tzd1, const double* tzd2, int precision, monc_tvd_ctx** out) {
+  g_err.clear();
+  if (!out) return fail(MONC_TVD_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (halo != 2) return fail(MONC_TVD_ERR_ARG, "halo depth %d unsupported: the TVD stencil needs exactly 2", halo);
+  if (nz < 4 || ny < 4 || nx < 4)
+    return fail(MONC_TVD_ERR_ARG, "local grid %dx%dx%d too small (need nz, ny, nx >= 4)", nz, ny, nx);
+  if (n_q < 0 || n_q > 64) return fail(MONC_TVD_ERR_ARG, "n_q %d out of range [0,64]", n_q);
+  if (precision != MONC_TVD_FP64 && precision != MONC_TVD_FP32)
+    return fail(MONC_TVD_ERR_ARG, "unknown precision %d", precision);
+  if (advect_flags & ~7) return fail(MONC_TVD_ERR_ARG, "unknown advect flags 0x%x", advect_flags);
+  if (!(cx > 0.0) || !(cy > 0.0)) return fail(MONC_TVD_ERR_ARG, "cx, cy must be positive");
+  const double* mets[MET_N] = {rdz, rdzn, dz, dzn, tzc1, tzc2, tzd1, tzd2};
+  for (int m = 0; m < MET_N; m++)
+    if (!mets[m]) return fail(MONC_TVD_ERR_ARG, "metric vector %d is null", m);
+  const size_t n_elem = (size_t)nz * (size_t)(ny + 4) * (size_t)(nx + 4);
+  if (n_elem > ((size_t)1 << 40)) return fail(MONC_TVD_ERR_ARG, "field too large");
+
+  monc_tvd_ctx* c = new monc_tvd_ctx();
+  c->nz = nz;
+  c->ny = ny;
+  c->nx = nx;
+  c->n_q = n_q;
+  c->advect = advect_flags;
+  c->precision = precision;
+  c->cx = cx;
+  c->cy = cy;
+  c->n_elem = n_elem;
+  c->elem_size = precision == MONC_TVD_FP64 ? 8 : 4;
+  c->nfields = 4 + n_q;
+  c->buf.assign((size_t)c->nfields * NVAR, nullptr);
+  c->owned.assign((size_t)c->nfields * NVAR, 0);
+  c->bound.assign((size_t)c->nfields, nullptr);
+  c->spare.assign((size_t)c->nfields, nullptr);
+  c->stream = nullptr;
+  c->dmet = nullptr;
+  std::vector<double> h((size_t)MET_N * nz);
+  for (int m = 0; m < MET_N; m++) std::memcpy(&h[(size_t)m * nz], mets[m], sizeof(double) * nz);
+  hipError_t e = hipMalloc(&c->dmet, h.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(c->dmet, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (c->dmet) (void)hipFree(c->dmet);
+    delete c;
+    return fail(MONC_TVD_ERR_HIP, "metric upload: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_destroy(monc_tvd_ctx* c) {
+  if (!c) return MONC_TVD_OK;
+  (void)hipStreamSynchronize(c->stream);
+  for (size_t n = 0; n < c->buf.size(); n++) {
+    void* p = c->buf[n];
+    if (!p) continue;
+    const int field = (int)(n / NVAR), variant = (int)(n % NVAR);
+    // after an odd number of steps a bound prognostic and its spare swapped
+    // roles; free only library memory
+    if (variant == MONC_TVD_PROG && c->bound[field]) {
+      if (p != c->bound[field]) (void)hipFree(p);
+      continue;
+    }
+    if (c->owned[n]) (void)hipFree(p);
+  }
+  for (int f = 0; f < c->nfields; f++)
+    if (c->spare[f] && c->spare[f] != c->bound[f]) (void)hipFree(c->spare[f]);
+  if (c->dmet) (void)hipFree(c->dmet);
+  delete c;
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_set_stream(monc_tvd_ctx* c, void* s) {
+  if (int rc = check_ctx(c)) return rc;
+  c->stream = (hipStream_t)s;
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_synchronize(monc_tvd_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_bind_field(monc_tvd_ctx* c, int field, int variant, void* dptr) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_fv(c, field, variant)) return rc;
+  if (!dptr) return fail(MONC_TVD_ERR_ARG, "null device pointer");
+  const size_t n = (size_t)field * NVAR + variant;
+  if (c->buf[n] && c->owned[n]) HIP_TRY(hipFree(c->buf[n]));
+  c->buf[n] = dptr;
+  c->owned[n] = 0;
+  if (variant == MONC_TVD_PROG) c->bound[field] = dptr;
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_field_device_ptr(monc_tvd_ctx* c, int field, int variant, void** dptr) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_fv(c, field, variant)) return rc;
+  if (!dptr) return fail(MONC_TVD_ERR_ARG, "null output pointer");
+  if (int rc = ensure(c, field, variant)) return rc;
+  *dptr = slot(c, field, variant);
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_set_field(monc_tvd_ctx* c, int field, int variant, const double* host) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_fv(c, field, variant)) return rc;
+  if (!host) return fail(MONC_TVD_ERR_ARG, "null host pointer");
+  if (int rc = ensure(c, field, variant)) return rc;
+  void* d = slot(c, field, variant);
+  if (c->precision == MONC_TVD_FP64) {
+    HIP_TRY(hipMemcpyAsync(d, host, c->n_elem * 8, hipMemcpyHostToDevice, c->stream));
+  } else {
+    std::vector<float> tmp(c->n_elem);
+    for (size_t n = 0; n < c->n_elem; n++) tmp[n] = (float)host[n];
+    HIP_TRY(hipMemcpyAsync(d, tmp.data(), c->n_elem * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_get_field(monc_tvd_ctx* c, int field, int variant, double* host) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_fv(c, field, variant)) return rc;
+  if (!host) return fail(MONC_TVD_ERR_ARG, "null host pointer");
+  if (int rc = need(c, field, variant)) return rc;
+  void* d = slot(c, field, variant);
+  if (c->precision == MONC_TVD_FP64) {
+    HIP_TRY(hipMemcpyAsync(host, d, c->n_elem * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(c->n_elem);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), d, c->n_elem * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t n = 0; n < c->n_elem; n++) host[n] = (double)tmp[n];
+  }
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_halo_fill(monc_tvd_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  for (int f = 0; f < c->nfields; f++)
+    for (int v : {MONC_TVD_PROG, MONC_TVD_ZFIELD}) {
+      void* p = slot(c, f, v);
+      if (!p) continue;
+      int rc = c->precision == MONC_TVD_FP64 ? launch_halo<double>(c, (double*)p) : launch_halo<float>(c, (float*)p);
+      if (rc) return rc;
+    }
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_tendencies(monc_tvd_ctx* c, double dtm, int field_stepping, int momentum_stepping, int scalar_stepping) {
+  if (int rc = check_ctx(c)) return rc;
+  for (int s : {field_stepping, momentum_stepping, scalar_stepping})
+    if (s != MONC_TVD_CENTRED_STEPPING && s != MONC_TVD_FORWARD_STEPPING)
+      return fail(MONC_TVD_ERR_ARG, "stepping flag %d is neither CENTRED (0) nor FORWARD (1)", s);
+  if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
+  return c->precision == MONC_TVD_FP64
+             ? tendencies_impl<double>(c, dtm, field_stepping, momentum_stepping, scalar_stepping)
+             : tendencies_impl<float>(c, dtm, field_stepping, momentum_stepping, scalar_stepping);
+}
+
+int monc_tvd_forward_steps(monc_tvd_ctx* c, double dtm, int nsteps) {
+  if (int rc = check_ctx(c)) return rc;
+  if (nsteps < 0) return fail(MONC_TVD_ERR_ARG, "nsteps %d < 0", nsteps);
+  if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
+  return c->precision == MONC_TVD_FP64 ? forward_steps_impl<double>(c, dtm, nsteps)
+                                       : forward_steps_impl<float>(c, dtm, nsteps);
+}
+
+}  // extern "C"

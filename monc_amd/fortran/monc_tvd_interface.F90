@@ -1,0 +1,100 @@
+!> iso_c_binding interface to libmonc_tvd.so (include/monc_tvd.h).
+!!
+!! This is the binding MONC's Fortran host uses to reach the gfx950 TVD
+!! advection kernels.  It follows the reference's own C-interop precedent
+!! (io/src/writers/io_state_writer.F90:26-60, io/src/forthread/forthread_ciface.F90):
+!! bind(C) interfaces with value arguments and type(c_ptr) handles.
+module monc_tvd_interface_mod
+  use iso_c_binding, only : c_int, c_double, c_ptr, c_char, c_null_char, c_f_pointer, c_associated
+  implicit none
+
+  integer(c_int), parameter :: MONC_TVD_OK = 0
+  integer(c_int), parameter :: MONC_TVD_CENTRED_STEPPING = 0, MONC_TVD_FORWARD_STEPPING = 1
+  integer(c_int), parameter :: MONC_TVD_FP64 = 0, MONC_TVD_FP32 = 1
+  integer(c_int), parameter :: MONC_TVD_ADVECT_FLOW = 1, MONC_TVD_ADVECT_THETA = 2, MONC_TVD_ADVECT_Q = 4
+  integer(c_int), parameter :: MONC_TVD_U = 0, MONC_TVD_V = 1, MONC_TVD_W = 2, MONC_TVD_TH = 3
+  integer(c_int), parameter :: MONC_TVD_PROG = 0, MONC_TVD_ZFIELD = 1, MONC_TVD_SOURCE = 2
+
+  interface
+    integer(c_int) function monc_tvd_create(nz, ny, nx, halo, n_q, advect_flags, cx, cy, rdz, rdzn, dz, dzn, &
+         tzc1, tzc2, tzd1, tzd2, precision, ctx) bind(C, name="monc_tvd_create")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: nz, ny, nx, halo, n_q, advect_flags, precision
+      real(c_double), value :: cx, cy
+      real(c_double), dimension(*), intent(in) :: rdz, rdzn, dz, dzn, tzc1, tzc2, tzd1, tzd2
+      type(c_ptr), intent(out) :: ctx
+    end function monc_tvd_create
+
+    integer(c_int) function monc_tvd_destroy(ctx) bind(C, name="monc_tvd_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function monc_tvd_destroy
+
+    integer(c_int) function monc_tvd_set_field(ctx, field, variant, host) bind(C, name="monc_tvd_set_field")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field, variant
+      real(c_double), dimension(*), intent(in) :: host
+    end function monc_tvd_set_field
+
+    integer(c_int) function monc_tvd_get_field(ctx, field, variant, host) bind(C, name="monc_tvd_get_field")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field, variant
+      real(c_double), dimension(*), intent(out) :: host
+    end function monc_tvd_get_field
+
+    integer(c_int) function monc_tvd_halo_fill(ctx) bind(C, name="monc_tvd_halo_fill")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function monc_tvd_halo_fill
+
+    integer(c_int) function monc_tvd_tendencies(ctx, dtm, field_stepping, momentum_stepping, scalar_stepping) &
+         bind(C, name="monc_tvd_tendencies")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dtm
+      integer(c_int), value :: field_stepping, momentum_stepping, scalar_stepping
+    end function monc_tvd_tendencies
+
+    integer(c_int) function monc_tvd_forward_steps(ctx, dtm, nsteps) bind(C, name="monc_tvd_forward_steps")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dtm
+      integer(c_int), value :: nsteps
+    end function monc_tvd_forward_steps
+
+    integer(c_int) function monc_tvd_synchronize(ctx) bind(C, name="monc_tvd_synchronize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function monc_tvd_synchronize
+
+    type(c_ptr) function monc_tvd_last_error_c() bind(C, name="monc_tvd_last_error")
+      import :: c_ptr
+    end function monc_tvd_last_error_c
+  end interface
+
+contains
+
+  !> The last error message as a Fortran string (for log_log(LOG_ERROR, ...))
+  function monc_tvd_last_error() result(msg)
+    character(len=:), allocatable :: msg
+    character(kind=c_char), dimension(:), pointer :: chars
+    type(c_ptr) :: p
+    integer :: n
+
+    p = monc_tvd_last_error_c()
+    msg = ""
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, chars, [4096])
+    n = 0
+    do while (n < 4096)
+      if (chars(n+1) == c_null_char) exit
+      n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    do n = 1, len(msg)
+      msg(n:n) = chars(n)
+    end do
+  end function monc_tvd_last_error
+end module monc_tvd_interface_mod

@@ -1,0 +1,188 @@
+"""Python mirror of MONC's tvd_advection component over libmonc_tvd.so.
+
+The reference component exposes init / timestep / finalise callbacks on the
+model state (components/tvdadvection/src/tvdadvection.F90:62-106, :286-481,
+:528-599, :485-524).  `TvdAdvection` keeps that surface for the test harness and
+the benchmark: `initialise` copies the grid metrics to the device, `timestep`
+adds the advection tendencies to the source fields, `finalise` frees the
+device state.  All numerics run in the HIP library; if the library is not
+built, constructing a `TvdAdvection` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .setup_state import State
+
+LIB_PATH = Path(__file__).resolve().parent / "libmonc_tvd.so"
+
+OK, ERR_ARG, ERR_HIP, ERR_STATE = 0, 1, 2, 3
+CENTRED_STEPPING, FORWARD_STEPPING = 0, 1  # state.F90:15
+FP64, FP32 = 0, 1
+ADVECT_FLOW, ADVECT_THETA, ADVECT_Q = 1, 2, 4
+U, V, W, TH = 0, 1, 2, 3
+PROG, ZFIELD, SOURCE = 0, 1, 2
+
+EXPORTS = (
+    "monc_tvd_create", "monc_tvd_destroy", "monc_tvd_set_field", "monc_tvd_get_field",
+    "monc_tvd_bind_field", "monc_tvd_field_device_ptr", "monc_tvd_set_stream",
+    "monc_tvd_synchronize", "monc_tvd_halo_fill", "monc_tvd_tendencies",
+    "monc_tvd_forward_steps", "monc_tvd_last_error",
+)
+
+
+def Q(n: int) -> int:
+    return 4 + n
+
+
+class TvdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"monc_tvd error {code}: {msg}")
+        self.code = code
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
+    """Load libmonc_tvd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not path.exists():
+        raise RuntimeError(f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(str(path))
+    c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.monc_tvd_create.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_double,
+                                    dp, dp, dp, dp, dp, dp, dp, dp, c_int, ctypes.POINTER(c_void_p)]
+    lib.monc_tvd_destroy.argtypes = [c_void_p]
+    lib.monc_tvd_set_field.argtypes = [c_void_p, c_int, c_int, dp]
+    lib.monc_tvd_get_field.argtypes = [c_void_p, c_int, c_int, dp]
+    lib.monc_tvd_bind_field.argtypes = [c_void_p, c_int, c_int, c_void_p]
+    lib.monc_tvd_field_device_ptr.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(c_void_p)]
+    lib.monc_tvd_set_stream.argtypes = [c_void_p, c_void_p]
+    lib.monc_tvd_synchronize.argtypes = [c_void_p]
+    lib.monc_tvd_halo_fill.argtypes = [c_void_p]
+    lib.monc_tvd_tendencies.argtypes = [c_void_p, c_double, c_int, c_int, c_int]
+    lib.monc_tvd_forward_steps.argtypes = [c_void_p, c_double, c_int]
+    lib.monc_tvd_last_error.argtypes = []
+    lib.monc_tvd_last_error.restype = ctypes.c_char_p
+    for name in EXPORTS:
+        if name != "monc_tvd_last_error":
+            getattr(lib, name).restype = c_int
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        raise TvdError(rc, load_library().monc_tvd_last_error().decode())
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class TvdAdvection:
+    """One device-resident local domain of the tvd_advection component."""
+
+    def __init__(self, nz: int, ny: int, nx: int, cx: float, cy: float, metrics: dict, n_q: int = 1,
+                 advect: int = ADVECT_FLOW | ADVECT_THETA | ADVECT_Q, precision: int = FP64, halo: int = 2):
+        self._lib = load_library()
+        self.nz, self.ny, self.nx, self.n_q = nz, ny, nx, n_q
+        self.precision = precision
+        self._keep = [np.ascontiguousarray(metrics[k], dtype=np.float64) for k in
+                      ("rdz", "rdzn", "dz", "dzn", "tzc1", "tzc2", "tzd1", "tzd2")]
+        for v in self._keep:
+            if v.shape != (nz,):
+                raise ValueError("metric vectors must have length nz")
+        h = ctypes.c_void_p()
+        _check(self._lib.monc_tvd_create(nz, ny, nx, halo, n_q, advect, cx, cy, *[_dptr(v) for v in self._keep],
+                                         precision, ctypes.byref(h)))
+        self._h = h
+
+    # the component surface -------------------------------------------------
+    @classmethod
+    def initialise(cls, state: State, precision: int = FP64,
+                   advect: int = ADVECT_FLOW | ADVECT_THETA | ADVECT_Q) -> "TvdAdvection":
+        """initialisation_callback: metrics to the device, fields staged."""
+        t = cls(state.nz, state.ny, state.nx, state.cx, state.cy, state.grid.metric_vectors(), n_q=state.n_q,
+                advect=advect, precision=precision)
+        for n, f in enumerate(state.fields):
+            t.set_field(n, PROG, f)
+        for n, f in enumerate(state.zfields):
+            t.set_field(n, ZFIELD, f)
+        return t
+
+    def timestep(self, dtm: float, field_stepping: int = FORWARD_STEPPING, momentum_stepping: int = CENTRED_STEPPING,
+                 scalar_stepping: int = CENTRED_STEPPING) -> None:
+        """timestep_callback over every column: s += advection tendencies."""
+        self.tendencies(dtm, field_stepping, momentum_stepping, scalar_stepping)
+
+    def finalise(self) -> None:
+        self.close()
+
+    # entry points ------------------------------------------------------------
+    def field_shape(self):
+        return (self.nx + 4, self.ny + 4, self.nz)
+
+    def set_field(self, field: int, variant: int, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != self.field_shape():
+            raise ValueError(f"field shape {a.shape} != {self.field_shape()}")
+        _check(self._lib.monc_tvd_set_field(self._h, field, variant, _dptr(a)))
+
+    def get_field(self, field: int, variant: int) -> np.ndarray:
+        out = np.empty(self.field_shape(), dtype=np.float64)
+        _check(self._lib.monc_tvd_get_field(self._h, field, variant, _dptr(out)))
+        return out
+
+    def zero_sources(self, fields: Sequence[int]) -> None:
+        z = np.zeros(self.field_shape())
+        for f in fields:
+            self.set_field(f, SOURCE, z)
+
+    def bind_field(self, field: int, variant: int, tensor) -> None:
+        """Bind a torch CUDA tensor of shape field_shape() and the context dtype."""
+        import torch
+        want = torch.float64 if self.precision == FP64 else torch.float32
+        if not tensor.is_cuda or tensor.dtype != want or tuple(tensor.shape) != self.field_shape() \
+                or not tensor.is_contiguous():
+            raise ValueError("bind_field needs a contiguous CUDA tensor of the context dtype and field shape")
+        _check(self._lib.monc_tvd_bind_field(self._h, field, variant, ctypes.c_void_p(tensor.data_ptr())))
+
+    def device_ptr(self, field: int, variant: int) -> int:
+        p = ctypes.c_void_p()
+        _check(self._lib.monc_tvd_field_device_ptr(self._h, field, variant, ctypes.byref(p)))
+        return int(p.value or 0)
+
+    def set_stream(self, stream_handle: int) -> None:
+        _check(self._lib.monc_tvd_set_stream(self._h, ctypes.c_void_p(stream_handle)))
+
+    def synchronize(self) -> None:
+        _check(self._lib.monc_tvd_synchronize(self._h))
+
+    def halo_fill(self) -> None:
+        _check(self._lib.monc_tvd_halo_fill(self._h))
+
+    def tendencies(self, dtm: float, field_stepping: int, momentum_stepping: int, scalar_stepping: int) -> None:
+        _check(self._lib.monc_tvd_tendencies(self._h, dtm, field_stepping, momentum_stepping, scalar_stepping))
+
+    def forward_steps(self, dtm: float, nsteps: int) -> None:
+        _check(self._lib.monc_tvd_forward_steps(self._h, dtm, nsteps))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.monc_tvd_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

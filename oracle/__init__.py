@@ -1,0 +1,159 @@
+"""ctypes front end of the C restatement in tvd_oracle.c -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may import
+this package, and only as the checker.  The product library never uses it.
+
+Parity status per oracle function (the checks live in tests/test_oracle.py):
+
+* oracle_tvd_timestep -- pinned by closed forms (constant field, zero flow,
+  linear field under uniform flow, 1-D ULTIMATE-QUICKEST of Leonard 1991 with
+  the universal limiter), invariants (mass conservation, translation
+  F+c -> s+c*D, periodic shift equivariance, x/y mirror and x<->y
+  transposition symmetry), and the y-flux wrap equivalence.  The reference
+  Fortran is not buildable here (it needs a Fortran `mpi` module compiled for
+  amdflang; the image only has a gfortran-format mpi.mod and no gfortran), so
+  no reference-executed vectors exist.  What remains unpinned by any check:
+  the result at exact ties (signed zeros, U == 0 exactly) follows the source
+  text only.
+* oracle_halo_fill -- pinned by the periodic-image identity on random data.
+* oracle_vertical_metrics -- pinned by the telescoping identities
+  4*rhon(k)*dz(k)*tzc1(k) = rho(k-1) and their tzd counterparts.
+* oracle_forward_step -- pinned by direct evaluation of field + dtm*s.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle_tvd.so"
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = ctypes.CDLL(str(LIB))
+        dp = ctypes.POINTER(ctypes.c_double)
+        pp = ctypes.POINTER(dp)
+        ci, cd = ctypes.c_int, ctypes.c_double
+        L.oracle_tvd_timestep.argtypes = [ci, ci, ci, cd, cd, dp, dp, dp, dp, dp, dp, dp, dp, cd, ci, ci, ci, ci, ci,
+                                          pp, pp, pp, ci, dp, dp, dp]
+        L.oracle_tvd_timestep.restype = ci
+        L.oracle_halo_fill.argtypes = [dp, ci, ci, ci]
+        L.oracle_vertical_metrics.argtypes = [ci, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.oracle_horizontal_c.argtypes = [cd]
+        L.oracle_horizontal_c.restype = cd
+        L.oracle_forward_step.argtypes = [dp, dp, dp, ci, ci, ci, cd]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _pp(arrs: Sequence[np.ndarray]):
+    dp = ctypes.POINTER(ctypes.c_double)
+    return (dp * len(arrs))(*[_p(a) for a in arrs])
+
+
+def tendencies(nz: int, ny: int, nx: int, cx: float, cy: float, metrics: dict, dtm: float, field_stepping: int,
+               momentum_stepping: int, scalar_stepping: int, fields: Sequence[np.ndarray],
+               zfields: Optional[Sequence[np.ndarray]] = None, sources: Optional[Sequence[np.ndarray]] = None,
+               advect: int = 7, rec_field: int = -1):
+    """One timestep_callback sweep.  Returns (sources, faces) where sources are
+    the accumulated s-fields (copies of `sources` or zeros) and faces is
+    (fx, fy, fz) for rec_field (or None)."""
+    n = len(fields)
+    shape = (nx + 4, ny + 4, nz)
+    fields = [np.ascontiguousarray(f, dtype=np.float64) for f in fields]
+    zfields = [np.ascontiguousarray(f, dtype=np.float64) for f in (zfields if zfields else fields)]
+    srcs = [np.zeros(shape) if sources is None else np.array(sources[i], dtype=np.float64, copy=True)
+            for i in range(n)]
+    m = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in metrics.items()}
+    faces = None
+    rec = [None, None, None]
+    if rec_field >= 0:
+        faces = tuple(np.full(shape, np.nan) for _ in range(3))
+        rec = [_p(a) for a in faces]
+    rc = lib().oracle_tvd_timestep(nz, ny, nx, cx, cy, _p(m["rdz"]), _p(m["rdzn"]), _p(m["dz"]), _p(m["dzn"]),
+                                   _p(m["tzc1"]), _p(m["tzc2"]), _p(m["tzd1"]), _p(m["tzd2"]), dtm,
+                                   field_stepping, momentum_stepping, scalar_stepping, advect, n - 4,
+                                   _pp(fields), _pp(zfields), _pp(srcs), rec_field, *rec)
+    if rc != 0:
+        raise RuntimeError(f"oracle_tvd_timestep failed with {rc}")
+    return srcs, faces
+
+
+def state_tendencies(state, dtm: float, field_stepping: int = 1, momentum_stepping: int = 1,
+                     scalar_stepping: int = 1, sources=None, advect: int = 7, rec_field: int = -1):
+    return tendencies(state.nz, state.ny, state.nx, state.cx, state.cy, state.grid.metric_vectors(), dtm,
+                      field_stepping, momentum_stepping, scalar_stepping, state.fields, state.zfields or None,
+                      sources, advect, rec_field)
+
+
+def halo_fill(f: np.ndarray) -> np.ndarray:
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    nx, ny, nz = f.shape[0] - 4, f.shape[1] - 4, f.shape[2]
+    lib().oracle_halo_fill(_p(f), nz, ny, nx)
+    return f
+
+
+def vertical_metrics(z, zn, rho, rhon) -> dict:
+    nz = len(z)
+    out = {k: np.zeros(nz) for k in ("dz", "dzn", "rdz", "rdzn", "tzc1", "tzc2", "tzd1", "tzd2")}
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (z, zn, rho, rhon)]
+    lib().oracle_vertical_metrics(nz, *[_p(x) for x in a], *[_p(out[k]) for k in
+                                  ("dz", "dzn", "rdz", "rdzn", "tzc1", "tzc2", "tzd1", "tzd2")])
+    return out
+
+
+def horizontal_c(dxx: float) -> float:
+    return lib().oracle_horizontal_c(dxx)
+
+
+def forward_step(f: np.ndarray, s: np.ndarray, dtm: float) -> np.ndarray:
+    """stepfields forward step over the interior columns; halo left as in f."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    s = np.array(s, dtype=np.float64, copy=True)
+    out = f.copy()
+    nx, ny, nz = f.shape[0] - 4, f.shape[1] - 4, f.shape[2]
+    lib().oracle_forward_step(_p(out), _p(f), _p(s), nz, ny, nx, dtm)
+    return out
+
+
+def forward_run(state, dtm: float, nsteps: int, advect: int = 7):
+    """nsteps of: halo fill, FORWARD tendencies from zero, forward step, for all
+    advected fields (the resident stepping the library fuses)."""
+    fields = [halo_fill(f.copy()) for f in state.fields]
+    for _ in range(nsteps):
+        srcs, _ = tendencies(state.nz, state.ny, state.nx, state.cx, state.cy, state.grid.metric_vectors(), dtm,
+                             1, 1, 1, fields, None, None, advect)
+        adv = _advected(len(fields), advect)
+        fields = [halo_fill(forward_step(f, srcs[n], dtm)) if n in adv else f for n, f in enumerate(fields)]
+    return fields
+
+
+def _advected(n: int, advect: int):
+    s = set()
+    if advect & 1:
+        s |= {0, 1, 2}
+    if advect & 2:
+        s.add(3)
+    if advect & 4:
+        s |= set(range(4, n))
+    return s

@@ -1,0 +1,21 @@
+"""Raw-file exchange with the Fortran drop-in driver (monc_amd/fortran)."""
+import numpy as np
+
+
+def write_input(path, state, fs, ms, ss, dtm, precision=0):
+    with open(path, "wb") as f:
+        np.array([state.nz, state.ny, state.nx, state.n_q, fs, ms, ss, precision], dtype=np.int32).tofile(f)
+        np.array([state.cx, state.cy, dtm], dtype=np.float64).tofile(f)
+        for k in ("rdz", "rdzn", "dz", "dzn", "tzc1", "tzc2", "tzd1", "tzd2"):
+            np.asarray(getattr(state.grid, k), dtype=np.float64).tofile(f)
+        for a in state.fields:
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        for a in (state.zfields or state.fields):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+
+
+def read_output(path, state):
+    data = np.fromfile(path, dtype=np.float64)
+    shape = state.shape()
+    n = int(np.prod(shape))
+    return [data[i * n:(i + 1) * n].reshape(shape) for i in range(len(state.fields))]

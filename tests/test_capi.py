@@ -1,0 +1,66 @@
+"""The C ABI library loads, exports every entry point include/monc_tvd.h
+declares, and rejects invalid arguments before touching a device (CPU only)."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from monc_amd import tvd
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "monc_tvd.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(monc_tvd_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_the_binding_set():
+    assert set(declared_functions()) == set(tvd.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = tvd.load_library()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def _create(nz=8, ny=6, nx=6, halo=2, n_q=1, flags=7, cx=0.02, cy=0.02, prec=0, null_metric=False):
+    lib = tvd.load_library()
+    v = np.ones(nz)
+    dp = v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    mets = [dp] * 8
+    if null_metric:
+        mets[3] = ctypes.POINTER(ctypes.c_double)()
+    h = ctypes.c_void_p()
+    rc = lib.monc_tvd_create(nz, ny, nx, halo, n_q, flags, cx, cy, *mets, prec, ctypes.byref(h))
+    return rc, lib.monc_tvd_last_error().decode(), h
+
+
+@pytest.mark.parametrize("kw,needle", [
+    (dict(halo=3), "halo"),
+    (dict(halo=1), "halo"),
+    (dict(nz=3), "too small"),
+    (dict(ny=3), "too small"),
+    (dict(prec=7), "precision"),
+    (dict(flags=8), "advect flags"),
+    (dict(cx=0.0), "positive"),
+    (dict(n_q=-1), "n_q"),
+    (dict(null_metric=True), "metric vector"),
+])
+def test_create_rejects_invalid_arguments(kw, needle):
+    rc, msg, h = _create(**kw)
+    assert rc == tvd.ERR_ARG
+    assert needle in msg
+    assert not h.value
+
+
+def test_null_context_is_an_argument_error():
+    lib = tvd.load_library()
+    assert lib.monc_tvd_halo_fill(None) == tvd.ERR_ARG
+    assert "null context" in lib.monc_tvd_last_error().decode()
+    assert lib.monc_tvd_tendencies(None, 1.0, 1, 1, 1) == tvd.ERR_ARG
+    assert lib.monc_tvd_destroy(None) == tvd.OK
