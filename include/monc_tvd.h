@@ -92,6 +92,13 @@ int monc_tvd_get_field(monc_tvd_ctx *ctx, int field, int variant, double *host);
 int monc_tvd_bind_field(monc_tvd_ctx *ctx, int field, int variant, void *device_ptr);
 int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **device_ptr);
 
+/* Launch shape (no counterpart in the reference: a tuning knob of this
+ * library).  kernel 0 = slab kernel (every face computed once, columns of up
+ * to 256 levels), 1 = one-cell-per-thread kernel; rows = cell rows per slab
+ * block (0: default for the column height); strip = x columns per slab block
+ * (0: default 16).  Shapes that are not compiled in are rejected. */
+int monc_tvd_set_tiling(monc_tvd_ctx *ctx, int kernel, int rows, int strip);
+
 /* Run subsequent work on this hipStream_t (NULL: the legacy default stream). */
 int monc_tvd_set_stream(monc_tvd_ctx *ctx, void *hip_stream);
 int monc_tvd_synchronize(monc_tvd_ctx *ctx);

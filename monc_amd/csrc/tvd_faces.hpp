@@ -24,6 +24,15 @@ __device__ __forceinline__ int64_t at(const Dims& d, int k, int j, int i) {
   return (int64_t)(k - 1) + d.sj * (int64_t)(j - 1) + d.si * (int64_t)(i - 1);
 }
 
+// Element (k,j,i) as column base + level.  When j and i are wave-uniform the
+// base lives in scalar registers and every neighbour load is a scalar base
+// plus the one 32-bit per-lane level offset.
+template <typename T>
+__device__ __forceinline__ T ld(const T* a, const Dims& d, int k, int j, int i) {
+  const T* col = a + (d.sj * (int64_t)(j - 1) + d.si * (int64_t)(i - 1));
+  return col[(unsigned)(k - 1)];
+}
+
 // Vertical metrics as passed to ultflx.  For w the reference passes
 // (rdz, rdzn, dzn) = (model rdzn, model rdz, model dz) and kdof = 1
 // (tvdadvection.F90:881-894); the kernel builds this view accordingly.
@@ -58,14 +67,14 @@ struct Flow {
   const T* a;
   Dims d;
   __device__ __forceinline__ T operator()(int k, int j, int i) const {
-    T x = a[at(d, k, j, i)];
+    T x = ld(a, d, k, j, i);
     if constexpr (DK == 0 && DJ == 0 && DI == 0) {
       return x;
     } else {
       if constexpr (DK == 1) {
         if (k == d.nz) return x;
       }
-      return T(0.5) * (x + a[at(d, k + DK, j + DJ, i + DI)]);
+      return T(0.5) * (x + ld(a, d, k + DK, j + DJ, i + DI));
     }
   }
 };
@@ -74,7 +83,7 @@ template <typename T>
 struct Adv {  // the advected field zf
   const T* a;
   Dims d;
-  __device__ __forceinline__ T operator()(int k, int j, int i) const { return a[at(d, k, j, i)]; }
+  __device__ __forceinline__ T operator()(int k, int j, int i) const { return ld(a, d, k, j, i); }
 };
 
 // ultimateflux.F90:556-606 calculate_flux_in_x / _y (ch = cx or cy)

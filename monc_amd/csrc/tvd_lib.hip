@@ -53,6 +53,8 @@ struct monc_tvd_ctx {
   std::vector<void*> bound;    // caller pointer of a bound PROG buffer (restored after stepping)
   std::vector<void*> spare;    // library-owned second buffer of each prognostic (stepping)
   hipStream_t stream;
+  int use_cell_kernel;         // force the one-cell-per-thread kernel
+  int slab_ty, slab_strip;     // slab tile shape (0: defaults)
 };
 
 namespace {
@@ -128,13 +130,51 @@ int kind_of(int field) {
   return field == MONC_TVD_U ? KIND_U : field == MONC_TVD_V ? KIND_V : field == MONC_TVD_W ? KIND_W : KIND_SCALAR;
 }
 
-// One launch of cell_kernel over fields[first, first+count).
+// Slab tile shapes compiled in: cell rows per block (TY) for each number of
+// 64-level chunks; the strip length along x is a launch argument.
+template <typename T, int MODE, int NCH, int TY>
+void launch_slab(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int strip) {
+  dim3 block(64, NCH, TY + 1);
+  dim3 grid((c->ny + TY - 1) / TY, (c->nx + strip - 1) / strip, 1);
+  hipLaunchKernelGGL((slab_kernel<T, MODE, NCH, TY>), grid, block, 0, c->stream, L, S, strip);
+}
+
+// Default rows per block for a column of nch chunks: the largest compiled
+// shape whose register budget (1024/threads-per-block waves) avoids spills.
+int default_ty(int nch) { return nch == 1 ? 7 : (nch == 2 ? 3 : 1); }
+
+bool ty_compiled(int nch, int ty) {
+  switch (nch) {
+    case 1: return ty == 3 || ty == 7;
+    case 2: return ty == 1 || ty == 3;
+    case 3: return ty == 1;
+    case 4: return ty == 1;
+    default: return false;
+  }
+}
+
+// One launch over a field set: the slab kernel for columns of up to 256
+// levels, the one-cell-per-thread kernel beyond (or when forced).
 template <typename T, int MODE>
 int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
-  const int nk = MODE == MODE_STEP ? c->nz : c->nz - 1;
-  dim3 block(64, 4, 1);
-  dim3 grid((nk + 63) / 64, (c->ny + 3) / 4, c->nx);
-  hipLaunchKernelGGL((cell_kernel<T, MODE>), grid, block, 0, c->stream, L, S);
+  const int nch = (c->nz + 63) / 64;
+  const int ty = c->slab_ty > 0 ? c->slab_ty : default_ty(nch);
+  const int strip = c->slab_strip > 0 ? c->slab_strip : 16;
+  const int key = c->use_cell_kernel || nch > 4 ? 0 : nch * 100 + ty;
+  switch (key) {
+    case 103: launch_slab<T, MODE, 1, 3>(c, L, S, strip); break;
+    case 107: launch_slab<T, MODE, 1, 7>(c, L, S, strip); break;
+    case 201: launch_slab<T, MODE, 2, 1>(c, L, S, strip); break;
+    case 203: launch_slab<T, MODE, 2, 3>(c, L, S, strip); break;
+    case 301: launch_slab<T, MODE, 3, 1>(c, L, S, strip); break;
+    case 401: launch_slab<T, MODE, 4, 1>(c, L, S, strip); break;
+    default: {
+      const int nk = MODE == MODE_STEP ? c->nz : c->nz - 1;
+      dim3 block(64, 4, 1);
+      dim3 grid((nk + 63) / 64, (c->ny + 3) / 4, c->nx);
+      hipLaunchKernelGGL((cell_kernel<T, MODE>), grid, block, 0, c->stream, L, S);
+    }
+  }
   HIP_TRY(hipGetLastError());
   return MONC_TVD_OK;
 }
@@ -285,6 +325,9 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags,
   c->spare.assign((size_t)c->nfields, nullptr);
   c->stream = nullptr;
   c->dmet = nullptr;
+  c->use_cell_kernel = 0;
+  c->slab_ty = 0;
+  c->slab_strip = 0;
   std::vector<double> h((size_t)MET_N * nz);
   for (int m = 0; m < MET_N; m++) std::memcpy(&h[(size_t)m * nz], mets[m], sizeof(double) * nz);
   hipError_t e = hipMalloc(&c->dmet, h.size() * sizeof(double));
@@ -317,6 +360,19 @@ int monc_tvd_destroy(monc_tvd_ctx* c) {
     if (c->spare[f] && c->spare[f] != c->bound[f]) (void)hipFree(c->spare[f]);
   if (c->dmet) (void)hipFree(c->dmet);
   delete c;
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_set_tiling(monc_tvd_ctx* c, int kernel, int rows, int strip) {
+  if (int rc = check_ctx(c)) return rc;
+  if (kernel != 0 && kernel != 1) return fail(MONC_TVD_ERR_ARG, "kernel %d is neither 0 (slab) nor 1 (cell)", kernel);
+  if (strip < 0 || strip > 4096) return fail(MONC_TVD_ERR_ARG, "strip %d out of range [0,4096]", strip);
+  const int nch = (c->nz + 63) / 64;
+  if (rows != 0 && !ty_compiled(nch, rows))
+    return fail(MONC_TVD_ERR_ARG, "no slab shape with %d rows for %d-level columns", rows, c->nz);
+  c->use_cell_kernel = kernel;
+  c->slab_ty = rows;
+  c->slab_strip = strip;
   return MONC_TVD_OK;
 }
 

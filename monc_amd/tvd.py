@@ -31,7 +31,7 @@ EXPORTS = (
     "monc_tvd_create", "monc_tvd_destroy", "monc_tvd_set_field", "monc_tvd_get_field",
     "monc_tvd_bind_field", "monc_tvd_field_device_ptr", "monc_tvd_set_stream",
     "monc_tvd_synchronize", "monc_tvd_halo_fill", "monc_tvd_tendencies",
-    "monc_tvd_forward_steps", "monc_tvd_last_error",
+    "monc_tvd_forward_steps", "monc_tvd_last_error", "monc_tvd_set_tiling",
 )
 
 
@@ -66,6 +66,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_bind_field.argtypes = [c_void_p, c_int, c_int, c_void_p]
     lib.monc_tvd_field_device_ptr.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(c_void_p)]
     lib.monc_tvd_set_stream.argtypes = [c_void_p, c_void_p]
+    lib.monc_tvd_set_tiling.argtypes = [c_void_p, c_int, c_int, c_int]
     lib.monc_tvd_synchronize.argtypes = [c_void_p]
     lib.monc_tvd_halo_fill.argtypes = [c_void_p]
     lib.monc_tvd_tendencies.argtypes = [c_void_p, c_double, c_int, c_int, c_int]
@@ -160,6 +161,9 @@ class TvdAdvection:
         p = ctypes.c_void_p()
         _check(self._lib.monc_tvd_field_device_ptr(self._h, field, variant, ctypes.byref(p)))
         return int(p.value or 0)
+
+    def set_tiling(self, kernel: int = 0, rows: int = 0, strip: int = 0) -> None:
+        _check(self._lib.monc_tvd_set_tiling(self._h, kernel, rows, strip))
 
     def set_stream(self, stream_handle: int) -> None:
         _check(self._lib.monc_tvd_set_stream(self._h, ctypes.c_void_p(stream_handle)))

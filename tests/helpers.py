@@ -18,10 +18,13 @@ def rel_err(a: np.ndarray, b: np.ndarray) -> float:
     return num / den if den > 0 else num
 
 
-def gpu_tendencies(state, dtm, fs, ms, ss, sources=None, precision=tvd.FP64, advect=7):
-    """Library timestep over a State; returns the s-fields (FP64 numpy)."""
+def gpu_tendencies(state, dtm, fs, ms, ss, sources=None, precision=tvd.FP64, advect=7, tiling=None):
+    """Library timestep over a State; returns the s-fields (FP64 numpy).
+    tiling = (kernel, rows, strip) for monc_tvd_set_tiling, or None."""
     t = tvd.TvdAdvection.initialise(state, precision=precision, advect=advect)
     try:
+        if tiling is not None:
+            t.set_tiling(*tiling)
         n = len(state.fields)
         for f in range(n):
             t.set_field(f, tvd.SOURCE, np.zeros(t.field_shape()) if sources is None else sources[f])
@@ -32,9 +35,11 @@ def gpu_tendencies(state, dtm, fs, ms, ss, sources=None, precision=tvd.FP64, adv
         t.close()
 
 
-def gpu_forward(state, dtm, nsteps, precision=tvd.FP64, advect=7):
+def gpu_forward(state, dtm, nsteps, precision=tvd.FP64, advect=7, tiling=None):
     t = tvd.TvdAdvection.initialise(state, precision=precision, advect=advect)
     try:
+        if tiling is not None:
+            t.set_tiling(*tiling)
         t.halo_fill()
         t.forward_steps(dtm, nsteps)
         return [t.get_field(f, tvd.PROG) for f in range(len(state.fields))]

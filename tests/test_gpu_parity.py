@@ -33,11 +33,24 @@ def _compare(gpu_s, ref_s, tol=TOL, rows=None):
 
 @pytest.mark.parametrize("fs,ms,sc", [(1, 1, 1), (0, 0, 0), (1, 0, 0), (0, 1, 0)])
 @pytest.mark.parametrize("anelastic", [None, 8000.0])
-def test_tiny_grid_stepping_modes(gpu, fs, ms, sc, anelastic):
+@pytest.mark.parametrize("tiling", [None, (1, 0, 0)])
+def test_tiny_grid_stepping_modes(gpu, fs, ms, sc, anelastic, tiling):
     st = ss.make_state(16, 10, 12, 50.0, 500.0, seed=3, n_q=2, anelastic_scale=anelastic)
     ref, _ = oracle.state_tendencies(st, 1.0, fs, ms, sc)
-    got = gpu_tendencies(st, 1.0, fs, ms, sc)
+    got = gpu_tendencies(st, 1.0, fs, ms, sc, tiling=tiling)
     _compare(got, ref)
+
+
+# every compiled slab shape, with strips that do and do not divide nx, and
+# column heights that fill 1..4 chunks of 64 levels partially and exactly
+@pytest.mark.parametrize("nz,rows", [(64, 3), (64, 7), (70, 1), (128, 3), (150, 1), (192, 1), (256, 1), (200, 1)])
+@pytest.mark.parametrize("strip", [5, 16])
+def test_slab_shapes(gpu, nz, rows, strip):
+    st = ss.make_state(nz, 9, 11, 50.0, 25.0 * (nz - 1), seed=nz + rows, n_q=1)
+    for fs in (1, 0):
+        ref, _ = oracle.state_tendencies(st, 1.0, fs, fs, fs)
+        got = gpu_tendencies(st, 1.0, fs, fs, fs, tiling=(0, rows, strip))
+        _compare(got, ref)
 
 
 @pytest.mark.parametrize("fs", [1, 0])
@@ -106,11 +119,12 @@ def test_halo_fill_matches_periodic_copy(gpu):
         t.close()
 
 
-def test_forward_steps_match_oracle_run(gpu):
+@pytest.mark.parametrize("tiling", [None, (1, 0, 0), (0, 3, 7)])
+def test_forward_steps_match_oracle_run(gpu, tiling):
     st = ss.make_state(20, 24, 20, 50.0, 600.0, seed=12, n_q=1, with_zfields=False)
     dt = ss.cfl_dt(st, 0.3)
     ref = oracle.forward_run(st, dt, 5)
-    got = gpu_forward(st, dt, 5)
+    got = gpu_forward(st, dt, 5, tiling=tiling)
     for n, (a, b) in enumerate(zip(got, ref)):
         e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))  # halos included
         assert e <= TOL, (n, e)
