@@ -1,14 +1,17 @@
 // Face-value arithmetic of MONC's TVD (ULTIMATE-QUICKEST) advection on gfx950.
 //
-// Every function here computes ONE face value of one advected field F for one
-// level, from the limiter algebra of
+// Each function computes the value on one face of one level for NS advected
+// fields that share the same flows, from the limiter algebra of
 //   components/tvdadvection/src/ultimateflux.F90:525-614 (limiter),
 //   :616-676 (upwind stencils), :678-761 (transverse gradient terms),
-//   :81-166 (x faces), :168-234 (y faces), :236-523 (z faces),
-// and the flux divergence of components/tvdadvection/src/tvdadvection.F90:911-960.
-// Index conventions follow the reference: 1-based (k,j,i) = (z,y,x), z fastest,
-// depth-2 halo in x and y.  The same templates serve the FP64 drop-in form and
-// the FP32 form (T = double or float); metric vectors are always read as FP64.
+//   :81-166 (x faces), :168-234 (y faces), :236-523 (z faces).
+// The velocity-only quantities (outflow Courant sum and its reciprocal, face
+// Courant number, transverse weights and upwind picks) are computed once per
+// face and reused by every field; NS = 1 for a momentum field, whose flows are
+// its own dual-grid interpolation.  Index conventions follow the reference:
+// 1-based (k,j,i) = (z,y,x), z fastest, depth-2 halo in x and y.  The same
+// templates serve the FP64 drop-in form and the FP32 form (T = double or
+// float); metric vectors are always stored in FP64.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -24,14 +27,39 @@ __device__ __forceinline__ int64_t at(const Dims& d, int k, int j, int i) {
   return (int64_t)(k - 1) + d.sj * (int64_t)(j - 1) + d.si * (int64_t)(i - 1);
 }
 
-// Element (k,j,i) as column base + level.  When j and i are wave-uniform the
-// base lives in scalar registers and every neighbour load is a scalar base
-// plus the one 32-bit per-lane level offset.
+// Raw buffer loads.  A field is addressed through a buffer resource (scalar
+// registers) with a 32-bit per-lane offset and a 32-bit scalar offset: a wave
+// of 64 levels at a wave-uniform column (j,i) puts (j,i) in the scalar offset
+// and the level in the lane offset, and constant level shifts fold into the
+// instruction's immediate offset, so neighbour loads cost no vector address
+// arithmetic.  Fields must be smaller than 4 GiB (checked on the host).
 template <typename T>
-__device__ __forceinline__ T ld(const T* a, const Dims& d, int k, int j, int i) {
-  const T* col = a + (d.sj * (int64_t)(j - 1) + d.si * (int64_t)(i - 1));
-  return col[(unsigned)(k - 1)];
+__device__ __forceinline__ T ldb(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
+
+// gfx9 raw-buffer descriptor word 3 (32-bit data, no swizzle)
+constexpr int BUFFER_RSRC_WORD3 = 0x00020000;
+
+template <typename T>
+struct Arr {
+  __amdgpu_buffer_rsrc_t r;
+  unsigned sjb, sib;  // byte strides of j and i
+  __device__ __forceinline__ Arr() {}
+  __device__ __forceinline__ Arr(const T* p, const Dims& d)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)0xffffffff, BUFFER_RSRC_WORD3)),
+        sjb((unsigned)d.sj * (unsigned)sizeof(T)), sib((unsigned)d.si * (unsigned)sizeof(T)) {}
+  // Level and row in the lane offset (one register per distinct row, level
+  // shifts as immediates), column in the scalar offset (one scalar register
+  // per distinct column).  i must be wave-uniform.
+  __device__ __forceinline__ T operator()(int k, int j, int i) const {
+    return ldb<T>(r, (unsigned)(k - 1) * (unsigned)sizeof(T) + (unsigned)(j - 1) * sjb, (unsigned)(i - 1) * sib);
+  }
+  __device__ __forceinline__ T lane(int k, int j, int i) const { return (*this)(k, j, i); }
+};
 
 // Vertical metrics as passed to ultflx.  For w the reference passes
 // (rdz, rdzn, dzn) = (model rdzn, model rdz, model dz) and kdof = 1
@@ -56,6 +84,10 @@ template <typename T>
 __device__ __forceinline__ T half_sign(T x) { return copysign(T(0.5), x); }
 template <typename T>
 __device__ __forceinline__ bool sign_pos(T x) { return !signbit(x); }
+template <typename T>
+__device__ __forceinline__ T pos(T x) { return tmax(T(0.0), x); }
+template <typename T>
+__device__ __forceinline__ T negabs(T x) { return fabs(tmin(T(0.0), x)); }
 
 // A flow field as ultflx sees it.  Scalars read the model array; momentum
 // fields read its dual-grid interpolation (stencil.F90:147-169):
@@ -64,275 +96,297 @@ __device__ __forceinline__ bool sign_pos(T x) { return !signbit(x); }
 // reached: faces bounding interior cells read flows at j <= ny+3, i <= nx+3.
 template <typename T, int DK, int DJ, int DI>
 struct Flow {
-  const T* a;
-  Dims d;
+  Arr<T> a;
+  int nz;
+  __device__ __forceinline__ Flow(const T* p, const Dims& d) : a(p, d), nz(d.nz) {}
+  // (j,i) wave-uniform
   __device__ __forceinline__ T operator()(int k, int j, int i) const {
-    T x = ld(a, d, k, j, i);
+    T x = a(k, j, i);
     if constexpr (DK == 0 && DJ == 0 && DI == 0) {
       return x;
+    } else if constexpr (DK == 1) {
+      // the shifted load is clamped to level nz, where the raw value is used
+      const int k1 = k < nz ? k + 1 : k;
+      T y = a(k1, j + DJ, i + DI);
+      return k < nz ? T(0.5) * (x + y) : x;
     } else {
-      if constexpr (DK == 1) {
-        if (k == d.nz) return x;
-      }
-      return T(0.5) * (x + ld(a, d, k + DK, j + DJ, i + DI));
+      return T(0.5) * (x + a(k + DK, j + DJ, i + DI));
     }
   }
 };
 
-template <typename T>
-struct Adv {  // the advected field zf
-  const T* a;
-  Dims d;
-  __device__ __forceinline__ T operator()(int k, int j, int i) const { return ld(a, d, k, j, i); }
+// NS advected fields (zf) read at the same positions.
+template <typename T, int NS = 1>
+struct Adv {
+  Arr<T> a[NS];
+  __device__ __forceinline__ T operator()(int n, int k, int j, int i) const { return a[n](k, j, i); }
+  __device__ __forceinline__ T lane(int n, int k, int j, int i) const { return a[n].lane(k, j, i); }
 };
 
-// ultimateflux.F90:556-606 calculate_flux_in_x / _y (ch = cx or cy)
+// Limiter of ultimateflux.F90:556-606 (calculate_flux_in_x/_y): the
+// velocity-only part, shared by every field advected by the same flows...
 template <typename T>
-__device__ __forceinline__ T limit_h(T vel, T dt, T sum_cfl_out, T fdels, T fd, T fc, T fu, T fgt1, T fgt2, T ch,
-                                     T fcurvs) {
-  const T r6 = T(1.0) / T(6.0);
-  T cflmod = fabs(vel * ch * dt);
-  T hrcfl = T(1.0) / (sum_cfl_out + T(1.e-30));
-  T hcfl = T(0.5) * cflmod;
-  T cflcoef = r6 * (T(1.0) - cflmod * cflmod);
-  T rfdels = T(1.0) / (fdels + T(1.e-30));
-  T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * fcurvs - fgt1 - fgt2;
-  T fnfs = (ffs - fu) * rfdels;
-  T fncs = (fc - fu) * rfdels;
-  T fnrefs = fncs * hrcfl;
-  T ftemp = tmax(fncs, tmin(tmin(fnfs, fnrefs), T(1.0)));
-  return ftemp * fdels + fu;
-}
+struct HLim {
+  T hrcfl, hcfl, cflcoef;
+  __device__ __forceinline__ HLim(T vel, T ch, T dt, T sum_cfl_out) {
+    const T r6 = T(1.0) / T(6.0);
+    T cflmod = fabs(vel * ch * dt);
+    hrcfl = T(1.0) / (sum_cfl_out + T(1.e-30));
+    hcfl = T(0.5) * cflmod;
+    cflcoef = r6 * (T(1.0) - cflmod * cflmod);
+  }
+  // ...and the per-field part; the upwind branch (|curvature| >= |delta|)
+  // gives fc (ultimateflux.F90:104-105).
+  __device__ __forceinline__ T face(T fu, T fc, T fd, T fgt1, T fgt2) const {
+    T fdels = fd - fu;
+    T fcurvs = fd - T(2.0) * fc + fu;
+    T rfdels = T(1.0) / (fdels + T(1.e-30));
+    T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * fcurvs - fgt1 - fgt2;
+    T fnfs = (ffs - fu) * rfdels;
+    T fncs = (fc - fu) * rfdels;
+    T fnrefs = fncs * hrcfl;
+    T ftemp = tmax(fncs, tmin(tmin(fnfs, fnrefs), T(1.0)));
+    T lim = ftemp * fdels + fu;
+    return fabs(fcurvs) >= fabs(fdels) ? fc : lim;
+  }
+};
 
-// ultimateflux.F90:525-554 calculate_flux_in_w (k = the level whose top face)
+// ultimateflux.F90:525-554 calculate_flux_in_w: velocity/metric part (k = the
+// level whose top face) and per-field part.
 template <typename T>
-__device__ __forceinline__ T limit_w(int k, T wval, T dt, T sum_cfl_out, T rdc, T fdels, T fd, T fc, T fu, T rdu,
-                                     T fgt1, T fgt2, const VMet& m) {
-  const T r6 = T(1.0) / T(6.0);
-  T dd = vm<T>(m.dzn, k + 1), rdd = vm<T>(m.rdzn, k + 1);
-  T cflmod = fabs(wval * rdd * dt);
-  T hrcfl = T(1.0) / (sum_cfl_out + T(1.e-30));
-  T hcfl = T(0.5) * cflmod;
-  T cflcoef = r6 * (T(1.0) - cflmod * cflmod) * (dd * dd * rdc);
-  T rfdels = T(1.0) / (fdels + T(1.e-30));
-  T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * ((fd - fc) * rdd - (fc - fu) * rdu) - fgt1 - fgt2;
-  T fnfs = (ffs - fu) * rfdels;
-  T fncs = (fc - fu) * rfdels;
-  T fnrefs = fncs * hrcfl;
-  T ftemp = tmax(fncs, tmin(tmin(fnfs, fnrefs), T(1.0)));
-  return ftemp * fdels + fu;
-}
-
-template <typename T>
-__device__ __forceinline__ T pos(T x) { return tmax(T(0.0), x); }
-template <typename T>
-__device__ __forceinline__ T negabs(T x) { return fabs(tmin(T(0.0), x)); }
+struct WLim {
+  T hrcfl, hcfl, cflcoef, rdd, rdu;
+  __device__ __forceinline__ WLim(int k, T wval, T dt, T sum_cfl_out, T rdc, T rdu_, const VMet& m) {
+    const T r6 = T(1.0) / T(6.0);
+    T dd = vm<T>(m.dzn, k + 1);
+    rdd = vm<T>(m.rdzn, k + 1);
+    rdu = rdu_;
+    T cflmod = fabs(wval * rdd * dt);
+    hrcfl = T(1.0) / (sum_cfl_out + T(1.e-30));
+    hcfl = T(0.5) * cflmod;
+    cflcoef = r6 * (T(1.0) - cflmod * cflmod) * (dd * dd * rdc);
+  }
+  __device__ __forceinline__ T face(T fu, T fc, T fd, T fgt1, T fgt2) const {
+    T fcurvs = fd - T(2.0) * fc + fu;
+    T fdels = fd - fu;
+    T rfdels = T(1.0) / (fdels + T(1.e-30));
+    T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * ((fd - fc) * rdd - (fc - fu) * rdu) - fgt1 - fgt2;
+    T fnfs = (ffs - fu) * rfdels;
+    T fncs = (fc - fu) * rfdels;
+    T fnrefs = fncs * hrcfl;
+    T ftemp = tmax(fncs, tmin(tmin(fnfs, fnrefs), T(1.0)));
+    T lim = ftemp * fdels + fu;
+    return fabs(fcurvs) >= fabs(fdels) ? fc : lim;
+  }
+};
 
 // Right x face of column (j,i) at level k: ultimateflux.F90:102-165.
-template <typename T, class FU, class FV, class FW>
-__device__ T face_x(int k, int j, int i, const Adv<T>& F, const FU& U, const FV& V, const FW& W, const VMet& m,
-                    int kdof, T dt, T cx, T cy, int nz) {
-  int kp1 = k + 1 < nz ? k + 1 : nz;
-  int kp0 = kp1 - 1;
-  T uk = U(k, j, i);
-  T sm = half_sign(uk);
-  T ap = T(0.5) + sm, an = T(0.5) - sm;
-  T fu = F(k, j, i - 1) * ap + F(k, j, i + 2) * an;
-  T fc = F(k, j, i) * ap + F(k, j, i + 1) * an;
-  T fd = F(k, j, i + 1) * ap + F(k, j, i) * an;
-  T fdels = fd - fu;
-  T fcurvs = fd - T(2.0) * fc + fu;
-  if (fabs(fcurvs) >= fabs(fdels)) return fc;
-  T s;
-  if (uk > T(0.0)) {
-    s = vm<T>(m.rdz, k) * (pos(W(k, j, i)) + negabs(W(k - 1, j, i)));
-    s = s + cy * (pos(V(k, j, i)) + negabs(V(k, j - 1, i)));
-    s = s + cx * (uk + negabs(U(k, j, i - 1)));
-  } else {
-    s = vm<T>(m.rdz, k) * (pos(W(k, j, i + 1)) + negabs(W(k - 1, j, i + 1)));
-    s = s + cy * (pos(V(k, j, i + 1)) + negabs(V(k, j - 1, i + 1)));
-    s = s + cx * (pos(U(k, j, i + 1)) - uk);
-  }
+template <typename T, int NS, class FU, class FV, class FW>
+__device__ __forceinline__ void face_x(int k, int j, int i, const Adv<T, NS>& F, const FU& U, const FV& V,
+                                       const FW& W, const VMet& m, int kdof, T dt, T cx, T cy, int nz, T out[NS]) {
+  const int kp1 = k + 1 < nz ? k + 1 : nz;
+  const int kp0 = kp1 - 1;
+  const T uk = U(k, j, i);
+  const T sm = half_sign(uk);
+  const T ap = T(0.5) + sm, an = T(0.5) - sm;
+  // outflow Courant sum, :108-139 (strict U > 0 chooses the column)
+  const bool upos = uk > T(0.0);
+  const T w0 = W(k, j, i), w0m = W(k - 1, j, i), w1 = W(k, j, i + 1), w1m = W(k - 1, j, i + 1);
+  const T v0 = V(k, j, i), v0m = V(k, j - 1, i), v1 = V(k, j, i + 1), v1m = V(k, j - 1, i + 1);
+  T s = vm<T>(m.rdz, k) * (pos(upos ? w0 : w1) + negabs(upos ? w0m : w1m));
+  s = s + cy * (pos(upos ? v0 : v1) + negabs(upos ? v0m : v1m));
+  s = s + cx * (upos ? uk + negabs(U(k, j, i - 1)) : pos(U(k, j, i + 1)) - uk);
   s = s * dt;
   // :678-695 gradient term in z, two source columns
-  T g1;
-  {
-    T sf = T(0.125) * (W(k, j, i) + W(k - 1, j, i) + W(k, j, i + 1) + W(k - 1, j, i + 1));
-    bool p = sign_pos(sf);
-    int ksu = p ? k - 1 : kp1;
-    int ksl = p ? k : kp0;
-    T tmp = (F(ksl, j, i) - F(ksu, j, i)) * ap + (F(ksl, j, i + 1) - F(ksu, j, i + 1)) * an;
-    g1 = fabs(sf) * vm<T>(m.rdz, k + kdof) * dt * tmp;
-  }
+  const T sz = T(0.125) * (w0 + w0m + w1 + w1m);
+  const bool pz = sign_pos(sz);
+  const int ksu = pz ? k - 1 : kp1;
+  const int ksl = pz ? k : kp0;
+  const T cz = fabs(sz) * vm<T>(m.rdz, k + kdof) * dt;
   // :711-761 gradient term in y, "data_two" form
-  T g2;
-  {
-    T sf = T(0.125) * (V(k, j, i + 1) + V(k, j, i) + V(k, j - 1, i + 1) + V(k, j - 1, i));
-    int js = sign_pos(sf) ? j - 1 : j + 1;
-    T t = F(k, js, i) * ap + F(k, js, i + 1) * an;
-    g2 = fabs(sf) * cy * dt * (fc - t);
+  const T sy = T(0.125) * (v1 + v0 + v1m + v0m);
+  const int js = sign_pos(sy) ? j - 1 : j + 1;
+  const T cyt = fabs(sy) * cy * dt;
+  const HLim<T> lim(uk, cx, dt, s);
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    const T fa = F(n, k, j, i), fb = F(n, k, j, i + 1);
+    const T fu = F(n, k, j, i - 1) * ap + F(n, k, j, i + 2) * an;
+    const T fc = fa * ap + fb * an;
+    const T fd = fb * ap + fa * an;
+    const T tmp = (F(n, ksl, j, i) - F(n, ksu, j, i)) * ap + (F(n, ksl, j, i + 1) - F(n, ksu, j, i + 1)) * an;
+    const T g1 = cz * tmp;
+    const T t = F.lane(n, k, js, i) * ap + F.lane(n, k, js, i + 1) * an;
+    const T g2 = cyt * (fc - t);
+    out[n] = lim.face(fu, fc, fd, g1, g2);
   }
-  return limit_h(uk, dt, s, fdels, fd, fc, fu, g1, g2, cx, fcurvs);
 }
 
 // North y face of row j at column i, level k: ultimateflux.F90:189-233
 // (j >= 2 always here, so jperiod = 0).
-template <typename T, class FU, class FV, class FW>
-__device__ T face_y(int k, int j, int i, const Adv<T>& F, const FU& U, const FV& V, const FW& W, const VMet& m,
-                    int kdof, T dt, T cx, T cy, int nz) {
-  int kp1 = k + 1 < nz ? k + 1 : nz;
-  int kp0 = kp1 - 1;
-  T vk = V(k, j, i);
-  T sm = half_sign(vk);
-  T ap = T(0.5) + sm, an = T(0.5) - sm;
-  int jo = sign_pos(vk) ? 0 : 1;
-  T fu = F(k, j - 1, i) * ap + F(k, j + 2, i) * an;
-  T fc = F(k, j, i) * ap + F(k, j + 1, i) * an;
-  T fd = F(k, j + 1, i) * ap + F(k, j, i) * an;
-  T fdels = fd - fu;
-  T fcurvs = fd - T(2.0) * fc + fu;
-  if (fabs(fcurvs) >= fabs(fdels)) return fc;
-  int jj = j + jo;
-  T s = vm<T>(m.rdz, k) * (pos(W(k, jj, i)) + negabs(W(k - 1, jj, i)));
-  s = s + cy * (pos(V(k, jj, i)) + negabs(V(k, jj - 1, i)));
-  s = s + cx * (pos(U(k, jj, i)) + negabs(U(k, jj, i - 1)));
+template <typename T, int NS, class FU, class FV, class FW>
+__device__ __forceinline__ void face_y(int k, int j, int i, const Adv<T, NS>& F, const FU& U, const FV& V,
+                                       const FW& W, const VMet& m, int kdof, T dt, T cx, T cy, int nz, T out[NS]) {
+  const int kp1 = k + 1 < nz ? k + 1 : nz;
+  const int kp0 = kp1 - 1;
+  const T vk = V(k, j, i);
+  const T sm = half_sign(vk);
+  const T ap = T(0.5) + sm, an = T(0.5) - sm;
+  const bool vpos = sign_pos(vk);
+  const int jj = vpos ? j : j + 1;  // j + jofset
+  const T w0 = W(k, j, i), w0m = W(k - 1, j, i), w1 = W(k, j + 1, i), w1m = W(k - 1, j + 1, i);
+  const T vn = V(k, j + 1, i), vs = V(k, j - 1, i);
+  const T u0 = U(k, j, i), u0m = U(k, j, i - 1), u1 = U(k, j + 1, i), u1m = U(k, j + 1, i - 1);
+  T s = vm<T>(m.rdz, k) * (pos(vpos ? w0 : w1) + negabs(vpos ? w0m : w1m));
+  s = s + cy * (pos(vpos ? vk : vn) + negabs(vpos ? vs : vk));
+  s = s + cx * (pos(vpos ? u0 : u1) + negabs(vpos ? u0m : u1m));
   s = s * dt;
-  T g1;
-  {
-    T sf = T(0.125) * (W(k, j, i) + W(k - 1, j, i) + W(k, j + 1, i) + W(k - 1, j + 1, i));
-    bool p = sign_pos(sf);
-    int ksu = p ? k - 1 : kp1;
-    int ksl = p ? k : kp0;
-    g1 = fabs(sf) * vm<T>(m.rdz, k + kdof) * dt * (F(ksl, jj, i) - F(ksu, jj, i));
+  const T sz = T(0.125) * (w0 + w0m + w1 + w1m);
+  const bool pz = sign_pos(sz);
+  const int ksu = pz ? k - 1 : kp1;
+  const int ksl = pz ? k : kp0;
+  const T cz = fabs(sz) * vm<T>(m.rdz, k + kdof) * dt;
+  const T sx = T(0.125) * (u0m + u0 + u1m + u1);
+  const T posuf = T(0.5) + half_sign(sx);
+  const T cxt = fabs(sx) * cx * dt;
+  const HLim<T> lim(vk, cy, dt, s);
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    const T fa = F(n, k, j, i), fb = F(n, k, j + 1, i);
+    const T fu = F(n, k, j - 1, i) * ap + F(n, k, j + 2, i) * an;
+    const T fc = fa * ap + fb * an;
+    const T fd = fb * ap + fa * an;
+    const T g1 = cz * (F.lane(n, ksl, jj, i) - F.lane(n, ksu, jj, i));
+    const T g2 = cxt * (fc - (F.lane(n, k, jj, i + 1) * (T(1.0) - posuf) + F.lane(n, k, jj, i - 1) * posuf));
+    out[n] = lim.face(fu, fc, fd, g1, g2);
   }
-  T g2;
-  {
-    T sf = T(0.125) * (U(k, j, i - 1) + U(k, j, i) + U(k, j + 1, i - 1) + U(k, j + 1, i));
-    T posuf = T(0.5) + half_sign(sf);
-    g2 = fabs(sf) * cx * dt * (fc - (F(k, jj, i + 1) * (T(1.0) - posuf) + F(k, jj, i - 1) * posuf));
-  }
-  return limit_h(vk, dt, s, fdels, fd, fc, fu, g1, g2, cy, fcurvs);
 }
 
-// Transverse terms of the z faces (:711-761 single-source form, :697-709 kpv=1)
+// Transverse weights of the z faces (:711-761 single-source form and
+// :697-709 with kpv = 1), shared by the fields.
 template <typename T, class FU, class FV>
-__device__ __forceinline__ void zface_grads(int k, int j, int i, int kk, T fc, const Adv<T>& F, const FU& U,
-                                            const FV& V, T dt, T cx, T cy, T& g1, T& g2) {
-  {
-    T sf = T(0.125) * (V(k, j - 1, i) + V(k, j, i) + V(k + 1, j - 1, i) + V(k + 1, j, i));
-    int js = sign_pos(sf) ? j - 1 : j + 1;
-    g1 = fabs(sf) * cy * dt * (fc - F(kk, js, i));
+struct ZGrad {
+  int js;
+  T cyt, posuf, cxt;
+  __device__ __forceinline__ ZGrad(int k, int j, int i, const FU& U, const FV& V, T dt, T cx, T cy) {
+    const T sy = T(0.125) * (V(k, j - 1, i) + V(k, j, i) + V(k + 1, j - 1, i) + V(k + 1, j, i));
+    js = sign_pos(sy) ? j - 1 : j + 1;
+    cyt = fabs(sy) * cy * dt;
+    const T sx = T(0.125) * (U(k, j, i - 1) + U(k, j, i) + U(k + 1, j, i - 1) + U(k + 1, j, i));
+    posuf = T(0.5) + half_sign(sx);
+    cxt = fabs(sx) * cx * dt;
   }
-  {
-    T sf = T(0.125) * (U(k, j, i - 1) + U(k, j, i) + U(k + 1, j, i - 1) + U(k + 1, j, i));
-    T posuf = T(0.5) + half_sign(sf);
-    g2 = fabs(sf) * cx * dt * (fc - (F(kk, j, i + 1) * (T(1.0) - posuf) + F(kk, j, i - 1) * posuf));
+  template <int NS>
+  __device__ __forceinline__ void apply(const Adv<T, NS>& F, int n, int kk, int j, int i, T fc, T& g1,
+                                        T& g2) const {
+    g1 = cyt * (fc - F.lane(n, kk, js, i));
+    g2 = cxt * (fc - (F(n, kk, j, i + 1) * (T(1.0) - posuf) + F(n, kk, j, i - 1) * posuf));
   }
-}
+};
 
 // Top face of level k (the reference's flux_z(k+1)) for 1 <= k <= nz-1.
 // k = 1 with kmin = 1 (w): :307-411; k = nz-1: :413-523; otherwise :236-305.
 // Scalars, u and v have kmin = 2 and flux_z(2) = 0 (:78).
-template <typename T, class FU, class FV, class FW>
-__device__ T face_z(int k, int j, int i, const Adv<T>& F, const FU& U, const FV& V, const FW& W, const VMet& m,
-                    int kdof, int kmin, T dt, T cx, T cy, int nz) {
+template <typename T, int NS, class FU, class FV, class FW>
+__device__ __forceinline__ void face_z(int k, int j, int i, const Adv<T, NS>& F, const FU& U, const FV& V,
+                                       const FW& W, const VMet& m, int kdof, int kmin, T dt, T cx, T cy, int nz,
+                                       T out[NS]) {
   if (k == 1) {
-    if (kmin > 1) return T(0.0);
-    T wk = W(1, j, i);
-    T fu, fc, fd;
-    bool up = wk >= T(0.0);
-    if (up) {
-      fu = F(1, j, i);
-      fc = F(1, j, i);
-      fd = F(2, j, i);
-    } else {
-      fu = F(3, j, i);
-      fc = F(2, j, i);
-      fd = F(1, j, i);
+    if (kmin > 1) {
+#pragma unroll
+      for (int n = 0; n < NS; n++) out[n] = T(0.0);
+      return;
     }
-    T fcurvs = fd - T(2.0) * fc + fu;
-    T fdels = fd - fu;
-    if (fabs(fcurvs) >= fabs(fdels)) return fc;
-    T s, rdu, rdc, g1, g2;
-    int kk;
+    const T wk = W(1, j, i);
+    const bool up = wk >= T(0.0);
+    T s, rdu, rdc;
     if (up) {
       s = wk * vm<T>(m.rdz, 1 + kdof);
       s = s + cy * (pos(V(1, j, i)) + negabs(V(1, j - 1, i)));
       s = s + cx * (pos(U(1, j, i)) + negabs(U(1, j, i - 1)));
-      kk = 1;
       rdu = vm<T>(m.rdzn, 2);
       rdc = vm<T>(m.rdz, 1 + kdof);
     } else {
       s = vm<T>(m.rdz, 2 + kdof) * (pos(W(2, j, i)) - wk);
       s = s + cy * (pos(V(2, j, i)) + negabs(V(2, j - 1, i)));
       s = s + cx * (pos(U(2, j, i)) + negabs(U(2, j, i - 1)));
-      kk = 2;
       rdu = vm<T>(m.rdzn, 3);
       rdc = vm<T>(m.rdz, 2 + kdof);
     }
     s = s * dt;
-    zface_grads(1, j, i, kk, fc, F, U, V, dt, cx, cy, g1, g2);
-    return limit_w(1, wk, dt, s, rdc, fdels, fd, fc, fu, rdu, g1, g2, m);
+    const int kk = up ? 1 : 2;
+    const ZGrad<T, FU, FV> zg(1, j, i, U, V, dt, cx, cy);
+    const WLim<T> lim(1, wk, dt, s, rdc, rdu, m);
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      const T f1 = F(n, 1, j, i), f2 = F(n, 2, j, i);
+      const T fu = up ? f1 : F(n, 3, j, i);
+      const T fc = up ? f1 : f2;
+      const T fd = up ? f2 : f1;
+      T g1, g2;
+      zg.apply(F, n, kk, j, i, fc, g1, g2);
+      out[n] = lim.face(fu, fc, fd, g1, g2);
+    }
+    return;
   }
   if (k == nz - 1) {
-    T wk = W(k, j, i);
-    bool up = wk >= T(0.0);
-    T fu, fc, fd;
-    if (up) {
-      fu = F(k - 1, j, i);
-      fc = F(k, j, i);
-      fd = F(k + 1, j, i);
-    } else {
-      fu = T(2.0) * F(k + 1, j, i) - F(k, j, i);
-      fc = F(k + 1, j, i);
-      fd = F(k, j, i);
-    }
-    T fcurvs = fd - T(2.0) * fc + fu;
-    T fdels = fd - fu;
-    if (fabs(fcurvs) >= fabs(fdels)) return fc;
-    T s, rdu, rdc, g1, g2;
-    int kk;
+    const T wk = W(k, j, i);
+    const bool up = wk >= T(0.0);
+    T s, rdu, rdc;
     if (up) {
       s = vm<T>(m.rdz, k + kdof) * (wk + negabs(W(k - 1, j, i)));
       s = s + cy * (pos(V(k, j, i)) + negabs(V(k, j - 1, i)));
       s = s + cx * (pos(U(k, j, i)) + negabs(U(k, j, i - 1)));
-      kk = k;
       rdu = vm<T>(m.rdzn, k);
       rdc = vm<T>(m.rdz, k + kdof);
     } else {
-      int kt = k + 1 + kdof < nz ? k + 1 + kdof : nz;
+      const int kt = k + 1 + kdof < nz ? k + 1 + kdof : nz;
       s = -(wk * vm<T>(m.rdz, kt));
       s = s + cy * (pos(V(k + 1, j, i)) + negabs(V(k + 1, j - 1, i)));
       s = s + cx * (pos(U(k + 1, j, i)) + negabs(U(k + 1, j, i - 1)));
-      kk = k + 1;
       rdu = vm<T>(m.rdzn, nz);
       rdc = vm<T>(m.rdz, kt);
     }
     s = s * dt;
-    zface_grads(k, j, i, kk, fc, F, U, V, dt, cx, cy, g1, g2);
-    return limit_w(k, wk, dt, s, rdc, fdels, fd, fc, fu, rdu, g1, g2, m);
+    const int kk = up ? k : k + 1;
+    const ZGrad<T, FU, FV> zg(k, j, i, U, V, dt, cx, cy);
+    const WLim<T> lim(k, wk, dt, s, rdc, rdu, m);
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      const T fk = F(n, k, j, i), fk1 = F(n, k + 1, j, i);
+      // W < 0: linear extrapolation to a level equidistant above the top
+      const T fu = up ? F(n, k - 1, j, i) : T(2.0) * fk1 - fk;
+      const T fc = up ? fk : fk1;
+      const T fd = up ? fk1 : fk;
+      T g1, g2;
+      zg.apply(F, n, kk, j, i, fc, g1, g2);
+      out[n] = lim.face(fu, fc, fd, g1, g2);
+    }
+    return;
   }
   // middle, 2 <= k <= nz-2
-  T wk = W(k, j, i);
-  T sm = half_sign(wk);
-  T ap = T(0.5) + sm, an = T(0.5) - sm;
-  int ko = sign_pos(wk) ? 0 : 1;
-  T fu = F(k - 1, j, i) * ap + F(k + 2, j, i) * an;
-  T fc = F(k + ko, j, i);
-  T fd = F(k + 1, j, i) * ap + F(k, j, i) * an;
-  T fcurvs = fd - T(2.0) * fc + fu;
-  T fdels = fd - fu;
-  if (fabs(fcurvs) >= fabs(fdels)) return fc;
-  int kk = k + ko;
+  const T wk = W(k, j, i);
+  const T sm = half_sign(wk);
+  const T ap = T(0.5) + sm, an = T(0.5) - sm;
+  const int kk = sign_pos(wk) ? k : k + 1;  // k + kofset
   T s = vm<T>(m.rdz, kk) * (pos(W(kk, j, i)) + negabs(W(kk - 1, j, i)));
   s = s + cy * (pos(V(kk, j, i)) + negabs(V(kk, j - 1, i)));
   s = s + cx * (pos(U(kk, j, i)) + negabs(U(kk, j, i - 1)));
   s = s * dt;
-  T g1, g2;
-  zface_grads(k, j, i, kk, fc, F, U, V, dt, cx, cy, g1, g2);
-  T rdu = vm<T>(m.rdzn, k) * ap + vm<T>(m.rdzn, k + 2) * an;
-  T rdc = vm<T>(m.rdz, k + kdof) * ap + vm<T>(m.rdz, k + 1 + kdof) * an;
-  return limit_w(k, wk, dt, s, rdc, fdels, fd, fc, fu, rdu, g1, g2, m);
+  const ZGrad<T, FU, FV> zg(k, j, i, U, V, dt, cx, cy);
+  const T rdu = vm<T>(m.rdzn, k) * ap + vm<T>(m.rdzn, k + 2) * an;
+  const T rdc = vm<T>(m.rdz, k + kdof) * ap + vm<T>(m.rdz, k + 1 + kdof) * an;
+  const WLim<T> lim(k, wk, dt, s, rdc, rdu, m);
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    const T fa = F(n, k, j, i), fb = F(n, k + 1, j, i);
+    const T fu = F(n, k - 1, j, i) * ap + F(n, k + 2, j, i) * an;
+    const T fc = F(n, kk, j, i);
+    const T fd = fb * ap + fa * an;
+    T g1, g2;
+    zg.apply(F, n, kk, j, i, fc, g1, g2);
+    out[n] = lim.face(fu, fc, fd, g1, g2);
+  }
 }
 
 }  // namespace tvd

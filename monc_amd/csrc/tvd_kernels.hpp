@@ -1,4 +1,4 @@
-// Kernels of the TVD advection path: periodic halo fill, the per-cell tendency
+// Kernels of the TVD advection path: periodic halo fill, the tendency sweep
 // (drop-in `s += ...` form) and the fused forward step.
 #pragma once
 #include "tvd_faces.hpp"
@@ -7,59 +7,6 @@ namespace tvd {
 
 enum FieldKind { KIND_U = 0, KIND_V = 1, KIND_W = 2, KIND_SCALAR = 3 };
 enum Mode { MODE_ACCUM = 0, MODE_STEP = 1 };
-
-template <typename T>
-struct CellArgs {
-  Dims d;
-  const T* u;       // flows (current fields)
-  const T* v;
-  const T* w;
-  const T* F;       // advected field (field or z-field)
-  const T* f;       // current field (MODE_STEP: base of the update)
-  T* out;           // MODE_ACCUM: s-field (read-modify-write); MODE_STEP: new field
-  VMet m;
-  T dt;             // ultflx dt (dtm or 2*dtm)
-  T dtm;            // update step (MODE_STEP)
-  T cx, cy;
-};
-
-// Wrapped index of a halo position for a depth-2 periodic halo:
-// 1 -> n+1, 2 -> n+2, n+3 -> 3, n+4 -> 4 (halocommunication.F90:1121-1168).
-__device__ __forceinline__ int wrap(int x, int n) {
-  return x <= 2 ? x + n : (x >= n + 3 ? x - n : x);
-}
-
-// The tendency of one interior cell (k,j,i) of one field: the six faces that
-// bound it and the three-term divergence of tvdadvection.F90:925-959.
-template <typename T, int KIND>
-__device__ __forceinline__ T cell_tendency(const CellArgs<T>& a, int k, int j, int i, T s0) {
-  constexpr int DK = KIND == KIND_W ? 1 : 0;
-  constexpr int DJ = KIND == KIND_V ? 1 : 0;
-  constexpr int DI = KIND == KIND_U ? 1 : 0;
-  const Flow<T, DK, DJ, DI> U{a.u, a.d}, V{a.v, a.d}, W{a.w, a.d};
-  const Adv<T> F{a.F, a.d};
-  const int nz = a.d.nz;
-  const int kdof = KIND == KIND_W ? 1 : 0;
-  const int kmin = KIND == KIND_W ? 1 : 2;
-  T fxl = face_x(k, j, i - 1, F, U, V, W, a.m, kdof, a.dt, a.cx, a.cy, nz);
-  T fxr = face_x(k, j, i, F, U, V, W, a.m, kdof, a.dt, a.cx, a.cy, nz);
-  T fys = face_y(k, j - 1, i, F, U, V, W, a.m, kdof, a.dt, a.cx, a.cy, nz);
-  T fyn = face_y(k, j, i, F, U, V, W, a.m, kdof, a.dt, a.cx, a.cy, nz);
-  T fzb = face_z(k - 1, j, i, F, U, V, W, a.m, kdof, kmin, a.dt, a.cx, a.cy, nz);
-  T s;
-  if (KIND != KIND_W && k == nz) {
-    // top row is overwritten, not accumulated (:942-959)
-    s = (V(k, j - 1, i) * fys - V(k, j, i) * fyn) * a.cy;
-    s = s + T(4.0) * vm<T>(a.m.t1, k) * W(k - 1, j, i) * fzb;
-    s = s + (U(k, j, i - 1) * fxl - U(k, j, i) * fxr) * a.cx;
-  } else {
-    T fzt = face_z(k, j, i, F, U, V, W, a.m, kdof, kmin, a.dt, a.cx, a.cy, nz);
-    s = s0 + (V(k, j - 1, i) * fys - V(k, j, i) * fyn) * a.cy;
-    s = s + T(4.0) * (W(k - 1, j, i) * fzb * vm<T>(a.m.t1, k) - W(k, j, i) * fzt * vm<T>(a.m.t2, k));
-    s = s + (U(k, j, i - 1) * fxl - U(k, j, i) * fxr) * a.cx;
-  }
-  return s;
-}
 
 // Largest number of fields one launch advects (u, v, w, th and up to 12
 // scalars); the host splits longer scalar lists over several launches.
@@ -86,49 +33,124 @@ struct LaunchArgs {
   T dtm, cx, cy;
 };
 
-template <typename T, int KIND, int MODE>
-__device__ __forceinline__ void do_field(const LaunchArgs<T>& L, const FieldSet<T>& S, int n, int k, int j, int i) {
-  CellArgs<T> a;
-  a.d = L.d;
-  a.u = L.u;
-  a.v = L.v;
-  a.w = L.w;
-  a.F = S.F[n];
-  a.f = S.f[n];
-  a.out = S.out[n];
-  a.m = KIND == KIND_W ? L.mw : L.mc;
-  a.dt = S.dt[n];
-  a.dtm = L.dtm;
-  a.cx = L.cx;
-  a.cy = L.cy;
+// Wrapped index of a halo position for a depth-2 periodic halo:
+// 1 -> n+1, 2 -> n+2, n+3 -> 3, n+4 -> 4 (halocommunication.F90:1121-1168).
+__device__ __forceinline__ int wrap(int x, int n) {
+  return x <= 2 ? x + n : (x >= n + 3 ? x - n : x);
+}
+
+// Flows of a field kind: raw for scalars, dual-interpolated along the primal
+// dimension of u (x), v (y) or w (z) for momentum.
+template <typename T, int KIND>
+struct Flows {
+  static constexpr int DK = KIND == KIND_W ? 1 : 0;
+  static constexpr int DJ = KIND == KIND_V ? 1 : 0;
+  static constexpr int DI = KIND == KIND_U ? 1 : 0;
+  using F = Flow<T, DK, DJ, DI>;
+  F U, V, W;
+  __device__ __forceinline__ Flows(const LaunchArgs<T>& L) : U(L.u, L.d), V(L.v, L.d), W(L.w, L.d) {}
+};
+
+// Flux-divergence weights of one cell (tvdadvection.F90:925-959), shared by
+// the fields advected by the same flows.
+template <typename T>
+struct DivW {
+  T vs, vn, wb, wt, t1, t2, ul, ur;
+  template <class FL>
+  __device__ __forceinline__ DivW(const FL& fl, const VMet& m, int k, int j, int i, int nz) {
+    vs = fl.V(k, j - 1, i);
+    vn = fl.V(k, j, i);
+    wb = fl.W(k - 1, j, i);
+    ul = fl.U(k, j, i - 1);
+    ur = fl.U(k, j, i);
+    t1 = vm<T>(m.t1, k);
+    const bool below_top = k < nz;
+    wt = below_top ? fl.W(k, j, i) : T(0);
+    t2 = below_top ? vm<T>(m.t2, k) : T(0);
+  }
+  // s += three terms in the reference's order; `top` = level nz of u, v and
+  // scalars, which is overwritten rather than accumulated (:942-959)
+  __device__ __forceinline__ T apply(bool top, T s0, T cx, T cy, T fxl, T fxr, T fys, T fyn, T fzb,
+                                     T fzt) const {
+    T s;
+    if (top) {
+      s = (vs * fys - vn * fyn) * cy;
+      s = s + T(4.0) * t1 * wb * fzb;
+      s = s + (ul * fxl - ur * fxr) * cx;
+    } else {
+      s = s0 + (vs * fys - vn * fyn) * cy;
+      s = s + T(4.0) * (wb * fzb * t1 - wt * fzt * t2);
+      s = s + (ul * fxl - ur * fxr) * cx;
+    }
+    return s;
+  }
+};
+
+// MODE_STEP output: out = f + dtm*s at the cell and at every halo position
+// that is its periodic image (ny, nx >= 4, so each image is unique).
+template <typename T>
+__device__ __forceinline__ void store_step(const Dims& d, T* o, const T* f, T dtm, T s, int k, int j, int i) {
+  const int64_t p = at(d, k, j, i);
+  const T val = f[p] + dtm * s;
+  const int jh = (j <= 4) ? j + d.ny : (j >= d.ny + 1 ? j - d.ny : 0);
+  const int ih = (i <= 4) ? i + d.nx : (i >= d.nx + 1 ? i - d.nx : 0);
+  o[p] = val;
+  if (jh) o[at(d, k, jh, i)] = val;
+  if (ih) o[at(d, k, j, ih)] = val;
+  if (jh && ih) o[at(d, k, jh, ih)] = val;
+}
+
+// ---------------------------------------------------------------------------
+// Cell kernel: one thread per cell, the six bounding faces recomputed per
+// cell.  Lanes along z (coalesced 64-level runs), 4 rows of y per block, one x
+// column per block row.  Used for columns taller than the slab kernel allows.
+template <typename T, int KIND, int NS, int MODE>
+__device__ __forceinline__ void cell_group(const LaunchArgs<T>& L, const FieldSet<T>& S, int n0, int k, int j,
+                                           int i) {
+  const Flows<T, KIND> fl(L);
+  const VMet m = KIND == KIND_W ? L.mw : L.mc;
   const int nz = L.d.nz;
+  const int kdof = KIND == KIND_W ? 1 : 0;
+  const int kmin = KIND == KIND_W ? 1 : 2;
   const int klast = KIND == KIND_W ? nz - 1 : nz;
-  if (MODE == MODE_ACCUM) {
-    if (k < 2 || k > klast) return;
-    const int64_t p = at(a.d, k, j, i);
-    a.out[p] = cell_tendency<T, KIND>(a, k, j, i, a.out[p]);
-  } else {
-    T s = T(0.0);
-    if (k >= 2 && k <= klast) s = cell_tendency<T, KIND>(a, k, j, i, T(0.0));
-    const T val = a.f[at(a.d, k, j, i)] + a.dtm * s;
-    const int ny = a.d.ny, nx = a.d.nx;
-    // the cell itself and every halo position that maps onto it (ny, nx >= 4)
-    const int jh = (j <= 4) ? j + ny : (j >= ny + 1 ? j - ny : 0);
-    const int ih = (i <= 4) ? i + nx : (i >= nx + 1 ? i - nx : 0);
-    a.out[at(a.d, k, j, i)] = val;
-    if (jh) a.out[at(a.d, k, jh, i)] = val;
-    if (ih) a.out[at(a.d, k, j, ih)] = val;
-    if (jh && ih) a.out[at(a.d, k, jh, ih)] = val;
+  const T dt = S.dt[n0];
+  Adv<T, NS> F{};
+#pragma unroll
+  for (int n = 0; n < NS; n++) F.a[n] = Arr<T>(S.F[n0 + n], L.d);
+  const bool cell = k >= 2 && k <= klast;
+  T fxl[NS], fxr[NS], fys[NS], fyn[NS], fzb[NS], fzt[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) fxl[n] = fxr[n] = fys[n] = fyn[n] = fzb[n] = fzt[n] = T(0);
+  const bool top = KIND != KIND_W && k == nz;
+  if (cell) {
+    face_x(k, j, i - 1, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fxl);
+    face_x(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fxr);
+    face_y(k, j - 1, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fys);
+    face_y(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fyn);
+    face_z(k - 1, j, i, F, fl.U, fl.V, fl.W, m, kdof, kmin, dt, L.cx, L.cy, nz, fzb);
+    if (!top) face_z(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, kmin, dt, L.cx, L.cy, nz, fzt);
+  }
+  const int64_t p = at(L.d, k, j, i);
+  if (cell) {  // the weights read flows at k-1: only for cells (k >= 2)
+    const DivW<T> dw(fl, m, k, j, i, nz);
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      if (MODE == MODE_ACCUM)
+        S.out[n0 + n][p] = dw.apply(top, S.out[n0 + n][p], L.cx, L.cy, fxl[n], fxr[n], fys[n], fyn[n], fzb[n],
+                                    fzt[n]);
+      else
+        store_step(L.d, S.out[n0 + n], S.f[n0 + n], L.dtm,
+                   dw.apply(top, T(0), L.cx, L.cy, fxl[n], fxr[n], fys[n], fyn[n], fzb[n], fzt[n]), k, j, i);
+    }
+  } else if (MODE == MODE_STEP) {
+#pragma unroll
+    for (int n = 0; n < NS; n++) store_step(L.d, S.out[n0 + n], S.f[n0 + n], L.dtm, T(0), k, j, i);
   }
 }
 
-// One thread per cell of the interior, every field of the set: lanes along z
-// (coalesced 64-level runs), 4 rows of y per block, one x column per block row.
-// MODE_ACCUM: levels 2..nz (2..nz-1 for w), out(k,j,i) += tendency, level nz
-// overwritten (tvdadvection.F90:942-959).
-// MODE_STEP: levels 1..nz, out = f + dtm * tendency with tendency 0 where the
-// reference leaves s untouched (k = 1, and k = nz for w), and the periodic halo
-// images of the updated cell are written too.
+// MODE_ACCUM: levels 2..nz (2..nz-1 for w), out += tendency, level nz
+// overwritten.  MODE_STEP: levels 1..nz, out = f + dtm * tendency (0 where the
+// reference leaves s untouched: k = 1, and k = nz for w), halo images too.
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) cell_kernel(LaunchArgs<T> L, FieldSet<T> S) {
   const int k0 = MODE == MODE_STEP ? 1 : 2;
@@ -138,10 +160,10 @@ __global__ void __launch_bounds__(256) cell_kernel(LaunchArgs<T> L, FieldSet<T> 
   if (j > L.d.ny + 2 || k > L.d.nz) return;
   for (int n = 0; n < S.nf; n++) {
     switch (S.kind[n]) {
-      case KIND_U: do_field<T, KIND_U, MODE>(L, S, n, k, j, i); break;
-      case KIND_V: do_field<T, KIND_V, MODE>(L, S, n, k, j, i); break;
-      case KIND_W: do_field<T, KIND_W, MODE>(L, S, n, k, j, i); break;
-      default: do_field<T, KIND_SCALAR, MODE>(L, S, n, k, j, i); break;
+      case KIND_U: cell_group<T, KIND_U, 1, MODE>(L, S, n, k, j, i); break;
+      case KIND_V: cell_group<T, KIND_V, 1, MODE>(L, S, n, k, j, i); break;
+      case KIND_W: cell_group<T, KIND_W, 1, MODE>(L, S, n, k, j, i); break;
+      default: cell_group<T, KIND_SCALAR, 1, MODE>(L, S, n, k, j, i); break;
     }
   }
 }
@@ -155,27 +177,30 @@ __global__ void __launch_bounds__(256) cell_kernel(LaunchArgs<T> L, FieldSet<T> 
 // (kept in a register as the next step's left face), the north y face and the
 // top z face.  The south y face comes from the row below through LDS (an extra
 // face-only wave row j0-1 supplies it for the first row), the bottom z face
-// from the lane below by a wave shuffle (across chunks through LDS).
+// from the lane below by a wave shuffle (across chunks through LDS).  Scalars
+// advected by the same flows are processed in pairs, sharing every
+// velocity-only term.
+constexpr int SLAB_NS = 2;
+
 template <typename T, int NCH, int TY>
 struct SlabLds {
-  T yn[2][TY + 1][64 * NCH];  // north y faces, double-buffered by step parity
-  T zt[2][TY + 1][NCH];       // top z face of the last level of each chunk
+  T yn[2][TY + 1][SLAB_NS][64 * NCH];  // north y faces, double-buffered by step parity
+  T zt[2][TY + 1][SLAB_NS][NCH];       // top z face of the last level of each chunk
 };
 
-template <typename T, int KIND, int MODE, int NCH, int TY>
-__device__ __forceinline__ void slab_field(const LaunchArgs<T>& L, const FieldSet<T>& S, int n, int i_begin,
+template <typename T, int KIND, int NS, int MODE, int NCH, int TY>
+__device__ __forceinline__ void slab_group(const LaunchArgs<T>& L, const FieldSet<T>& S, int n0, int i_begin,
                                            int i_end, SlabLds<T, NCH, TY>& lds) {
-  constexpr int DK = KIND == KIND_W ? 1 : 0;
-  constexpr int DJ = KIND == KIND_V ? 1 : 0;
-  constexpr int DI = KIND == KIND_U ? 1 : 0;
-  const Flow<T, DK, DJ, DI> U{L.u, L.d}, V{L.v, L.d}, W{L.w, L.d};
-  const Adv<T> F{S.F[n], L.d};
+  const Flows<T, KIND> fl(L);
   const VMet m = KIND == KIND_W ? L.mw : L.mc;
-  const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;
+  const int nz = L.d.nz, ny = L.d.ny;
   const int kdof = KIND == KIND_W ? 1 : 0;
   const int kmin = KIND == KIND_W ? 1 : 2;
   const int kmax = KIND == KIND_W ? nz - 1 : nz;  // last level with x/y faces and a cell
-  const T dt = S.dt[n];
+  const T dt = S.dt[n0];
+  Adv<T, NS> F{};
+#pragma unroll
+  for (int n = 0; n < NS; n++) F.a[n] = Arr<T>(S.F[n0 + n], L.d);
   // a wave is 64 levels of one (chunk, row): c and r are wave-uniform
   const int lane = threadIdx.x;
   const int c = __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -187,75 +212,65 @@ __device__ __forceinline__ void slab_field(const LaunchArgs<T>& L, const FieldSe
   const bool hface = row_ok && k >= 2 && k <= kmax;  // level has x/y faces
   const bool cell = cell_row && k >= 2 && k <= kmax;
   const bool zface = cell_row && k <= nz - 1;
-  T fxl = T(0);
-  if (cell_row && hface) fxl = face_x(k, j, i_begin - 1, F, U, V, W, m, kdof, dt, L.cx, L.cy, nz);
+  const bool top = KIND != KIND_W && k == nz;
+  T fxl[NS], fxr[NS], fyn[NS], fzt[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) fxl[n] = T(0);
+  if (cell_row && hface) face_x(k, j, i_begin - 1, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fxl);
   for (int i = i_begin; i < i_end; i++) {
     const int b = i & 1;
-    T fyn = T(0), fxr = T(0), fzt = T(0);
-    // the fences keep the scheduler from interleaving the three face
-    // computations, which bounds the live registers to one face's worth
-    if (hface) fyn = face_y(k, j, i, F, U, V, W, m, kdof, dt, L.cx, L.cy, nz);
-    __builtin_amdgcn_sched_barrier(0);
-    if (cell_row && hface) fxr = face_x(k, j, i, F, U, V, W, m, kdof, dt, L.cx, L.cy, nz);
-    __builtin_amdgcn_sched_barrier(0);
-    if (zface) fzt = face_z(k, j, i, F, U, V, W, m, kdof, kmin, dt, L.cx, L.cy, nz);
-    __builtin_amdgcn_sched_barrier(0);
-    lds.yn[b][r][64 * c + lane] = fyn;
-    if (lane == 63) lds.zt[b][r][c] = fzt;
-    __syncthreads();
-    T fzb = __shfl_up(fzt, 1);
-    if (lane == 0) fzb = c > 0 ? lds.zt[b][r][c - 1] : T(0);
-    if (cell_row && k <= nz) {
-      T s;
-      const int64_t p = at(L.d, k, j, i);
-      if (cell) {
-        const T fys = lds.yn[b][r - 1][64 * c + lane];
-        const T s0 = MODE == MODE_ACCUM ? S.out[n][p] : T(0);
-        if (KIND != KIND_W && k == nz) {
-          s = (V(k, j - 1, i) * fys - V(k, j, i) * fyn) * L.cy;
-          s = s + T(4.0) * vm<T>(m.t1, k) * W(k - 1, j, i) * fzb;
-          s = s + (U(k, j, i - 1) * fxl - U(k, j, i) * fxr) * L.cx;
-        } else {
-          s = s0 + (V(k, j - 1, i) * fys - V(k, j, i) * fyn) * L.cy;
-          s = s + T(4.0) * (W(k - 1, j, i) * fzb * vm<T>(m.t1, k) - W(k, j, i) * fzt * vm<T>(m.t2, k));
-          s = s + (U(k, j, i - 1) * fxl - U(k, j, i) * fxr) * L.cx;
-        }
-      } else {
-        s = T(0);
-      }
-      if (MODE == MODE_ACCUM) {
-        if (cell) S.out[n][p] = s;
-      } else {
-        const T val = S.f[n][p] + L.dtm * s;
-        const int jh = (j <= 4) ? j + ny : (j >= ny + 1 ? j - ny : 0);
-        const int ih = (i <= 4) ? i + nx : (i >= nx + 1 ? i - nx : 0);
-        T* o = S.out[n];
-        o[p] = val;
-        if (jh) o[at(L.d, k, jh, i)] = val;
-        if (ih) o[at(L.d, k, j, ih)] = val;
-        if (jh && ih) o[at(L.d, k, jh, ih)] = val;
-      }
+#pragma unroll
+    for (int n = 0; n < NS; n++) fxr[n] = fyn[n] = fzt[n] = T(0);
+    if (hface) face_y(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fyn);
+    if (cell_row && hface) face_x(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fxr);
+    if (zface) face_z(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, kmin, dt, L.cx, L.cy, nz, fzt);
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      lds.yn[b][r][n][64 * c + lane] = fyn[n];
+      if (lane == 63) lds.zt[b][r][n][c] = fzt[n];
     }
-    fxl = fxr;
+    __syncthreads();
+    T fzb[NS], fys[NS];
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      fzb[n] = __shfl_up(fzt[n], 1);
+      if (lane == 0) fzb[n] = c > 0 ? lds.zt[b][r][n][c - 1] : T(0);
+      fys[n] = r > 0 ? lds.yn[b][r - 1][n][64 * c + lane] : T(0);
+    }
+    if (cell) {  // the weights read flows at k-1: only for cells (k >= 2)
+      const DivW<T> dw(fl, m, k, j, i, nz);
+      const int64_t p = at(L.d, k, j, i);
+#pragma unroll
+      for (int n = 0; n < NS; n++) {
+        if (MODE == MODE_ACCUM)
+          S.out[n0 + n][p] = dw.apply(top, S.out[n0 + n][p], L.cx, L.cy, fxl[n], fxr[n], fys[n], fyn[n], fzb[n],
+                                      fzt[n]);
+        else
+          store_step(L.d, S.out[n0 + n], S.f[n0 + n], L.dtm,
+                     dw.apply(top, T(0), L.cx, L.cy, fxl[n], fxr[n], fys[n], fyn[n], fzb[n], fzt[n]), k, j, i);
+      }
+    } else if (MODE == MODE_STEP && cell_row && k <= nz) {
+#pragma unroll
+      for (int n = 0; n < NS; n++) store_step(L.d, S.out[n0 + n], S.f[n0 + n], L.dtm, T(0), k, j, i);
+    }
+#pragma unroll
+    for (int n = 0; n < NS; n++) fxl[n] = fxr[n];
   }
-  __syncthreads();  // the next field reuses the LDS buffers
+  __syncthreads();  // the next group reuses the LDS buffers
 }
 
-template <typename T, int MODE, int NCH, int TY>
+// One launch advects one field kind: a momentum field (NS = 1) or a group of
+// NS scalars; S.F[0..NS) etc. hold that group.  Separate launches keep each
+// kernel's register demand to one kind's working set.
+template <typename T, int KIND, int NS, int MODE, int NCH, int TY>
 __global__ void __launch_bounds__(64 * NCH * (TY + 1)) slab_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip) {
   __shared__ SlabLds<T, NCH, TY> lds;
   const int i_begin = 3 + (int)blockIdx.y * strip;
   const int i_end = min(i_begin + strip, L.d.nx + 3);
-  for (int n = 0; n < S.nf; n++) {
-    switch (S.kind[n]) {
-      case KIND_U: slab_field<T, KIND_U, MODE, NCH, TY>(L, S, n, i_begin, i_end, lds); break;
-      case KIND_V: slab_field<T, KIND_V, MODE, NCH, TY>(L, S, n, i_begin, i_end, lds); break;
-      case KIND_W: slab_field<T, KIND_W, MODE, NCH, TY>(L, S, n, i_begin, i_end, lds); break;
-      default: slab_field<T, KIND_SCALAR, MODE, NCH, TY>(L, S, n, i_begin, i_end, lds); break;
-    }
-  }
+  slab_group<T, KIND, NS, MODE, NCH, TY>(L, S, 0, i_begin, i_end, lds);
 }
 
+// ---------------------------------------------------------------------------
 // Periodic fill of the depth-2 x/y halo of one field, corners included.  One
 // thread per (k, halo column); the halo columns are the 4 rows j in {1,2,ny+3,
 // ny+4} over every i, then the 4 columns i in {1,2,nx+3,nx+4} over interior j.

@@ -55,6 +55,7 @@ struct monc_tvd_ctx {
   hipStream_t stream;
   int use_cell_kernel;         // force the one-cell-per-thread kernel
   int slab_ty, slab_strip;     // slab tile shape (0: defaults)
+  int slab_ok;                 // fields small enough for 32-bit buffer offsets
 };
 
 namespace {
@@ -132,15 +133,15 @@ int kind_of(int field) {
 
 // Slab tile shapes compiled in: cell rows per block (TY) for each number of
 // 64-level chunks; the strip length along x is a launch argument.
-template <typename T, int MODE, int NCH, int TY>
+template <typename T, int KIND, int NS, int MODE, int NCH, int TY>
 void launch_slab(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int strip) {
   dim3 block(64, NCH, TY + 1);
   dim3 grid((c->ny + TY - 1) / TY, (c->nx + strip - 1) / strip, 1);
-  hipLaunchKernelGGL((slab_kernel<T, MODE, NCH, TY>), grid, block, 0, c->stream, L, S, strip);
+  hipLaunchKernelGGL((slab_kernel<T, KIND, NS, MODE, NCH, TY>), grid, block, 0, c->stream, L, S, strip);
 }
 
 // Default rows per block for a column of nch chunks: the largest compiled
-// shape whose register budget (1024/threads-per-block waves) avoids spills.
+// shape within 512 threads per block.
 int default_ty(int nch) { return nch == 1 ? 7 : (nch == 2 ? 3 : 1); }
 
 bool ty_compiled(int nch, int ty) {
@@ -153,29 +154,57 @@ bool ty_compiled(int nch, int ty) {
   }
 }
 
-// One launch over a field set: the slab kernel for columns of up to 256
-// levels, the one-cell-per-thread kernel beyond (or when forced).
+template <typename T, int KIND, int NS, int MODE>
+void launch_slab_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int key, int strip) {
+  switch (key) {
+    case 103: launch_slab<T, KIND, NS, MODE, 1, 3>(c, L, S, strip); break;
+    case 107: launch_slab<T, KIND, NS, MODE, 1, 7>(c, L, S, strip); break;
+    case 201: launch_slab<T, KIND, NS, MODE, 2, 1>(c, L, S, strip); break;
+    case 203: launch_slab<T, KIND, NS, MODE, 2, 3>(c, L, S, strip); break;
+    case 301: launch_slab<T, KIND, NS, MODE, 3, 1>(c, L, S, strip); break;
+    case 401: launch_slab<T, KIND, NS, MODE, 4, 1>(c, L, S, strip); break;
+  }
+}
+
+// Launches over a field set: per field kind (u, v, w, scalars in pairs) the
+// slab kernel for columns of up to 256 levels; the one-cell-per-thread kernel
+// (all fields in one launch) beyond that or when forced.
 template <typename T, int MODE>
 int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
   const int nch = (c->nz + 63) / 64;
   const int ty = c->slab_ty > 0 ? c->slab_ty : default_ty(nch);
   const int strip = c->slab_strip > 0 ? c->slab_strip : 16;
-  const int key = c->use_cell_kernel || nch > 4 ? 0 : nch * 100 + ty;
-  switch (key) {
-    case 103: launch_slab<T, MODE, 1, 3>(c, L, S, strip); break;
-    case 107: launch_slab<T, MODE, 1, 7>(c, L, S, strip); break;
-    case 201: launch_slab<T, MODE, 2, 1>(c, L, S, strip); break;
-    case 203: launch_slab<T, MODE, 2, 3>(c, L, S, strip); break;
-    case 301: launch_slab<T, MODE, 3, 1>(c, L, S, strip); break;
-    case 401: launch_slab<T, MODE, 4, 1>(c, L, S, strip); break;
-    default: {
-      const int nk = MODE == MODE_STEP ? c->nz : c->nz - 1;
-      dim3 block(64, 4, 1);
-      dim3 grid((nk + 63) / 64, (c->ny + 3) / 4, c->nx);
-      hipLaunchKernelGGL((cell_kernel<T, MODE>), grid, block, 0, c->stream, L, S);
-    }
+  if (c->use_cell_kernel || nch > 4 || !c->slab_ok) {
+    const int nk = MODE == MODE_STEP ? c->nz : c->nz - 1;
+    dim3 block(64, 4, 1);
+    dim3 grid((nk + 63) / 64, (c->ny + 3) / 4, c->nx);
+    hipLaunchKernelGGL((cell_kernel<T, MODE>), grid, block, 0, c->stream, L, S);
+    HIP_TRY(hipGetLastError());
+    return MONC_TVD_OK;
   }
-  HIP_TRY(hipGetLastError());
+  const int key = nch * 100 + ty;
+  for (int n = 0; n < S.nf;) {
+    FieldSet<T> G;
+    const int pair = S.kind[n] == KIND_SCALAR && n + 1 < S.nf && S.kind[n + 1] == KIND_SCALAR && S.dt[n + 1] == S.dt[n];
+    G.nf = pair ? 2 : 1;
+    for (int m = 0; m < G.nf; m++) {
+      G.kind[m] = S.kind[n + m];
+      G.F[m] = S.F[n + m];
+      G.f[m] = S.f[n + m];
+      G.out[m] = S.out[n + m];
+      G.dt[m] = S.dt[n + m];
+    }
+    switch (S.kind[n]) {
+      case KIND_U: launch_slab_shape<T, KIND_U, 1, MODE>(c, L, G, key, strip); break;
+      case KIND_V: launch_slab_shape<T, KIND_V, 1, MODE>(c, L, G, key, strip); break;
+      case KIND_W: launch_slab_shape<T, KIND_W, 1, MODE>(c, L, G, key, strip); break;
+      default:
+        if (pair) launch_slab_shape<T, KIND_SCALAR, 2, MODE>(c, L, G, key, strip);
+        else launch_slab_shape<T, KIND_SCALAR, 1, MODE>(c, L, G, key, strip);
+    }
+    HIP_TRY(hipGetLastError());
+    n += G.nf;
+  }
   return MONC_TVD_OK;
 }
 
@@ -328,6 +357,7 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags,
   c->use_cell_kernel = 0;
   c->slab_ty = 0;
   c->slab_strip = 0;
+  c->slab_ok = n_elem * c->elem_size < ((size_t)1 << 32) - ((size_t)1 << 20);
   std::vector<double> h((size_t)MET_N * nz);
   for (int m = 0; m < MET_N; m++) std::memcpy(&h[(size_t)m * nz], mets[m], sizeof(double) * nz);
   hipError_t e = hipMalloc(&c->dmet, h.size() * sizeof(double));
