@@ -5,13 +5,16 @@ Workload (the plain FP64 form at real shape): a 512x512x256 local domain
 seeded distributions of the synthetic set-up, advection-only forward-Euler
 steps at CFL 0.3.  One step = the TVD tendencies of all five fields from the
 pre-step state plus the update field <- field + dt*s and the periodic halo of
-the new state, fused in one kernel launch (monc_tvd_forward_steps).  The same
-step in FP32 is reported beside it.
+the new state (monc_tvd_forward_steps): four launches of the wave kernel, one
+each for u, v, w and the (th, q) pair.  The same step in FP32 is reported
+beside it.
 
 Output: one JSON line.  value = interior cell-updates per second (all five
-fields) of the FP64 form; roofline = the step kernel's algorithmic bytes over
-its average duration (HIP events on the launch stream); cpu_baseline = the C
-restatement (oracle) timed on a bounded sample on this host's CPU.
+fields) of the FP64 form, wall clock over the timed steps; roofline = the
+dominant kernel's algorithmic bytes per launch over its average duration
+(HIP events recorded by the library around every launch on its stream);
+cpu_baseline = the C restatement (oracle) timed on a bounded sample on this
+host's CPU.
 """
 from __future__ import annotations
 
@@ -32,6 +35,12 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+# Algorithmic HBM bytes per interior cell of one launch of each step kernel
+# (element size es): u, v and w kernels read u, v, w and write their field;
+# the scalar kernel reads u, v, w and its two scalars and writes both.
+KIND_FIELDS = {"u": 4, "v": 4, "w": 4, "scalars": 7, "cell": 10}
+
+
 def run_form(state, precision: int, steps: int, warmup: int, dt: float):
     import torch
 
@@ -44,31 +53,40 @@ def run_form(state, precision: int, steps: int, warmup: int, dt: float):
         t.halo_fill()
         for _ in range(warmup):
             t.forward_steps(dt, 1)
-        torch.cuda.synchronize()
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t.synchronize()
+        t.set_profiling(True)
+        t.kernel_times(reset=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for s in range(steps):
-            evs[s][0].record(stream)
-            t.forward_steps(dt, 1)  # exactly one kernel launch per step
-            evs[s][1].record(stream)
+        t.forward_steps(dt, steps)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        kms = [a.elapsed_time(b) for a, b in evs]
-        return wall, kms
+        kt = t.kernel_times(reset=True)
+        t.set_profiling(False)
+        return wall, kt
     finally:
         t.close()
 
 
+def roofline(kt: dict, cells: int, es: int):
+    """Dominant kernel of the step: the kind with the largest total time."""
+    kind = max((k for k in kt if k in KIND_FIELDS), key=lambda k: kt[k][0])
+    ms_total, n = kt[kind]
+    avg_ms = ms_total / n
+    bytes_launch = KIND_FIELDS[kind] * es * cells
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    return kind, avg_ms, bytes_launch, achieved
+
+
 def cpu_baseline(nz: int, seconds_hint: float = 12.0):
     """The oracle (single-threaded C restatement) on a bounded sample of the
-    same workload: one forward step of a 160x160 column block at full depth."""
+    same workload: one forward step of a 288x288 column block at full depth."""
     import numpy as np
 
     import oracle
     from monc_amd import setup_state as ss
 
-    n = 160
+    n = 288
     st = ss.make_state(nz, n, n, 25.0, 5000.0, seed=0, n_q=1, with_zfields=False)
     dt = ss.cfl_dt(st, 0.3)
     t0 = time.perf_counter()
@@ -112,12 +130,11 @@ def main() -> None:
     cells = nx * ny * nz
 
     log("bench: FP64 form")
-    wall64, k64 = run_form(st, tvd.FP64, args.steps, args.warmup, dt)
+    wall64, kt64 = run_form(st, tvd.FP64, args.steps, args.warmup, dt)
     ms_step = wall64 / args.steps * 1e3
     value = cells * args.steps / wall64 / 1e9
-    kavg = sum(k64) / len(k64)
-    bytes_cell_64 = 80.0  # read u,v,w,th,q + write 5, 8 B each
-    achieved = bytes_cell_64 * cells / (kavg * 1e-3) / 1e9
+    kind, kavg, kbytes, achieved = roofline(kt64, cells, 8)
+    step_kernel_ms = sum(v[0] for k, v in kt64.items() if k in KIND_FIELDS) / args.steps
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -137,19 +154,24 @@ def main() -> None:
                    "nx": nx, "ny": ny, "nz": nz, "fields": 5, "dt_s": dt},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "tvd::cell_kernel<double, MODE_STEP>", "kernel_ms": round(kavg, 4),
-                     "bytes_per_cell_update": bytes_cell_64},
+                     "kernel": f"wave_kernel ({kind})", "kernel_ms": round(kavg, 4),
+                     "algorithmic_bytes_per_launch": kbytes,
+                     "step": {"kernel_ms": round(step_kernel_ms, 4), "bytes_per_cell_update": 80,
+                              "achieved_GBs": round(80 * cells / (step_kernel_ms * 1e-3) / 1e9, 2)},
+                     "kernels_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kt64.items()}},
     }
     if not args.no_fp32:
         log("bench: FP32 form")
-        wall32, k32 = run_form(st, tvd.FP32, args.steps, args.warmup, dt)
+        wall32, kt32 = run_form(st, tvd.FP32, args.steps, args.warmup, dt)
         v32 = cells * args.steps / wall32 / 1e9
-        k32avg = sum(k32) / len(k32)
-        a32 = 40.0 * cells / (k32avg * 1e-3) / 1e9
-        out["fp32"] = {"value": round(v32, 4), "unit": "Gcell-updates/s", "ms_per_step": round(wall32 / args.steps
-                                                                                                * 1e3, 4),
-                       "kernel_ms": round(k32avg, 4), "achieved_GBs": round(a32, 2),
-                       "frac": round(a32 / HBM_PEAK_GBS, 4), "speedup_vs_fp64": round(v32 / value, 3)}
+        kind32, kavg32, kbytes32, a32 = roofline(kt32, cells, 4)
+        step32 = sum(v[0] for k, v in kt32.items() if k in KIND_FIELDS) / args.steps
+        out["fp32"] = {"value": round(v32, 4), "unit": "Gcell-updates/s",
+                       "ms_per_step": round(wall32 / args.steps * 1e3, 4),
+                       "dominant_kernel": kind32, "kernel_ms": round(kavg32, 4), "achieved_GBs": round(a32, 2),
+                       "frac": round(a32 / HBM_PEAK_GBS, 4),
+                       "step_achieved_GBs": round(40 * cells / (step32 * 1e-3) / 1e9, 2),
+                       "speedup_vs_fp64": round(v32 / value, 3)}
     if not args.no_cpu:
         log("bench: CPU baseline (oracle) on a bounded sample")
         out["cpu_baseline"] = cpu_baseline(nz)

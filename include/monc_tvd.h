@@ -93,11 +93,23 @@ int monc_tvd_bind_field(monc_tvd_ctx *ctx, int field, int variant, void *device_
 int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **device_ptr);
 
 /* Launch shape (no counterpart in the reference: a tuning knob of this
- * library).  kernel 0 = slab kernel (every face computed once, columns of up
- * to 256 levels), 1 = one-cell-per-thread kernel; rows = cell rows per slab
- * block (0: default for the column height); strip = x columns per slab block
- * (0: default 16).  Shapes that are not compiled in are rejected. */
+ * library).  kernel 0 = x-marching kernels for columns of up to 256 levels,
+ * 1 = one-cell-per-thread kernel.  For kernel 0, rows = -1 picks the default
+ * shape, 0 the wave kernel (one wave per row, no block synchronisation), and
+ * rows > 0 the slab kernel with that many cell rows per block (y faces shared
+ * through LDS); strip = x columns per block (0: default 16).  Shapes that are
+ * not compiled in are rejected. */
 int monc_tvd_set_tiling(monc_tvd_ctx *ctx, int kernel, int rows, int strip);
+
+/* Per-kernel timing (no counterpart in the reference: measurement support).
+ * With profiling on, every launch is bracketed by HIP events on the context
+ * stream; monc_tvd_kernel_times synchronises, folds the events into per-kind
+ * totals and returns the first n of them: milliseconds and launch counts for
+ * kinds 0 = u, 1 = v, 2 = w, 3 = scalars, 4 = halo fill, 5 = one-cell-per-
+ * thread kernel (all fields).  reset != 0 clears the totals afterwards. */
+#define MONC_TVD_NKERNEL_KINDS 6
+int monc_tvd_set_profiling(monc_tvd_ctx *ctx, int on);
+int monc_tvd_kernel_times(monc_tvd_ctx *ctx, double *ms, long long *launches, int n, int reset);
 
 /* Run subsequent work on this hipStream_t (NULL: the legacy default stream). */
 int monc_tvd_set_stream(monc_tvd_ctx *ctx, void *hip_stream);

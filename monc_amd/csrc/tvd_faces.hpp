@@ -84,6 +84,20 @@ template <typename T>
 __device__ __forceinline__ T half_sign(T x) { return copysign(T(0.5), x); }
 template <typename T>
 __device__ __forceinline__ bool sign_pos(T x) { return !signbit(x); }
+// 1/x for the limiter's guarded denominators (x = delta + 1e-30 or
+// S + 1e-30).  FP64: hardware reciprocal refined by two Newton steps (within
+// an ulp of the correctly rounded quotient); FP32: the hardware reciprocal
+// (1 ulp).  x = 0 never reaches a selected result: S >= 0, and a face whose
+// delta + 1e-30 vanishes takes the upwind branch.
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <typename T>
 __device__ __forceinline__ T pos(T x) { return tmax(T(0.0), x); }
 template <typename T>
@@ -131,7 +145,7 @@ struct HLim {
   __device__ __forceinline__ HLim(T vel, T ch, T dt, T sum_cfl_out) {
     const T r6 = T(1.0) / T(6.0);
     T cflmod = fabs(vel * ch * dt);
-    hrcfl = T(1.0) / (sum_cfl_out + T(1.e-30));
+    hrcfl = recip(sum_cfl_out + T(1.e-30));
     hcfl = T(0.5) * cflmod;
     cflcoef = r6 * (T(1.0) - cflmod * cflmod);
   }
@@ -140,7 +154,7 @@ struct HLim {
   __device__ __forceinline__ T face(T fu, T fc, T fd, T fgt1, T fgt2) const {
     T fdels = fd - fu;
     T fcurvs = fd - T(2.0) * fc + fu;
-    T rfdels = T(1.0) / (fdels + T(1.e-30));
+    T rfdels = recip(fdels + T(1.e-30));
     T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * fcurvs - fgt1 - fgt2;
     T fnfs = (ffs - fu) * rfdels;
     T fncs = (fc - fu) * rfdels;
@@ -162,14 +176,14 @@ struct WLim {
     rdd = vm<T>(m.rdzn, k + 1);
     rdu = rdu_;
     T cflmod = fabs(wval * rdd * dt);
-    hrcfl = T(1.0) / (sum_cfl_out + T(1.e-30));
+    hrcfl = recip(sum_cfl_out + T(1.e-30));
     hcfl = T(0.5) * cflmod;
     cflcoef = r6 * (T(1.0) - cflmod * cflmod) * (dd * dd * rdc);
   }
   __device__ __forceinline__ T face(T fu, T fc, T fd, T fgt1, T fgt2) const {
     T fcurvs = fd - T(2.0) * fc + fu;
     T fdels = fd - fu;
-    T rfdels = T(1.0) / (fdels + T(1.e-30));
+    T rfdels = recip(fdels + T(1.e-30));
     T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * ((fd - fc) * rdd - (fc - fu) * rdu) - fgt1 - fgt2;
     T fnfs = (ffs - fu) * rfdels;
     T fncs = (fc - fu) * rfdels;

@@ -271,6 +271,85 @@ __global__ void __launch_bounds__(64 * NCH * (TY + 1)) slab_kernel(LaunchArgs<T>
 }
 
 // ---------------------------------------------------------------------------
+// Wave kernel: one wave owns one row j over all levels (NSUB sub-levels of 64
+// per lane) and marches along an x strip; waves never synchronise.  The x face
+// is carried between steps (computed once), the z face reaches the level above
+// by a wave shuffle (and from the last lane of one sub-level to the first lane
+// of the next by readlane), and each row computes both of its y faces.
+template <typename T, int KIND, int NS, int MODE, int NSUB>
+__global__ void __launch_bounds__(256) wave_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip) {
+  const Flows<T, KIND> fl(L);
+  const VMet m = KIND == KIND_W ? L.mw : L.mc;
+  const int nz = L.d.nz, ny = L.d.ny;
+  const int kdof = KIND == KIND_W ? 1 : 0;
+  const int kmin = KIND == KIND_W ? 1 : 2;
+  const int kmax = KIND == KIND_W ? nz - 1 : nz;
+  const T dt = S.dt[0];
+  Adv<T, NS> F{};
+#pragma unroll
+  for (int n = 0; n < NS; n++) F.a[n] = Arr<T>(S.F[n], L.d);
+  const int lane = threadIdx.x;
+  const int j = 3 + (int)blockIdx.x * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
+  if (j > ny + 2) return;  // whole wave
+  const int i_begin = 3 + (int)blockIdx.y * strip;
+  const int i_end = min(i_begin + strip, L.d.nx + 3);
+  T fxl[NSUB][NS];
+#pragma unroll
+  for (int q = 0; q < NSUB; q++) {
+    const int k = 1 + 64 * q + lane;
+#pragma unroll
+    for (int n = 0; n < NS; n++) fxl[q][n] = T(0);
+    if (k >= 2 && k <= kmax) face_x(k, j, i_begin - 1, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fxl[q]);
+  }
+  for (int i = i_begin; i < i_end; i++) {
+    T prev_top[NS];  // top z face of the last level of the previous sub-level
+#pragma unroll
+    for (int n = 0; n < NS; n++) prev_top[n] = T(0);
+#pragma unroll
+    for (int q = 0; q < NSUB; q++) {
+      const int k = 1 + 64 * q + lane;
+      const bool cell = k >= 2 && k <= kmax;
+      const bool top = KIND != KIND_W && k == nz;
+      T fxr[NS], fyn[NS], fys[NS], fzt[NS], fzb[NS];
+#pragma unroll
+      for (int n = 0; n < NS; n++) fxr[n] = fyn[n] = fys[n] = fzt[n] = T(0);
+      if (cell) {
+        face_y(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fyn);
+        face_y(k, j - 1, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fys);
+        face_x(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fxr);
+      }
+      if (k <= nz - 1) face_z(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, kmin, dt, L.cx, L.cy, nz, fzt);
+#pragma unroll
+      for (int n = 0; n < NS; n++) {
+        fzb[n] = __shfl_up(fzt[n], 1);
+        if (lane == 0) fzb[n] = prev_top[n];
+        prev_top[n] = __shfl(fzt[n], 63);
+      }
+      if (cell) {
+        const DivW<T> dw(fl, m, k, j, i, nz);
+        const int64_t p = at(L.d, k, j, i);
+#pragma unroll
+        for (int n = 0; n < NS; n++) {
+          if (MODE == MODE_ACCUM)
+            S.out[n][p] = dw.apply(top, S.out[n][p], L.cx, L.cy, fxl[q][n], fxr[n], fys[n], fyn[n], fzb[n], fzt[n]);
+          else
+            store_step(L.d, S.out[n], S.f[n], L.dtm,
+                       dw.apply(top, T(0), L.cx, L.cy, fxl[q][n], fxr[n], fys[n], fyn[n], fzb[n], fzt[n]), k, j, i);
+        }
+      } else if (MODE == MODE_STEP && k <= nz) {
+#pragma unroll
+        for (int n = 0; n < NS; n++) store_step(L.d, S.out[n], S.f[n], L.dtm, T(0), k, j, i);
+      }
+#pragma unroll
+      for (int n = 0; n < NS; n++) fxl[q][n] = fxr[n];
+      // sub-levels are independent work; keeping the scheduler from
+      // interleaving them bounds the registers to one sub-level's set
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Periodic fill of the depth-2 x/y halo of one field, corners included.  One
 // thread per (k, halo column); the halo columns are the 4 rows j in {1,2,ny+3,
 // ny+4} over every i, then the 4 columns i in {1,2,nx+3,nx+4} over interior j.

@@ -56,9 +56,33 @@ struct monc_tvd_ctx {
   int use_cell_kernel;         // force the one-cell-per-thread kernel
   int slab_ty, slab_strip;     // slab tile shape (0: defaults)
   int slab_ok;                 // fields small enough for 32-bit buffer offsets
+  // optional per-kernel timing: HIP events recorded around every launch on
+  // the context stream, folded into per-kind totals on query
+  int profiling;
+  struct Timed { int kind; hipEvent_t a, b; };
+  std::vector<Timed> pending;
+  double prof_ms[MONC_TVD_NKERNEL_KINDS];
+  long long prof_n[MONC_TVD_NKERNEL_KINDS];
 };
 
 namespace {
+
+// Event pair around a launch when profiling is on.
+struct LaunchTimer {
+  monc_tvd_ctx* c;
+  int kind;
+  hipEvent_t a = nullptr, b = nullptr;
+  LaunchTimer(monc_tvd_ctx* c_, int kind_) : c(c_), kind(kind_) {
+    if (!c->profiling) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    (void)hipEventRecord(a, c->stream);
+  }
+  ~LaunchTimer() {
+    if (!c->profiling || !a || !b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({kind, a, b});
+  }
+};
 
 int check_ctx(monc_tvd_ctx* c) {
   if (!c) return fail(MONC_TVD_ERR_ARG, "null context");
@@ -140,9 +164,9 @@ void launch_slab(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, 
   hipLaunchKernelGGL((slab_kernel<T, KIND, NS, MODE, NCH, TY>), grid, block, 0, c->stream, L, S, strip);
 }
 
-// Default rows per block for a column of nch chunks: the largest compiled
-// shape within 512 threads per block.
-int default_ty(int nch) { return nch == 1 ? 7 : (nch == 2 ? 3 : 1); }
+// Default shape: the wave kernel (rows = 0), fastest measured for every
+// column height on MI355X.
+int default_ty(int) { return 0; }
 
 bool ty_compiled(int nch, int ty) {
   switch (nch) {
@@ -154,9 +178,21 @@ bool ty_compiled(int nch, int ty) {
   }
 }
 
+template <typename T, int KIND, int NS, int MODE, int NSUB>
+void launch_wave(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int strip) {
+  dim3 block(64, 4, 1);
+  dim3 grid((c->ny + 3) / 4, (c->nx + strip - 1) / strip, 1);
+  hipLaunchKernelGGL((wave_kernel<T, KIND, NS, MODE, NSUB>), grid, block, 0, c->stream, L, S, strip);
+}
+
+// key: nch*100 + rows for the slab kernel; nch*100 + 0 for the wave kernel
 template <typename T, int KIND, int NS, int MODE>
 void launch_slab_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int key, int strip) {
   switch (key) {
+    case 100: launch_wave<T, KIND, NS, MODE, 1>(c, L, S, strip); break;
+    case 200: launch_wave<T, KIND, NS, MODE, 2>(c, L, S, strip); break;
+    case 300: launch_wave<T, KIND, NS, MODE, 3>(c, L, S, strip); break;
+    case 400: launch_wave<T, KIND, NS, MODE, 4>(c, L, S, strip); break;
     case 103: launch_slab<T, KIND, NS, MODE, 1, 3>(c, L, S, strip); break;
     case 107: launch_slab<T, KIND, NS, MODE, 1, 7>(c, L, S, strip); break;
     case 201: launch_slab<T, KIND, NS, MODE, 2, 1>(c, L, S, strip); break;
@@ -172,12 +208,13 @@ void launch_slab_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T
 template <typename T, int MODE>
 int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
   const int nch = (c->nz + 63) / 64;
-  const int ty = c->slab_ty > 0 ? c->slab_ty : default_ty(nch);
+  const int ty = c->slab_ty >= 0 ? c->slab_ty : default_ty(nch);
   const int strip = c->slab_strip > 0 ? c->slab_strip : 16;
   if (c->use_cell_kernel || nch > 4 || !c->slab_ok) {
     const int nk = MODE == MODE_STEP ? c->nz : c->nz - 1;
     dim3 block(64, 4, 1);
     dim3 grid((nk + 63) / 64, (c->ny + 3) / 4, c->nx);
+    LaunchTimer timer(c, 5);
     hipLaunchKernelGGL((cell_kernel<T, MODE>), grid, block, 0, c->stream, L, S);
     HIP_TRY(hipGetLastError());
     return MONC_TVD_OK;
@@ -194,6 +231,7 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
       G.out[m] = S.out[n + m];
       G.dt[m] = S.dt[n + m];
     }
+    LaunchTimer timer(c, pair || S.kind[n] == KIND_SCALAR ? 3 : S.kind[n]);
     switch (S.kind[n]) {
       case KIND_U: launch_slab_shape<T, KIND_U, 1, MODE>(c, L, G, key, strip); break;
       case KIND_V: launch_slab_shape<T, KIND_V, 1, MODE>(c, L, G, key, strip); break;
@@ -214,6 +252,7 @@ int launch_halo(monc_tvd_ctx* c, T* f) {
   const int64_t nh = 4 * (int64_t)(c->nx + 4) + 4 * (int64_t)c->ny;
   dim3 block(64, 4, 1);
   dim3 grid((c->nz + 63) / 64, (unsigned)((nh + 3) / 4), 1);
+  LaunchTimer timer(c, 4);
   hipLaunchKernelGGL((halo_kernel<T>), grid, block, 0, c->stream, f, d);
   HIP_TRY(hipGetLastError());
   return MONC_TVD_OK;
@@ -355,9 +394,14 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags,
   c->stream = nullptr;
   c->dmet = nullptr;
   c->use_cell_kernel = 0;
-  c->slab_ty = 0;
+  c->slab_ty = -1;
   c->slab_strip = 0;
   c->slab_ok = n_elem * c->elem_size < ((size_t)1 << 32) - ((size_t)1 << 20);
+  c->profiling = 0;
+  for (int k = 0; k < MONC_TVD_NKERNEL_KINDS; k++) {
+    c->prof_ms[k] = 0.0;
+    c->prof_n[k] = 0;
+  }
   std::vector<double> h((size_t)MET_N * nz);
   for (int m = 0; m < MET_N; m++) std::memcpy(&h[(size_t)m * nz], mets[m], sizeof(double) * nz);
   hipError_t e = hipMalloc(&c->dmet, h.size() * sizeof(double));
@@ -374,6 +418,10 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags,
 int monc_tvd_destroy(monc_tvd_ctx* c) {
   if (!c) return MONC_TVD_OK;
   (void)hipStreamSynchronize(c->stream);
+  for (auto& t : c->pending) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
   for (size_t n = 0; n < c->buf.size(); n++) {
     void* p = c->buf[n];
     if (!p) continue;
@@ -398,11 +446,43 @@ int monc_tvd_set_tiling(monc_tvd_ctx* c, int kernel, int rows, int strip) {
   if (kernel != 0 && kernel != 1) return fail(MONC_TVD_ERR_ARG, "kernel %d is neither 0 (slab) nor 1 (cell)", kernel);
   if (strip < 0 || strip > 4096) return fail(MONC_TVD_ERR_ARG, "strip %d out of range [0,4096]", strip);
   const int nch = (c->nz + 63) / 64;
-  if (rows != 0 && !ty_compiled(nch, rows))
+  if (rows > 0 && !ty_compiled(nch, rows))
     return fail(MONC_TVD_ERR_ARG, "no slab shape with %d rows for %d-level columns", rows, c->nz);
   c->use_cell_kernel = kernel;
-  c->slab_ty = rows;
+  c->slab_ty = rows;  // -1 default, 0 wave kernel, > 0 slab rows
   c->slab_strip = strip;
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_set_profiling(monc_tvd_ctx* c, int on) {
+  if (int rc = check_ctx(c)) return rc;
+  c->profiling = on ? 1 : 0;
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_kernel_times(monc_tvd_ctx* c, double* ms, long long* launches, int n, int reset) {
+  if (int rc = check_ctx(c)) return rc;
+  if (n < 0 || (n > 0 && (!ms || !launches))) return fail(MONC_TVD_ERR_ARG, "null output arrays");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (auto& t : c->pending) {
+    float el = 0.f;
+    hipError_t e = hipEventElapsedTime(&el, t.a, t.b);
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+    if (e != hipSuccess) continue;
+    c->prof_ms[t.kind] += el;
+    c->prof_n[t.kind] += 1;
+  }
+  c->pending.clear();
+  for (int k = 0; k < n && k < MONC_TVD_NKERNEL_KINDS; k++) {
+    ms[k] = c->prof_ms[k];
+    launches[k] = c->prof_n[k];
+  }
+  if (reset)
+    for (int k = 0; k < MONC_TVD_NKERNEL_KINDS; k++) {
+      c->prof_ms[k] = 0.0;
+      c->prof_n[k] = 0;
+    }
   return MONC_TVD_OK;
 }
 

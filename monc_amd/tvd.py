@@ -32,7 +32,9 @@ EXPORTS = (
     "monc_tvd_bind_field", "monc_tvd_field_device_ptr", "monc_tvd_set_stream",
     "monc_tvd_synchronize", "monc_tvd_halo_fill", "monc_tvd_tendencies",
     "monc_tvd_forward_steps", "monc_tvd_last_error", "monc_tvd_set_tiling",
+    "monc_tvd_set_profiling", "monc_tvd_kernel_times",
 )
+KERNEL_KINDS = ("u", "v", "w", "scalars", "halo", "cell")
 
 
 def Q(n: int) -> int:
@@ -67,6 +69,9 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_field_device_ptr.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(c_void_p)]
     lib.monc_tvd_set_stream.argtypes = [c_void_p, c_void_p]
     lib.monc_tvd_set_tiling.argtypes = [c_void_p, c_int, c_int, c_int]
+    lib.monc_tvd_set_profiling.argtypes = [c_void_p, c_int]
+    lib.monc_tvd_kernel_times.argtypes = [c_void_p, ctypes.POINTER(c_double), ctypes.POINTER(ctypes.c_longlong),
+                                          c_int, c_int]
     lib.monc_tvd_synchronize.argtypes = [c_void_p]
     lib.monc_tvd_halo_fill.argtypes = [c_void_p]
     lib.monc_tvd_tendencies.argtypes = [c_void_p, c_double, c_int, c_int, c_int]
@@ -162,8 +167,21 @@ class TvdAdvection:
         _check(self._lib.monc_tvd_field_device_ptr(self._h, field, variant, ctypes.byref(p)))
         return int(p.value or 0)
 
-    def set_tiling(self, kernel: int = 0, rows: int = 0, strip: int = 0) -> None:
+    def set_tiling(self, kernel: int = 0, rows: int = -1, strip: int = 0) -> None:
+        """kernel 0 = x-marching kernels (rows -1 default shape, 0 wave kernel,
+        > 0 slab rows), 1 = one-cell-per-thread kernel; strip = x columns per block."""
         _check(self._lib.monc_tvd_set_tiling(self._h, kernel, rows, strip))
+
+    def set_profiling(self, on: bool) -> None:
+        _check(self._lib.monc_tvd_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self, reset: bool = True) -> dict:
+        """{kind: (total ms, launches)} from the HIP events of profiled launches."""
+        n = len(KERNEL_KINDS)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_longlong * n)()
+        _check(self._lib.monc_tvd_kernel_times(self._h, ms, cnt, n, 1 if reset else 0))
+        return {KERNEL_KINDS[k]: (ms[k], cnt[k]) for k in range(n) if cnt[k]}
 
     def set_stream(self, stream_handle: int) -> None:
         _check(self._lib.monc_tvd_set_stream(self._h, ctypes.c_void_p(stream_handle)))
