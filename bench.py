@@ -109,6 +109,9 @@ def main() -> None:
     ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed work, run three 512 MiB device-to-device copies: the known-size "
+                         "reference that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -172,6 +175,13 @@ def main() -> None:
                        "frac": round(a32 / HBM_PEAK_GBS, 4),
                        "step_achieved_GBs": round(40 * cells / (step32 * 1e-3) / 1e9, 2),
                        "speedup_vs_fp64": round(v32 / value, 3)}
+    if args.calibrate_copy:
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     if not args.no_cpu:
         log("bench: CPU baseline (oracle) on a bounded sample")
         out["cpu_baseline"] = cpu_baseline(nz)

@@ -144,6 +144,14 @@ int monc_tvd_tendencies(monc_tvd_ctx *ctx, double dtm, int field_stepping, int m
  */
 int monc_tvd_forward_steps(monc_tvd_ctx *ctx, double dtm, int nsteps);
 
+/* FP64 conservation sum of one field over the interior columns:
+ * out = sum_k weights[k-1] * sum_{j,i} f(k,j,i), accumulated in FP64 on the
+ * device whatever the context precision (weights: host, length nz; e.g.
+ * rhon(k)*dz(k) for k >= 2 and 0 at k = 1 gives the scalar mass that
+ * tvdadvection.F90:925-959 conserves).  No counterpart in the reference: the
+ * reference has no global diagnostic of this path.  Synchronous. */
+int monc_tvd_weighted_sum(monc_tvd_ctx *ctx, int field, int variant, const double *weights, double *out);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char *monc_tvd_last_error(void);
 

@@ -98,6 +98,13 @@ __device__ __forceinline__ double recip(double x) {
 }
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// The reference's guard (ultimateflux.F90:548,551,576,579,604,607) adds 1e-30
+// to the limiter denominators.  FP64 keeps it exactly; FP32 adds it with the
+// denominator's sign, so a value of -1e-30 cannot cancel to zero.  The two
+// differ only for |delta| below ~1e-23, far under FP32 field resolution.
+__device__ __forceinline__ double guard(double x) { return x + 1.e-30; }
+__device__ __forceinline__ float guard(float x) { return x + copysignf(1.e-30f, x); }
+
 template <typename T>
 __device__ __forceinline__ T pos(T x) { return tmax(T(0.0), x); }
 template <typename T>
@@ -154,7 +161,7 @@ struct HLim {
   __device__ __forceinline__ T face(T fu, T fc, T fd, T fgt1, T fgt2) const {
     T fdels = fd - fu;
     T fcurvs = fd - T(2.0) * fc + fu;
-    T rfdels = recip(fdels + T(1.e-30));
+    T rfdels = recip(guard(fdels));
     T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * fcurvs - fgt1 - fgt2;
     T fnfs = (ffs - fu) * rfdels;
     T fncs = (fc - fu) * rfdels;
@@ -183,7 +190,7 @@ struct WLim {
   __device__ __forceinline__ T face(T fu, T fc, T fd, T fgt1, T fgt2) const {
     T fcurvs = fd - T(2.0) * fc + fu;
     T fdels = fd - fu;
-    T rfdels = recip(fdels + T(1.e-30));
+    T rfdels = recip(guard(fdels));
     T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * ((fd - fc) * rdd - (fc - fu) * rdu) - fgt1 - fgt2;
     T fnfs = (ffs - fu) * rfdels;
     T fncs = (fc - fu) * rfdels;

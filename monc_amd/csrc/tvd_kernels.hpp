@@ -375,4 +375,25 @@ __global__ void __launch_bounds__(256) halo_kernel(T* f, Dims d) {
   f[at(d, k, j, i)] = f[at(d, k, wrap(j, d.ny), wrap(i, d.nx))];
 }
 
+// FP64 weighted sum over the interior, sum_k w(k) sum_{j,i} f(k,j,i): one
+// partial per block (lanes along z, 4 rows, one column), reduced by the host
+// in a fixed order so the result does not depend on scheduling.
+template <typename T>
+__global__ void __launch_bounds__(256) wsum_kernel(const T* f, Dims d, const double* w, double* partial) {
+  __shared__ double red[256];
+  const int j = 3 + blockIdx.y * 4 + threadIdx.y;
+  const int i = 3 + blockIdx.z;
+  double acc = 0.0;
+  if (j <= d.ny + 2)
+    for (int k = 1 + (int)threadIdx.x; k <= d.nz; k += 64) acc += w[k - 1] * (double)f[at(d, k, j, i)];
+  const int t = threadIdx.x + 64 * threadIdx.y;
+  red[t] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
+    __syncthreads();
+  }
+  if (t == 0) partial[blockIdx.y + gridDim.y * (size_t)blockIdx.z] = red[0];
+}
+
 }  // namespace tvd

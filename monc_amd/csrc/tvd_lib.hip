@@ -454,6 +454,39 @@ int monc_tvd_set_tiling(monc_tvd_ctx* c, int kernel, int rows, int strip) {
   return MONC_TVD_OK;
 }
 
+int monc_tvd_weighted_sum(monc_tvd_ctx* c, int field, int variant, const double* weights, double* out) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_fv(c, field, variant)) return rc;
+  if (!weights || !out) return fail(MONC_TVD_ERR_ARG, "null weights or output");
+  if (int rc = need(c, field, variant)) return rc;
+  const int gy = (c->ny + 3) / 4, gz = c->nx;
+  const size_t np = (size_t)gy * gz;
+  double *dw = nullptr, *dp = nullptr;
+  HIP_TRY(hipMalloc(&dw, sizeof(double) * c->nz));
+  hipError_t e = hipMalloc(&dp, sizeof(double) * np);
+  if (e == hipSuccess) e = hipMemcpyAsync(dw, weights, sizeof(double) * c->nz, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    dim3 block(64, 4, 1), grid(1, gy, gz);
+    if (c->precision == MONC_TVD_FP64)
+      hipLaunchKernelGGL((wsum_kernel<double>), grid, block, 0, c->stream, (const double*)slot(c, field, variant),
+                         dims_of(c), dw, dp);
+    else
+      hipLaunchKernelGGL((wsum_kernel<float>), grid, block, 0, c->stream, (const float*)slot(c, field, variant),
+                         dims_of(c), dw, dp);
+    e = hipGetLastError();
+  }
+  std::vector<double> h(np);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dp, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (dw) (void)hipFree(dw);
+  if (dp) (void)hipFree(dp);
+  if (e != hipSuccess) return fail(MONC_TVD_ERR_HIP, "weighted sum: %s", hipGetErrorString(e));
+  double acc = 0.0;
+  for (size_t n = 0; n < np; n++) acc += h[n];
+  *out = acc;
+  return MONC_TVD_OK;
+}
+
 int monc_tvd_set_profiling(monc_tvd_ctx* c, int on) {
   if (int rc = check_ctx(c)) return rc;
   c->profiling = on ? 1 : 0;

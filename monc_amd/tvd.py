@@ -32,7 +32,7 @@ EXPORTS = (
     "monc_tvd_bind_field", "monc_tvd_field_device_ptr", "monc_tvd_set_stream",
     "monc_tvd_synchronize", "monc_tvd_halo_fill", "monc_tvd_tendencies",
     "monc_tvd_forward_steps", "monc_tvd_last_error", "monc_tvd_set_tiling",
-    "monc_tvd_set_profiling", "monc_tvd_kernel_times",
+    "monc_tvd_set_profiling", "monc_tvd_kernel_times", "monc_tvd_weighted_sum",
 )
 KERNEL_KINDS = ("u", "v", "w", "scalars", "halo", "cell")
 
@@ -70,6 +70,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_set_stream.argtypes = [c_void_p, c_void_p]
     lib.monc_tvd_set_tiling.argtypes = [c_void_p, c_int, c_int, c_int]
     lib.monc_tvd_set_profiling.argtypes = [c_void_p, c_int]
+    lib.monc_tvd_weighted_sum.argtypes = [c_void_p, c_int, c_int, dp, dp]
     lib.monc_tvd_kernel_times.argtypes = [c_void_p, ctypes.POINTER(c_double), ctypes.POINTER(ctypes.c_longlong),
                                           c_int, c_int]
     lib.monc_tvd_synchronize.argtypes = [c_void_p]
@@ -171,6 +172,15 @@ class TvdAdvection:
         """kernel 0 = x-marching kernels (rows -1 default shape, 0 wave kernel,
         > 0 slab rows), 1 = one-cell-per-thread kernel; strip = x columns per block."""
         _check(self._lib.monc_tvd_set_tiling(self._h, kernel, rows, strip))
+
+    def weighted_sum(self, field: int, variant: int, weights: np.ndarray) -> float:
+        """sum_k w(k) sum_{j,i} f(k,j,i) over the interior, accumulated in FP64 on the device."""
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        if w.shape != (self.nz,):
+            raise ValueError("weights must have length nz")
+        out = np.zeros(1)
+        _check(self._lib.monc_tvd_weighted_sum(self._h, field, variant, _dptr(w), _dptr(out)))
+        return float(out[0])
 
     def set_profiling(self, on: bool) -> None:
         _check(self._lib.monc_tvd_set_profiling(self._h, 1 if on else 0))

@@ -1,0 +1,102 @@
+"""Precision form: FP32 fields and arithmetic (FP64 metric vectors, FP64
+conservation sums) against the FP64 path, which the parity tests pin to the
+oracle at 1e-12.
+
+Reported per field: relative L2 and L-infinity error of the fields after 1 and
+100 forward-Euler steps (CFL 0.3) and of the tendencies of one call, and the
+drift of the th and q masses sum_k rhon dz sum_ij f over 100 steps.  Bounds:
+fields <= 1e-5 relative (L-infinity) after 1 and 100 steps; tendencies <=
+1e-4 relative (L-infinity over max |s|); FP32 mass drift <= 1e-6 relative.
+"""
+import json
+
+import numpy as np
+import pytest
+
+import oracle
+from monc_amd import setup_state as ss
+from monc_amd import tvd
+from tests.helpers import gpu_tendencies, interior
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["u", "v", "w", "th", "q"]
+
+
+def _errs(a, b):
+    a, b = interior(a), interior(b)
+    d = a - b
+    return {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(b)),
+            "rel_linf": float(np.max(np.abs(d)) / np.max(np.abs(b)))}
+
+
+def _mass_weights(g):
+    w = g.rhon * g.dz
+    w[0] = 0.0
+    return w
+
+
+def _run(st, precision, dt, nsteps):
+    t = tvd.TvdAdvection.initialise(st, precision=precision)
+    try:
+        w = _mass_weights(st.grid)
+        t.halo_fill()
+        m0 = [t.weighted_sum(n, tvd.PROG, w) for n in (3, 4)]
+        out = {}
+        done = 0
+        for n in sorted(set(nsteps)):
+            t.forward_steps(dt, n - done)
+            done = n
+            out[n] = [t.get_field(f, tvd.PROG) for f in range(len(st.fields))]
+        m1 = [t.weighted_sum(n, tvd.PROG, w) for n in (3, 4)]
+        return out, m0, m1
+    finally:
+        t.close()
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 64, 50.0, 2000.0), (256, 512, 512, 25.0, 5000.0)],
+                         ids=["64cubed", "512x512x256"])
+def test_fp32_error_vs_fp64_after_1_and_100_steps(gpu, shape):
+    nz, ny, nx, dxy, ztop = shape
+    st = ss.make_state(nz, ny, nx, dxy, ztop, seed=0, n_q=1, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    f64, m64a, m64b = _run(st, tvd.FP64, dt, [1, 100])
+    f32, m32a, m32b = _run(st, tvd.FP32, dt, [1, 100])
+    report = {"shape": f"{nx}x{ny}x{nz}", "dt": dt}
+    for n in (1, 100):
+        report[f"after_{n}"] = {NAMES[f]: _errs(f32[n][f], f64[n][f]) for f in range(5)}
+    report["mass_drift_fp64"] = [abs(b - a) / abs(a) for a, b in zip(m64a, m64b)]
+    report["mass_drift_fp32"] = [abs(b - a) / abs(a) for a, b in zip(m32a, m32b)]
+    print("FP32_REPORT " + json.dumps(report))
+    for n in (1, 100):
+        for f in range(5):
+            assert report[f"after_{n}"][NAMES[f]]["rel_linf"] <= 1e-5, (n, NAMES[f], report[f"after_{n}"])
+    assert max(report["mass_drift_fp64"]) <= 1e-12
+    assert max(report["mass_drift_fp32"]) <= 1e-6
+
+
+def test_fp32_tendencies_vs_oracle(gpu):
+    st = ss.make_state(64, 64, 64, 50.0, 2000.0, seed=0, n_q=1)
+    ref, _ = oracle.state_tendencies(st, 1.0, 1, 1, 1)
+    got = gpu_tendencies(st, 1.0, 1, 1, 1, precision=tvd.FP32)
+    errs = {}
+    for f in range(5):
+        a, b = interior(got[f]), interior(ref[f])
+        errs[NAMES[f]] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    print("FP32_TENDENCY_REPORT " + json.dumps(errs))
+    assert all(e <= 1e-4 for e in errs.values()), errs
+
+
+def test_weighted_sum_matches_host(gpu):
+    st = ss.make_state(20, 12, 10, 50.0, 500.0, seed=4, n_q=1, with_zfields=False)
+    w = np.linspace(0.5, 2.0, st.nz)
+    for prec in (tvd.FP64, tvd.FP32):
+        t = tvd.TvdAdvection.initialise(st, precision=prec)
+        try:
+            for f in (0, 3, 4):
+                a = st.fields[f] if prec == tvd.FP64 else st.fields[f].astype(np.float32).astype(np.float64)
+                exp = float((interior(a).sum(axis=(0, 1)) * w).sum())
+                got = t.weighted_sum(f, tvd.PROG, w)
+                assert abs(got - exp) <= 1e-13 * abs(exp)
+        finally:
+            t.close()
