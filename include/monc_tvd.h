@@ -62,6 +62,10 @@ extern "C" {
 #define MONC_TVD_PROG 0
 #define MONC_TVD_ZFIELD 1
 #define MONC_TVD_SOURCE 2
+/* differentiation support (no counterpart in the reference): the adjoint of a
+ * field (interior values, zero halo) and its tangent (periodic halo) */
+#define MONC_TVD_ADJOINT 3
+#define MONC_TVD_TANGENT 4
 
 typedef struct monc_tvd_ctx monc_tvd_ctx;
 
@@ -151,6 +155,36 @@ int monc_tvd_forward_steps(monc_tvd_ctx *ctx, double dtm, int nsteps);
  * tvdadvection.F90:925-959 conserves).  No counterpart in the reference: the
  * reference has no global diagnostic of this path.  Synchronous. */
 int monc_tvd_weighted_sum(monc_tvd_ctx *ctx, int field, int variant, const double *weights, double *out);
+
+/*
+ * Adjoint of one monc_tvd_forward_steps step (no counterpart in the
+ * reference: the differentiable form of the path).  With the prognostics
+ * (MONC_TVD_PROG) holding the state X the step starts from and the
+ * MONC_TVD_ADJOINT fields holding the adjoint lambda' of the state after it
+ * (interior values, zero halo), replaces them by lambda = lambda' +
+ * dtm * J(X)^T lambda', J the Jacobian of the tendencies; halo contributions
+ * are folded back onto their periodic sources.  The adjoint fields of u, v, w
+ * are allocated (zero) if missing.  Branches of the limiter are re-derived
+ * from X, so they are those the forward step took.  Asynchronous.
+ */
+int monc_tvd_adjoint_step(monc_tvd_ctx *ctx, double dtm);
+
+/*
+ * Adjoint of monc_tvd_halo_fill (no counterpart in the reference): adds the
+ * halo entries of every MONC_TVD_ADJOINT field to their periodic sources and
+ * zeroes the halo.  Applied to the adjoint of a state whose halo entries were
+ * themselves used (a loss over the full arrays) before the first
+ * monc_tvd_adjoint_step.  Asynchronous.
+ */
+int monc_tvd_adjoint_fold(monc_tvd_ctx *ctx);
+
+/*
+ * Tangent-linear model of monc_tvd_forward_steps (FP64 contexts only; no
+ * counterpart in the reference).  Advances the state (MONC_TVD_PROG) and its
+ * tangent (MONC_TVD_TANGENT, periodic halo filled) together nsteps times:
+ * dX' = dX + dtm * J(X) dX.  Asynchronous.
+ */
+int monc_tvd_tangent_steps(monc_tvd_ctx *ctx, double dtm, int nsteps);
 
 /* Message of the last failure on the calling thread ("" if none). */
 const char *monc_tvd_last_error(void);

@@ -1,0 +1,140 @@
+"""Differentiable forward-Euler rollouts of the TVD advection step.
+
+`TvdRollout` owns one device-resident domain (FP32 or FP64) and exposes an
+n-step rollout X_0 -> X_n (X = u, v, w, th, q on the (nx+4, ny+4, nz) arrays
+with their periodic halo) as a `torch.autograd.Function`:
+
+* forward: the state is advanced by the library's fused step
+  (monc_tvd_forward_steps) and a copy of it is kept every `checkpoint_every`
+  steps;
+* backward: segment by segment from the last, the states inside the segment
+  are recomputed from its checkpoint, then the hand-written adjoint step
+  (monc_tvd_adjoint_step) carries lambda back one step at a time.  The
+  limiter's branches are re-derived from each recomputed state, which is
+  bitwise the state the forward pass stepped from, so the adjoint follows the
+  branches the forward pass took.
+
+Memory: ceil(n/k) checkpoints plus k segment states, each one state (all
+fields with halo) in the context precision.
+
+The incoming gradient may be non-zero on halo entries (a loss over whole
+arrays): it is folded onto the periodic sources first (monc_tvd_adjoint_fold).
+The gradient returned for the initial state is zero on the halo, because the
+rollout overwrites the input halo with the periodic image of its interior.
+"""
+from __future__ import annotations
+
+from typing import List, Sequence
+
+import torch
+
+from . import tvd
+from .setup_state import State
+
+
+class TvdRollout:
+    """Device domain + checkpoint buffers for differentiable rollouts."""
+
+    def __init__(self, state: State, precision: int = tvd.FP32, checkpoint_every: int = 8,
+                 advect: int = tvd.ADVECT_FLOW | tvd.ADVECT_THETA | tvd.ADVECT_Q):
+        if checkpoint_every < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        if not torch.cuda.is_available():
+            raise RuntimeError("TvdRollout needs a GPU: the rollout and its adjoint run only in the HIP library")
+        self.nfields = len(state.fields)
+        self.precision = precision
+        self.dtype = torch.float64 if precision == tvd.FP64 else torch.float32
+        self.k = checkpoint_every
+        self.t = tvd.TvdAdvection(state.nz, state.ny, state.nx, state.cx, state.cy, state.grid.metric_vectors(),
+                                  n_q=state.n_q, advect=advect, precision=precision)
+        self.shape = self.t.field_shape()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.prog = [torch.zeros(self.shape, dtype=self.dtype, device=dev) for _ in range(self.nfields)]
+        self.adj = [torch.zeros(self.shape, dtype=self.dtype, device=dev) for _ in range(self.nfields)]
+        self._bind_prog(self.prog)
+        for f in range(self.nfields):
+            self.t.bind_field(f, tvd.ADJOINT, self.adj[f])
+        self._seg: List[List[torch.Tensor]] = []  # recompute buffers, reused across segments
+
+    # helpers -------------------------------------------------------------------
+    def _bind_prog(self, fields: Sequence[torch.Tensor]) -> None:
+        for f, a in enumerate(fields):
+            self.t.bind_field(f, tvd.PROG, a)
+
+    def _sync_stream(self) -> None:
+        self.t.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    def _load(self, fields: Sequence[torch.Tensor]) -> None:
+        for p, f in zip(self.prog, fields):
+            p.copy_(f)
+
+    def _snapshot(self) -> List[torch.Tensor]:
+        return [p.clone() for p in self.prog]
+
+    def _seg_buffers(self, n: int) -> List[List[torch.Tensor]]:
+        while len(self._seg) < n:
+            self._seg.append([torch.empty_like(p) for p in self.prog])
+        return self._seg
+
+    # public --------------------------------------------------------------------
+    def rollout(self, fields: Sequence[torch.Tensor], dt: float, nsteps: int):
+        """X_n as a tuple of tensors, differentiable w.r.t. `fields` (X_0)."""
+        return RolloutFn.apply(self, float(dt), int(nsteps), *fields)
+
+    @torch.no_grad()
+    def forward_only(self, fields: Sequence[torch.Tensor], dt: float, nsteps: int) -> List[torch.Tensor]:
+        self._sync_stream()
+        self._load(fields)
+        self.t.halo_fill()
+        self.t.forward_steps(dt, nsteps)
+        return self._snapshot()
+
+    def close(self) -> None:
+        self.t.close()
+        self._seg = []
+
+
+class RolloutFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, eng: TvdRollout, dt: float, nsteps: int, *fields):
+        eng._sync_stream()
+        eng._load(fields)
+        eng.t.halo_fill()
+        ckpts = []
+        done = 0
+        while done < nsteps:
+            ckpts.append((done, eng._snapshot()))
+            n = min(eng.k, nsteps - done)
+            eng.t.forward_steps(dt, n)
+            done += n
+        ctx.eng, ctx.dt, ctx.nsteps, ctx.ckpts = eng, dt, nsteps, ckpts
+        return tuple(eng._snapshot())
+
+    @staticmethod
+    def backward(ctx, *grads):
+        eng, dt = ctx.eng, ctx.dt
+        eng._sync_stream()
+        for a, g in zip(eng.adj, grads):
+            if g is None:
+                a.zero_()
+            else:
+                a.copy_(g)
+        eng.t.adjoint_fold()
+        seg = eng._seg_buffers(eng.k)
+        for start, ck in reversed(ctx.ckpts):
+            n = min(eng.k, ctx.nsteps - start)
+            # recompute X_start .. X_{start+n-1} in the working buffers
+            eng._bind_prog(eng.prog)
+            eng._load(ck)
+            for m in range(n):
+                if m:
+                    eng.t.forward_steps(dt, 1)
+                for s, p in zip(seg[m], eng.prog):
+                    s.copy_(p)
+            # adjoint sweep over the segment, newest state first
+            for m in reversed(range(n)):
+                eng._bind_prog(seg[m])
+                eng.t.adjoint_step(dt)
+        eng._bind_prog(eng.prog)
+        ctx.ckpts = None
+        return (None, None, None) + tuple(a.clone() for a in eng.adj)

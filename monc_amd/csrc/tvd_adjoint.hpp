@@ -1,0 +1,618 @@
+// Hand-written adjoint of one forward-Euler TVD step.
+//
+// The forward step is X' = X + dtm * T(X) for every advected field (u, v, w,
+// th, q), T the tendency of tvdadvection.F90:528-960 with forward stepping
+// (ultflx dt = dtm, advected field = the prognostic).  Given the adjoint of the
+// new state, lambda' (interior), the adjoint of the old state is
+//   lambda = lambda' + dtm * J(X)^T lambda'
+// assembled face by face: each face owner recomputes its face value from the
+// state (so every branch -- upwind or limited, which of the limiter's min/max
+// arms, flow signs, upwind picks of the transverse terms -- is re-derived from
+// the same inputs and is the one the forward pass took), receives the face's
+// adjoint from the two cells it bounds, and scatters the reverse-mode partials
+// to the field and flow adjoints with hardware floating-point atomics.  A fold
+// then returns the adjoint that landed on halo points to their periodic
+// sources (the adjoint of the periodic fill).
+//
+// Derivative conventions at kinks follow the tangent-linear code obtained by
+// instantiating the forward templates with Dual: max/min differentiate the
+// chosen operand, d|x| = sign-bit(x) ? -dx : dx.
+#pragma once
+#include "tvd_kernels.hpp"
+
+namespace tvd {
+
+template <typename T>
+__device__ __forceinline__ T sgn_abs(T x) { return signbit(x) ? T(-1.0) : T(1.0); }
+// derivatives of pos(x) = max(0,x) and negabs(x) = |min(0,x)| as coded
+template <typename T>
+__device__ __forceinline__ T dpos(T x) { return T(0.0) > x ? T(0.0) : T(1.0); }
+template <typename T>
+__device__ __forceinline__ T dnegabs(T x) { return T(0.0) < x ? T(0.0) : sgn_abs(x); }
+
+template <typename T>
+__device__ __forceinline__ void atomic_add(T* p, T v) {
+  if (v != T(0)) unsafeAtomicAdd(p, v);
+}
+
+// Adjoint of a field: atomic scatter with the field's indexing.
+template <typename T>
+struct Acc {
+  T* a;
+  Dims d;
+  __device__ __forceinline__ void add(int k, int j, int i, T v) const { atomic_add(a + at(d, k, j, i), v); }
+};
+
+// Adjoint of Flow<T,DK,DJ,DI>: the dual-grid interpolation scatters to both points.
+template <typename T, int DK, int DJ, int DI>
+struct AccFlow {
+  Acc<T> a;
+  int nz;
+  __device__ __forceinline__ void add(int k, int j, int i, T v) const {
+    if constexpr (DK == 0 && DJ == 0 && DI == 0) {
+      a.add(k, j, i, v);
+    } else if constexpr (DK == 1) {
+      if (k < nz) {
+        a.add(k, j, i, T(0.5) * v);
+        a.add(k + 1, j, i, T(0.5) * v);
+      } else {
+        a.add(k, j, i, v);
+      }
+    } else {
+      a.add(k, j, i, T(0.5) * v);
+      a.add(k, j + DJ, i + DI, T(0.5) * v);
+    }
+  }
+};
+
+// Adjoints of the limiter inputs.
+template <typename T>
+struct LimAdj {
+  T fu, fc, fd, g1, g2, hrcfl, hcfl, cflcoef;
+};
+
+// Reverse of HLim::face.  `q_rdd`, `q_rdu` non-zero select the z form's
+// curvature term (fd-fc)*rdd - (fc-fu)*rdu instead of fd - 2fc + fu.
+template <typename T, bool ZFORM>
+__device__ __forceinline__ LimAdj<T> lim_adjoint(T lam, T fu, T fc, T fd, T g1, T g2, T hrcfl, T hcfl, T cflcoef,
+                                                 T rdd, T rdu) {
+  LimAdj<T> r{};
+  const T fdels = fd - fu;
+  const T fcurvs = fd - T(2.0) * fc + fu;
+  if (fabs(fcurvs) >= fabs(fdels)) {  // upwind branch: the face is fc
+    r.fc = lam;
+    return r;
+  }
+  const T rf = recip(guard(fdels));
+  const T curv = ZFORM ? (fd - fc) * rdd - (fc - fu) * rdu : fcurvs;
+  const T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * curv - g1 - g2;
+  const T fnfs = (ffs - fu) * rf;
+  const T fncs = (fc - fu) * rf;
+  const T fnrefs = fncs * hrcfl;
+  const T m1 = tmin(fnfs, fnrefs);
+  const T m2 = tmin(m1, T(1.0));
+  const T ftemp = tmax(fncs, m2);
+  // lim = ftemp * fdels + fu
+  const T lftemp = lam * fdels;
+  T lfdels = lam * ftemp;
+  T lfu = lam;
+  const bool pick_ncs = fncs > m2;
+  T lfncs = pick_ncs ? lftemp : T(0.0);
+  const T lm2 = pick_ncs ? T(0.0) : lftemp;
+  const T lm1 = m1 < T(1.0) ? lm2 : T(0.0);
+  const bool pick_nfs = fnfs < fnrefs;
+  const T lfnfs = pick_nfs ? lm1 : T(0.0);
+  const T lfnrefs = pick_nfs ? T(0.0) : lm1;
+  lfncs = lfncs + lfnrefs * hrcfl;
+  r.hrcfl = lfnrefs * fncs;
+  const T lffs = lfnfs * rf;
+  lfu = lfu - lfnfs * rf;
+  T lrf = lfnfs * (ffs - fu);
+  T lfc = lfncs * rf;
+  lfu = lfu - lfncs * rf;
+  lrf = lrf + lfncs * (fc - fu);
+  lfdels = lfdels - (rf * rf) * lrf;
+  // ffs = 0.5(fd+fc) - hcfl(fd-fc) - cflcoef*curv - g1 - g2
+  T lfd = T(0.5) * lffs - hcfl * lffs;
+  lfc = lfc + T(0.5) * lffs + hcfl * lffs;
+  r.hcfl = -lffs * (fd - fc);
+  r.cflcoef = -lffs * curv;
+  const T lcurv = -lffs * cflcoef;
+  r.g1 = -lffs;
+  r.g2 = -lffs;
+  if (ZFORM) {
+    lfd = lfd + lcurv * rdd;
+    lfc = lfc - lcurv * (rdd + rdu);
+    lfu = lfu + lcurv * rdu;
+  } else {
+    lfd = lfd + lcurv;
+    lfc = lfc - T(2.0) * lcurv;
+    lfu = lfu + lcurv;
+  }
+  lfd = lfd + lfdels;
+  lfu = lfu - lfdels;
+  r.fu = lfu;
+  r.fc = lfc;
+  r.fd = lfd;
+  return r;
+}
+
+// Adjoint of cflmod = |vel*ch*dt| through hcfl = cflmod/2 and
+// cflcoef = r6*(1 - cflmod^2)*geom: returns d/dvel.
+template <typename T>
+__device__ __forceinline__ T courant_adj(T vel, T ch, T dt, T geom, T lhcfl, T lcflcoef) {
+  const T r6 = T(1.0 / 6.0);
+  const T x = vel * ch * dt;
+  const T cflmod = fabs(x);
+  const T lcfl = T(0.5) * lhcfl - r6 * T(2.0) * cflmod * geom * lcflcoef;
+  return lcfl * sgn_abs(x) * ch * dt;
+}
+
+// Right x face of column (j,i) at level k: value and reverse.  Returns the
+// face value (the divergence adjoint of the flow coefficient needs it).
+template <typename T, class FU, class FV, class FW, class AU, class AV, class AW>
+__device__ T face_x_adj(int k, int j, int i, const Adv<T, 1>& F, const Acc<T>& aF, const FU& U, const FV& V,
+                        const FW& W, const AU& aU, const AV& aV, const AW& aW, const VMet& m, int kdof, T dt, T cx,
+                        T cy, int nz, T lam) {
+  const int kp1 = k + 1 < nz ? k + 1 : nz;
+  const int kp0 = kp1 - 1;
+  const T uk = U(k, j, i);
+  const T sm = half_sign(uk);
+  const T ap = T(0.5) + sm, an = T(0.5) - sm;
+  const bool upos = uk > T(0.0);
+  const T w0 = W(k, j, i), w0m = W(k - 1, j, i), w1 = W(k, j, i + 1), w1m = W(k - 1, j, i + 1);
+  const T v0 = V(k, j, i), v0m = V(k, j - 1, i), v1 = V(k, j, i + 1), v1m = V(k, j - 1, i + 1);
+  const T um = U(k, j, i - 1), up1 = U(k, j, i + 1);
+  const T wa = upos ? w0 : w1, wb = upos ? w0m : w1m;
+  const T va = upos ? v0 : v1, vb = upos ? v0m : v1m;
+  const T rz0 = vm<T>(m.rdz, k);
+  T s = rz0 * (pos(wa) + negabs(wb));
+  s = s + cy * (pos(va) + negabs(vb));
+  s = s + cx * (upos ? uk + negabs(um) : pos(up1) - uk);
+  s = s * dt;
+  const T sz = T(0.125) * (w0 + w0m + w1 + w1m);
+  const bool pz = sign_pos(sz);
+  const int ksu = pz ? k - 1 : kp1;
+  const int ksl = pz ? k : kp0;
+  const T rzk = vm<T>(m.rdz, k + kdof);
+  const T cz = fabs(sz) * rzk * dt;
+  const T sy = T(0.125) * (v1 + v0 + v1m + v0m);
+  const int js = sign_pos(sy) ? j - 1 : j + 1;
+  const T cyt = fabs(sy) * cy * dt;
+  const HLim<T> lim(uk, cx, dt, s);
+  const T fa = F(0, k, j, i), fb = F(0, k, j, i + 1);
+  const T fm = F(0, k, j, i - 1), fp = F(0, k, j, i + 2);
+  const T fu = fm * ap + fp * an, fc = fa * ap + fb * an, fd = fb * ap + fa * an;
+  const T l0 = F(0, ksl, j, i), u0 = F(0, ksu, j, i), l1 = F(0, ksl, j, i + 1), u1 = F(0, ksu, j, i + 1);
+  const T tmp = (l0 - u0) * ap + (l1 - u1) * an;
+  const T g1 = cz * tmp;
+  const T s0 = F.lane(0, k, js, i), s1 = F.lane(0, k, js, i + 1);
+  const T t = s0 * ap + s1 * an;
+  const T g2 = cyt * (fc - t);
+  const T face = lim.face(fu, fc, fd, g1, g2);
+  if (lam == T(0.0)) return face;
+  const LimAdj<T> r = lim_adjoint<T, false>(lam, fu, fc, fd, g1, g2, lim.hrcfl, lim.hcfl, lim.cflcoef, T(0), T(0));
+  // transverse terms
+  const T lcyt = r.g2 * (fc - t);
+  const T lfc = r.fc + r.g2 * cyt;
+  const T lt = -(r.g2 * cyt);
+  const T lcz = r.g1 * tmp;
+  const T ltmp = r.g1 * cz;
+  // advected field
+  aF.add(k, j, i, lfc * ap + r.fd * an);
+  aF.add(k, j, i + 1, lfc * an + r.fd * ap);
+  aF.add(k, j, i - 1, r.fu * ap);
+  aF.add(k, j, i + 2, r.fu * an);
+  aF.add(ksl, j, i, ltmp * ap);
+  aF.add(ksu, j, i, -(ltmp * ap));
+  aF.add(ksl, j, i + 1, ltmp * an);
+  aF.add(ksu, j, i + 1, -(ltmp * an));
+  aF.add(k, js, i, lt * ap);
+  aF.add(k, js, i + 1, lt * an);
+  // flows
+  const T lS = -(lim.hrcfl * lim.hrcfl) * r.hrcfl;
+  T luk = courant_adj(uk, cx, dt, T(1.0), r.hcfl, r.cflcoef);
+  const T lsz = T(0.125) * (lcz * sgn_abs(sz) * rzk * dt);
+  const T lsy = T(0.125) * (lcyt * sgn_abs(sy) * cy * dt);
+  const T lS3 = lS * dt;
+  const T lA = lS3 * rz0, lB = lS3 * cy, lC = lS3 * cx;
+  const T lwa = lA * dpos(wa), lwb = lA * dnegabs(wb);
+  const T lva = lB * dpos(va), lvb = lB * dnegabs(vb);
+  T lum = T(0.0), lup1 = T(0.0);
+  if (upos) {
+    luk = luk + lC;
+    lum = lC * dnegabs(um);
+  } else {
+    luk = luk - lC;
+    lup1 = lC * dpos(up1);
+  }
+  aW.add(k, j, i, lsz + (upos ? lwa : T(0.0)));
+  aW.add(k - 1, j, i, lsz + (upos ? lwb : T(0.0)));
+  aW.add(k, j, i + 1, lsz + (upos ? T(0.0) : lwa));
+  aW.add(k - 1, j, i + 1, lsz + (upos ? T(0.0) : lwb));
+  aV.add(k, j, i, lsy + (upos ? lva : T(0.0)));
+  aV.add(k, j - 1, i, lsy + (upos ? lvb : T(0.0)));
+  aV.add(k, j, i + 1, lsy + (upos ? T(0.0) : lva));
+  aV.add(k, j - 1, i + 1, lsy + (upos ? T(0.0) : lvb));
+  aU.add(k, j, i, luk);
+  aU.add(k, j, i - 1, lum);
+  aU.add(k, j, i + 1, lup1);
+  return face;
+}
+
+// North y face of row j at column i, level k: value and reverse.
+template <typename T, class FU, class FV, class FW, class AU, class AV, class AW>
+__device__ T face_y_adj(int k, int j, int i, const Adv<T, 1>& F, const Acc<T>& aF, const FU& U, const FV& V,
+                        const FW& W, const AU& aU, const AV& aV, const AW& aW, const VMet& m, int kdof, T dt, T cx,
+                        T cy, int nz, T lam) {
+  const int kp1 = k + 1 < nz ? k + 1 : nz;
+  const int kp0 = kp1 - 1;
+  const T vk = V(k, j, i);
+  const T sm = half_sign(vk);
+  const T ap = T(0.5) + sm, an = T(0.5) - sm;
+  const bool vpos = sign_pos(vk);
+  const int jj = vpos ? j : j + 1;
+  const T w0 = W(k, j, i), w0m = W(k - 1, j, i), w1 = W(k, j + 1, i), w1m = W(k - 1, j + 1, i);
+  const T vn = V(k, j + 1, i), vs = V(k, j - 1, i);
+  const T u0 = U(k, j, i), u0m = U(k, j, i - 1), u1 = U(k, j + 1, i), u1m = U(k, j + 1, i - 1);
+  const T wa = vpos ? w0 : w1, wb = vpos ? w0m : w1m;
+  const T va = vpos ? vk : vn, vb = vpos ? vs : vk;
+  const T ua = vpos ? u0 : u1, ub = vpos ? u0m : u1m;
+  const T rz0 = vm<T>(m.rdz, k);
+  T s = rz0 * (pos(wa) + negabs(wb));
+  s = s + cy * (pos(va) + negabs(vb));
+  s = s + cx * (pos(ua) + negabs(ub));
+  s = s * dt;
+  const T sz = T(0.125) * (w0 + w0m + w1 + w1m);
+  const bool pz = sign_pos(sz);
+  const int ksu = pz ? k - 1 : kp1;
+  const int ksl = pz ? k : kp0;
+  const T rzk = vm<T>(m.rdz, k + kdof);
+  const T cz = fabs(sz) * rzk * dt;
+  const T sx = T(0.125) * (u0m + u0 + u1m + u1);
+  const T posuf = T(0.5) + half_sign(sx);
+  const T cxt = fabs(sx) * cx * dt;
+  const HLim<T> lim(vk, cy, dt, s);
+  const T fa = F(0, k, j, i), fb = F(0, k, j + 1, i);
+  const T fm = F(0, k, j - 1, i), fp = F(0, k, j + 2, i);
+  const T fu = fm * ap + fp * an, fc = fa * ap + fb * an, fd = fb * ap + fa * an;
+  const T fl = F.lane(0, ksl, jj, i), fs = F.lane(0, ksu, jj, i);
+  const T g1 = cz * (fl - fs);
+  const T fe = F.lane(0, k, jj, i + 1), fw = F.lane(0, k, jj, i - 1);
+  const T tx = fe * (T(1.0) - posuf) + fw * posuf;
+  const T g2 = cxt * (fc - tx);
+  const T face = lim.face(fu, fc, fd, g1, g2);
+  if (lam == T(0.0)) return face;
+  const LimAdj<T> r = lim_adjoint<T, false>(lam, fu, fc, fd, g1, g2, lim.hrcfl, lim.hcfl, lim.cflcoef, T(0), T(0));
+  const T lcxt = r.g2 * (fc - tx);
+  const T lfc = r.fc + r.g2 * cxt;
+  const T ltx = -(r.g2 * cxt);
+  const T lcz = r.g1 * (fl - fs);
+  const T ldiff = r.g1 * cz;
+  aF.add(k, j, i, lfc * ap + r.fd * an);
+  aF.add(k, j + 1, i, lfc * an + r.fd * ap);
+  aF.add(k, j - 1, i, r.fu * ap);
+  aF.add(k, j + 2, i, r.fu * an);
+  aF.add(ksl, jj, i, ldiff);
+  aF.add(ksu, jj, i, -ldiff);
+  aF.add(k, jj, i + 1, ltx * (T(1.0) - posuf));
+  aF.add(k, jj, i - 1, ltx * posuf);
+  const T lS = -(lim.hrcfl * lim.hrcfl) * r.hrcfl;
+  T lvk = courant_adj(vk, cy, dt, T(1.0), r.hcfl, r.cflcoef);
+  const T lsz = T(0.125) * (lcz * sgn_abs(sz) * rzk * dt);
+  const T lsx = T(0.125) * (lcxt * sgn_abs(sx) * cx * dt);
+  const T lS3 = lS * dt;
+  const T lA = lS3 * rz0, lB = lS3 * cy, lC = lS3 * cx;
+  const T lwa = lA * dpos(wa), lwb = lA * dnegabs(wb);
+  const T lva = lB * dpos(va), lvb = lB * dnegabs(vb);
+  const T lua = lC * dpos(ua), lub = lC * dnegabs(ub);
+  aW.add(k, j, i, lsz + (vpos ? lwa : T(0.0)));
+  aW.add(k - 1, j, i, lsz + (vpos ? lwb : T(0.0)));
+  aW.add(k, j + 1, i, lsz + (vpos ? T(0.0) : lwa));
+  aW.add(k - 1, j + 1, i, lsz + (vpos ? T(0.0) : lwb));
+  // va = vk or vn, vb = vs or vk
+  if (vpos) {
+    lvk = lvk + lva;
+    aV.add(k, j - 1, i, lvb);
+  } else {
+    aV.add(k, j + 1, i, lva);
+    lvk = lvk + lvb;
+  }
+  aV.add(k, j, i, lvk);
+  aU.add(k, j, i, lsx + (vpos ? lua : T(0.0)));
+  aU.add(k, j, i - 1, lsx + (vpos ? lub : T(0.0)));
+  aU.add(k, j + 1, i, lsx + (vpos ? T(0.0) : lua));
+  aU.add(k, j + 1, i - 1, lsx + (vpos ? T(0.0) : lub));
+  return face;
+}
+
+// Reverse of the z faces' transverse terms (ZGrad) for one field.
+template <typename T, class FU, class FV, class AU, class AV>
+__device__ __forceinline__ void zgrad_adj(int k, int j, int i, int kk, T fc, const Adv<T, 1>& F, const Acc<T>& aF,
+                                          const FU& U, const FV& V, const AU& aU, const AV& aV, T dt, T cx, T cy,
+                                          T lg1, T lg2, T& lfc) {
+  const T v00 = V(k, j - 1, i), v01 = V(k, j, i), v10 = V(k + 1, j - 1, i), v11 = V(k + 1, j, i);
+  const T sy = T(0.125) * (v00 + v01 + v10 + v11);
+  const int js = sign_pos(sy) ? j - 1 : j + 1;
+  const T cyt = fabs(sy) * cy * dt;
+  const T u00 = U(k, j, i - 1), u01 = U(k, j, i), u10 = U(k + 1, j, i - 1), u11 = U(k + 1, j, i);
+  const T sx = T(0.125) * (u00 + u01 + u10 + u11);
+  const T posuf = T(0.5) + half_sign(sx);
+  const T cxt = fabs(sx) * cx * dt;
+  const T fsj = F.lane(0, kk, js, i);
+  const T fe = F(0, kk, j, i + 1), fw = F(0, kk, j, i - 1);
+  const T tx = fe * (T(1.0) - posuf) + fw * posuf;
+  // g1 = cyt*(fc - fsj), g2 = cxt*(fc - tx)
+  lfc = lfc + lg1 * cyt + lg2 * cxt;
+  aF.add(kk, js, i, -(lg1 * cyt));
+  aF.add(kk, j, i + 1, -(lg2 * cxt) * (T(1.0) - posuf));
+  aF.add(kk, j, i - 1, -(lg2 * cxt) * posuf);
+  const T lsy = T(0.125) * (lg1 * (fc - fsj) * sgn_abs(sy) * cy * dt);
+  const T lsx = T(0.125) * (lg2 * (fc - tx) * sgn_abs(sx) * cx * dt);
+  aV.add(k, j - 1, i, lsy);
+  aV.add(k, j, i, lsy);
+  aV.add(k + 1, j - 1, i, lsy);
+  aV.add(k + 1, j, i, lsy);
+  aU.add(k, j, i - 1, lsx);
+  aU.add(k, j, i, lsx);
+  aU.add(k + 1, j, i - 1, lsx);
+  aU.add(k + 1, j, i, lsx);
+}
+
+// Top face of level k (flux_z(k+1)), 1 <= k <= nz-1: value and reverse.
+template <typename T, class FU, class FV, class FW, class AU, class AV, class AW>
+__device__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F, const Acc<T>& aF, const FU& U, const FV& V,
+                        const FW& W, const AU& aU, const AV& aV, const AW& aW, const VMet& m, int kdof, int kmin, T dt,
+                        T cx, T cy, int nz, T lam) {
+  if (k == 1 && kmin > 1) return T(0.0);
+  const T wk = W(k, j, i);
+  T s, rdu, rdc, fu, fc, fd;
+  int kk;
+  // S = ((a0 + cy*B) + cx*C) * dt with a0 the W part; record its inputs
+  int kw0 = k, kw1 = k - 1, kv = k;  // W(kw0), W(kw1), V(kv, j/j-1), U(kv, i/i-1)
+  int mode;                          // 0 middle, 1 top/bottom up, 2 top down, 3 bottom down
+  T rzw;
+  const bool bottom = k == 1, top = k == nz - 1;
+  T ap = T(0.0), an = T(0.0);
+  if (bottom) {
+    const bool up = wk >= T(0.0);
+    const T f1 = F(0, 1, j, i), f2 = F(0, 2, j, i);
+    if (up) {
+      mode = 1;
+      kv = 1;
+      rzw = vm<T>(m.rdz, 1 + kdof);
+      s = wk * rzw;
+      rdu = vm<T>(m.rdzn, 2);
+      rdc = vm<T>(m.rdz, 1 + kdof);
+      fu = f1, fc = f1, fd = f2;
+      kk = 1;
+    } else {
+      mode = 3;
+      kv = 2;
+      kw0 = 2;
+      rzw = vm<T>(m.rdz, 2 + kdof);
+      s = rzw * (pos(W(2, j, i)) - wk);
+      rdu = vm<T>(m.rdzn, 3);
+      rdc = vm<T>(m.rdz, 2 + kdof);
+      fu = F(0, 3, j, i), fc = f2, fd = f1;
+      kk = 2;
+    }
+  } else if (top) {
+    const bool up = wk >= T(0.0);
+    const T fk = F(0, k, j, i), fk1 = F(0, k + 1, j, i);
+    if (up) {
+      mode = 1;
+      kv = k;
+      rzw = vm<T>(m.rdz, k + kdof);
+      s = rzw * (wk + negabs(W(k - 1, j, i)));
+      rdu = vm<T>(m.rdzn, k);
+      rdc = vm<T>(m.rdz, k + kdof);
+      fu = F(0, k - 1, j, i), fc = fk, fd = fk1;
+      kk = k;
+    } else {
+      mode = 2;
+      kv = k + 1;
+      const int kt = k + 1 + kdof < nz ? k + 1 + kdof : nz;
+      rzw = vm<T>(m.rdz, kt);
+      s = -(wk * rzw);
+      rdu = vm<T>(m.rdzn, nz);
+      rdc = rzw;
+      fu = T(2.0) * fk1 - fk, fc = fk1, fd = fk;
+      kk = k + 1;
+    }
+  } else {
+    mode = 0;
+    const T sm = half_sign(wk);
+    ap = T(0.5) + sm, an = T(0.5) - sm;
+    kk = sign_pos(wk) ? k : k + 1;
+    kv = kk;
+    kw0 = kk;
+    kw1 = kk - 1;
+    rzw = vm<T>(m.rdz, kk);
+    s = rzw * (pos(W(kk, j, i)) + negabs(W(kk - 1, j, i)));
+    rdu = vm<T>(m.rdzn, k) * ap + vm<T>(m.rdzn, k + 2) * an;
+    rdc = vm<T>(m.rdz, k + kdof) * ap + vm<T>(m.rdz, k + 1 + kdof) * an;
+    fu = F(0, k - 1, j, i) * ap + F(0, k + 2, j, i) * an;
+    fc = F(0, kk, j, i);
+    fd = F(0, k + 1, j, i) * ap + F(0, k, j, i) * an;
+  }
+  const T vb0 = V(kv, j, i), vb1 = V(kv, j - 1, i), ub0 = U(kv, j, i), ub1 = U(kv, j, i - 1);
+  s = s + cy * (pos(vb0) + negabs(vb1));
+  s = s + cx * (pos(ub0) + negabs(ub1));
+  s = s * dt;
+  const ZGrad<T, FU, FV> zg(k, j, i, U, V, dt, cx, cy);
+  T g1, g2;
+  zg.apply(F, 0, kk, j, i, fc, g1, g2);
+  const WLim<T> lim(k, wk, dt, s, rdc, rdu, m);
+  const T face = lim.face(fu, fc, fd, g1, g2);
+  if (lam == T(0.0)) return face;
+  const LimAdj<T> r = lim_adjoint<T, true>(lam, fu, fc, fd, g1, g2, lim.hrcfl, lim.hcfl, lim.cflcoef, lim.rdd,
+                                           lim.rdu);
+  T lfc = r.fc;
+  zgrad_adj(k, j, i, kk, fc, F, aF, U, V, aU, aV, dt, cx, cy, r.g1, r.g2, lfc);
+  // stencil values
+  if (mode == 0) {
+    aF.add(k - 1, j, i, r.fu * ap);
+    aF.add(k + 2, j, i, r.fu * an);
+    aF.add(kk, j, i, lfc);
+    aF.add(k + 1, j, i, r.fd * ap);
+    aF.add(k, j, i, r.fd * an);
+  } else if (bottom) {
+    if (mode == 1) {  // fu = fc = F(1), fd = F(2)
+      aF.add(1, j, i, r.fu + lfc);
+      aF.add(2, j, i, r.fd);
+    } else {  // fu = F(3), fc = F(2), fd = F(1)
+      aF.add(3, j, i, r.fu);
+      aF.add(2, j, i, lfc);
+      aF.add(1, j, i, r.fd);
+    }
+  } else {
+    if (mode == 1) {  // fu = F(k-1), fc = F(k), fd = F(k+1)
+      aF.add(k - 1, j, i, r.fu);
+      aF.add(k, j, i, lfc);
+      aF.add(k + 1, j, i, r.fd);
+    } else {  // fu = 2F(k+1) - F(k), fc = F(k+1), fd = F(k)
+      aF.add(k + 1, j, i, T(2.0) * r.fu + lfc);
+      aF.add(k, j, i, r.fd - r.fu);
+    }
+  }
+  // flows: courant number, outflow sum
+  const T rdd = lim.rdd;
+  const T dd = vm<T>(m.dzn, k + 1);
+  T lwk = courant_adj(wk, rdd, dt, dd * dd * rdc, r.hcfl, r.cflcoef);
+  const T lS = -(lim.hrcfl * lim.hrcfl) * r.hrcfl;
+  const T lS3 = lS * dt;
+  const T lB = lS3 * cy, lC = lS3 * cx;
+  aV.add(kv, j, i, lB * dpos(vb0));
+  aV.add(kv, j - 1, i, lB * dnegabs(vb1));
+  aU.add(kv, j, i, lC * dpos(ub0));
+  aU.add(kv, j, i - 1, lC * dnegabs(ub1));
+  const T lW = lS3;  // adjoint of the W part a0 of S
+  if (mode == 0) {
+    const T wa = W(kw0, j, i), wb = W(kw1, j, i);
+    aW.add(kw0, j, i, lW * rzw * dpos(wa));
+    aW.add(kw1, j, i, lW * rzw * dnegabs(wb));
+  } else if (bottom && mode == 1) {  // a0 = wk * rzw
+    lwk = lwk + lW * rzw;
+  } else if (bottom) {  // a0 = rzw * (pos(W(2)) - wk)
+    aW.add(2, j, i, lW * rzw * dpos(W(2, j, i)));
+    lwk = lwk - lW * rzw;
+  } else if (mode == 1) {  // a0 = rzw * (wk + negabs(W(k-1)))
+    lwk = lwk + lW * rzw;
+    aW.add(k - 1, j, i, lW * rzw * dnegabs(W(k - 1, j, i)));
+  } else {  // a0 = -(wk * rzw)
+    lwk = lwk - lW * rzw;
+  }
+  aW.add(k, j, i, lwk);
+  return face;
+}
+
+template <typename T>
+struct AdjArgs {
+  const T* F;        // advected field (forward stepping: the prognostic)
+  const T* lin;      // adjoint of the new state for this field (interior)
+  T* lf;             // adjoint accumulator of this field
+  T* lu;             // adjoint accumulators of the flows u, v, w
+  T* lv;
+  T* lw;
+  T dtm;
+};
+
+// One thread per (k, j, i) over levels 1..nz, rows 2..ny+2, columns 2..nx+2:
+// the owner of the right x face, north y face and top z face of that point.
+template <typename T, int KIND>
+__global__ void __launch_bounds__(256) adjoint_kernel(LaunchArgs<T> L, AdjArgs<T> A) {
+  const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;
+  const int k = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 2 + blockIdx.y * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int i = 2 + blockIdx.z;
+  if (j > ny + 2 || k > nz) return;
+  const Flows<T, KIND> fl(L);
+  constexpr int DK = Flows<T, KIND>::DK, DJ = Flows<T, KIND>::DJ, DI = Flows<T, KIND>::DI;
+  const AccFlow<T, DK, DJ, DI> aU{{A.lu, L.d}, nz}, aV{{A.lv, L.d}, nz}, aW{{A.lw, L.d}, nz};
+  const Acc<T> aF{A.lf, L.d};
+  Adv<T, 1> F{};
+  F.a[0] = Arr<T>(A.F, L.d);
+  const VMet m = KIND == KIND_W ? L.mw : L.mc;
+  const int kdof = KIND == KIND_W ? 1 : 0;
+  const int kmin = KIND == KIND_W ? 1 : 2;
+  const int kmax = KIND == KIND_W ? nz - 1 : nz;
+  const T dt = A.dtm;
+  // adjoint lambda'(cell) of the new state, zero where no tendency is formed.
+  // Differences of lambda' are taken before scaling by dtm: lambda' of a
+  // smooth loss (e.g. 2 th) is large against its differences, and scaling
+  // first would round each term before the cancellation.
+  auto lraw = [&](int kk, int jj, int ii) -> T {
+    const bool cell = jj >= 3 && jj <= ny + 2 && ii >= 3 && ii <= nx + 2 && kk >= 2 && kk <= kmax;
+    return cell ? A.lin[at(L.d, kk, jj, ii)] : T(0.0);
+  };
+  const bool row_in = j >= 3, col_in = i >= 3;
+  const T lself = lraw(k, j, i);
+  // x face (k, j, i+1/2): s(i) -= U fx cx, s(i+1) += U fx cx
+  if (row_in && k >= 2 && k <= kmax) {
+    const T d = A.dtm * (lraw(k, j, i + 1) - lself);
+    const T uf = fl.U(k, j, i);
+    const T fx = face_x_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, dt, L.cx, L.cy, nz, uf * L.cx * d);
+    aU.add(k, j, i, fx * L.cx * d);
+  }
+  // y face (k, j+1/2, i)
+  if (col_in && k >= 2 && k <= kmax) {
+    const T d = A.dtm * (lraw(k, j + 1, i) - lself);
+    const T vf = fl.V(k, j, i);
+    const T fy = face_y_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, dt, L.cx, L.cy, nz, vf * L.cy * d);
+    aV.add(k, j, i, fy * L.cy * d);
+  }
+  // z face at level k (flux_z(k+1)): s(k) -= 4 W Fz t2(k), s(k+1) += 4 W Fz t1(k+1);
+  // t1(k+1) l(k+1) - t2(k) l(k) = t1(k+1) (l(k+1) - l(k)) + (t1(k+1) - t2(k)) l(k),
+  // the metric difference formed in FP64
+  if (row_in && col_in && k <= nz - 1 && !(k == 1 && kmin > 1)) {
+    const T dmet = (T)(m.t1[k] - m.t2[k - 1]);
+    const T e = T(4.0) * A.dtm * (vm<T>(m.t1, k + 1) * (lraw(k + 1, j, i) - lself) + dmet * lself);
+    const T wf = fl.W(k, j, i);
+    const T fz = face_z_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, kmin, dt, L.cx, L.cy, nz, wf * e);
+    aW.add(k, j, i, fz * e);
+  }
+}
+
+// Adjoint of the periodic halo fill, in two passes so every target is
+// written by one thread: rows first (over all columns, corners included),
+// then columns over the interior rows.  Halo entries are zeroed.
+template <typename T>
+__global__ void __launch_bounds__(256) fold_rows_kernel(T* f, Dims d) {
+  const int k = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int h = blockIdx.y * 4 + threadIdx.y;  // 0..4*(nx+4)-1
+  if (k > d.nz || h >= 4 * (d.nx + 4)) return;
+  const int r = h / (d.nx + 4), i = 1 + h % (d.nx + 4);
+  const int j = r < 2 ? 1 + r : d.ny + 1 + r;
+  const int64_t src = at(d, k, j, i), dst = at(d, k, wrap(j, d.ny), i);
+  f[dst] = f[dst] + f[src];
+  f[src] = T(0.0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) fold_cols_kernel(T* f, Dims d) {
+  const int k = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int h = blockIdx.y * 4 + threadIdx.y;  // 0..4*ny-1
+  if (k > d.nz || h >= 4 * d.ny) return;
+  const int c = h / d.ny, j = 3 + h % d.ny;
+  const int i = c < 2 ? 1 + c : d.nx + 1 + c;
+  const int64_t src = at(d, k, j, i), dst = at(d, k, j, wrap(i, d.nx));
+  f[dst] = f[dst] + f[src];
+  f[src] = T(0.0);
+}
+
+// Tangent-linear support: pack (value, tangent) pairs and split them again.
+__global__ void __launch_bounds__(256) pack_dual_kernel(Dual* out, const double* v, const double* dv, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) out[t] = Dual(v[t], dv[t]);
+}
+__global__ void __launch_bounds__(256) unpack_dual_kernel(double* v, double* dv, const Dual* in, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) {
+    const Dual x = in[t];
+    v[t] = x.v;
+    dv[t] = x.d;
+  }
+}
+
+}  // namespace tvd

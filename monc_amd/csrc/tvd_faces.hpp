@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tvd_dual.hpp"
+
 namespace tvd {
 
 struct Dims {
@@ -35,7 +37,9 @@ __device__ __forceinline__ int64_t at(const Dims& d, int k, int j, int i) {
 // arithmetic.  Fields must be smaller than 4 GiB (checked on the host).
 template <typename T>
 __device__ __forceinline__ T ldb(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  if constexpr (sizeof(T) == 8)
+  if constexpr (sizeof(T) == 16)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  else if constexpr (sizeof(T) == 8)
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
   else
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
@@ -97,6 +101,10 @@ __device__ __forceinline__ double recip(double x) {
   return __builtin_fma(r, e, r);
 }
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ Dual recip(Dual x) {
+  const double r = recip(x.v);
+  return Dual(r, -(r * r) * x.d);
+}
 
 // The reference's guard (ultimateflux.F90:548,551,576,579,604,607) adds 1e-30
 // to the limiter denominators.  FP64 keeps it exactly; FP32 adds it with the
@@ -104,6 +112,7 @@ __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x
 // differ only for |delta| below ~1e-23, far under FP32 field resolution.
 __device__ __forceinline__ double guard(double x) { return x + 1.e-30; }
 __device__ __forceinline__ float guard(float x) { return x + copysignf(1.e-30f, x); }
+__device__ __forceinline__ Dual guard(Dual x) { return Dual(x.v + 1.e-30, x.d); }
 
 template <typename T>
 __device__ __forceinline__ T pos(T x) { return tmax(T(0.0), x); }
@@ -150,7 +159,7 @@ template <typename T>
 struct HLim {
   T hrcfl, hcfl, cflcoef;
   __device__ __forceinline__ HLim(T vel, T ch, T dt, T sum_cfl_out) {
-    const T r6 = T(1.0) / T(6.0);
+    const T r6 = T(1.0 / 6.0);
     T cflmod = fabs(vel * ch * dt);
     hrcfl = recip(sum_cfl_out + T(1.e-30));
     hcfl = T(0.5) * cflmod;
@@ -178,7 +187,7 @@ template <typename T>
 struct WLim {
   T hrcfl, hcfl, cflcoef, rdd, rdu;
   __device__ __forceinline__ WLim(int k, T wval, T dt, T sum_cfl_out, T rdc, T rdu_, const VMet& m) {
-    const T r6 = T(1.0) / T(6.0);
+    const T r6 = T(1.0 / 6.0);
     T dd = vm<T>(m.dzn, k + 1);
     rdd = vm<T>(m.rdzn, k + 1);
     rdu = rdu_;

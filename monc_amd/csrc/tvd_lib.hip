@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/monc_tvd.h"
+#include "tvd_adjoint.hpp"
 #include "tvd_kernels.hpp"
 
 using namespace tvd;
@@ -37,7 +38,7 @@ int fail(int code, const char* fmt, ...) {
   } while (0)
 
 enum { MET_RDZ, MET_RDZN, MET_DZ, MET_DZN, MET_TZC1, MET_TZC2, MET_TZD1, MET_TZD2, MET_N };
-constexpr int NVAR = 3;
+constexpr int NVAR = 5;  // PROG, ZFIELD, SOURCE, ADJOINT, TANGENT
 
 }  // namespace
 
@@ -52,6 +53,8 @@ struct monc_tvd_ctx {
   std::vector<char> owned;     // library-allocated?
   std::vector<void*> bound;    // caller pointer of a bound PROG buffer (restored after stepping)
   std::vector<void*> spare;    // library-owned second buffer of each prognostic (stepping)
+  std::vector<void*> adj_tmp;  // copy of the incoming adjoint of each field (adjoint step)
+  std::vector<void*> dual[2];  // packed (value, tangent) state, double-buffered (tangent steps)
   hipStream_t stream;
   int use_cell_kernel;         // force the one-cell-per-thread kernel
   int slab_ty, slab_strip;     // slab tile shape (0: defaults)
@@ -349,6 +352,123 @@ int forward_steps_impl(monc_tvd_ctx* c, double dtm, int nsteps) {
   return MONC_TVD_OK;
 }
 
+template <typename T>
+int fold_halo(monc_tvd_ctx* c, T* f) {
+  Dims d = dims_of(c);
+  dim3 block(64, 4, 1);
+  dim3 g1((c->nz + 63) / 64, (unsigned)((4 * (c->nx + 4) + 3) / 4), 1);
+  hipLaunchKernelGGL((fold_rows_kernel<T>), g1, block, 0, c->stream, f, d);
+  HIP_TRY(hipGetLastError());
+  dim3 g2((c->nz + 63) / 64, (unsigned)((4 * c->ny + 3) / 4), 1);
+  hipLaunchKernelGGL((fold_cols_kernel<T>), g2, block, 0, c->stream, f, d);
+  HIP_TRY(hipGetLastError());
+  return MONC_TVD_OK;
+}
+
+template <typename T>
+int adjoint_step_impl(monc_tvd_ctx* c, double dtm) {
+  int rc;
+  std::vector<int> fields = advected_fields(c);
+  for (int fl : {MONC_TVD_U, MONC_TVD_V, MONC_TVD_W}) {
+    if ((rc = need(c, fl, MONC_TVD_PROG))) return rc;
+    if ((rc = ensure(c, fl, MONC_TVD_ADJOINT))) return rc;  // flows receive adjoint from every field
+  }
+  for (int fl : fields) {
+    if ((rc = need(c, fl, MONC_TVD_PROG)) || (rc = need(c, fl, MONC_TVD_ADJOINT))) return rc;
+    if (!c->adj_tmp[fl]) HIP_TRY(hipMalloc(&c->adj_tmp[fl], c->n_elem * c->elem_size));
+  }
+  // the incoming adjoint is read while the accumulators (which start from it:
+  // the identity part of X' = X + dtm T(X)) receive the transposed Jacobian
+  for (int fl : fields)
+    HIP_TRY(hipMemcpyAsync(c->adj_tmp[fl], slot(c, fl, MONC_TVD_ADJOINT), c->n_elem * c->elem_size,
+                           hipMemcpyDeviceToDevice, c->stream));
+  const LaunchArgs<T> L = launch_args<T>(c, dtm);
+  dim3 block(64, 4, 1);
+  dim3 grid((c->nz + 63) / 64, (c->ny + 1 + 3) / 4, c->nx + 1);
+  for (int fl : fields) {
+    AdjArgs<T> A;
+    A.F = (const T*)slot(c, fl, MONC_TVD_PROG);
+    A.lin = (const T*)c->adj_tmp[fl];
+    A.lf = (T*)slot(c, fl, MONC_TVD_ADJOINT);
+    A.lu = (T*)slot(c, MONC_TVD_U, MONC_TVD_ADJOINT);
+    A.lv = (T*)slot(c, MONC_TVD_V, MONC_TVD_ADJOINT);
+    A.lw = (T*)slot(c, MONC_TVD_W, MONC_TVD_ADJOINT);
+    A.dtm = (T)dtm;
+    LaunchTimer timer(c, kind_of(fl));
+    switch (kind_of(fl)) {
+      case KIND_U: hipLaunchKernelGGL((adjoint_kernel<T, KIND_U>), grid, block, 0, c->stream, L, A); break;
+      case KIND_V: hipLaunchKernelGGL((adjoint_kernel<T, KIND_V>), grid, block, 0, c->stream, L, A); break;
+      case KIND_W: hipLaunchKernelGGL((adjoint_kernel<T, KIND_W>), grid, block, 0, c->stream, L, A); break;
+      default: hipLaunchKernelGGL((adjoint_kernel<T, KIND_SCALAR>), grid, block, 0, c->stream, L, A); break;
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  for (int f = 0; f < c->nfields; f++) {
+    void* p = slot(c, f, MONC_TVD_ADJOINT);
+    if (p && (rc = fold_halo<T>(c, (T*)p))) return rc;
+  }
+  return MONC_TVD_OK;
+}
+
+int tangent_steps_impl(monc_tvd_ctx* c, double dtm, int nsteps) {
+  int rc;
+  std::vector<int> fields = advected_fields(c);
+  std::vector<int> state = {MONC_TVD_U, MONC_TVD_V, MONC_TVD_W};
+  for (int fl : fields)
+    if (fl > MONC_TVD_W) state.push_back(fl);
+  const int64_t n = (int64_t)c->n_elem;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  for (int fl : state) {
+    if ((rc = need(c, fl, MONC_TVD_PROG)) || (rc = need(c, fl, MONC_TVD_TANGENT))) return rc;
+    for (int b = 0; b < 2; b++)
+      if (!c->dual[b][fl]) HIP_TRY(hipMalloc(&c->dual[b][fl], c->n_elem * sizeof(Dual)));
+    hipLaunchKernelGGL(pack_dual_kernel, dim3(nb), dim3(256), 0, c->stream, (Dual*)c->dual[0][fl],
+                       (const double*)slot(c, fl, MONC_TVD_PROG), (const double*)slot(c, fl, MONC_TVD_TANGENT), n);
+    HIP_TRY(hipGetLastError());
+  }
+  // flows that are not advected keep their (value, tangent) across steps
+  for (int fl : {MONC_TVD_U, MONC_TVD_V, MONC_TVD_W})
+    if (!(c->advect & MONC_TVD_ADVECT_FLOW))
+      HIP_TRY(hipMemcpyAsync(c->dual[1][fl], c->dual[0][fl], c->n_elem * sizeof(Dual), hipMemcpyDeviceToDevice,
+                             c->stream));
+  int cur = 0;
+  for (int s = 0; s < nsteps; s++) {
+    LaunchArgs<Dual> L;
+    L.d = dims_of(c);
+    L.u = (const Dual*)c->dual[cur][MONC_TVD_U];
+    L.v = (const Dual*)c->dual[cur][MONC_TVD_V];
+    L.w = (const Dual*)c->dual[cur][MONC_TVD_W];
+    L.mc = vmet_of(c, false);
+    L.mw = vmet_of(c, true);
+    L.dtm = Dual(dtm);
+    L.cx = Dual(c->cx);
+    L.cy = Dual(c->cy);
+    for (size_t first = 0; first < fields.size(); first += MAX_FIELDS) {
+      FieldSet<Dual> S;
+      S.nf = (int)std::min(fields.size() - first, (size_t)MAX_FIELDS);
+      for (int m = 0; m < S.nf; m++) {
+        const int fl = fields[first + m];
+        S.kind[m] = kind_of(fl);
+        S.F[m] = (const Dual*)c->dual[cur][fl];
+        S.f[m] = S.F[m];
+        S.out[m] = (Dual*)c->dual[1 - cur][fl];
+        S.dt[m] = Dual(dtm);
+      }
+      dim3 block(64, 4, 1);
+      dim3 grid((c->nz + 63) / 64, (c->ny + 3) / 4, c->nx);
+      hipLaunchKernelGGL((cell_kernel<Dual, MODE_STEP>), grid, block, 0, c->stream, L, S);
+      HIP_TRY(hipGetLastError());
+    }
+    cur = 1 - cur;  // flows that are not advected hold the same pair in both buffers
+  }
+  for (int fl : state) {
+    hipLaunchKernelGGL(unpack_dual_kernel, dim3(nb), dim3(256), 0, c->stream, (double*)slot(c, fl, MONC_TVD_PROG),
+                       (double*)slot(c, fl, MONC_TVD_TANGENT), (const Dual*)c->dual[cur][fl], n);
+    HIP_TRY(hipGetLastError());
+  }
+  return MONC_TVD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -391,6 +511,9 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags,
   c->owned.assign((size_t)c->nfields * NVAR, 0);
   c->bound.assign((size_t)c->nfields, nullptr);
   c->spare.assign((size_t)c->nfields, nullptr);
+  c->adj_tmp.assign((size_t)c->nfields, nullptr);
+  c->dual[0].assign((size_t)c->nfields, nullptr);
+  c->dual[1].assign((size_t)c->nfields, nullptr);
   c->stream = nullptr;
   c->dmet = nullptr;
   c->use_cell_kernel = 0;
@@ -434,8 +557,12 @@ int monc_tvd_destroy(monc_tvd_ctx* c) {
     }
     if (c->owned[n]) (void)hipFree(p);
   }
-  for (int f = 0; f < c->nfields; f++)
+  for (int f = 0; f < c->nfields; f++) {
     if (c->spare[f] && c->spare[f] != c->bound[f]) (void)hipFree(c->spare[f]);
+    if (c->adj_tmp[f]) (void)hipFree(c->adj_tmp[f]);
+    for (int b = 0; b < 2; b++)
+      if (c->dual[b][f]) (void)hipFree(c->dual[b][f]);
+  }
   if (c->dmet) (void)hipFree(c->dmet);
   delete c;
   return MONC_TVD_OK;
@@ -591,7 +718,7 @@ int monc_tvd_get_field(monc_tvd_ctx* c, int field, int variant, double* host) {
 int monc_tvd_halo_fill(monc_tvd_ctx* c) {
   if (int rc = check_ctx(c)) return rc;
   for (int f = 0; f < c->nfields; f++)
-    for (int v : {MONC_TVD_PROG, MONC_TVD_ZFIELD}) {
+    for (int v : {MONC_TVD_PROG, MONC_TVD_ZFIELD, MONC_TVD_TANGENT}) {
       void* p = slot(c, f, v);
       if (!p) continue;
       int rc = c->precision == MONC_TVD_FP64 ? launch_halo<double>(c, (double*)p) : launch_halo<float>(c, (float*)p);
@@ -609,6 +736,31 @@ int monc_tvd_tendencies(monc_tvd_ctx* c, double dtm, int field_stepping, int mom
   return c->precision == MONC_TVD_FP64
              ? tendencies_impl<double>(c, dtm, field_stepping, momentum_stepping, scalar_stepping)
              : tendencies_impl<float>(c, dtm, field_stepping, momentum_stepping, scalar_stepping);
+}
+
+int monc_tvd_adjoint_step(monc_tvd_ctx* c, double dtm) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
+  return c->precision == MONC_TVD_FP64 ? adjoint_step_impl<double>(c, dtm) : adjoint_step_impl<float>(c, dtm);
+}
+
+int monc_tvd_adjoint_fold(monc_tvd_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  for (int f = 0; f < c->nfields; f++) {
+    void* p = slot(c, f, MONC_TVD_ADJOINT);
+    if (!p) continue;
+    const int rc = c->precision == MONC_TVD_FP64 ? fold_halo<double>(c, (double*)p) : fold_halo<float>(c, (float*)p);
+    if (rc) return rc;
+  }
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_tangent_steps(monc_tvd_ctx* c, double dtm, int nsteps) {
+  if (int rc = check_ctx(c)) return rc;
+  if (c->precision != MONC_TVD_FP64) return fail(MONC_TVD_ERR_ARG, "the tangent-linear model runs in FP64 only");
+  if (nsteps < 0) return fail(MONC_TVD_ERR_ARG, "nsteps %d < 0", nsteps);
+  if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
+  return tangent_steps_impl(c, dtm, nsteps);
 }
 
 int monc_tvd_forward_steps(monc_tvd_ctx* c, double dtm, int nsteps) {

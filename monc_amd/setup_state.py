@@ -155,15 +155,50 @@ class State:
         return (self.nx + 4, self.ny + 4, self.nz)
 
 
+def nondivergent_flow(nz: int, ny: int, nx: int, dxy: float, z: np.ndarray, rng: np.random.Generator,
+                      u_mean: float = 5.0, uamp: float = 1.5, wamp: float = 0.3, modes: int = 6):
+    """Interior (nx, ny, nz) u, v, w whose discrete divergence (the scalar flux
+    divergence of the advection scheme for rho = 1: (u(i-1)-u(i))/dx +
+    (v(j-1)-v(j))/dy + (w(k-1)-w(k))/dz(k)) vanishes at every cell k >= 2.
+    Horizontal part from a streamfunction psi at cell corners (u = -d psi/dy,
+    v = d psi/dx); vertical part from chi at (u face, w level) with chi = 0 at
+    levels 1 and nz (u -= d chi/dz, w = d chi/dx), so w = 0 there.  Fourier
+    modes with seeded wavenumbers and phases, periodic in x and y."""
+    x = np.arange(nx)[:, None, None] / nx
+    y = np.arange(ny)[None, :, None] / ny
+    zf = ((z - z[0]) / (z[-1] - z[0]))[None, None, :]
+    psi = np.zeros((nx, ny, nz))
+    chi = np.zeros((nx, ny, nz))
+    for _ in range(modes):
+        kx, ky = rng.integers(1, 4, size=2)
+        ph, ph2, kz = rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi), rng.uniform(0.5, 2.0)
+        a = uamp * ny * dxy / (2 * np.pi * ky * np.sqrt(modes))
+        psi += a * np.sin(2 * np.pi * (kx * x + ky * y) + ph) * (1.0 + 0.5 * np.sin(2 * np.pi * kz * zf + ph2))
+        b = wamp * nx * dxy / (2 * np.pi * kx * np.sqrt(modes))
+        chi += b * np.sin(2 * np.pi * (kx * x + ky * y) + ph2) * np.sin(np.pi * zf)
+    chi[:, :, 0] = 0.0
+    chi[:, :, -1] = 0.0
+    rdz = np.zeros(nz)
+    rdz[1:] = 1.0 / (z[1:] - z[:-1])
+    u = u_mean - (psi - np.roll(psi, 1, axis=1)) / dxy
+    u[:, :, 1:] -= (chi[:, :, 1:] - chi[:, :, :-1]) * rdz[None, None, 1:]
+    v = (psi - np.roll(psi, 1, axis=0)) / dxy
+    w = (chi - np.roll(chi, 1, axis=0)) / dxy
+    return u, v, w
+
+
 def make_state(nz: int, ny: int, nx: int, dxy: float, ztop: float, seed: int, n_q: int = 1,
                anelastic_scale: Optional[float] = None, u_mean: float = 5.0, lapse: float = 0.003,
-               with_zfields: bool = True) -> State:
+               with_zfields: bool = True, flow: str = "noise") -> State:
     """Seeded synthetic state: u = u_mean + N(0,1), v = N(0,1), w = N(0,0.5) with
     w = 0 at k = 1 and nz, th = 300 + lapse*zn + U(-0.1,0.1),
     q = 0.01*exp(-zn/2000) + U(0,1e-4) (extra q fields reuse the q recipe).
     Draw order u, v, w, th noise, q noise, each as the (nz, ny, nx) interior in
     Fortran order, then the periodic halo fill.  z-fields (the t-1 level for
-    centred stepping) are the same fields plus a second seeded perturbation."""
+    centred stepping) are the same fields plus a second seeded perturbation.
+    flow = "nondivergent" replaces u, v, w (after the same draws) by the
+    discretely divergence-free flow of `nondivergent_flow`, drawn next from
+    the same generator (dx = dy = dxy)."""
     z = tanh_stretched_z(nz, ztop)
     g = vertical_grid(z, anelastic_scale)
     rng = np.random.default_rng(seed)
@@ -182,6 +217,10 @@ def make_state(nz: int, ny: int, nx: int, dxy: float, ztop: float, seed: int, n_
     w[:, :, -1] = 0.0
     th = 300.0 + lapse * zn + rng.uniform(-0.1, 0.1, shp)
     qs = [0.01 * np.exp(-zn / 2000.0) + rng.uniform(0.0, 1e-4, shp) for _ in range(max(n_q, 0))]
+    if flow == "nondivergent":
+        u, v, w = nondivergent_flow(nz, ny, nx, dxy, z, rng, u_mean=u_mean)
+    elif flow != "noise":
+        raise ValueError(f"unknown flow kind {flow!r}")
     fields = [interior(a) for a in [u, v, w, th, *qs]]
     zfields: List[np.ndarray] = []
     if with_zfields:

@@ -25,7 +25,7 @@ CENTRED_STEPPING, FORWARD_STEPPING = 0, 1  # state.F90:15
 FP64, FP32 = 0, 1
 ADVECT_FLOW, ADVECT_THETA, ADVECT_Q = 1, 2, 4
 U, V, W, TH = 0, 1, 2, 3
-PROG, ZFIELD, SOURCE = 0, 1, 2
+PROG, ZFIELD, SOURCE, ADJOINT, TANGENT = 0, 1, 2, 3, 4
 
 EXPORTS = (
     "monc_tvd_create", "monc_tvd_destroy", "monc_tvd_set_field", "monc_tvd_get_field",
@@ -33,6 +33,7 @@ EXPORTS = (
     "monc_tvd_synchronize", "monc_tvd_halo_fill", "monc_tvd_tendencies",
     "monc_tvd_forward_steps", "monc_tvd_last_error", "monc_tvd_set_tiling",
     "monc_tvd_set_profiling", "monc_tvd_kernel_times", "monc_tvd_weighted_sum",
+    "monc_tvd_adjoint_step", "monc_tvd_adjoint_fold", "monc_tvd_tangent_steps",
 )
 KERNEL_KINDS = ("u", "v", "w", "scalars", "halo", "cell")
 
@@ -77,6 +78,9 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_halo_fill.argtypes = [c_void_p]
     lib.monc_tvd_tendencies.argtypes = [c_void_p, c_double, c_int, c_int, c_int]
     lib.monc_tvd_forward_steps.argtypes = [c_void_p, c_double, c_int]
+    lib.monc_tvd_adjoint_step.argtypes = [c_void_p, c_double]
+    lib.monc_tvd_adjoint_fold.argtypes = [c_void_p]
+    lib.monc_tvd_tangent_steps.argtypes = [c_void_p, c_double, c_int]
     lib.monc_tvd_last_error.argtypes = []
     lib.monc_tvd_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
@@ -207,6 +211,15 @@ class TvdAdvection:
 
     def forward_steps(self, dtm: float, nsteps: int) -> None:
         _check(self._lib.monc_tvd_forward_steps(self._h, dtm, nsteps))
+
+    def adjoint_step(self, dtm: float) -> None:
+        _check(self._lib.monc_tvd_adjoint_step(self._h, dtm))
+
+    def adjoint_fold(self) -> None:
+        _check(self._lib.monc_tvd_adjoint_fold(self._h))
+
+    def tangent_steps(self, dtm: float, nsteps: int) -> None:
+        _check(self._lib.monc_tvd_tangent_steps(self._h, dtm, nsteps))
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -1,0 +1,449 @@
+"""Hand-written adjoint and tangent-linear model of the forward-Euler TVD step.
+
+Checks, all through the C ABI on the GPU:
+* the tangent-linear model (the forward templates instantiated on dual
+  numbers) carries the same values as the forward step and its tangent
+  matches central finite differences of the oracle's own rollout;
+* dot-product test: <lambda_0, dX_0> = <lambda_n, dX_n> with dX_n from the
+  tangent-linear model and lambda_0 from n adjoint steps (FP64, 1e-11);
+* the autograd rollout with checkpointing and recompute returns the same
+  gradient as the plain adjoint sweep, and folds halo-dependent losses;
+* the C4 case (64-step rollout on 256x256x128, loss = sum th^2 over a
+  32x32x16 subvolume, gradients w.r.t. the initial th and u): the FP32
+  adjoint agrees with FP64 tangent-linear directional derivatives and with the
+  FP64 adjoint to 1e-4, and 200 central finite-difference probes
+  (eps = 1e-3, FP64 rollouts) away from switch points agree with the adjoint.
+"""
+import json
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from monc_amd import setup_state as ss
+from monc_amd import tvd
+from tests.helpers import interior
+
+pytestmark = pytest.mark.gpu
+
+
+def _halo(a):
+    return ss.halo_fill(a.copy())
+
+
+def _random_interior(shape, rng, scale=1.0, zero_w_ends=False):
+    a = np.zeros(shape)
+    nx, ny, nz = shape[0] - 4, shape[1] - 4, shape[2]
+    a[2:nx + 2, 2:ny + 2, :] = scale * rng.standard_normal((nx, ny, nz))
+    if zero_w_ends:
+        a[:, :, 0] = 0.0
+        a[:, :, -1] = 0.0
+    return a
+
+
+def _direction(st, rng, fields=None):
+    """Tangent direction: per-field scaled noise (halo filled); w kept zero at
+    levels 1 and nz like the state."""
+    scale = [0.3, 0.3, 0.1, 0.05, 1e-5]
+    out = []
+    for n, f in enumerate(st.fields):
+        if fields is not None and n not in fields:
+            out.append(np.zeros_like(f))
+            continue
+        out.append(_halo(_random_interior(f.shape, rng, scale[min(n, 4)], zero_w_ends=(n == 2))))
+    return out
+
+
+def tangent_run(st, dx, dt, nsteps):
+    t = tvd.TvdAdvection.initialise(st, precision=tvd.FP64)
+    try:
+        for n, d in enumerate(dx):
+            t.set_field(n, tvd.TANGENT, d)
+        t.halo_fill()
+        t.tangent_steps(dt, nsteps)
+        return ([t.get_field(n, tvd.PROG) for n in range(len(dx))],
+                [t.get_field(n, tvd.TANGENT) for n in range(len(dx))])
+    finally:
+        t.close()
+
+
+def adjoint_run(st, lam, dt, nsteps, precision=tvd.FP64):
+    """n adjoint steps by a plain sweep (states stored on the host)."""
+    t = tvd.TvdAdvection.initialise(st, precision=precision)
+    try:
+        t.halo_fill()
+        states = []
+        for _ in range(nsteps):
+            states.append([t.get_field(n, tvd.PROG) for n in range(len(lam))])
+            t.forward_steps(dt, 1)
+        for n, l in enumerate(lam):
+            t.set_field(n, tvd.ADJOINT, l)
+        for m in reversed(range(nsteps)):
+            for n, x in enumerate(states[m]):
+                t.set_field(n, tvd.PROG, x)
+            t.adjoint_step(dt)
+        return [t.get_field(n, tvd.ADJOINT) for n in range(len(lam))]
+    finally:
+        t.close()
+
+
+def _dot(a, b):
+    return float(sum(np.sum(x * y) for x, y in zip(a, b)))
+
+
+CASES = [(12, 12, 16, None), (20, 14, 10, None), (70, 8, 9, 1000.0), (33, 10, 12, 2000.0)]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[2]}x{c[1]}x{c[0]}" for c in CASES])
+def test_tangent_values_match_forward(gpu, case):
+    nz, ny, nx, an = case
+    st = ss.make_state(nz, ny, nx, 50.0, 25.0 * nz, seed=1, anelastic_scale=an, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    rng = np.random.default_rng(2)
+    x, _ = tangent_run(st, _direction(st, rng), dt, 3)
+    t = tvd.TvdAdvection.initialise(st, precision=tvd.FP64)
+    try:
+        t.halo_fill()
+        t.forward_steps(dt, 3)
+        ref = [t.get_field(n, tvd.PROG) for n in range(5)]
+    finally:
+        t.close()
+    for n in range(5):
+        assert np.max(np.abs(x[n] - ref[n])) <= 1e-12 * np.max(np.abs(ref[n])), n
+
+
+def test_tangent_vs_oracle_central_differences(gpu):
+    """TL tangent vs (oracle(x + e d) - oracle(x - e d)) / 2e, e = 1e-6."""
+    st = ss.make_state(12, 12, 16, 50.0, 300.0, seed=3, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    rng = np.random.default_rng(4)
+    dx = _direction(st, rng)
+    _, tan = tangent_run(st, dx, dt, 2)
+    eps = 1e-6
+
+    def run(sign):
+        s2 = ss.State(st.nz, st.ny, st.nx, st.cx, st.cy, st.grid,
+                      [f + sign * eps * d for f, d in zip(st.fields, dx)])
+        return oracle.forward_run(s2, dt, 2)
+
+    fp, fm = run(1.0), run(-1.0)
+    errs = []
+    for n in range(5):
+        fd = (interior(fp[n]) - interior(fm[n])) / (2 * eps)
+        tl = interior(tan[n])
+        errs.append(float(np.linalg.norm(fd - tl) / np.linalg.norm(tl)))
+    print("TL_FD " + json.dumps(errs))
+    assert max(errs) <= 1e-5, errs
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[2]}x{c[1]}x{c[0]}" for c in CASES])
+@pytest.mark.parametrize("nsteps", [1, 4])
+def test_adjoint_dot_product(gpu, case, nsteps):
+    nz, ny, nx, an = case
+    st = ss.make_state(nz, ny, nx, 50.0, 25.0 * nz, seed=5, anelastic_scale=an, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    rng = np.random.default_rng(6)
+    dx0 = _direction(st, rng)
+    _, dxn = tangent_run(st, dx0, dt, nsteps)
+    lam_n = [_random_interior(f.shape, rng) for f in st.fields]
+    lam0 = adjoint_run(st, lam_n, dt, nsteps)
+    for l in lam0:  # halo folded away
+        assert np.all(l[:2] == 0) and np.all(l[-2:] == 0) and np.all(l[:, :2] == 0) and np.all(l[:, -2:] == 0)
+    lhs = _dot(lam0, dx0)
+    rhs = _dot(lam_n, dxn)
+    scale = sum(abs(float(np.sum(a * b))) for a, b in zip(lam_n, dxn)) + abs(rhs)
+    assert abs(lhs - rhs) <= 1e-11 * scale, (lhs, rhs)
+
+
+@pytest.mark.parametrize("advect", [tvd.ADVECT_THETA | tvd.ADVECT_Q, tvd.ADVECT_FLOW])
+def test_adjoint_dot_product_partial_advection(gpu, advect):
+    st = ss.make_state(16, 10, 12, 50.0, 400.0, seed=7, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    rng = np.random.default_rng(8)
+    dx0 = _direction(st, rng)
+    t = tvd.TvdAdvection.initialise(st, precision=tvd.FP64, advect=advect)
+    try:
+        for n, d in enumerate(dx0):
+            t.set_field(n, tvd.TANGENT, d)
+        t.halo_fill()
+        t.tangent_steps(dt, 1)
+        dxn = [t.get_field(n, tvd.TANGENT) for n in range(5)]
+        lam_n = [_random_interior(f.shape, rng) for f in st.fields]
+        t2 = tvd.TvdAdvection.initialise(st, precision=tvd.FP64, advect=advect)
+        try:
+            t2.halo_fill()
+            for n, l in enumerate(lam_n):
+                t2.set_field(n, tvd.ADJOINT, l)
+            t2.adjoint_step(dt)
+            lam0 = [t2.get_field(n, tvd.ADJOINT) for n in range(5)]
+        finally:
+            t2.close()
+    finally:
+        t.close()
+    lhs, rhs = _dot(lam0, dx0), _dot(lam_n, dxn)
+    assert abs(lhs - rhs) <= 1e-11 * abs(rhs), (lhs, rhs)
+
+
+def test_fp32_adjoint_vs_fp64(gpu):
+    """One adjoint step from the same FP32-representable state: FP32 agrees
+    with FP64 to 1e-6 (relative L2 over the 99% of points that agree best);
+    the rest are points next to a face where a limiter decision
+    (upwind/limited branch, min/max arm, sign pick) comes out differently in
+    FP32 and FP64 arithmetic, and points off by more than 1e-4 of the largest
+    value are at most 0.2% of the domain."""
+    st = ss.make_state(64, 48, 40, 50.0, 2000.0, seed=9, with_zfields=False)
+    st.fields = [f.astype(np.float32).astype(np.float64) for f in st.fields]
+    dt = ss.cfl_dt(st, 0.3)
+    rng = np.random.default_rng(10)
+    lam_n = [_random_interior(f.shape, rng) for f in st.fields]
+    a64 = adjoint_run(st, lam_n, dt, 1, tvd.FP64)
+    a32 = adjoint_run(st, lam_n, dt, 1, tvd.FP32)
+    rep = []
+    for a, b in zip(a32, a64):
+        d = np.abs(interior(a) - interior(b)).ravel()
+        ref = interior(b).ravel()
+        best = np.sort(d)[: int(0.99 * d.size)]
+        rep.append({"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(ref)),
+                    "rel_l2_best99": float(np.linalg.norm(best) / np.linalg.norm(ref)),
+                    "switch_fraction": float(np.mean(d > 1e-4 * np.abs(ref).max()))})
+    print("ADJ32 " + json.dumps(rep))
+    for r in rep:
+        assert r["rel_l2_best99"] <= 1e-6 and r["switch_fraction"] <= 2e-3, rep
+
+
+def test_autograd_rollout_checkpointing(gpu):
+    import torch
+    from monc_amd.adjoint import TvdRollout
+
+    st = ss.make_state(24, 16, 20, 50.0, 600.0, seed=11, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    rng = np.random.default_rng(12)
+    lam = [_random_interior(f.shape, rng) for f in st.fields]
+    ref = adjoint_run(st, lam, dt, 7)
+    for k in (1, 3, 7, 10):
+        eng = TvdRollout(st, precision=tvd.FP64, checkpoint_every=k)
+        try:
+            x0 = [torch.tensor(f, device=gpu, requires_grad=True) for f in st.fields]
+            xn = eng.rollout(x0, dt, 7)
+            loss = sum((a * torch.tensor(l, device=gpu)).sum() for a, l in zip(xn, lam))
+            loss.backward()
+            for n in range(5):
+                g = x0[n].grad.cpu().numpy()
+                assert np.max(np.abs(g - ref[n])) <= 1e-12 * np.max(np.abs(ref[n])), (k, n)
+        finally:
+            eng.close()
+    # a loss that reads halo entries: the fold returns them to their sources
+    eng = TvdRollout(st, precision=tvd.FP64, checkpoint_every=2)
+    try:
+        x0 = [torch.tensor(f, device=gpu, requires_grad=True) for f in st.fields]
+        xn = eng.rollout(x0, dt, 3)
+        loss = (xn[3][:, :, 5] ** 2).sum() + xn[0][0:2].sum()
+        loss.backward()
+        lam_full = [np.zeros(f.shape) for f in st.fields]
+        xn_np = [a.detach().cpu().numpy() for a in xn]
+        lam_full[3][:, :, 5] = 2 * xn_np[3][:, :, 5]
+        lam_full[0][0:2] = 1.0
+        folded = [_fold(l) for l in lam_full]
+        ref = adjoint_run(st, folded, dt, 3)
+        for n in range(5):
+            g = x0[n].grad.cpu().numpy()
+            assert np.max(np.abs(g - ref[n])) <= 1e-12 * max(np.max(np.abs(ref[n])), 1e-300), n
+    finally:
+        eng.close()
+
+
+def _fold(a):
+    """Adjoint of the periodic halo fill on the host (rows, then columns)."""
+    a = a.copy()
+    nx, ny = a.shape[0] - 4, a.shape[1] - 4
+    for h, s in ((0, ny), (1, ny + 1), (ny + 2, 2), (ny + 3, 3)):
+        a[:, s] += a[:, h]
+        a[:, h] = 0
+    for h, s in ((0, nx), (1, nx + 1), (nx + 2, 2), (nx + 3, 3)):
+        a[s, 2:ny + 2] += a[h, 2:ny + 2]
+        a[h, 2:ny + 2] = 0
+    return a
+
+
+# ---------------------------------------------------------------------------
+# C4: 64-step rollout on 256x256x128, loss = sum th^2 over a 32x32x16 subvolume
+C4 = dict(nz=128, ny=256, nx=256, steps=64)
+SUB = (slice(2 + 112, 2 + 144), slice(2 + 112, 2 + 144), slice(56, 72))
+
+
+def _c4_state():
+    """The short-range distributions (u = 5 + N(0,1), v = N(0,1), w = N(0,0.5),
+    th = 300 + 0.003 z + U(-0.1,0.1), q) at 256x256x128, dx = dy = 50 m."""
+    st = ss.make_state(C4["nz"], C4["ny"], C4["nx"], 50.0, 3200.0, seed=13, with_zfields=False)
+    return st, ss.cfl_dt(st, 0.3)
+
+
+def _c4_gradient(st, dt, precision, k=8):
+    import torch
+    from monc_amd.adjoint import TvdRollout
+
+    eng = TvdRollout(st, precision=precision, checkpoint_every=k)
+    try:
+        dtype = eng.dtype
+        x0 = [torch.tensor(f, device="cuda", dtype=dtype, requires_grad=(n in (0, 3)))
+              for n, f in enumerate(st.fields)]
+        for _ in range(2):  # warm-up pass, then the timed one
+            for x in x0:
+                x.grad = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            xn = eng.rollout(x0, dt, C4["steps"])
+            loss = (xn[3][SUB].double() ** 2).sum()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            loss.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+        return (float(loss), x0[3].grad.double().cpu().numpy(), x0[0].grad.double().cpu().numpy(),
+                t1 - t0, t2 - t1)
+    finally:
+        eng.close()
+
+
+@pytest.fixture(scope="module")
+def c4(gpu):
+    st, dt = _c4_state()
+    l32, g32_th, g32_u, tf32, tb32 = _c4_gradient(st, dt, tvd.FP32)
+    l64, g64_th, g64_u, tf64, tb64 = _c4_gradient(st, dt, tvd.FP64)
+    return dict(st=st, dt=dt, l32=l32, l64=l64, g32=(g32_th, g32_u), g64=(g64_th, g64_u),
+                t=dict(fwd32=tf32, bwd32=tb32, fwd64=tf64, bwd64=tb64))
+
+
+def _bump(st, amp):
+    """Smooth direction: a Gaussian bump (12 cells wide, 8 levels deep) at the
+    centre of the loss subvolume, periodic halo filled."""
+    nx, ny, nz = st.nx, st.ny, st.nz
+    x = np.arange(nx)[:, None, None] - 128.0
+    y = np.arange(ny)[None, :, None] - 128.0
+    z = np.arange(nz)[None, None, :] - 63.5
+    a = np.zeros(st.fields[0].shape)
+    a[2:nx + 2, 2:ny + 2, :] = amp * np.exp(-(x * x + y * y) / (2 * 12.0 ** 2) - z * z / (2 * 8.0 ** 2))
+    return _halo(a)
+
+
+def test_c4_fp32_adjoint_vs_fp64_tangent_linear(c4):
+    """Directional derivatives of the C4 loss from the FP64 tangent-linear
+    model against <gradient, direction> from the FP64 and FP32 adjoints, for a
+    grid-scale noise direction and a smooth bump per control (th, u), plus the
+    relative L2 distance between the FP32 and FP64 gradients.  The FP32
+    gradient is the exact adjoint of the FP32 trajectory; it differs from the
+    FP64 one where the two trajectories take different limiter branches
+    (isolated points, which dominate the L2 distance and the noise direction)."""
+    st, dt = c4["st"], c4["dt"]
+    rng = np.random.default_rng(14)
+    report = {"timing_s": c4["t"], "bwd_over_fwd_fp32": c4["t"]["bwd32"] / c4["t"]["fwd32"],
+              "bwd_over_fwd_fp64": c4["t"]["bwd64"] / c4["t"]["fwd64"]}
+    for name, fid, gi, amp in (("th", 3, 0, 0.05), ("u", 0, 1, 0.3)):
+        g32, g64 = interior(c4["g32"][gi]), interior(c4["g64"][gi])
+        report[name] = {"grad_rel_l2_32_vs_64": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))}
+        for dname, d in (("noise", _direction(st, rng, fields=[fid])[fid]), ("smooth", _bump(st, amp))):
+            dx = [np.zeros_like(f) for f in st.fields]
+            dx[fid] = d
+            xn, tan = tangent_run(st, dx, dt, C4["steps"])
+            tl = float(np.sum(2.0 * xn[3][SUB] * tan[3][SUB]))
+            ad32 = float(np.sum(c4["g32"][gi] * d))
+            ad64 = float(np.sum(c4["g64"][gi] * d))
+            report[name][dname] = {"tl64": tl, "adj64": ad64, "adj32": ad32,
+                                   "rel_adj64_vs_tl": abs(ad64 - tl) / abs(tl),
+                                   "rel_adj32_vs_tl": abs(ad32 - tl) / abs(tl)}
+    print("C4_REPORT " + json.dumps(report))
+    for name in ("th", "u"):
+        r = report[name]
+        for dname in ("noise", "smooth"):
+            assert r[dname]["rel_adj64_vs_tl"] <= 1e-10, (name, dname, r)
+        assert r["smooth"]["rel_adj32_vs_tl"] <= 1e-4, (name, r)
+        assert r["grad_rel_l2_32_vs_64"] <= 3e-3, (name, r)
+    assert report["bwd_over_fwd_fp32"] <= 3.0, report
+
+
+@pytest.mark.timeout(900)
+def test_c4_central_differences_200_probes(c4):
+    """FP64 central differences with eps = 1e-3 at 200 probes (140 on th, 60
+    on u) drawn at random where the gradient is not negligible.  Loss
+    differences are formed as sum (th_h - th_0)(th_h + th_0) over the
+    subvolume, so they carry no cancellation against the loss itself.  Each
+    probe also takes the eps/2 differences; it is near a switch point (a
+    limiter branch, max/min arm or sign pick changing inside [-eps, eps]) and
+    replaced by another when the eps and eps/2 central differences differ by
+    more than 1e-5 relative, or the eps and eps/2 second differences by more
+    than 25% (a kink at s inside the interval moves the first by c s / 2 eps
+    and doubles the second).  A perturbation of u moves the limiter inputs of
+    every face it advects, so about 3 in 4 u probes are near a switch against
+    1 in 6 for th.  Reported: the eps = 1e-3 difference and its Richardson
+    extrapolation (4 D(eps/2) - D(eps)) / 3 against the FP64 and FP32 adjoint
+    gradients, relative to the largest probe derivative."""
+    import torch
+    from monc_amd.adjoint import TvdRollout
+
+    st, dt = c4["st"], c4["dt"]
+    eps = 1e-3
+    rng = np.random.default_rng(15)
+    eng = TvdRollout(st, precision=tvd.FP64, checkpoint_every=C4["steps"])
+    results = {"th": [], "u": []}
+    skipped = {"th": 0, "u": 0}
+    try:
+        x0 = [torch.tensor(f, device="cuda") for f in st.fields]
+        th0 = eng.forward_only(x0, dt, C4["steps"])[3][SUB].clone()
+
+        def dloss(xs):
+            th = eng.forward_only(xs, dt, C4["steps"])[3][SUB]
+            return float(((th - th0) * (th + th0)).sum())
+
+        for name, fid, gi, want, cap in (("th", 3, 0, 140, 220), ("u", 0, 1, 60, 320)):
+            g64 = c4["g64"][gi]
+            mag = np.abs(interior(g64))
+            cand = np.argwhere(mag > 1e-3 * mag.max())
+            order = rng.permutation(len(cand))
+            tried = 0
+            for idx in order:
+                if len(results[name]) == want or tried >= cap:
+                    break
+                tried += 1
+                i, j, k = (int(v) for v in cand[idx])
+                p = (i + 2, j + 2, k)
+                base = x0[fid][p].item()
+                dl = {}
+                for h in (eps, -eps, eps / 2, -eps / 2):
+                    x0[fid][p] = base + h
+                    dl[h] = dloss(x0)
+                x0[fid][p] = base
+                d1 = (dl[eps] - dl[-eps]) / (2 * eps)
+                d2 = (dl[eps / 2] - dl[-eps / 2]) / eps
+                s1 = (dl[eps] + dl[-eps]) / eps ** 2
+                s2 = (dl[eps / 2] + dl[-eps / 2]) / (eps / 2) ** 2
+                if abs(d1 - d2) > 1e-5 * abs(d2) or abs(s1 - s2) > 0.25 * max(abs(s1), abs(s2)) + 0.05:
+                    skipped[name] += 1
+                    continue
+                rich = (4 * d2 - d1) / 3
+                results[name].append((d1, rich, float(g64[p]), float(c4["g32"][gi][p])))
+                if len(results[name]) % 20 == 0:
+                    print(f"C4_FD {name}: {len(results[name])} probes done", flush=True)
+            print(f"C4_FD {name}: {len(results[name])} probes, {skipped[name]} near switch points", flush=True)
+    finally:
+        eng.close()
+    report = {}
+    for name in ("th", "u"):
+        r = np.array(results[name])
+        scale = np.abs(r[:, 0]).max()
+        e32 = np.abs(r[:, 3] - r[:, 1]) / scale
+        report[name] = {"probes": len(r), "near_switch_skipped": skipped[name],
+                        "fd_vs_adj64": float(np.max(np.abs(r[:, 2] - r[:, 0])) / scale),
+                        "richardson_vs_adj64": float(np.max(np.abs(r[:, 2] - r[:, 1])) / scale),
+                        "fd_vs_adj32": float(np.max(np.abs(r[:, 3] - r[:, 0])) / scale),
+                        "richardson_vs_adj32": float(np.max(e32)),
+                        "richardson_vs_adj32_p95": float(np.percentile(e32, 95)),
+                        "median_rel_richardson_vs_adj32": float(np.median(np.abs(r[:, 3] - r[:, 1])
+                                                                          / np.abs(r[:, 1])))}
+    print("C4_FD_REPORT " + json.dumps(report))
+    assert report["th"]["probes"] == 140 and report["u"]["probes"] == 60, report
+    for n in report:
+        assert report[n]["fd_vs_adj64"] <= 1e-4, report
+        assert report[n]["richardson_vs_adj64"] <= 1e-4, report
+        assert report[n]["richardson_vs_adj32_p95"] <= 1e-4, report
+        assert report[n]["richardson_vs_adj32"] <= 1e-3, report

@@ -92,6 +92,19 @@ def test_constant_scalar_gives_c_times_divergence(anel):
     np.testing.assert_allclose(inner(s[4]), -2.5 * inner(D), rtol=0, atol=1e-12 * np.abs(2.5 * D).max())
 
 
+def test_nondivergent_flow_is_discretely_divergence_free():
+    """The synthetic divergence-free flow: the scheme's own divergence of it
+    (the tendency of a constant scalar) vanishes to rounding at every cell."""
+    st = ss.make_state(16, 12, 20, 25.0, 400.0, seed=3, flow="nondivergent", with_zfields=False)
+    assert np.abs(inner(st.fields[0]) - 5.0).max() > 0.5 and np.abs(inner(st.fields[2])).max() > 0.05
+    assert np.all(st.fields[2][:, :, 0] == 0) and np.all(st.fields[2][:, :, -1] == 0)
+    st.fields[3][:] = 300.0
+    s, _ = tend(st, dtm=1.0)
+    D = _divergence_scalar(st)
+    assert np.abs(inner(D)).max() <= 1e-14
+    assert np.abs(inner(s[3])).max() <= 1e-11
+
+
 def _hat(a, shift, nz):
     """Dual-grid interpolation of a flow (stencil.F90:147-169) for the
     positions used by interior cells (no raw-copy edges except z at nz)."""
