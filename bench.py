@@ -5,8 +5,8 @@ Workload (the plain FP64 form at real shape): a 512x512x256 local domain
 seeded distributions of the synthetic set-up, advection-only forward-Euler
 steps at CFL 0.3.  One step = the TVD tendencies of all five fields from the
 pre-step state plus the update field <- field + dt*s and the periodic halo of
-the new state (monc_tvd_forward_steps): four launches of the wave kernel, one
-each for u, v, w and the (th, q) pair.  The same step in FP32 is reported
+the new state (monc_tvd_forward_steps): four launches of the split wave
+kernel, one each for u, v, w and the (th, q) pair.  The same step in FP32 is reported
 beside it.
 
 Output: one JSON line.  value = interior cell-updates per second (all five
@@ -157,7 +157,7 @@ def main() -> None:
                    "nx": nx, "ny": ny, "nz": nz, "fields": 5, "dt_s": dt},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"wave_kernel ({kind})", "kernel_ms": round(kavg, 4),
+                     "kernel": f"split_kernel ({kind})", "kernel_ms": round(kavg, 4),
                      "algorithmic_bytes_per_launch": kbytes,
                      "step": {"kernel_ms": round(step_kernel_ms, 4), "bytes_per_cell_update": 80,
                               "achieved_GBs": round(80 * cells / (step_kernel_ms * 1e-3) / 1e9, 2)},

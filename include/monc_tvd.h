@@ -99,10 +99,12 @@ int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **
 /* Launch shape (no counterpart in the reference: a tuning knob of this
  * library).  kernel 0 = x-marching kernels for columns of up to 256 levels,
  * 1 = one-cell-per-thread kernel.  For kernel 0, rows = -1 picks the default
- * shape, 0 the wave kernel (one wave per row, no block synchronisation), and
- * rows > 0 the slab kernel with that many cell rows per block (y faces shared
- * through LDS); strip = x columns per block (0: default 16).  Shapes that are
- * not compiled in are rejected. */
+ * shape (the split wave kernel), 0 the wave kernel (one wave per row over all
+ * its levels, no block synchronisation), -2 the split wave kernel (one wave
+ * per 64 levels of a row, the z face crossing sub-levels passed through LDS),
+ * and rows > 0 the slab kernel with that many cell rows per block (y faces
+ * shared through LDS); strip = x columns per block (0: default 16).  Shapes
+ * that are not compiled in are rejected. */
 int monc_tvd_set_tiling(monc_tvd_ctx *ctx, int kernel, int rows, int strip);
 
 /* Per-kernel timing (no counterpart in the reference: measurement support).

@@ -174,7 +174,8 @@ class TvdAdvection:
 
     def set_tiling(self, kernel: int = 0, rows: int = -1, strip: int = 0) -> None:
         """kernel 0 = x-marching kernels (rows -1 default shape, 0 wave kernel,
-        > 0 slab rows), 1 = one-cell-per-thread kernel; strip = x columns per block."""
+        -2 split wave kernel, > 0 slab rows), 1 = one-cell-per-thread kernel;
+        strip = x columns per block."""
         _check(self._lib.monc_tvd_set_tiling(self._h, kernel, rows, strip))
 
     def weighted_sum(self, field: int, variant: int, weights: np.ndarray) -> float:
