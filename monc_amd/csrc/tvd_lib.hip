@@ -14,6 +14,7 @@
 #include "../../include/monc_tvd.h"
 #include "tvd_adjoint.hpp"
 #include "tvd_kernels.hpp"
+#include "tvd_march.hpp"
 
 using namespace tvd;
 
@@ -167,12 +168,9 @@ void launch_slab(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, 
   hipLaunchKernelGGL((slab_kernel<T, KIND, NS, MODE, NCH, TY>), grid, block, 0, c->stream, L, S, strip);
 }
 
-// Default shape: the wave kernel (rows = 0), fastest measured for every
-// column height on MI355X.
-// Default x-marching kernel: the split wave kernel (one sub-level of 64
-// levels per wave; measured fastest at 128x256x256 and 256x512x512 in FP32
-// and FP64).
-int default_ty(int) { return 99; }
+// Default x-marching kernel: the march kernel (every face computed once;
+// 1.45x the split wave kernel at 512x512x256 in FP32 and FP64 on MI355X).
+int default_ty(int) { return 98; }
 
 bool ty_compiled(int nch, int ty) {
   switch (nch) {
@@ -201,8 +199,24 @@ void launch_split(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
   hipLaunchKernelGGL((split_kernel<T, KIND, NS, MODE, NCH, R>), grid, block, 0, c->stream, L, S, strip);
 }
 
+// March kernel: NCH waves per row, R rows per block; a 1-D grid of
+// tiles_y * tiles_x tiles (the kernel maps blocks to tiles XCD by XCD).
+template <int NCH, int ES>
+constexpr int march_rows() {
+  return NCH == 1 ? 8 : NCH == 2 ? 4 : NCH == 3 ? 4 : (ES == 8 ? 3 : 4);
+}
+
+template <typename T, int KIND, int NS, int MODE, int NCH>
+void launch_march(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int strip) {
+  constexpr int R = march_rows<NCH, (int)sizeof(T)>();
+  const int tiles_y = (c->ny + R - 1) / R, tiles_x = (c->nx + strip - 1) / strip;
+  dim3 block(64, NCH * R, 1);
+  hipLaunchKernelGGL((march_kernel<T, KIND, NS, MODE, NCH, R>), dim3(tiles_y * tiles_x), block, 0, c->stream, L, S,
+                     strip, tiles_y);
+}
+
 // key: nch*100 + rows for the slab kernel; nch*100 + 0 for the wave kernel;
-// nch*100 + 99 for the split wave kernel
+// nch*100 + 99 for the split wave kernel; nch*100 + 98 for the march kernel
 template <typename T, int KIND, int NS, int MODE>
 void launch_slab_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int key, int strip) {
   switch (key) {
@@ -214,6 +228,10 @@ void launch_slab_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T
     case 299: launch_split<T, KIND, NS, MODE, 2>(c, L, S, strip); break;
     case 399: launch_split<T, KIND, NS, MODE, 3>(c, L, S, strip); break;
     case 499: launch_split<T, KIND, NS, MODE, 4>(c, L, S, strip); break;
+    case 198: launch_march<T, KIND, NS, MODE, 1>(c, L, S, strip); break;
+    case 298: launch_march<T, KIND, NS, MODE, 2>(c, L, S, strip); break;
+    case 398: launch_march<T, KIND, NS, MODE, 3>(c, L, S, strip); break;
+    case 498: launch_march<T, KIND, NS, MODE, 4>(c, L, S, strip); break;
     case 103: launch_slab<T, KIND, NS, MODE, 1, 3>(c, L, S, strip); break;
     case 107: launch_slab<T, KIND, NS, MODE, 1, 7>(c, L, S, strip); break;
     case 201: launch_slab<T, KIND, NS, MODE, 2, 1>(c, L, S, strip); break;
@@ -229,7 +247,7 @@ void launch_slab_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T
 template <typename T, int MODE>
 int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
   const int nch = (c->nz + 63) / 64;
-  const int ty = c->slab_ty == -2 ? 99 : c->slab_ty >= 0 ? c->slab_ty : default_ty(nch);
+  const int ty = c->slab_ty == -3 ? 98 : c->slab_ty == -2 ? 99 : c->slab_ty >= 0 ? c->slab_ty : default_ty(nch);
   const int strip = c->slab_strip > 0 ? c->slab_strip : 16;
   if (c->use_cell_kernel || nch > 4 || !c->slab_ok) {
     const int nk = MODE == MODE_STEP ? c->nz : c->nz - 1;
@@ -591,10 +609,10 @@ int monc_tvd_set_tiling(monc_tvd_ctx* c, int kernel, int rows, int strip) {
   if (kernel != 0 && kernel != 1) return fail(MONC_TVD_ERR_ARG, "kernel %d is neither 0 (slab) nor 1 (cell)", kernel);
   if (strip < 0 || strip > 4096) return fail(MONC_TVD_ERR_ARG, "strip %d out of range [0,4096]", strip);
   const int nch = (c->nz + 63) / 64;
-  if (rows < -2 || (rows > 0 && !ty_compiled(nch, rows)))
+  if (rows < -3 || (rows > 0 && !ty_compiled(nch, rows)))
     return fail(MONC_TVD_ERR_ARG, "no slab shape with %d rows for %d-level columns", rows, c->nz);
   c->use_cell_kernel = kernel;
-  c->slab_ty = rows;  // -1 default, 0 wave kernel, -2 split wave kernel, > 0 slab rows
+  c->slab_ty = rows;  // -1 default, 0 wave kernel, -2 split wave kernel, -3 march kernel, > 0 slab rows
   c->slab_strip = strip;
   return MONC_TVD_OK;
 }

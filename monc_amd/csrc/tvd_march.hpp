@@ -1,0 +1,452 @@
+// March kernel: the default x-marching kernel of the TVD step on gfx950.
+//
+// A block owns R cell rows (j0 .. j0+R-1) over all levels: NCH waves of 64
+// levels per row (lanes along z), R*NCH waves, and marches along an x strip.
+// Per column i every cell (k, j, i) computes each face it owns exactly once:
+//   * the right x face, carried in registers to the next column as its left face;
+//   * the north y face, handed to the row above through LDS (the first row of
+//     the block also computes its south face);
+//   * the top z face, handed to the level above by a wave shuffle and across
+//     64-level sub-columns through LDS.
+// One barrier per column orders the LDS exchange (double-buffered by column
+// parity).
+//
+// The face arithmetic is that of tvd_faces.hpp (ultimateflux.F90:81-614) with
+// three changes of form that keep every value identical:
+//   * the reference's upwind weights ap = 1/2 + sign(1/2, vel), an = 1 - ap are
+//     exactly 0 or 1, so ap*x + an*y is the selection ap ? x : y;
+//   * the three vertical branches of ultflx (bottom k = 1 for w, middle, top
+//     k = nz-1; :236-523) are one computation whose few differing operands are
+//     chosen per lane: no wave runs two branches;
+//   * the per-level metric factors of each branch are tabulated once per block
+//     in LDS in the working precision.
+// Loads use buffer instructions with the wave-uniform row and column in the
+// scalar offset and the per-lane level in the vector offset (levels clamped to
+// [1, nz], so every load stays inside its own column).
+#pragma once
+#include "tvd_kernels.hpp"
+
+namespace tvd {
+
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ void stb(__amdgpu_buffer_rsrc_t r, T v, unsigned vo, unsigned so) {
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vo, so, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 0);
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)0xffffffff, BUFFER_RSRC_WORD3);
+}
+
+// Byte strides shared by every field of a launch.
+struct Geo {
+  unsigned sjb, sib;
+  __device__ __forceinline__ unsigned so(int j, int i) const {
+    return (unsigned)(j - 1) * sjb + (unsigned)(i - 1) * sib;
+  }
+};
+
+// A flow as a field kind sees it (stencil.F90:147-169): raw for scalars, the
+// dual-grid interpolation 0.5*(f + f shifted one step along u's x, v's y or
+// w's z) for momentum.  A level is given by its byte offset o and that of the
+// level above it, o1, clamped to nz: at k = nz the interpolation of w reads
+// 0.5*(f + f) = f, the reference's raw copy (:160-163).
+template <typename T, int KIND>
+struct MFlow {
+  static constexpr int DK = KIND == KIND_W ? 1 : 0;
+  static constexpr int DJ = KIND == KIND_V ? 1 : 0;
+  static constexpr int DI = KIND == KIND_U ? 1 : 0;
+  __amdgpu_buffer_rsrc_t r;
+  Geo g;
+  __device__ __forceinline__ T operator()(unsigned o, unsigned o1, int j, int i) const {
+    const unsigned s = g.so(j, i);
+    if constexpr (DK) return T(0.5) * (ldb<T>(r, o, s) + ldb<T>(r, o1, s));
+    else if constexpr (DJ) return T(0.5) * (ldb<T>(r, o, s) + ldb<T>(r, o, s + g.sjb));
+    else if constexpr (DI) return T(0.5) * (ldb<T>(r, o, s) + ldb<T>(r, o, s + g.sib));
+    else return ldb<T>(r, o, s);
+  }
+};
+
+// Per-lane byte offsets of the levels a lane reads, each clamped to [1, nz].
+struct Levels {
+  unsigned k, km1, kp1, kp0, kp2;  // k, k-1, k+1, (k+1 clamped) - 1, k+2
+};
+
+// Metric factors per level, tabulated in LDS for the field kind (for w the
+// reference passes (rdzn, rdz, dz) in the roles of (rdz, rdzn, dzn), kdof = 1;
+// tvdadvection.F90:881-894).  Indices clamped to [1, nz].
+enum {
+  M_RDZ,     // rdz(k): x/y outflow sums
+  M_RDZKD,   // rdz(k+kdof): x/y vertical gradient terms
+  M_ZS_UP,   // z outflow-sum factor, W >= 0: rdz(k); rdz(1+kdof) at k = 1, rdz(k+kdof) at k = nz-1
+  M_ZS_DN,   // W < 0: rdz(k+1); rdz(2+kdof) at k = 1
+  M_RDC_UP,  // rdz(k+kdof)
+  M_RDC_DN,  // rdz(k+1+kdof)
+  M_RDU_UP,  // rdzn(k); rdzn(2) at k = 1
+  M_RDU_DN,  // rdzn(k+2) (rdzn(nz) at k = nz-1)
+  M_RDD,     // rdzn(k+1)
+  M_DD2,     // dzn(k+1)^2
+  M_T1,      // tzc1 / tzd1 (k)
+  M_T2,      // tzc2 / tzd2 (k); 0 at k = nz
+  M_COUNT
+};
+
+template <typename T>
+__device__ __forceinline__ T met_value(int m, int k, int nz, int kdof, const VMet& v) {
+  auto c = [nz](int x) { return x < 1 ? 1 : (x > nz ? nz : x); };
+  switch (m) {
+    case M_RDZ: return vm<T>(v.rdz, c(k));
+    case M_RDZKD: return vm<T>(v.rdz, c(k + kdof));
+    case M_ZS_UP: return vm<T>(v.rdz, c(k == 1 ? 1 + kdof : (k == nz - 1 ? k + kdof : k)));
+    case M_ZS_DN: return vm<T>(v.rdz, c(k == 1 ? 2 + kdof : k + 1));
+    case M_RDC_UP: return vm<T>(v.rdz, c(k + kdof));
+    case M_RDC_DN: return vm<T>(v.rdz, c(k + 1 + kdof));
+    case M_RDU_UP: return vm<T>(v.rdzn, c(k == 1 ? 2 : k));
+    case M_RDU_DN: return vm<T>(v.rdzn, c(k + 2));
+    case M_RDD: return vm<T>(v.rdzn, c(k + 1));
+    case M_DD2: {
+      const T dd = vm<T>(v.dzn, c(k + 1));
+      return dd * dd;
+    }
+    case M_T1: return vm<T>(v.t1, c(k));
+    default: return k < nz ? vm<T>(v.t2, c(k)) : T(0);
+  }
+}
+
+// Flows at the level of the lane (and k-1, k+1) around column (j, i) that the
+// three faces of cell (k, j, i) and its divergence read.
+template <typename T>
+struct FlowsAt {
+  T u_m, u_0, u_p, u1_m, u1_0;      // U(k) at (j,i-1) (j,i) (j,i+1) (j+1,i-1) (j+1,i)
+  T v_0, v_s, v_0p, v_sp, v_n;      // V(k) at (j,i) (j-1,i) (j,i+1) (j-1,i+1) (j+1,i)
+  T w_0, w_p, w_n;                  // W(k) at (j,i) (j,i+1) (j+1,i)
+  T wm_0, wm_p, wm_n;               // W(k-1) at the same
+  T uk_m, uk_0, vk_s, vk_0, wk_0;   // U(k+1) at (j,i-1) (j,i); V(k+1) at (j-1,i) (j,i); W(k+1) at (j,i)
+};
+
+// Right x face of column (j, i) at the lane's level (ultimateflux.F90:102-165).
+template <typename T, int NS>
+__device__ __forceinline__ void mx_face(const __amdgpu_buffer_rsrc_t (&F)[NS], const Geo& g, const Levels& lv, int j,
+                                        int i, T u_m, T u_0, T u_p, T v_0, T v_s, T v_0p, T v_sp, T w_0, T w_p,
+                                        T wm_0, T wm_p, T rdz_k, T rdz_kd, T dt, T cx, T cy, T out[NS]) {
+  const bool upos = u_0 > T(0.0);
+  T s = rdz_k * (pos(upos ? w_0 : w_p) + negabs(upos ? wm_0 : wm_p));
+  s = s + cy * (pos(upos ? v_0 : v_0p) + negabs(upos ? v_s : v_sp));
+  s = s + cx * (upos ? u_0 + negabs(u_m) : pos(u_p) - u_0);
+  s = s * dt;
+  const T sz = T(0.125) * (w_0 + wm_0 + w_p + wm_p);
+  const bool pz = sign_pos(sz);
+  const unsigned osl = pz ? lv.k : lv.kp0;   // ksl
+  const unsigned osu = pz ? lv.km1 : lv.kp1;  // ksu
+  const T cz = fabs(sz) * rdz_kd * dt;
+  const T sy = T(0.125) * (v_0p + v_0 + v_sp + v_s);
+  const unsigned ojs = lv.k + (sign_pos(sy) ? 0u : 2u * g.sjb);  // row j-1 or j+1, from row j-1
+  const T cyt = fabs(sy) * cy * dt;
+  const HLim<T> lim(u_0, cx, dt, s);
+  const bool ap = sign_pos(u_0);
+  const unsigned s_m = g.so(j, i - 1), s_0 = g.so(j, i), s_1 = g.so(j, i + 1), s_2 = g.so(j, i + 2);
+  const unsigned sj0 = g.so(j - 1, i), sj1 = g.so(j - 1, i + 1);
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    const T fa = ldb<T>(F[n], lv.k, s_0), fb = ldb<T>(F[n], lv.k, s_1);
+    const T fu = ap ? ldb<T>(F[n], lv.k, s_m) : ldb<T>(F[n], lv.k, s_2);
+    const T fc = ap ? fa : fb;
+    const T fd = ap ? fb : fa;
+    const T d0 = ldb<T>(F[n], osl, s_0) - ldb<T>(F[n], osu, s_0);
+    const T d1 = ldb<T>(F[n], osl, s_1) - ldb<T>(F[n], osu, s_1);
+    const T g1 = cz * (ap ? d0 : d1);
+    const T t = ap ? ldb<T>(F[n], ojs, sj0) : ldb<T>(F[n], ojs, sj1);
+    const T g2 = cyt * (fc - t);
+    out[n] = lim.face(fu, fc, fd, g1, g2);
+  }
+}
+
+// North y face of row j at column i (ultimateflux.F90:189-233); vk, vn, vs =
+// V at rows j, j+1, j-1; w0/w0m, w1/w1m = W(k), W(k-1) at rows j, j+1; u0/u0m,
+// u1/u1m = U at (row j, i), (row j, i-1), (row j+1, i), (row j+1, i-1).
+template <typename T, int NS>
+__device__ __forceinline__ void my_face(const __amdgpu_buffer_rsrc_t (&F)[NS], const Geo& g, const Levels& lv, int j,
+                                        int i, T vk, T vn, T vs, T w0, T w0m, T w1, T w1m, T u0, T u0m, T u1, T u1m,
+                                        T rdz_k, T rdz_kd, T dt, T cx, T cy, T out[NS]) {
+  const bool vpos = sign_pos(vk);
+  T s = rdz_k * (pos(vpos ? w0 : w1) + negabs(vpos ? w0m : w1m));
+  s = s + cy * (pos(vpos ? vk : vn) + negabs(vpos ? vs : vk));
+  s = s + cx * (pos(vpos ? u0 : u1) + negabs(vpos ? u0m : u1m));
+  s = s * dt;
+  const T sz = T(0.125) * (w0 + w0m + w1 + w1m);
+  const bool pz = sign_pos(sz);
+  const unsigned orow = vpos ? 0u : g.sjb;  // row jj = j + jofset
+  const unsigned osl = (pz ? lv.k : lv.kp0) + orow;
+  const unsigned osu = (pz ? lv.km1 : lv.kp1) + orow;
+  const T cz = fabs(sz) * rdz_kd * dt;
+  const T sx = T(0.125) * (u0m + u0 + u1m + u1);
+  const bool xup = sign_pos(sx);
+  const unsigned ot = lv.k + orow + (xup ? 0u : 2u * g.sib);  // (jj, i-1) or (jj, i+1), from column i-1
+  const T cxt = fabs(sx) * cx * dt;
+  const HLim<T> lim(vk, cy, dt, s);
+  const unsigned s_s = g.so(j - 1, i), s_0 = g.so(j, i), s_1 = g.so(j + 1, i), s_2 = g.so(j + 2, i);
+  const unsigned s_im = g.so(j, i - 1);
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    const T fa = ldb<T>(F[n], lv.k, s_0), fb = ldb<T>(F[n], lv.k, s_1);
+    const T fu = vpos ? ldb<T>(F[n], lv.k, s_s) : ldb<T>(F[n], lv.k, s_2);
+    const T fc = vpos ? fa : fb;
+    const T fd = vpos ? fb : fa;
+    const T g1 = cz * (ldb<T>(F[n], osl, s_0) - ldb<T>(F[n], osu, s_0));
+    const T g2 = cxt * (fc - ldb<T>(F[n], ot, s_im));
+    out[n] = lim.face(fu, fc, fd, g1, g2);
+  }
+}
+
+// Top z face of the lane's level k (the reference's flux_z(k+1)),
+// ultimateflux.F90:236-523, all three branches in one: `edge` marks k = 1 and
+// k = nz-1, whose upwind test is W >= 0 (the middle one takes the sign, so they
+// differ for W = -0), `top` marks k = nz-1 (W < 0 extrapolates linearly above
+// the top and has no inflow from above), `bot` marks k = 1 (no level below).
+template <typename T, int NS>
+__device__ __forceinline__ void mz_face(const __amdgpu_buffer_rsrc_t (&F)[NS], const Geo& g, const Levels& lv, int j,
+                                        int i, const FlowsAt<T>& q, bool edge, bool top, bool bot, T zs_up, T zs_dn,
+                                        T rdc_up, T rdc_dn, T rdu_up, T rdu_dn, T rdd, T dd2, T dt, T cx, T cy,
+                                        T out[NS]) {
+  const T wk = q.w_0;
+  const bool up = edge ? wk >= T(0.0) : sign_pos(wk);
+  // outflow sum of the upwind cell kk = k (up) or k+1
+  const T wa = up ? pos(wk) : (top ? T(0.0) : pos(q.wk_0));
+  const T wb = up ? (bot ? T(0.0) : negabs(q.wm_0)) : negabs(wk);
+  T s = (up ? zs_up : zs_dn) * (wa + wb);
+  s = s + cy * (pos(up ? q.v_0 : q.vk_0) + negabs(up ? q.v_s : q.vk_s));
+  s = s + cx * (pos(up ? q.u_0 : q.uk_0) + negabs(up ? q.u_m : q.uk_m));
+  s = s * dt;
+  // transverse gradient weights at level k (:697-761)
+  const T sy = T(0.125) * (q.v_s + q.v_0 + q.vk_s + q.vk_0);
+  const T cyt = fabs(sy) * cy * dt;
+  const T sx = T(0.125) * (q.u_m + q.u_0 + q.uk_m + q.uk_0);
+  const bool xup = sign_pos(sx);
+  const T cxt = fabs(sx) * cx * dt;
+  const unsigned okk = up ? lv.k : lv.kp1;
+  const unsigned ojs = okk + (sign_pos(sy) ? 0u : 2u * g.sjb);  // (kk, j-1 or j+1), from row j-1
+  const unsigned oxs = okk + (xup ? 0u : 2u * g.sib);           // (kk, i-1 or i+1), from column i-1
+  // limiter, velocity and metric part (calculate_flux_in_w, :525-554)
+  const T rdc = up ? rdc_up : rdc_dn;
+  const T rdu = up ? rdu_up : rdu_dn;
+  const T r6 = T(1.0 / 6.0);
+  const T cflmod = fabs(wk * rdd * dt);
+  const T hrcfl = recip(s + T(1.e-30));
+  const T hcfl = T(0.5) * cflmod;
+  const T cflcoef = r6 * (T(1.0) - cflmod * cflmod) * (dd2 * rdc);
+  const unsigned s_0 = g.so(j, i), s_jm = g.so(j - 1, i), s_im = g.so(j, i - 1);
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    const T fk = ldb<T>(F[n], lv.k, s_0), fk1 = ldb<T>(F[n], lv.kp1, s_0);
+    const T fdn = top ? T(2.0) * fk1 - fk : ldb<T>(F[n], lv.kp2, s_0);
+    const T fu = up ? ldb<T>(F[n], lv.km1, s_0) : fdn;
+    const T fc = up ? fk : fk1;
+    const T fd = up ? fk1 : fk;
+    const T g1 = cyt * (fc - ldb<T>(F[n], ojs, s_jm));
+    const T g2 = cxt * (fc - ldb<T>(F[n], oxs, s_im));
+    const T fcurvs = fd - T(2.0) * fc + fu;
+    const T fdels = fd - fu;
+    const T rfdels = recip(guard(fdels));
+    const T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * ((fd - fc) * rdd - (fc - fu) * rdu) - g1 - g2;
+    const T fnfs = (ffs - fu) * rfdels;
+    const T fncs = (fc - fu) * rfdels;
+    const T fnrefs = fncs * hrcfl;
+    const T ftemp = tmax(fncs, tmin(tmin(fnfs, fnrefs), T(1.0)));
+    const T lim = ftemp * fdels + fu;
+    out[n] = fabs(fcurvs) >= fabs(fdels) ? fc : lim;
+  }
+}
+
+template <typename T, int KIND, int NS, int MODE, int NCH, int R>
+__global__ void __launch_bounds__(64 * NCH * R) march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
+  constexpr int NK = 64 * NCH;
+  constexpr bool IS_W = KIND == KIND_W;
+  constexpr int kdof = IS_W ? 1 : 0;
+  __shared__ T met[M_COUNT][NK];
+  __shared__ T yn[2][R][NS][NK];
+  __shared__ T zt[2][R][NCH][NS];
+  const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;
+  const VMet m = IS_W ? L.mw : L.mc;
+  const int tid = threadIdx.x + 64 * threadIdx.y;
+  for (int e = tid; e < M_COUNT * NK; e += 64 * NCH * R) {
+    const int mm = e / NK, kk = 1 + e % NK;
+    met[mm][kk - 1] = met_value<T>(mm, kk, nz, kdof, m);
+  }
+  // tile of this block: consecutive blocks of one XCD (blocks are dealt to the
+  // 8 XCDs round-robin) take consecutive tiles, so that rows shared by
+  // neighbouring tiles meet in one L2
+  const int nb = (int)gridDim.x, bid = (int)blockIdx.x;
+  const int full = nb - nb % 8;
+  const int t = bid < full ? (bid % 8) * (full / 8) + bid / 8 : bid;
+  const int ty = t % tiles_y, tx = t / tiles_y;
+
+  const int lane = threadIdx.x;
+  const int wv = (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int qc = wv % NCH, r = wv / NCH;
+  const int j = 3 + ty * R + r;
+  const bool active = j <= ny + 2;  // wave-uniform
+  const int k = 1 + 64 * qc + lane;
+  const int kmax = IS_W ? nz - 1 : nz;
+  const bool cell = k >= 2 && k <= kmax;
+  const bool has_z = IS_W ? (k >= 1 && k <= nz - 1) : (k >= 2 && k <= nz - 1);
+  const bool edge = k == 1 || k == nz - 1;
+  const bool ztop = k == nz - 1, zbot = k == 1;
+  const bool top = !IS_W && k == nz;
+  const int i_begin = 3 + tx * strip;
+  const int i_end = min(i_begin + strip, nx + 3);
+  const T dt = S.dt[0], cx = L.cx, cy = L.cy;
+
+  const unsigned es = sizeof(T);
+  auto off = [&](int kk) { return (unsigned)((kk < 1 ? 1 : (kk > nz ? nz : kk)) - 1) * es; };
+  Levels lv;
+  lv.k = off(k);
+  lv.km1 = off(k - 1);
+  lv.kp1 = off(k + 1);
+  lv.kp0 = off((k + 1 > nz ? nz : k + 1) - 1);
+  lv.kp2 = off(k + 2);
+  Geo g;
+  g.sjb = (unsigned)L.d.sj * es;
+  g.sib = (unsigned)L.d.si * es;
+  MFlow<T, KIND> U{rsrc(L.u), g}, V{rsrc(L.v), g}, W{rsrc(L.w), g};
+  __amdgpu_buffer_rsrc_t F[NS], Fo[NS], Ff[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    F[n] = rsrc(S.F[n]);
+    Fo[n] = rsrc(S.out[n]);
+    Ff[n] = rsrc(S.f[n]);
+  }
+  __syncthreads();
+  const int kl = (k <= NK ? k : NK) - 1;
+  const T rdz_k = met[M_RDZ][kl], rdz_kd = met[M_RDZKD][kl];
+
+  T fxl[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) fxl[n] = T(0);
+  if (active) {
+    const int i = i_begin - 1;
+    mx_face<T, NS>(F, g, lv, j, i, U(lv.k, lv.kp1, j, i - 1), U(lv.k, lv.kp1, j, i), U(lv.k, lv.kp1, j, i + 1),
+                   V(lv.k, lv.kp1, j, i), V(lv.k, lv.kp1, j - 1, i), V(lv.k, lv.kp1, j, i + 1),
+                   V(lv.k, lv.kp1, j - 1, i + 1), W(lv.k, lv.kp1, j, i), W(lv.k, lv.kp1, j, i + 1),
+                   W(lv.km1, lv.k, j, i), W(lv.km1, lv.k, j, i + 1), rdz_k, rdz_kd, dt, cx, cy, fxl);
+  }
+  int par = 0;
+  for (int i = i_begin; i < i_end; i++) {
+    T fxr[NS], fyn[NS], fys[NS], fzt[NS], fzb[NS];
+#pragma unroll
+    for (int n = 0; n < NS; n++) fxr[n] = fyn[n] = fys[n] = fzt[n] = T(0);
+    FlowsAt<T> q;
+    if (active) {
+      q.u_m = U(lv.k, lv.kp1, j, i - 1);
+      q.u_0 = U(lv.k, lv.kp1, j, i);
+      q.u_p = U(lv.k, lv.kp1, j, i + 1);
+      q.u1_m = U(lv.k, lv.kp1, j + 1, i - 1);
+      q.u1_0 = U(lv.k, lv.kp1, j + 1, i);
+      q.v_0 = V(lv.k, lv.kp1, j, i);
+      q.v_s = V(lv.k, lv.kp1, j - 1, i);
+      q.v_0p = V(lv.k, lv.kp1, j, i + 1);
+      q.v_sp = V(lv.k, lv.kp1, j - 1, i + 1);
+      q.v_n = V(lv.k, lv.kp1, j + 1, i);
+      q.w_0 = W(lv.k, lv.kp1, j, i);
+      q.w_p = W(lv.k, lv.kp1, j, i + 1);
+      q.w_n = W(lv.k, lv.kp1, j + 1, i);
+      q.wm_0 = W(lv.km1, lv.k, j, i);
+      q.wm_p = W(lv.km1, lv.k, j, i + 1);
+      q.wm_n = W(lv.km1, lv.k, j + 1, i);
+      q.uk_m = U(lv.kp1, lv.kp2, j, i - 1);
+      q.uk_0 = U(lv.kp1, lv.kp2, j, i);
+      q.vk_s = V(lv.kp1, lv.kp2, j - 1, i);
+      q.vk_0 = V(lv.kp1, lv.kp2, j, i);
+      q.wk_0 = W(lv.kp1, lv.kp2, j, i);
+      my_face<T, NS>(F, g, lv, j, i, q.v_0, q.v_n, q.v_s, q.w_0, q.wm_0, q.w_n, q.wm_n, q.u_0, q.u_m, q.u1_0,
+                     q.u1_m, rdz_k, rdz_kd, dt, cx, cy, fyn);
+      if (r == 0) {  // south face of the block's first row
+        my_face<T, NS>(F, g, lv, j - 1, i, q.v_s, q.v_0, V(lv.k, lv.kp1, j - 2, i), W(lv.k, lv.kp1, j - 1, i),
+                       W(lv.km1, lv.k, j - 1, i), q.w_0, q.wm_0, U(lv.k, lv.kp1, j - 1, i),
+                       U(lv.k, lv.kp1, j - 1, i - 1), q.u_0, q.u_m, rdz_k, rdz_kd, dt, cx, cy, fys);
+      }
+      mx_face<T, NS>(F, g, lv, j, i, q.u_m, q.u_0, q.u_p, q.v_0, q.v_s, q.v_0p, q.v_sp, q.w_0, q.w_p, q.wm_0,
+                     q.wm_p, rdz_k, rdz_kd, dt, cx, cy, fxr);
+      T zf[NS];
+      mz_face<T, NS>(F, g, lv, j, i, q, edge, ztop, zbot, met[M_ZS_UP][kl], met[M_ZS_DN][kl], met[M_RDC_UP][kl],
+                     met[M_RDC_DN][kl], met[M_RDU_UP][kl], met[M_RDU_DN][kl], met[M_RDD][kl], met[M_DD2][kl], dt,
+                     cx, cy, zf);
+#pragma unroll
+      for (int n = 0; n < NS; n++) fzt[n] = has_z ? zf[n] : T(0);
+    }
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      yn[par][r][n][64 * qc + lane] = fyn[n];
+      fzb[n] = __shfl_up(fzt[n], 1);
+    }
+    if constexpr (NCH > 1) {
+      if (lane == 63) {
+#pragma unroll
+        for (int n = 0; n < NS; n++) zt[par][r][qc][n] = fzt[n];
+      }
+    }
+    __syncthreads();
+    if (r > 0) {
+#pragma unroll
+      for (int n = 0; n < NS; n++) fys[n] = yn[par][r - 1][n][64 * qc + lane];
+    }
+    if constexpr (NCH > 1) {
+      if (lane == 0 && qc > 0) {
+#pragma unroll
+        for (int n = 0; n < NS; n++) fzb[n] = zt[par][r][qc - 1][n];
+      }
+    }
+    par ^= 1;
+    if (active && k <= nz) {
+      const unsigned so = g.so(j, i);
+      if (cell) {
+        const T t1 = met[M_T1][kl], t2 = met[M_T2][kl];
+        const T wt = k < nz ? q.w_0 : T(0);
+#pragma unroll
+        for (int n = 0; n < NS; n++) {
+          T sv;
+          const T s0 = MODE == MODE_ACCUM ? ldb<T>(Fo[n], lv.k, so) : T(0);
+          if (top) {
+            sv = (q.v_s * fys[n] - q.v_0 * fyn[n]) * cy;
+            sv = sv + T(4.0) * t1 * q.wm_0 * fzb[n];
+            sv = sv + (q.u_m * fxl[n] - q.u_0 * fxr[n]) * cx;
+          } else {
+            sv = s0 + (q.v_s * fys[n] - q.v_0 * fyn[n]) * cy;
+            sv = sv + T(4.0) * (q.wm_0 * fzb[n] * t1 - wt * fzt[n] * t2);
+            sv = sv + (q.u_m * fxl[n] - q.u_0 * fxr[n]) * cx;
+          }
+          if constexpr (MODE == MODE_ACCUM) {
+            stb<T>(Fo[n], sv, lv.k, so);
+          } else {
+            const T val = ldb<T>(Ff[n], lv.k, so) + L.dtm * sv;
+            stb<T>(Fo[n], val, lv.k, so);
+            const int jh = (j <= 4) ? j + ny : (j >= ny + 1 ? j - ny : 0);
+            const int ih = (i <= 4) ? i + nx : (i >= nx + 1 ? i - nx : 0);
+            if (jh) stb<T>(Fo[n], val, lv.k, g.so(jh, i));
+            if (ih) stb<T>(Fo[n], val, lv.k, g.so(j, ih));
+            if (jh && ih) stb<T>(Fo[n], val, lv.k, g.so(jh, ih));
+          }
+        }
+      } else if constexpr (MODE == MODE_STEP) {
+        // levels the reference leaves untouched (k = 1; k = nz for w): s = 0
+#pragma unroll
+        for (int n = 0; n < NS; n++) {
+          const T val = ldb<T>(Ff[n], lv.k, so);
+          stb<T>(Fo[n], val, lv.k, so);
+          const int jh = (j <= 4) ? j + ny : (j >= ny + 1 ? j - ny : 0);
+          const int ih = (i <= 4) ? i + nx : (i >= nx + 1 ? i - nx : 0);
+          if (jh) stb<T>(Fo[n], val, lv.k, g.so(jh, i));
+          if (ih) stb<T>(Fo[n], val, lv.k, g.so(j, ih));
+          if (jh && ih) stb<T>(Fo[n], val, lv.k, g.so(jh, ih));
+        }
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < NS; n++) fxl[n] = fxr[n];
+  }
+}
+
+}  // namespace tvd

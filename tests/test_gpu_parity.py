@@ -33,7 +33,7 @@ def _compare(gpu_s, ref_s, tol=TOL, rows=None):
 
 @pytest.mark.parametrize("fs,ms,sc", [(1, 1, 1), (0, 0, 0), (1, 0, 0), (0, 1, 0)])
 @pytest.mark.parametrize("anelastic", [None, 8000.0])
-@pytest.mark.parametrize("tiling", [None, (1, 0, 0), (0, 0, 5), (0, 3, 5), (0, -2, 5)])
+@pytest.mark.parametrize("tiling", [None, (1, 0, 0), (0, 0, 5), (0, 3, 5), (0, -2, 5), (0, -3, 5), (0, -3, 16)])
 def test_tiny_grid_stepping_modes(gpu, fs, ms, sc, anelastic, tiling):
     st = ss.make_state(16, 10, 12, 50.0, 500.0, seed=3, n_q=2, anelastic_scale=anelastic)
     ref, _ = oracle.state_tendencies(st, 1.0, fs, ms, sc)
@@ -45,7 +45,9 @@ def test_tiny_grid_stepping_modes(gpu, fs, ms, sc, anelastic, tiling):
 # column heights that fill 1..4 chunks of 64 levels partially and exactly
 @pytest.mark.parametrize("nz,rows", [(64, 3), (64, 7), (70, 1), (128, 3), (150, 1), (192, 1), (256, 1), (200, 1),
                                      (64, 0), (70, 0), (128, 0), (150, 0), (256, 0), (200, 0),
-                                     (64, -2), (70, -2), (128, -2), (150, -2), (256, -2), (200, -2)])
+                                     (64, -2), (70, -2), (128, -2), (150, -2), (256, -2), (200, -2),
+                                     (64, -3), (70, -3), (128, -3), (150, -3), (256, -3), (200, -3), (4, -3),
+                                     (5, -3), (63, -3), (65, -3), (127, -3), (129, -3), (191, -3), (193, -3)])
 @pytest.mark.parametrize("strip", [5, 16])
 def test_slab_shapes(gpu, nz, rows, strip):
     st = ss.make_state(nz, 9, 11, 50.0, 25.0 * (nz - 1), seed=nz + rows, n_q=1)
@@ -121,7 +123,7 @@ def test_halo_fill_matches_periodic_copy(gpu):
         t.close()
 
 
-@pytest.mark.parametrize("tiling", [None, (1, 0, 0), (0, 3, 7), (0, 0, 7), (0, -2, 7)])
+@pytest.mark.parametrize("tiling", [None, (1, 0, 0), (0, 3, 7), (0, 0, 7), (0, -2, 7), (0, -3, 7), (0, -3, 3)])
 def test_forward_steps_match_oracle_run(gpu, tiling):
     st = ss.make_state(20, 24, 20, 50.0, 600.0, seed=12, n_q=1, with_zfields=False)
     dt = ss.cfl_dt(st, 0.3)
