@@ -41,7 +41,7 @@ def log(msg: str) -> None:
 KIND_FIELDS = {"u": 4, "v": 4, "w": 4, "scalars": 7, "cell": 10}
 
 
-def run_form(state, precision: int, steps: int, warmup: int, dt: float, rows: int = -1):
+def run_form(state, precision: int, steps: int, warmup: int, dt: float, kernel: int = 0, rows: int = 0):
     import torch
 
     from monc_amd import tvd
@@ -50,7 +50,7 @@ def run_form(state, precision: int, steps: int, warmup: int, dt: float, rows: in
     try:
         stream = torch.cuda.current_stream()
         t.set_stream(stream.cuda_stream)
-        t.set_tiling(0, rows, 0)
+        t.set_tiling(kernel, rows, 0)
         t.halo_fill()
         for _ in range(warmup):
             t.forward_steps(dt, 1)
@@ -108,8 +108,8 @@ def main() -> None:
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--nz", type=int, default=256)
-    ap.add_argument("--rows", type=int, default=-1,
-                    help="x-marching kernel: -1 default, -3 march, -2 split, 0 wave, > 0 slab rows")
+    ap.add_argument("--kernel", type=int, default=0, help="0 march (default), 1 cell, 2 split")
+    ap.add_argument("--rows", type=int, default=0, help="march kernel cell rows per block (0: default)")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -136,7 +136,7 @@ def main() -> None:
     cells = nx * ny * nz
 
     log("bench: FP64 form")
-    wall64, kt64 = run_form(st, tvd.FP64, args.steps, args.warmup, dt, args.rows)
+    wall64, kt64 = run_form(st, tvd.FP64, args.steps, args.warmup, dt, args.kernel, args.rows)
     ms_step = wall64 / args.steps * 1e3
     value = cells * args.steps / wall64 / 1e9
     kind, kavg, kbytes, achieved = roofline(kt64, cells, 8)
@@ -168,7 +168,7 @@ def main() -> None:
     }
     if not args.no_fp32:
         log("bench: FP32 form")
-        wall32, kt32 = run_form(st, tvd.FP32, args.steps, args.warmup, dt, args.rows)
+        wall32, kt32 = run_form(st, tvd.FP32, args.steps, args.warmup, dt, args.kernel, args.rows)
         v32 = cells * args.steps / wall32 / 1e9
         kind32, kavg32, kbytes32, a32 = roofline(kt32, cells, 4)
         step32 = sum(v[0] for k, v in kt32.items() if k in KIND_FIELDS) / args.steps

@@ -97,15 +97,56 @@ int monc_tvd_bind_field(monc_tvd_ctx *ctx, int field, int variant, void *device_
 int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **device_ptr);
 
 /* Launch shape (no counterpart in the reference: a tuning knob of this
- * library).  kernel 0 = x-marching kernels for columns of up to 256 levels,
- * 1 = one-cell-per-thread kernel.  For kernel 0, rows = -1 picks the default
- * shape (the split wave kernel), 0 the wave kernel (one wave per row over all
- * its levels, no block synchronisation), -2 the split wave kernel (one wave
- * per 64 levels of a row, the z face crossing sub-levels passed through LDS),
- * and rows > 0 the slab kernel with that many cell rows per block (y faces
- * shared through LDS); strip = x columns per block (0: default 16).  Shapes
- * that are not compiled in are rejected. */
+ * library).  kernel MONC_TVD_KERNEL_MARCH (default, columns of up to 256
+ * levels): each face computed once, rows = cell rows per block (0: the
+ * default for the column height; otherwise one of the shapes listed by
+ * monc_tvd_march_shapes), strip = x columns per block (0: 16).
+ * MONC_TVD_KERNEL_CELL: one thread per cell, every face recomputed (any
+ * column height; the fallback beyond 256 levels or 4 GiB per field).
+ * MONC_TVD_KERNEL_SPLIT: one wave per 64 levels of a row, both y faces of a
+ * row computed by it (the earlier default, kept for comparison). */
+#define MONC_TVD_KERNEL_MARCH 0
+#define MONC_TVD_KERNEL_CELL 1
+#define MONC_TVD_KERNEL_SPLIT 2
 int monc_tvd_set_tiling(monc_tvd_ctx *ctx, int kernel, int rows, int strip);
+/* The march shapes (cell rows per block) compiled for this context's column
+ * height, default first: writes up to n of them to rows and returns how many
+ * exist (0 when the march kernel cannot run this context), or a negative
+ * status. */
+int monc_tvd_march_shapes(monc_tvd_ctx *ctx, int *rows, int n);
+
+/* Tendency diagnostics of the component (tvdadvection.F90:1064-1188, the
+ * published fields tend_*_tvdadvection_3d_local and _profile_total_local of
+ * :71-103) and the vertical face values handed to profile_diagnostics
+ * (tvd_dgs_terms%adv_*_dgs, :615-644, 666-671, 712-717).  flags:
+ *   MONC_TVD_DIAG_TENDENCY  every monc_tvd_tendencies call records, for each
+ *       advected field, tend_3d = s_after - s_before over the interior
+ *       columns (save_precomponent_tendencies / compute_component_tendencies)
+ *       and its per-level total over the interior columns; with th advected
+ *       and rprefrcp given (length nz, the reference's
+ *       vertical%rprefrcp), also tabs = s_after*rprefrcp - s_before*rprefrcp;
+ *   MONC_TVD_DIAG_ZFACE     every call records each advected field's flux_z
+ *       per interior column: flux_z(k), the face between levels k-1 and k
+ *       (0 at k = 1), in the field layout.
+ * The march and cell kernels record z faces; the split kernel does not
+ * (a call that needs it fails).  flags = 0 turns recording off. */
+#define MONC_TVD_DIAG_TENDENCY 1
+#define MONC_TVD_DIAG_ZFACE 2
+int monc_tvd_set_diagnostics(monc_tvd_ctx *ctx, int flags, const double *rprefrcp);
+
+/* Read a recorded diagnostic into host memory (FP64):
+ *   MONC_TVD_DIAG_3D          tend_3d of `field`, shape (nz, ny, nx), no halo;
+ *   MONC_TVD_DIAG_PROFILE     its per-level total, length nz;
+ *   MONC_TVD_DIAG_TABS_3D     tabs tendency (field ignored), (nz, ny, nx);
+ *   MONC_TVD_DIAG_TABS_PROFILE its total, length nz;
+ *   MONC_TVD_DIAG_ZFACES      flux_z of `field`, shape (nz, ny+4, nx+4).
+ * Fails with MONC_TVD_ERR_STATE when that diagnostic was never recorded. */
+#define MONC_TVD_DIAG_3D 0
+#define MONC_TVD_DIAG_PROFILE 1
+#define MONC_TVD_DIAG_TABS_3D 2
+#define MONC_TVD_DIAG_TABS_PROFILE 3
+#define MONC_TVD_DIAG_ZFACES 4
+int monc_tvd_get_diagnostic(monc_tvd_ctx *ctx, int field, int which, double *host);
 
 /* Per-kernel timing (no counterpart in the reference: measurement support).
  * With profiling on, every launch is bracketed by HIP events on the context

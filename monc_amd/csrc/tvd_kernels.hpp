@@ -20,6 +20,7 @@ struct FieldSet {
   const T* f[MAX_FIELDS];  // current prognostic (MODE_STEP update base)
   T* out[MAX_FIELDS];      // MODE_ACCUM: source field; MODE_STEP: next state
   T dt[MAX_FIELDS];        // ultflx dt of the field
+  T* zf[MAX_FIELDS];       // MODE_ACCUM: flux_z copy of the field (tvd_dgs_terms), or null
 };
 
 template <typename T>
@@ -131,6 +132,15 @@ __device__ __forceinline__ void cell_group(const LaunchArgs<T>& L, const FieldSe
     if (!top) face_z(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, kmin, dt, L.cx, L.cy, nz, fzt);
   }
   const int64_t p = at(L.d, k, j, i);
+  if (MODE == MODE_ACCUM && S.zf[n0]) {  // flux_z(k): the face below level k
+    if (cell) {
+      S.zf[n0][p] = fzb[0];
+    } else if (KIND == KIND_W && k == nz) {
+      T fz[NS];
+      face_z(k - 1, j, i, F, fl.U, fl.V, fl.W, m, kdof, kmin, dt, L.cx, L.cy, nz, fz);
+      S.zf[n0][p] = fz[0];
+    }
+  }
   if (cell) {  // the weights read flows at k-1: only for cells (k >= 2)
     const DivW<T> dw(fl, m, k, j, i, nz);
 #pragma unroll
@@ -164,187 +174,6 @@ __global__ void __launch_bounds__(256) cell_kernel(LaunchArgs<T> L, FieldSet<T> 
       case KIND_V: cell_group<T, KIND_V, 1, MODE>(L, S, n, k, j, i); break;
       case KIND_W: cell_group<T, KIND_W, 1, MODE>(L, S, n, k, j, i); break;
       default: cell_group<T, KIND_SCALAR, 1, MODE>(L, S, n, k, j, i); break;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Slab kernel: every face computed once.
-//
-// A block owns TY cell rows j0..j0+TY-1 over all levels (NCH chunks of 64
-// levels, lanes along z) and marches along an x strip.  Each thread (level k,
-// row j) computes per step i only the faces its cell owns: the right x face
-// (kept in a register as the next step's left face), the north y face and the
-// top z face.  The south y face comes from the row below through LDS (an extra
-// face-only wave row j0-1 supplies it for the first row), the bottom z face
-// from the lane below by a wave shuffle (across chunks through LDS).  Scalars
-// advected by the same flows are processed in pairs, sharing every
-// velocity-only term.
-constexpr int SLAB_NS = 2;
-
-template <typename T, int NCH, int TY>
-struct SlabLds {
-  T yn[2][TY + 1][SLAB_NS][64 * NCH];  // north y faces, double-buffered by step parity
-  T zt[2][TY + 1][SLAB_NS][NCH];       // top z face of the last level of each chunk
-};
-
-template <typename T, int KIND, int NS, int MODE, int NCH, int TY>
-__device__ __forceinline__ void slab_group(const LaunchArgs<T>& L, const FieldSet<T>& S, int n0, int i_begin,
-                                           int i_end, SlabLds<T, NCH, TY>& lds) {
-  const Flows<T, KIND> fl(L);
-  const VMet m = KIND == KIND_W ? L.mw : L.mc;
-  const int nz = L.d.nz, ny = L.d.ny;
-  const int kdof = KIND == KIND_W ? 1 : 0;
-  const int kmin = KIND == KIND_W ? 1 : 2;
-  const int kmax = KIND == KIND_W ? nz - 1 : nz;  // last level with x/y faces and a cell
-  const T dt = S.dt[n0];
-  Adv<T, NS> F{};
-#pragma unroll
-  for (int n = 0; n < NS; n++) F.a[n] = Arr<T>(S.F[n0 + n], L.d);
-  // a wave is 64 levels of one (chunk, row): c and r are wave-uniform
-  const int lane = threadIdx.x;
-  const int c = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int r = __builtin_amdgcn_readfirstlane(threadIdx.z);
-  const int k = 1 + 64 * c + lane;
-  const int j = 3 + (int)blockIdx.x * TY - 1 + r;
-  const bool row_ok = j <= ny + 2;
-  const bool cell_row = r > 0 && row_ok;
-  const bool hface = row_ok && k >= 2 && k <= kmax;  // level has x/y faces
-  const bool cell = cell_row && k >= 2 && k <= kmax;
-  const bool zface = cell_row && k <= nz - 1;
-  const bool top = KIND != KIND_W && k == nz;
-  T fxl[NS], fxr[NS], fyn[NS], fzt[NS];
-#pragma unroll
-  for (int n = 0; n < NS; n++) fxl[n] = T(0);
-  if (cell_row && hface) face_x(k, j, i_begin - 1, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fxl);
-  for (int i = i_begin; i < i_end; i++) {
-    const int b = i & 1;
-#pragma unroll
-    for (int n = 0; n < NS; n++) fxr[n] = fyn[n] = fzt[n] = T(0);
-    if (hface) face_y(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fyn);
-    if (cell_row && hface) face_x(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fxr);
-    if (zface) face_z(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, kmin, dt, L.cx, L.cy, nz, fzt);
-#pragma unroll
-    for (int n = 0; n < NS; n++) {
-      lds.yn[b][r][n][64 * c + lane] = fyn[n];
-      if (lane == 63) lds.zt[b][r][n][c] = fzt[n];
-    }
-    __syncthreads();
-    T fzb[NS], fys[NS];
-#pragma unroll
-    for (int n = 0; n < NS; n++) {
-      fzb[n] = __shfl_up(fzt[n], 1);
-      if (lane == 0) fzb[n] = c > 0 ? lds.zt[b][r][n][c - 1] : T(0);
-      fys[n] = r > 0 ? lds.yn[b][r - 1][n][64 * c + lane] : T(0);
-    }
-    if (cell) {  // the weights read flows at k-1: only for cells (k >= 2)
-      const DivW<T> dw(fl, m, k, j, i, nz);
-      const int64_t p = at(L.d, k, j, i);
-#pragma unroll
-      for (int n = 0; n < NS; n++) {
-        if (MODE == MODE_ACCUM)
-          S.out[n0 + n][p] = dw.apply(top, S.out[n0 + n][p], L.cx, L.cy, fxl[n], fxr[n], fys[n], fyn[n], fzb[n],
-                                      fzt[n]);
-        else
-          store_step(L.d, S.out[n0 + n], S.f[n0 + n], L.dtm,
-                     dw.apply(top, T(0), L.cx, L.cy, fxl[n], fxr[n], fys[n], fyn[n], fzb[n], fzt[n]), k, j, i);
-      }
-    } else if (MODE == MODE_STEP && cell_row && k <= nz) {
-#pragma unroll
-      for (int n = 0; n < NS; n++) store_step(L.d, S.out[n0 + n], S.f[n0 + n], L.dtm, T(0), k, j, i);
-    }
-#pragma unroll
-    for (int n = 0; n < NS; n++) fxl[n] = fxr[n];
-  }
-  __syncthreads();  // the next group reuses the LDS buffers
-}
-
-// One launch advects one field kind: a momentum field (NS = 1) or a group of
-// NS scalars; S.F[0..NS) etc. hold that group.  Separate launches keep each
-// kernel's register demand to one kind's working set.
-template <typename T, int KIND, int NS, int MODE, int NCH, int TY>
-__global__ void __launch_bounds__(64 * NCH * (TY + 1)) slab_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip) {
-  __shared__ SlabLds<T, NCH, TY> lds;
-  const int i_begin = 3 + (int)blockIdx.y * strip;
-  const int i_end = min(i_begin + strip, L.d.nx + 3);
-  slab_group<T, KIND, NS, MODE, NCH, TY>(L, S, 0, i_begin, i_end, lds);
-}
-
-// ---------------------------------------------------------------------------
-// Wave kernel: one wave owns one row j over all levels (NSUB sub-levels of 64
-// per lane) and marches along an x strip; waves never synchronise.  The x face
-// is carried between steps (computed once), the z face reaches the level above
-// by a wave shuffle (and from the last lane of one sub-level to the first lane
-// of the next by readlane), and each row computes both of its y faces.
-template <typename T, int KIND, int NS, int MODE, int NSUB>
-__global__ void __launch_bounds__(256) wave_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip) {
-  const Flows<T, KIND> fl(L);
-  const VMet m = KIND == KIND_W ? L.mw : L.mc;
-  const int nz = L.d.nz, ny = L.d.ny;
-  const int kdof = KIND == KIND_W ? 1 : 0;
-  const int kmin = KIND == KIND_W ? 1 : 2;
-  const int kmax = KIND == KIND_W ? nz - 1 : nz;
-  const T dt = S.dt[0];
-  Adv<T, NS> F{};
-#pragma unroll
-  for (int n = 0; n < NS; n++) F.a[n] = Arr<T>(S.F[n], L.d);
-  const int lane = threadIdx.x;
-  const int j = 3 + (int)blockIdx.x * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
-  if (j > ny + 2) return;  // whole wave
-  const int i_begin = 3 + (int)blockIdx.y * strip;
-  const int i_end = min(i_begin + strip, L.d.nx + 3);
-  T fxl[NSUB][NS];
-#pragma unroll
-  for (int q = 0; q < NSUB; q++) {
-    const int k = 1 + 64 * q + lane;
-#pragma unroll
-    for (int n = 0; n < NS; n++) fxl[q][n] = T(0);
-    if (k >= 2 && k <= kmax) face_x(k, j, i_begin - 1, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fxl[q]);
-  }
-  for (int i = i_begin; i < i_end; i++) {
-    T prev_top[NS];  // top z face of the last level of the previous sub-level
-#pragma unroll
-    for (int n = 0; n < NS; n++) prev_top[n] = T(0);
-#pragma unroll
-    for (int q = 0; q < NSUB; q++) {
-      const int k = 1 + 64 * q + lane;
-      const bool cell = k >= 2 && k <= kmax;
-      const bool top = KIND != KIND_W && k == nz;
-      T fxr[NS], fyn[NS], fys[NS], fzt[NS], fzb[NS];
-#pragma unroll
-      for (int n = 0; n < NS; n++) fxr[n] = fyn[n] = fys[n] = fzt[n] = T(0);
-      if (cell) {
-        face_y(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fyn);
-        face_y(k, j - 1, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fys);
-        face_x(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fxr);
-      }
-      if (k <= nz - 1) face_z(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, kmin, dt, L.cx, L.cy, nz, fzt);
-#pragma unroll
-      for (int n = 0; n < NS; n++) {
-        fzb[n] = __shfl_up(fzt[n], 1);
-        if (lane == 0) fzb[n] = prev_top[n];
-        prev_top[n] = __shfl(fzt[n], 63);
-      }
-      if (cell) {
-        const DivW<T> dw(fl, m, k, j, i, nz);
-        const int64_t p = at(L.d, k, j, i);
-#pragma unroll
-        for (int n = 0; n < NS; n++) {
-          if (MODE == MODE_ACCUM)
-            S.out[n][p] = dw.apply(top, S.out[n][p], L.cx, L.cy, fxl[q][n], fxr[n], fys[n], fyn[n], fzb[n], fzt[n]);
-          else
-            store_step(L.d, S.out[n], S.f[n], L.dtm,
-                       dw.apply(top, T(0), L.cx, L.cy, fxl[q][n], fxr[n], fys[n], fyn[n], fzb[n], fzt[n]), k, j, i);
-        }
-      } else if (MODE == MODE_STEP && k <= nz) {
-#pragma unroll
-        for (int n = 0; n < NS; n++) store_step(L.d, S.out[n], S.f[n], L.dtm, T(0), k, j, i);
-      }
-#pragma unroll
-      for (int n = 0; n < NS; n++) fxl[q][n] = fxr[n];
-      // sub-levels are independent work; keeping the scheduler from
-      // interleaving them bounds the registers to one sub-level's set
-      __builtin_amdgcn_sched_barrier(0);
     }
   }
 }

@@ -15,6 +15,7 @@
 #include "tvd_adjoint.hpp"
 #include "tvd_kernels.hpp"
 #include "tvd_march.hpp"
+#include "tvd_diag.hpp"
 
 using namespace tvd;
 
@@ -57,9 +58,9 @@ struct monc_tvd_ctx {
   std::vector<void*> adj_tmp;  // copy of the incoming adjoint of each field (adjoint step)
   std::vector<void*> dual[2];  // packed (value, tangent) state, double-buffered (tangent steps)
   hipStream_t stream;
-  int use_cell_kernel;         // force the one-cell-per-thread kernel
-  int slab_ty, slab_strip;     // slab tile shape (0: defaults)
-  int slab_ok;                 // fields small enough for 32-bit buffer offsets
+  int kernel;                  // MONC_TVD_KERNEL_MARCH, _CELL or _SPLIT
+  int rows, strip;             // march tile shape: cell rows per block, x columns per block (0: defaults)
+  int buffer_ok;               // fields small enough for 32-bit buffer offsets
   // optional per-kernel timing: HIP events recorded around every launch on
   // the context stream, folded into per-kind totals on query
   int profiling;
@@ -67,6 +68,17 @@ struct monc_tvd_ctx {
   std::vector<Timed> pending;
   double prof_ms[MONC_TVD_NKERNEL_KINDS];
   long long prof_n[MONC_TVD_NKERNEL_KINDS];
+  // tendency diagnostics (per field): s before the call, tend_3d (nz,ny,nx),
+  // per-level totals (FP64), flux_z copies; tabs from th and rprefrcp
+  int diag_flags;
+  double* rprefrcp;            // device, nz (or null)
+  std::vector<void*> diag_before, diag_3d, diag_zf;
+  std::vector<double*> diag_prof;
+  std::vector<char> diag_have;  // tend_3d/profile recorded
+  void* tabs_3d;
+  double* tabs_prof;
+  int tabs_have;
+  double* diag_partial;        // ny*nz FP64 scratch of the level totals
 };
 
 namespace {
@@ -159,38 +171,8 @@ int kind_of(int field) {
   return field == MONC_TVD_U ? KIND_U : field == MONC_TVD_V ? KIND_V : field == MONC_TVD_W ? KIND_W : KIND_SCALAR;
 }
 
-// Slab tile shapes compiled in: cell rows per block (TY) for each number of
-// 64-level chunks; the strip length along x is a launch argument.
-template <typename T, int KIND, int NS, int MODE, int NCH, int TY>
-void launch_slab(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int strip) {
-  dim3 block(64, NCH, TY + 1);
-  dim3 grid((c->ny + TY - 1) / TY, (c->nx + strip - 1) / strip, 1);
-  hipLaunchKernelGGL((slab_kernel<T, KIND, NS, MODE, NCH, TY>), grid, block, 0, c->stream, L, S, strip);
-}
-
-// Default x-marching kernel: the march kernel (every face computed once;
-// 1.45x the split wave kernel at 512x512x256 in FP32 and FP64 on MI355X).
-int default_ty(int) { return 98; }
-
-bool ty_compiled(int nch, int ty) {
-  switch (nch) {
-    case 1: return ty == 3 || ty == 7;
-    case 2: return ty == 1 || ty == 3;
-    case 3: return ty == 1;
-    case 4: return ty == 1;
-    default: return false;
-  }
-}
-
-template <typename T, int KIND, int NS, int MODE, int NSUB>
-void launch_wave(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int strip) {
-  dim3 block(64, 4, 1);
-  dim3 grid((c->ny + 3) / 4, (c->nx + strip - 1) / strip, 1);
-  hipLaunchKernelGGL((wave_kernel<T, KIND, NS, MODE, NSUB>), grid, block, 0, c->stream, L, S, strip);
-}
-
-// Split wave kernel: NCH waves per row, R rows per block (256 threads, or 192
-// for three sub-levels).
+// Split wave kernel (kernel 2): NCH waves per row, R rows per block (256
+// threads, or 192 for three sub-levels).
 template <typename T, int KIND, int NS, int MODE, int NCH>
 void launch_split(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int strip) {
   constexpr int R = NCH == 1 ? 4 : NCH == 2 ? 2 : 1;
@@ -199,45 +181,59 @@ void launch_split(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
   hipLaunchKernelGGL((split_kernel<T, KIND, NS, MODE, NCH, R>), grid, block, 0, c->stream, L, S, strip);
 }
 
-// March kernel: NCH waves per row, R rows per block; a 1-D grid of
-// tiles_y * tiles_x tiles (the kernel maps blocks to tiles XCD by XCD).
-template <int NCH, int ES>
-constexpr int march_rows() {
-  return NCH == 1 ? 8 : NCH == 2 ? 4 : NCH == 3 ? 4 : (ES == 8 ? 3 : 4);
+// March kernel (kernel 0): NCH waves per row, R cell rows per block, on a 1-D
+// grid of tiles_y * tiles_x tiles (the kernel deals tiles to XCDs).  WPE, the
+// waves per SIMD the compiler keeps registers for, lets two or more blocks
+// share a CU, so that one block's loads overlap another's arithmetic across
+// the per-column barrier.
+template <int ES, int NCH, int R>
+constexpr int march_wpe() {
+  return ES == 4 ? (NCH * R >= 16 ? 8 : 6) : ((16 % (NCH * R)) == 0 ? 4 : 3);
 }
 
-template <typename T, int KIND, int NS, int MODE, int NCH>
+template <typename T, int KIND, int NS, int MODE, int NCH, int R>
 void launch_march(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int strip) {
-  constexpr int R = march_rows<NCH, (int)sizeof(T)>();
+  constexpr int WPE = march_wpe<(int)sizeof(T), NCH, R>();
   const int tiles_y = (c->ny + R - 1) / R, tiles_x = (c->nx + strip - 1) / strip;
   dim3 block(64, NCH * R, 1);
-  hipLaunchKernelGGL((march_kernel<T, KIND, NS, MODE, NCH, R>), dim3(tiles_y * tiles_x), block, 0, c->stream, L, S,
-                     strip, tiles_y);
+  hipLaunchKernelGGL((march_kernel<T, KIND, NS, MODE, NCH, R, WPE>), dim3(tiles_y * tiles_x), block, 0, c->stream,
+                     L, S, strip, tiles_y);
 }
 
-// key: nch*100 + rows for the slab kernel; nch*100 + 0 for the wave kernel;
-// nch*100 + 99 for the split wave kernel; nch*100 + 98 for the march kernel
+// Cell rows per block of the march kernel compiled for each column height
+// (NCH sub-columns of 64 levels); the first entry is the default.
+constexpr int MARCH_ROWS[4][4] = {{8, 4, 0, 0}, {4, 2, 0, 0}, {4, 2, 0, 0}, {4, 2, 3, 1}};
+
+bool march_rows_ok(int nch, int rows) {
+  if (nch < 1 || nch > 4) return false;
+  for (int r : MARCH_ROWS[nch - 1])
+    if (r && r == rows) return true;
+  return false;
+}
+
 template <typename T, int KIND, int NS, int MODE>
-void launch_slab_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int key, int strip) {
-  switch (key) {
-    case 100: launch_wave<T, KIND, NS, MODE, 1>(c, L, S, strip); break;
-    case 200: launch_wave<T, KIND, NS, MODE, 2>(c, L, S, strip); break;
-    case 300: launch_wave<T, KIND, NS, MODE, 3>(c, L, S, strip); break;
-    case 400: launch_wave<T, KIND, NS, MODE, 4>(c, L, S, strip); break;
-    case 199: launch_split<T, KIND, NS, MODE, 1>(c, L, S, strip); break;
-    case 299: launch_split<T, KIND, NS, MODE, 2>(c, L, S, strip); break;
-    case 399: launch_split<T, KIND, NS, MODE, 3>(c, L, S, strip); break;
-    case 499: launch_split<T, KIND, NS, MODE, 4>(c, L, S, strip); break;
-    case 198: launch_march<T, KIND, NS, MODE, 1>(c, L, S, strip); break;
-    case 298: launch_march<T, KIND, NS, MODE, 2>(c, L, S, strip); break;
-    case 398: launch_march<T, KIND, NS, MODE, 3>(c, L, S, strip); break;
-    case 498: launch_march<T, KIND, NS, MODE, 4>(c, L, S, strip); break;
-    case 103: launch_slab<T, KIND, NS, MODE, 1, 3>(c, L, S, strip); break;
-    case 107: launch_slab<T, KIND, NS, MODE, 1, 7>(c, L, S, strip); break;
-    case 201: launch_slab<T, KIND, NS, MODE, 2, 1>(c, L, S, strip); break;
-    case 203: launch_slab<T, KIND, NS, MODE, 2, 3>(c, L, S, strip); break;
-    case 301: launch_slab<T, KIND, NS, MODE, 3, 1>(c, L, S, strip); break;
-    case 401: launch_slab<T, KIND, NS, MODE, 4, 1>(c, L, S, strip); break;
+void launch_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int nch, int kernel, int rows,
+                  int strip) {
+  if (kernel == MONC_TVD_KERNEL_SPLIT) {
+    switch (nch) {
+      case 1: launch_split<T, KIND, NS, MODE, 1>(c, L, S, strip); break;
+      case 2: launch_split<T, KIND, NS, MODE, 2>(c, L, S, strip); break;
+      case 3: launch_split<T, KIND, NS, MODE, 3>(c, L, S, strip); break;
+      default: launch_split<T, KIND, NS, MODE, 4>(c, L, S, strip); break;
+    }
+    return;
+  }
+  switch (nch * 100 + rows) {
+    case 108: launch_march<T, KIND, NS, MODE, 1, 8>(c, L, S, strip); break;
+    case 104: launch_march<T, KIND, NS, MODE, 1, 4>(c, L, S, strip); break;
+    case 204: launch_march<T, KIND, NS, MODE, 2, 4>(c, L, S, strip); break;
+    case 202: launch_march<T, KIND, NS, MODE, 2, 2>(c, L, S, strip); break;
+    case 304: launch_march<T, KIND, NS, MODE, 3, 4>(c, L, S, strip); break;
+    case 302: launch_march<T, KIND, NS, MODE, 3, 2>(c, L, S, strip); break;
+    case 404: launch_march<T, KIND, NS, MODE, 4, 4>(c, L, S, strip); break;
+    case 402: launch_march<T, KIND, NS, MODE, 4, 2>(c, L, S, strip); break;
+    case 403: launch_march<T, KIND, NS, MODE, 4, 3>(c, L, S, strip); break;
+    case 401: launch_march<T, KIND, NS, MODE, 4, 1>(c, L, S, strip); break;
   }
 }
 
@@ -247,9 +243,8 @@ void launch_slab_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T
 template <typename T, int MODE>
 int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
   const int nch = (c->nz + 63) / 64;
-  const int ty = c->slab_ty == -3 ? 98 : c->slab_ty == -2 ? 99 : c->slab_ty >= 0 ? c->slab_ty : default_ty(nch);
-  const int strip = c->slab_strip > 0 ? c->slab_strip : 16;
-  if (c->use_cell_kernel || nch > 4 || !c->slab_ok) {
+  const int strip = c->strip > 0 ? c->strip : 16;
+  if (c->kernel == MONC_TVD_KERNEL_CELL || nch > 4 || !c->buffer_ok) {
     const int nk = MODE == MODE_STEP ? c->nz : c->nz - 1;
     dim3 block(64, 4, 1);
     dim3 grid((nk + 63) / 64, (c->ny + 3) / 4, c->nx);
@@ -258,9 +253,9 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
     HIP_TRY(hipGetLastError());
     return MONC_TVD_OK;
   }
-  const int key = nch * 100 + ty;
+  const int rows = c->rows > 0 ? c->rows : MARCH_ROWS[nch - 1][0];
   for (int n = 0; n < S.nf;) {
-    FieldSet<T> G;
+    FieldSet<T> G{};
     const int pair = S.kind[n] == KIND_SCALAR && n + 1 < S.nf && S.kind[n + 1] == KIND_SCALAR && S.dt[n + 1] == S.dt[n];
     G.nf = pair ? 2 : 1;
     for (int m = 0; m < G.nf; m++) {
@@ -269,15 +264,18 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
       G.f[m] = S.f[n + m];
       G.out[m] = S.out[n + m];
       G.dt[m] = S.dt[n + m];
+      G.zf[m] = S.zf[n + m];
     }
+    if (c->kernel == MONC_TVD_KERNEL_SPLIT && G.zf[0])
+      return fail(MONC_TVD_ERR_ARG, "z-face diagnostics need the march or cell kernel, not the split kernel");
     LaunchTimer timer(c, pair || S.kind[n] == KIND_SCALAR ? 3 : S.kind[n]);
     switch (S.kind[n]) {
-      case KIND_U: launch_slab_shape<T, KIND_U, 1, MODE>(c, L, G, key, strip); break;
-      case KIND_V: launch_slab_shape<T, KIND_V, 1, MODE>(c, L, G, key, strip); break;
-      case KIND_W: launch_slab_shape<T, KIND_W, 1, MODE>(c, L, G, key, strip); break;
+      case KIND_U: launch_shape<T, KIND_U, 1, MODE>(c, L, G, nch, c->kernel, rows, strip); break;
+      case KIND_V: launch_shape<T, KIND_V, 1, MODE>(c, L, G, nch, c->kernel, rows, strip); break;
+      case KIND_W: launch_shape<T, KIND_W, 1, MODE>(c, L, G, nch, c->kernel, rows, strip); break;
       default:
-        if (pair) launch_slab_shape<T, KIND_SCALAR, 2, MODE>(c, L, G, key, strip);
-        else launch_slab_shape<T, KIND_SCALAR, 1, MODE>(c, L, G, key, strip);
+        if (pair) launch_shape<T, KIND_SCALAR, 2, MODE>(c, L, G, nch, c->kernel, rows, strip);
+        else launch_shape<T, KIND_SCALAR, 1, MODE>(c, L, G, nch, c->kernel, rows, strip);
     }
     HIP_TRY(hipGetLastError());
     n += G.nf;
@@ -318,6 +316,66 @@ int need(monc_tvd_ctx* c, int field, int variant) {
   return MONC_TVD_OK;
 }
 
+// Diagnostics around one tendency call: snapshot the s-fields before it
+// (save_precomponent_tendencies), allocate the z-face buffers.
+int diag_before_call(monc_tvd_ctx* c, const std::vector<int>& fields) {
+  if (!c->diag_flags) return MONC_TVD_OK;
+  const size_t bytes = c->n_elem * c->elem_size;
+  for (int fl : fields) {
+    if (c->diag_flags & MONC_TVD_DIAG_TENDENCY) {
+      if (!c->diag_before[fl]) HIP_TRY(hipMalloc(&c->diag_before[fl], bytes));
+      HIP_TRY(hipMemcpyAsync(c->diag_before[fl], slot(c, fl, MONC_TVD_SOURCE), bytes, hipMemcpyDeviceToDevice,
+                             c->stream));
+    }
+    if ((c->diag_flags & MONC_TVD_DIAG_ZFACE) && !c->diag_zf[fl]) {
+      HIP_TRY(hipMalloc(&c->diag_zf[fl], bytes));
+      HIP_TRY(hipMemsetAsync(c->diag_zf[fl], 0, bytes, c->stream));
+    }
+  }
+  return MONC_TVD_OK;
+}
+
+// tend_3d = after - before and its per-level totals (compute_component_tendencies).
+template <typename T>
+int tend_and_profile(monc_tvd_ctx* c, const T* after, const T* before, const double* r, T* t3d, double* prof) {
+  const Dims d = dims_of(c);
+  hipLaunchKernelGGL((tend3d_kernel<T>), dim3((c->nz + 63) / 64, (c->ny + 3) / 4, c->nx), dim3(64, 4, 1), 0,
+                     c->stream, after, before, r, t3d, d);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL((profile_rows_kernel<T>), dim3((c->nz + 63) / 64, c->ny, 1), dim3(64, 1, 1), 0, c->stream,
+                     (const T*)t3d, c->nz, c->ny, c->nx, c->diag_partial);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(profile_sum_kernel, dim3((c->nz + 63) / 64, 1, 1), dim3(64, 1, 1), 0, c->stream,
+                     (const double*)c->diag_partial, c->nz, c->ny, prof);
+  HIP_TRY(hipGetLastError());
+  return MONC_TVD_OK;
+}
+
+template <typename T>
+int diag_after_call(monc_tvd_ctx* c, const std::vector<int>& fields) {
+  if (!(c->diag_flags & MONC_TVD_DIAG_TENDENCY)) return MONC_TVD_OK;
+  const size_t n3d = (size_t)c->nz * c->ny * c->nx;
+  if (!c->diag_partial) HIP_TRY(hipMalloc(&c->diag_partial, sizeof(double) * (size_t)c->nz * c->ny));
+  int rc;
+  for (int fl : fields) {
+    if (!c->diag_3d[fl]) HIP_TRY(hipMalloc(&c->diag_3d[fl], n3d * sizeof(T)));
+    if (!c->diag_prof[fl]) HIP_TRY(hipMalloc(&c->diag_prof[fl], sizeof(double) * c->nz));
+    if ((rc = tend_and_profile<T>(c, (const T*)slot(c, fl, MONC_TVD_SOURCE), (const T*)c->diag_before[fl], nullptr,
+                                  (T*)c->diag_3d[fl], c->diag_prof[fl])))
+      return rc;
+    c->diag_have[fl] = 1;
+    if (fl == MONC_TVD_TH && c->rprefrcp) {
+      if (!c->tabs_3d) HIP_TRY(hipMalloc(&c->tabs_3d, n3d * sizeof(T)));
+      if (!c->tabs_prof) HIP_TRY(hipMalloc(&c->tabs_prof, sizeof(double) * c->nz));
+      if ((rc = tend_and_profile<T>(c, (const T*)slot(c, fl, MONC_TVD_SOURCE), (const T*)c->diag_before[fl],
+                                    c->rprefrcp, (T*)c->tabs_3d, c->tabs_prof)))
+        return rc;
+      c->tabs_have = 1;
+    }
+  }
+  return MONC_TVD_OK;
+}
+
 template <typename T>
 int tendencies_impl(monc_tvd_ctx* c, double dtm, int field_stepping, int momentum_stepping, int scalar_stepping) {
   int rc;
@@ -330,9 +388,10 @@ int tendencies_impl(monc_tvd_ctx* c, double dtm, int field_stepping, int momentu
   for (int fl : fields) {
     if ((rc = need(c, fl, adv_var)) || (rc = need(c, fl, MONC_TVD_SOURCE))) return rc;
   }
+  if ((rc = diag_before_call(c, fields))) return rc;
   const LaunchArgs<T> L = launch_args<T>(c, dtm);
   for (size_t first = 0; first < fields.size(); first += MAX_FIELDS) {
-    FieldSet<T> S;
+    FieldSet<T> S{};
     S.nf = (int)std::min(fields.size() - first, (size_t)MAX_FIELDS);
     for (int n = 0; n < S.nf; n++) {
       const int fl = fields[first + n];
@@ -341,10 +400,11 @@ int tendencies_impl(monc_tvd_ctx* c, double dtm, int field_stepping, int momentu
       S.f[n] = (const T*)slot(c, fl, MONC_TVD_PROG);
       S.out[n] = (T*)slot(c, fl, MONC_TVD_SOURCE);
       S.dt[n] = (T)(fl <= MONC_TVD_W ? dt_mom : dt_sca);
+      S.zf[n] = (c->diag_flags & MONC_TVD_DIAG_ZFACE) ? (T*)c->diag_zf[fl] : nullptr;
     }
     if ((rc = launch_set<T, MODE_ACCUM>(c, L, S))) return rc;
   }
-  return MONC_TVD_OK;
+  return diag_after_call<T>(c, fields);
 }
 
 template <typename T>
@@ -362,7 +422,7 @@ int forward_steps_impl(monc_tvd_ctx* c, double dtm, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     const LaunchArgs<T> L = launch_args<T>(c, dtm);
     for (size_t first = 0; first < fields.size(); first += MAX_FIELDS) {
-      FieldSet<T> S;
+      FieldSet<T> S{};
       S.nf = (int)std::min(fields.size() - first, (size_t)MAX_FIELDS);
       for (int n = 0; n < S.nf; n++) {
         const int fl = fields[first + n];
@@ -480,7 +540,7 @@ int tangent_steps_impl(monc_tvd_ctx* c, double dtm, int nsteps) {
     L.cx = Dual(c->cx);
     L.cy = Dual(c->cy);
     for (size_t first = 0; first < fields.size(); first += MAX_FIELDS) {
-      FieldSet<Dual> S;
+      FieldSet<Dual> S{};
       S.nf = (int)std::min(fields.size() - first, (size_t)MAX_FIELDS);
       for (int m = 0; m < S.nf; m++) {
         const int fl = fields[first + m];
@@ -552,11 +612,22 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags,
   c->dual[1].assign((size_t)c->nfields, nullptr);
   c->stream = nullptr;
   c->dmet = nullptr;
-  c->use_cell_kernel = 0;
-  c->slab_ty = -1;
-  c->slab_strip = 0;
-  c->slab_ok = n_elem * c->elem_size < ((size_t)1 << 32) - ((size_t)1 << 20);
+  c->kernel = MONC_TVD_KERNEL_MARCH;
+  c->rows = 0;
+  c->strip = 0;
+  c->buffer_ok = n_elem * c->elem_size < ((size_t)1 << 32) - ((size_t)1 << 20);
   c->profiling = 0;
+  c->diag_flags = 0;
+  c->rprefrcp = nullptr;
+  c->diag_before.assign((size_t)c->nfields, nullptr);
+  c->diag_3d.assign((size_t)c->nfields, nullptr);
+  c->diag_zf.assign((size_t)c->nfields, nullptr);
+  c->diag_prof.assign((size_t)c->nfields, nullptr);
+  c->diag_have.assign((size_t)c->nfields, 0);
+  c->tabs_3d = nullptr;
+  c->tabs_prof = nullptr;
+  c->tabs_have = 0;
+  c->diag_partial = nullptr;
   for (int k = 0; k < MONC_TVD_NKERNEL_KINDS; k++) {
     c->prof_ms[k] = 0.0;
     c->prof_n[k] = 0;
@@ -599,6 +670,12 @@ int monc_tvd_destroy(monc_tvd_ctx* c) {
     for (int b = 0; b < 2; b++)
       if (c->dual[b][f]) (void)hipFree(c->dual[b][f]);
   }
+  for (int f = 0; f < c->nfields; f++) {
+    for (void* p : {c->diag_before[f], c->diag_3d[f], c->diag_zf[f], (void*)c->diag_prof[f]})
+      if (p) (void)hipFree(p);
+  }
+  for (void* p : {(void*)c->rprefrcp, c->tabs_3d, (void*)c->tabs_prof, (void*)c->diag_partial})
+    if (p) (void)hipFree(p);
   if (c->dmet) (void)hipFree(c->dmet);
   delete c;
   return MONC_TVD_OK;
@@ -606,15 +683,31 @@ int monc_tvd_destroy(monc_tvd_ctx* c) {
 
 int monc_tvd_set_tiling(monc_tvd_ctx* c, int kernel, int rows, int strip) {
   if (int rc = check_ctx(c)) return rc;
-  if (kernel != 0 && kernel != 1) return fail(MONC_TVD_ERR_ARG, "kernel %d is neither 0 (slab) nor 1 (cell)", kernel);
+  if (kernel != MONC_TVD_KERNEL_MARCH && kernel != MONC_TVD_KERNEL_CELL && kernel != MONC_TVD_KERNEL_SPLIT)
+    return fail(MONC_TVD_ERR_ARG, "kernel %d is not 0 (march), 1 (cell) or 2 (split)", kernel);
   if (strip < 0 || strip > 4096) return fail(MONC_TVD_ERR_ARG, "strip %d out of range [0,4096]", strip);
   const int nch = (c->nz + 63) / 64;
-  if (rows < -3 || (rows > 0 && !ty_compiled(nch, rows)))
-    return fail(MONC_TVD_ERR_ARG, "no slab shape with %d rows for %d-level columns", rows, c->nz);
-  c->use_cell_kernel = kernel;
-  c->slab_ty = rows;  // -1 default, 0 wave kernel, -2 split wave kernel, -3 march kernel, > 0 slab rows
-  c->slab_strip = strip;
+  if (kernel == MONC_TVD_KERNEL_MARCH && rows != 0 && !march_rows_ok(nch, rows))
+    return fail(MONC_TVD_ERR_ARG, "no march shape with %d rows for %d-level columns", rows, c->nz);
+  if (rows < 0) return fail(MONC_TVD_ERR_ARG, "rows %d < 0", rows);
+  c->kernel = kernel;
+  c->rows = kernel == MONC_TVD_KERNEL_MARCH ? rows : 0;
+  c->strip = strip;
   return MONC_TVD_OK;
+}
+
+int monc_tvd_march_shapes(monc_tvd_ctx* c, int* rows, int n) {
+  if (int rc = check_ctx(c)) return rc;
+  if (n < 0 || (n > 0 && !rows)) return fail(MONC_TVD_ERR_ARG, "null output array");
+  const int nch = (c->nz + 63) / 64;
+  int m = 0;
+  if (nch >= 1 && nch <= 4 && c->buffer_ok)
+    for (int r : MARCH_ROWS[nch - 1])
+      if (r) {
+        if (m < n) rows[m] = r;
+        m++;
+      }
+  return m;
 }
 
 int monc_tvd_weighted_sum(monc_tvd_ctx* c, int field, int variant, const double* weights, double* out) {
@@ -805,6 +898,71 @@ int monc_tvd_forward_steps(monc_tvd_ctx* c, double dtm, int nsteps) {
   if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
   return c->precision == MONC_TVD_FP64 ? forward_steps_impl<double>(c, dtm, nsteps)
                                        : forward_steps_impl<float>(c, dtm, nsteps);
+}
+
+int monc_tvd_set_diagnostics(monc_tvd_ctx* c, int flags, const double* rprefrcp) {
+  if (int rc = check_ctx(c)) return rc;
+  if (flags & ~(MONC_TVD_DIAG_TENDENCY | MONC_TVD_DIAG_ZFACE))
+    return fail(MONC_TVD_ERR_ARG, "unknown diagnostics flags 0x%x", flags);
+  if (rprefrcp) {
+    if (!c->rprefrcp) HIP_TRY(hipMalloc(&c->rprefrcp, sizeof(double) * c->nz));
+    HIP_TRY(hipMemcpyAsync(c->rprefrcp, rprefrcp, sizeof(double) * c->nz, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  c->diag_flags = flags;
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_get_diagnostic(monc_tvd_ctx* c, int field, int which, double* host) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!host) return fail(MONC_TVD_ERR_ARG, "null host pointer");
+  const bool tabs = which == MONC_TVD_DIAG_TABS_3D || which == MONC_TVD_DIAG_TABS_PROFILE;
+  if (!tabs && (field < 0 || field >= c->nfields))
+    return fail(MONC_TVD_ERR_ARG, "field id %d out of range [0,%d)", field, c->nfields);
+  const void* src = nullptr;
+  size_t n = 0;
+  bool fp64 = false;
+  switch (which) {
+    case MONC_TVD_DIAG_3D:
+      if (!c->diag_have[field]) break;
+      src = c->diag_3d[field];
+      n = (size_t)c->nz * c->ny * c->nx;
+      break;
+    case MONC_TVD_DIAG_PROFILE:
+      if (!c->diag_have[field]) break;
+      src = c->diag_prof[field];
+      n = c->nz;
+      fp64 = true;
+      break;
+    case MONC_TVD_DIAG_TABS_3D:
+      if (!c->tabs_have) break;
+      src = c->tabs_3d;
+      n = (size_t)c->nz * c->ny * c->nx;
+      break;
+    case MONC_TVD_DIAG_TABS_PROFILE:
+      if (!c->tabs_have) break;
+      src = c->tabs_prof;
+      n = c->nz;
+      fp64 = true;
+      break;
+    case MONC_TVD_DIAG_ZFACES:
+      src = c->diag_zf[field];
+      n = c->n_elem;
+      break;
+    default:
+      return fail(MONC_TVD_ERR_ARG, "unknown diagnostic %d", which);
+  }
+  if (!src) return fail(MONC_TVD_ERR_STATE, "diagnostic %d of field %d was never recorded", which, field);
+  if (fp64 || c->precision == MONC_TVD_FP64) {
+    HIP_TRY(hipMemcpyAsync(host, src, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(n);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), src, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < n; q++) host[q] = (double)tmp[q];
+  }
+  return MONC_TVD_OK;
 }
 
 }  // extern "C"

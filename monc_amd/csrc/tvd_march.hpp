@@ -152,14 +152,18 @@ __device__ __forceinline__ void mx_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
   const unsigned sj0 = g.so(j - 1, i), sj1 = g.so(j - 1, i + 1);
 #pragma unroll
   for (int n = 0; n < NS; n++) {
+    // every load is unconditional and only values are selected: a load whose
+    // scalar offset were chosen per lane would have to be serialised
     const T fa = ldb<T>(F[n], lv.k, s_0), fb = ldb<T>(F[n], lv.k, s_1);
-    const T fu = ap ? ldb<T>(F[n], lv.k, s_m) : ldb<T>(F[n], lv.k, s_2);
+    const T fm = ldb<T>(F[n], lv.k, s_m), f2 = ldb<T>(F[n], lv.k, s_2);
+    const T fu = ap ? fm : f2;
     const T fc = ap ? fa : fb;
     const T fd = ap ? fb : fa;
     const T d0 = ldb<T>(F[n], osl, s_0) - ldb<T>(F[n], osu, s_0);
     const T d1 = ldb<T>(F[n], osl, s_1) - ldb<T>(F[n], osu, s_1);
     const T g1 = cz * (ap ? d0 : d1);
-    const T t = ap ? ldb<T>(F[n], ojs, sj0) : ldb<T>(F[n], ojs, sj1);
+    const T t0 = ldb<T>(F[n], ojs, sj0), t1 = ldb<T>(F[n], ojs, sj1);
+    const T t = ap ? t0 : t1;
     const T g2 = cyt * (fc - t);
     out[n] = lim.face(fu, fc, fd, g1, g2);
   }
@@ -193,7 +197,8 @@ __device__ __forceinline__ void my_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
 #pragma unroll
   for (int n = 0; n < NS; n++) {
     const T fa = ldb<T>(F[n], lv.k, s_0), fb = ldb<T>(F[n], lv.k, s_1);
-    const T fu = vpos ? ldb<T>(F[n], lv.k, s_s) : ldb<T>(F[n], lv.k, s_2);
+    const T fs = ldb<T>(F[n], lv.k, s_s), f2 = ldb<T>(F[n], lv.k, s_2);
+    const T fu = vpos ? fs : f2;
     const T fc = vpos ? fa : fb;
     const T fd = vpos ? fb : fa;
     const T g1 = cz * (ldb<T>(F[n], osl, s_0) - ldb<T>(F[n], osu, s_0));
@@ -242,8 +247,9 @@ __device__ __forceinline__ void mz_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
 #pragma unroll
   for (int n = 0; n < NS; n++) {
     const T fk = ldb<T>(F[n], lv.k, s_0), fk1 = ldb<T>(F[n], lv.kp1, s_0);
-    const T fdn = top ? T(2.0) * fk1 - fk : ldb<T>(F[n], lv.kp2, s_0);
-    const T fu = up ? ldb<T>(F[n], lv.km1, s_0) : fdn;
+    const T fkm = ldb<T>(F[n], lv.km1, s_0), fk2 = ldb<T>(F[n], lv.kp2, s_0);
+    const T fdn = top ? T(2.0) * fk1 - fk : fk2;
+    const T fu = up ? fkm : fdn;
     const T fc = up ? fk : fk1;
     const T fd = up ? fk1 : fk;
     const T g1 = cyt * (fc - ldb<T>(F[n], ojs, s_jm));
@@ -261,8 +267,9 @@ __device__ __forceinline__ void mz_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
   }
 }
 
-template <typename T, int KIND, int NS, int MODE, int NCH, int R>
-__global__ void __launch_bounds__(64 * NCH * R) march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
+template <typename T, int KIND, int NS, int MODE, int NCH, int R, int WPE>
+__global__ void __launch_bounds__(64 * NCH * R) __attribute__((amdgpu_waves_per_eu(WPE)))
+march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
   constexpr int NK = 64 * NCH;
   constexpr bool IS_W = KIND == KIND_W;
   constexpr int kdof = IS_W ? 1 : 0;
@@ -400,47 +407,41 @@ __global__ void __launch_bounds__(64 * NCH * R) march_kernel(LaunchArgs<T> L, Fi
       }
     }
     par ^= 1;
-    if (active && k <= nz) {
+    if (active) {
+      // scalar offsets of the cell and of its periodic images, computed outside
+      // the per-lane branches
       const unsigned so = g.so(j, i);
-      if (cell) {
-        const T t1 = met[M_T1][kl], t2 = met[M_T2][kl];
-        const T wt = k < nz ? q.w_0 : T(0);
+      const int jh = (j <= 4) ? j + ny : (j >= ny + 1 ? j - ny : 0);
+      const int ih = (i <= 4) ? i + nx : (i >= nx + 1 ? i - nx : 0);
+      const unsigned soj = g.so(jh ? jh : j, i), soi = g.so(j, ih ? ih : i), soji = g.so(jh ? jh : j, ih ? ih : i);
+      const T t1 = met[M_T1][kl], t2 = met[M_T2][kl];
+      const T wt = k < nz ? q.w_0 : T(0);
 #pragma unroll
-        for (int n = 0; n < NS; n++) {
-          T sv;
-          const T s0 = MODE == MODE_ACCUM ? ldb<T>(Fo[n], lv.k, so) : T(0);
-          if (top) {
-            sv = (q.v_s * fys[n] - q.v_0 * fyn[n]) * cy;
-            sv = sv + T(4.0) * t1 * q.wm_0 * fzb[n];
-            sv = sv + (q.u_m * fxl[n] - q.u_0 * fxr[n]) * cx;
-          } else {
-            sv = s0 + (q.v_s * fys[n] - q.v_0 * fyn[n]) * cy;
-            sv = sv + T(4.0) * (q.wm_0 * fzb[n] * t1 - wt * fzt[n] * t2);
-            sv = sv + (q.u_m * fxl[n] - q.u_0 * fxr[n]) * cx;
-          }
-          if constexpr (MODE == MODE_ACCUM) {
-            stb<T>(Fo[n], sv, lv.k, so);
-          } else {
-            const T val = ldb<T>(Ff[n], lv.k, so) + L.dtm * sv;
-            stb<T>(Fo[n], val, lv.k, so);
-            const int jh = (j <= 4) ? j + ny : (j >= ny + 1 ? j - ny : 0);
-            const int ih = (i <= 4) ? i + nx : (i >= nx + 1 ? i - nx : 0);
-            if (jh) stb<T>(Fo[n], val, lv.k, g.so(jh, i));
-            if (ih) stb<T>(Fo[n], val, lv.k, g.so(j, ih));
-            if (jh && ih) stb<T>(Fo[n], val, lv.k, g.so(jh, ih));
-          }
+      for (int n = 0; n < NS; n++) {
+        T sv;
+        const T s0 = MODE == MODE_ACCUM ? ldb<T>(Fo[n], lv.k, so) : T(0);
+        const T fcur = MODE == MODE_STEP ? ldb<T>(Ff[n], lv.k, so) : T(0);
+        if (top) {
+          sv = (q.v_s * fys[n] - q.v_0 * fyn[n]) * cy;
+          sv = sv + T(4.0) * t1 * q.wm_0 * fzb[n];
+          sv = sv + (q.u_m * fxl[n] - q.u_0 * fxr[n]) * cx;
+        } else {
+          sv = s0 + (q.v_s * fys[n] - q.v_0 * fyn[n]) * cy;
+          sv = sv + T(4.0) * (q.wm_0 * fzb[n] * t1 - wt * fzt[n] * t2);
+          sv = sv + (q.u_m * fxl[n] - q.u_0 * fxr[n]) * cx;
         }
-      } else if constexpr (MODE == MODE_STEP) {
-        // levels the reference leaves untouched (k = 1; k = nz for w): s = 0
-#pragma unroll
-        for (int n = 0; n < NS; n++) {
-          const T val = ldb<T>(Ff[n], lv.k, so);
-          stb<T>(Fo[n], val, lv.k, so);
-          const int jh = (j <= 4) ? j + ny : (j >= ny + 1 ? j - ny : 0);
-          const int ih = (i <= 4) ? i + nx : (i >= nx + 1 ? i - nx : 0);
-          if (jh) stb<T>(Fo[n], val, lv.k, g.so(jh, i));
-          if (ih) stb<T>(Fo[n], val, lv.k, g.so(j, ih));
-          if (jh && ih) stb<T>(Fo[n], val, lv.k, g.so(jh, ih));
+        if constexpr (MODE == MODE_ACCUM) {
+          if (cell) stb<T>(Fo[n], sv, lv.k, so);
+          if (S.zf[n] && k <= nz) stb<T>(rsrc(S.zf[n]), k == 1 ? T(0) : fzb[n], lv.k, so);
+        } else {
+          // levels the reference leaves untouched (k = 1; k = nz for w) keep f
+          const T val = cell ? fcur + L.dtm * sv : fcur;
+          if (k <= nz) {
+            stb<T>(Fo[n], val, lv.k, so);
+            if (jh) stb<T>(Fo[n], val, lv.k, soj);
+            if (ih) stb<T>(Fo[n], val, lv.k, soi);
+            if (jh && ih) stb<T>(Fo[n], val, lv.k, soji);
+          }
         }
       }
     }

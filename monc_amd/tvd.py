@@ -33,8 +33,12 @@ EXPORTS = (
     "monc_tvd_synchronize", "monc_tvd_halo_fill", "monc_tvd_tendencies",
     "monc_tvd_forward_steps", "monc_tvd_last_error", "monc_tvd_set_tiling",
     "monc_tvd_set_profiling", "monc_tvd_kernel_times", "monc_tvd_weighted_sum",
-    "monc_tvd_adjoint_step", "monc_tvd_adjoint_fold", "monc_tvd_tangent_steps",
+    "monc_tvd_adjoint_step", "monc_tvd_adjoint_fold", "monc_tvd_tangent_steps", "monc_tvd_march_shapes",
+    "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic",
 )
+DIAG_TENDENCY, DIAG_ZFACE = 1, 2
+DIAG_3D, DIAG_PROFILE, DIAG_TABS_3D, DIAG_TABS_PROFILE, DIAG_ZFACES = 0, 1, 2, 3, 4
+KERNEL_MARCH, KERNEL_CELL, KERNEL_SPLIT = 0, 1, 2
 KERNEL_KINDS = ("u", "v", "w", "scalars", "halo", "cell")
 
 
@@ -70,6 +74,9 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_field_device_ptr.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(c_void_p)]
     lib.monc_tvd_set_stream.argtypes = [c_void_p, c_void_p]
     lib.monc_tvd_set_tiling.argtypes = [c_void_p, c_int, c_int, c_int]
+    lib.monc_tvd_march_shapes.argtypes = [c_void_p, ctypes.POINTER(c_int), c_int]
+    lib.monc_tvd_set_diagnostics.argtypes = [c_void_p, c_int, dp]
+    lib.monc_tvd_get_diagnostic.argtypes = [c_void_p, c_int, c_int, dp]
     lib.monc_tvd_set_profiling.argtypes = [c_void_p, c_int]
     lib.monc_tvd_weighted_sum.argtypes = [c_void_p, c_int, c_int, dp, dp]
     lib.monc_tvd_kernel_times.argtypes = [c_void_p, ctypes.POINTER(c_double), ctypes.POINTER(ctypes.c_longlong),
@@ -172,11 +179,42 @@ class TvdAdvection:
         _check(self._lib.monc_tvd_field_device_ptr(self._h, field, variant, ctypes.byref(p)))
         return int(p.value or 0)
 
-    def set_tiling(self, kernel: int = 0, rows: int = -1, strip: int = 0) -> None:
-        """kernel 0 = x-marching kernels (rows -1 default shape, 0 wave kernel,
-        -2 split wave kernel, > 0 slab rows), 1 = one-cell-per-thread kernel;
-        strip = x columns per block."""
+    def set_tiling(self, kernel: int = KERNEL_MARCH, rows: int = 0, strip: int = 0) -> None:
+        """kernel KERNEL_MARCH (default; rows = cell rows per block, 0 the
+        default shape), KERNEL_CELL or KERNEL_SPLIT; strip = x columns per
+        block (0: 16)."""
         _check(self._lib.monc_tvd_set_tiling(self._h, kernel, rows, strip))
+
+    def march_shapes(self) -> list:
+        """Cell rows per block of the march shapes compiled for this column height, default first."""
+        buf = (ctypes.c_int * 16)()
+        n = self._lib.monc_tvd_march_shapes(self._h, buf, 16)
+        if n < 0:
+            _check(n)
+        return [buf[i] for i in range(min(n, 16))]
+
+    def set_diagnostics(self, flags: int, rprefrcp: Optional[np.ndarray] = None) -> None:
+        """DIAG_TENDENCY and/or DIAG_ZFACE recording on every tendencies() call;
+        rprefrcp (length nz) enables the tabs tendency."""
+        r = None
+        if rprefrcp is not None:
+            r = np.ascontiguousarray(rprefrcp, dtype=np.float64)
+            if r.shape != (self.nz,):
+                raise ValueError("rprefrcp must have length nz")
+        _check(self._lib.monc_tvd_set_diagnostics(self._h, flags, _dptr(r) if r is not None else None))
+
+    def get_diagnostic(self, field: int, which: int) -> np.ndarray:
+        """A recorded diagnostic: DIAG_3D / DIAG_TABS_3D as (nx, ny, nz) arrays
+        (x slowest, the (z,y,x) column-major layout), DIAG_PROFILE /
+        DIAG_TABS_PROFILE as (nz,), DIAG_ZFACES in the field shape."""
+        if which in (DIAG_3D, DIAG_TABS_3D):
+            out = np.zeros((self.nx, self.ny, self.nz))
+        elif which in (DIAG_PROFILE, DIAG_TABS_PROFILE):
+            out = np.zeros(self.nz)
+        else:
+            out = np.zeros(self.field_shape())
+        _check(self._lib.monc_tvd_get_diagnostic(self._h, field, which, _dptr(out)))
+        return out
 
     def weighted_sum(self, field: int, variant: int, weights: np.ndarray) -> float:
         """sum_k w(k) sum_{j,i} f(k,j,i) over the interior, accumulated in FP64 on the device."""

@@ -19,6 +19,8 @@ Parity status per oracle function (the checks live in tests/test_oracle.py):
 * oracle_vertical_metrics -- pinned by the telescoping identities
   4*rhon(k)*dz(k)*tzc1(k) = rho(k-1) and their tzd counterparts.
 * oracle_forward_step -- pinned by direct evaluation of field + dtm*s.
+* oracle_profile_total -- pinned by exact integer sums (every partial sum
+  exactly representable) and by comparison with a pairwise sum on random data.
 """
 from __future__ import annotations
 
@@ -57,6 +59,7 @@ def lib() -> ctypes.CDLL:
         L.oracle_horizontal_c.argtypes = [cd]
         L.oracle_horizontal_c.restype = cd
         L.oracle_forward_step.argtypes = [dp, dp, dp, ci, ci, ci, cd]
+        L.oracle_profile_total.argtypes = [dp, ci, ci, ci, dp]
         _lib = L
     return _lib
 
@@ -157,3 +160,30 @@ def _advected(n: int, advect: int):
     if advect & 4:
         s |= set(range(4, n))
     return s
+
+
+def profile_total(t3d: np.ndarray) -> np.ndarray:
+    """Per-level total of a (nx, ny, nz) tendency over its columns, in the
+    reference's column order."""
+    t = np.ascontiguousarray(t3d, dtype=np.float64)
+    nx, ny, nz = t.shape
+    out = np.zeros(nz)
+    lib().oracle_profile_total(_p(t), nz, ny, nx, _p(out))
+    return out
+
+
+def tendency_diagnostics(state, dtm: float, field_stepping: int, momentum_stepping: int, scalar_stepping: int,
+                         sources, rprefrcp=None, advect: int = 7):
+    """save_precomponent_tendencies / compute_component_tendencies around one
+    sweep: per field tend_3d = s_after - s_before over the interior columns,
+    its profile total, and (with rprefrcp) the tabs tendency of th.  Returns
+    (after, tend3d, profiles, tabs3d, tabs_profile)."""
+    after, _ = state_tendencies(state, dtm, field_stepping, momentum_stepping, scalar_stepping, sources, advect)
+    t3d = [a[2:-2, 2:-2, :] - b[2:-2, 2:-2, :] for a, b in zip(after, sources)]
+    prof = [profile_total(t) for t in t3d]
+    tabs = tabs_p = None
+    if rprefrcp is not None:
+        r = np.asarray(rprefrcp, dtype=np.float64)
+        tabs = after[3][2:-2, 2:-2, :] * r - sources[3][2:-2, 2:-2, :] * r
+        tabs_p = profile_total(tabs)
+    return after, t3d, prof, tabs, tabs_p

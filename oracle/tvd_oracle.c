@@ -637,3 +637,14 @@ void oracle_forward_step(double *znew, const double *f, double *s, int nz, int n
       for (int k = 1; k <= nz; k++) A(znew, k, j, i) = A(f, k, j, i) + dtm * A(s, k, j, i);
     }
 }
+
+/* tvdadvection.F90:1131-1163: the per-level total of a 3-D tendency over the
+ * interior columns, accumulated column by column in the order the timestepper
+ * visits them (timestepper.F90:65-85: x outer, y inner).  t has the
+ * reference's tend_3d layout (nz, ny, nx). */
+void oracle_profile_total(const double *t, int nz, int ny, int nx, double *out) {
+  for (int k = 0; k < nz; k++) out[k] = 0.0;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) out[k] = out[k] + t[k + (size_t)nz * ((size_t)j + (size_t)ny * (size_t)i)];
+}

@@ -33,7 +33,7 @@ def _compare(gpu_s, ref_s, tol=TOL, rows=None):
 
 @pytest.mark.parametrize("fs,ms,sc", [(1, 1, 1), (0, 0, 0), (1, 0, 0), (0, 1, 0)])
 @pytest.mark.parametrize("anelastic", [None, 8000.0])
-@pytest.mark.parametrize("tiling", [None, (1, 0, 0), (0, 0, 5), (0, 3, 5), (0, -2, 5), (0, -3, 5), (0, -3, 16)])
+@pytest.mark.parametrize("tiling", [None, (1, 0, 0), (2, 0, 5), (0, 4, 5), (0, 8, 16), (0, 4, 1)])
 def test_tiny_grid_stepping_modes(gpu, fs, ms, sc, anelastic, tiling):
     st = ss.make_state(16, 10, 12, 50.0, 500.0, seed=3, n_q=2, anelastic_scale=anelastic)
     ref, _ = oracle.state_tendencies(st, 1.0, fs, ms, sc)
@@ -41,19 +41,30 @@ def test_tiny_grid_stepping_modes(gpu, fs, ms, sc, anelastic, tiling):
     _compare(got, ref)
 
 
-# every compiled slab shape, with strips that do and do not divide nx, and
-# column heights that fill 1..4 chunks of 64 levels partially and exactly
-@pytest.mark.parametrize("nz,rows", [(64, 3), (64, 7), (70, 1), (128, 3), (150, 1), (192, 1), (256, 1), (200, 1),
-                                     (64, 0), (70, 0), (128, 0), (150, 0), (256, 0), (200, 0),
-                                     (64, -2), (70, -2), (128, -2), (150, -2), (256, -2), (200, -2),
-                                     (64, -3), (70, -3), (128, -3), (150, -3), (256, -3), (200, -3), (4, -3),
-                                     (5, -3), (63, -3), (65, -3), (127, -3), (129, -3), (191, -3), (193, -3)])
+# every compiled march shape (cell rows per block) for column heights that
+# fill 1..4 sub-columns of 64 levels partially and exactly, with strips that
+# do and do not divide nx; the split kernel at the same heights
+MARCH_CASES = [(nz, rows) for nz in (4, 5, 63, 64) for rows in (8, 4)]
+MARCH_CASES += [(nz, rows) for nz in (65, 70, 127, 128, 129, 150, 191, 192) for rows in (4, 2)]
+MARCH_CASES += [(nz, rows) for nz in (193, 200, 256) for rows in (4, 2, 3, 1)]
+
+
+@pytest.mark.parametrize("nz,rows", MARCH_CASES)
 @pytest.mark.parametrize("strip", [5, 16])
-def test_slab_shapes(gpu, nz, rows, strip):
+def test_march_shapes(gpu, nz, rows, strip):
     st = ss.make_state(nz, 9, 11, 50.0, 25.0 * (nz - 1), seed=nz + rows, n_q=1)
     for fs in (1, 0):
         ref, _ = oracle.state_tendencies(st, 1.0, fs, fs, fs)
-        got = gpu_tendencies(st, 1.0, fs, fs, fs, tiling=(0, rows, strip))
+        got = gpu_tendencies(st, 1.0, fs, fs, fs, tiling=(tvd.KERNEL_MARCH, rows, strip))
+        _compare(got, ref)
+
+
+@pytest.mark.parametrize("nz", [64, 70, 128, 150, 256, 200])
+def test_split_kernel(gpu, nz):
+    st = ss.make_state(nz, 9, 11, 50.0, 25.0 * (nz - 1), seed=nz, n_q=1)
+    for fs in (1, 0):
+        ref, _ = oracle.state_tendencies(st, 1.0, fs, fs, fs)
+        got = gpu_tendencies(st, 1.0, fs, fs, fs, tiling=(tvd.KERNEL_SPLIT, 0, 7))
         _compare(got, ref)
 
 
@@ -123,7 +134,7 @@ def test_halo_fill_matches_periodic_copy(gpu):
         t.close()
 
 
-@pytest.mark.parametrize("tiling", [None, (1, 0, 0), (0, 3, 7), (0, 0, 7), (0, -2, 7), (0, -3, 7), (0, -3, 3)])
+@pytest.mark.parametrize("tiling", [None, (1, 0, 0), (2, 0, 7), (0, 8, 7), (0, 4, 3)])
 def test_forward_steps_match_oracle_run(gpu, tiling):
     st = ss.make_state(20, 24, 20, 50.0, 600.0, seed=12, n_q=1, with_zfields=False)
     dt = ss.cfl_dt(st, 0.3)

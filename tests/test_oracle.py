@@ -472,3 +472,23 @@ def test_forward_step():
     exp[2:-2, 2:-2, 0] = f[2:-2, 2:-2, 0]  # sfield(1) = 0
     np.testing.assert_array_equal(out[2:-2, 2:-2], exp[2:-2, 2:-2])
     np.testing.assert_array_equal(out[:2], f[:2])
+
+
+def test_profile_total_pinned():
+    """oracle_profile_total: exact on integer data (every partial sum is
+    representable), and within round-off of an exactly rounded sum
+    (math.fsum) on random data; column order does not change exact sums."""
+    import math
+    rng = np.random.default_rng(7)
+    t = rng.integers(-1000, 1000, (9, 7, 12)).astype(np.float64)
+    np.testing.assert_array_equal(oracle.profile_total(t), t.sum(axis=(0, 1)))
+    t = rng.standard_normal((9, 7, 12))
+    got = oracle.profile_total(t)
+    for k in range(t.shape[2]):
+        ref = math.fsum(t[:, :, k].ravel())
+        assert abs(got[k] - ref) <= 1e-14 * np.abs(t[:, :, k]).sum()
+    # a planted mistake (y outer instead of x outer) changes nothing for exact
+    # sums but the routine must still see every column once
+    t = np.zeros((3, 4, 2))
+    t[2, 3, 1] = 1.0
+    assert oracle.profile_total(t).tolist() == [0.0, 1.0]
