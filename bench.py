@@ -5,9 +5,10 @@ Workload (the plain FP64 form at real shape): a 512x512x256 local domain
 seeded distributions of the synthetic set-up, advection-only forward-Euler
 steps at CFL 0.3.  One step = the TVD tendencies of all five fields from the
 pre-step state plus the update field <- field + dt*s and the periodic halo of
-the new state (monc_tvd_forward_steps): four launches of the split wave
-kernel, one each for u, v, w and the (th, q) pair.  The same step in FP32 is reported
-beside it.
+the new state (monc_tvd_forward_steps): four launches of the march
+kernel, one each for u, v, w and the (th, q) pair, with tile shapes picked by
+the library's autotuner before the untimed warm-up.  The same step in FP32 is
+reported beside it.
 
 Output: one JSON line.  value = interior cell-updates per second (all five
 fields) of the FP64 form, wall clock over the timed steps; roofline = the
@@ -39,9 +40,11 @@ def log(msg: str) -> None:
 # (element size es): u, v and w kernels read u, v, w and write their field;
 # the scalar kernel reads u, v, w and its two scalars and writes both.
 KIND_FIELDS = {"u": 4, "v": 4, "w": 4, "scalars": 7, "cell": 10}
+KERNEL_NAMES = {0: "march_kernel", 1: "cell_kernel", 2: "split_kernel"}
 
 
-def run_form(state, precision: int, steps: int, warmup: int, dt: float, kernel: int = 0, rows: int = 0):
+def run_form(state, precision: int, steps: int, warmup: int, dt: float, kernel: int = 0, rows: int = 0,
+             tune: bool = True):
     import torch
 
     from monc_amd import tvd
@@ -52,6 +55,7 @@ def run_form(state, precision: int, steps: int, warmup: int, dt: float, kernel: 
         t.set_stream(stream.cuda_stream)
         t.set_tiling(kernel, rows, 0)
         t.halo_fill()
+        tuned = t.autotune(dt, reps=2) if tune else None
         for _ in range(warmup):
             t.forward_steps(dt, 1)
         t.synchronize()
@@ -64,7 +68,7 @@ def run_form(state, precision: int, steps: int, warmup: int, dt: float, kernel: 
         wall = time.perf_counter() - t0
         kt = t.kernel_times(reset=True)
         t.set_profiling(False)
-        return wall, kt
+        return wall, kt, tuned
     finally:
         t.close()
 
@@ -110,6 +114,7 @@ def main() -> None:
     ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--kernel", type=int, default=0, help="0 march (default), 1 cell, 2 split")
     ap.add_argument("--rows", type=int, default=0, help="march kernel cell rows per block (0: default)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile-shape autotuner (untimed)")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -136,7 +141,8 @@ def main() -> None:
     cells = nx * ny * nz
 
     log("bench: FP64 form")
-    wall64, kt64 = run_form(st, tvd.FP64, args.steps, args.warmup, dt, args.kernel, args.rows)
+    tune = not args.no_tune and args.kernel == 0 and args.rows == 0
+    wall64, kt64, tuned64 = run_form(st, tvd.FP64, args.steps, args.warmup, dt, args.kernel, args.rows, tune)
     ms_step = wall64 / args.steps * 1e3
     value = cells * args.steps / wall64 / 1e9
     kind, kavg, kbytes, achieved = roofline(kt64, cells, 8)
@@ -160,15 +166,16 @@ def main() -> None:
                    "nx": nx, "ny": ny, "nz": nz, "fields": 5, "dt_s": dt},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"split_kernel ({kind})", "kernel_ms": round(kavg, 4),
+                     "kernel": f"{KERNEL_NAMES[args.kernel]} ({kind})", "kernel_ms": round(kavg, 4),
                      "algorithmic_bytes_per_launch": kbytes,
                      "step": {"kernel_ms": round(step_kernel_ms, 4), "bytes_per_cell_update": 80,
                               "achieved_GBs": round(80 * cells / (step_kernel_ms * 1e-3) / 1e9, 2)},
                      "kernels_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kt64.items()}},
+        "tile_shapes": {"fp64": tuned64},
     }
     if not args.no_fp32:
         log("bench: FP32 form")
-        wall32, kt32 = run_form(st, tvd.FP32, args.steps, args.warmup, dt, args.kernel, args.rows)
+        wall32, kt32, tuned32 = run_form(st, tvd.FP32, args.steps, args.warmup, dt, args.kernel, args.rows, tune)
         v32 = cells * args.steps / wall32 / 1e9
         kind32, kavg32, kbytes32, a32 = roofline(kt32, cells, 4)
         step32 = sum(v[0] for k, v in kt32.items() if k in KIND_FIELDS) / args.steps
@@ -178,6 +185,7 @@ def main() -> None:
                        "frac": round(a32 / HBM_PEAK_GBS, 4),
                        "step_achieved_GBs": round(40 * cells / (step32 * 1e-3) / 1e9, 2),
                        "speedup_vs_fp64": round(v32 / value, 3)}
+        out["tile_shapes"]["fp32"] = tuned32
     if args.calibrate_copy:
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)

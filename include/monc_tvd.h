@@ -100,7 +100,7 @@ int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **
  * library).  kernel MONC_TVD_KERNEL_MARCH (default, columns of up to 256
  * levels): each face computed once, rows = cell rows per block (0: the
  * default for the column height; otherwise one of the shapes listed by
- * monc_tvd_march_shapes), strip = x columns per block (0: 16).
+ * monc_tvd_march_shapes), strip = x columns per block (0: 32).
  * MONC_TVD_KERNEL_CELL: one thread per cell, every face recomputed (any
  * column height; the fallback beyond 256 levels or 4 GiB per field).
  * MONC_TVD_KERNEL_SPLIT: one wave per 64 levels of a row, both y faces of a
@@ -109,6 +109,15 @@ int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **
 #define MONC_TVD_KERNEL_CELL 1
 #define MONC_TVD_KERNEL_SPLIT 2
 int monc_tvd_set_tiling(monc_tvd_ctx *ctx, int kernel, int rows, int strip);
+/* Tile-shape autotuner (no counterpart in the reference).  For each kernel
+ * of the forward step (u, v, w, the scalar group) it times every compiled
+ * march shape and strip length (8, 16, 32, 64) on the context's own fields,
+ * reps launches each after one warm-up, writing into library scratch (the
+ * state is not advanced), and keeps the fastest per kernel.  chosen (8 ints,
+ * may be null) receives rows and strip per kernel in the order u, v, w,
+ * scalars.  Needs the prognostics of u, v, w and the advected fields. */
+int monc_tvd_autotune(monc_tvd_ctx *ctx, double dtm, int reps, int *chosen);
+
 /* The march shapes (cell rows per block) compiled for this context's column
  * height, default first: writes up to n of them to rows and returns how many
  * exist (0 when the march kernel cannot run this context), or a negative

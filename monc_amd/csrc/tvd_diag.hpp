@@ -12,6 +12,9 @@ namespace tvd {
 template <typename T>
 __global__ void __launch_bounds__(256) tend3d_kernel(const T* after, const T* before, const double* r, T* out,
                                                      Dims d) {
+  // products rounded separately, as the reference forms them (no fused
+  // multiply-add: at an untouched level after == before and v must be 0)
+#pragma clang fp contract(off)
   const int k = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 3 + blockIdx.y * 4 + threadIdx.y;
   const int i = 3 + blockIdx.z;

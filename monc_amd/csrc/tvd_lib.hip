@@ -59,7 +59,8 @@ struct monc_tvd_ctx {
   std::vector<void*> dual[2];  // packed (value, tangent) state, double-buffered (tangent steps)
   hipStream_t stream;
   int kernel;                  // MONC_TVD_KERNEL_MARCH, _CELL or _SPLIT
-  int rows, strip;             // march tile shape: cell rows per block, x columns per block (0: defaults)
+  int rows[4], strip[4];       // march tile shape per kind (u, v, w, scalars): cell rows and x columns
+                               // per block (0: defaults)
   int buffer_ok;               // fields small enough for 32-bit buffer offsets
   // optional per-kernel timing: HIP events recorded around every launch on
   // the context stream, folded into per-kind totals on query
@@ -188,7 +189,7 @@ void launch_split(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
 // the per-column barrier.
 template <int ES, int NCH, int R>
 constexpr int march_wpe() {
-  return ES == 4 ? (NCH * R >= 16 ? 8 : 6) : ((16 % (NCH * R)) == 0 ? 4 : 3);
+  return ES == 4 ? (NCH * R >= 16 ? 8 : 6) : (NCH * R >= 16 ? 4 : 3);
 }
 
 template <typename T, int KIND, int NS, int MODE, int NCH, int R>
@@ -202,7 +203,21 @@ void launch_march(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
 
 // Cell rows per block of the march kernel compiled for each column height
 // (NCH sub-columns of 64 levels); the first entry is the default.
-constexpr int MARCH_ROWS[4][4] = {{8, 4, 0, 0}, {4, 2, 0, 0}, {4, 2, 0, 0}, {4, 2, 3, 1}};
+constexpr int MARCH_ROWS[4][4] = {{12, 8, 4, 0}, {6, 4, 2, 0}, {4, 2, 0, 0}, {3, 2, 4, 1}};
+
+// Default shape per precision and column height, from timings of the forward
+// step on MI355X (64, 128 and 256 levels over 256^2 and 512^2 columns): rows
+// per block, and x columns per block chosen so that a launch has at least
+// 1024 blocks (4 per CU) when the domain allows it.
+int default_rows(int precision, int nch) {
+  static const int fp32[4] = {8, 4, 4, 3}, fp64[4] = {8, 2, 2, 3};
+  return (precision == MONC_TVD_FP64 ? fp64 : fp32)[nch - 1];
+}
+
+int default_strip(int ny, int nx, int rows) {
+  const long tiles_y = (ny + rows - 1) / rows;
+  return tiles_y * ((nx + 31) / 32) >= 1024 ? 32 : 16;
+}
 
 bool march_rows_ok(int nch, int rows) {
   if (nch < 1 || nch > 4) return false;
@@ -224,8 +239,10 @@ void launch_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
     return;
   }
   switch (nch * 100 + rows) {
+    case 112: launch_march<T, KIND, NS, MODE, 1, 12>(c, L, S, strip); break;
     case 108: launch_march<T, KIND, NS, MODE, 1, 8>(c, L, S, strip); break;
     case 104: launch_march<T, KIND, NS, MODE, 1, 4>(c, L, S, strip); break;
+    case 206: launch_march<T, KIND, NS, MODE, 2, 6>(c, L, S, strip); break;
     case 204: launch_march<T, KIND, NS, MODE, 2, 4>(c, L, S, strip); break;
     case 202: launch_march<T, KIND, NS, MODE, 2, 2>(c, L, S, strip); break;
     case 304: launch_march<T, KIND, NS, MODE, 3, 4>(c, L, S, strip); break;
@@ -243,7 +260,6 @@ void launch_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
 template <typename T, int MODE>
 int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
   const int nch = (c->nz + 63) / 64;
-  const int strip = c->strip > 0 ? c->strip : 16;
   if (c->kernel == MONC_TVD_KERNEL_CELL || nch > 4 || !c->buffer_ok) {
     const int nk = MODE == MODE_STEP ? c->nz : c->nz - 1;
     dim3 block(64, 4, 1);
@@ -253,7 +269,6 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
     HIP_TRY(hipGetLastError());
     return MONC_TVD_OK;
   }
-  const int rows = c->rows > 0 ? c->rows : MARCH_ROWS[nch - 1][0];
   for (int n = 0; n < S.nf;) {
     FieldSet<T> G{};
     const int pair = S.kind[n] == KIND_SCALAR && n + 1 < S.nf && S.kind[n + 1] == KIND_SCALAR && S.dt[n + 1] == S.dt[n];
@@ -268,7 +283,10 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
     }
     if (c->kernel == MONC_TVD_KERNEL_SPLIT && G.zf[0])
       return fail(MONC_TVD_ERR_ARG, "z-face diagnostics need the march or cell kernel, not the split kernel");
-    LaunchTimer timer(c, pair || S.kind[n] == KIND_SCALAR ? 3 : S.kind[n]);
+    const int kk = pair || S.kind[n] == KIND_SCALAR ? 3 : S.kind[n];
+    const int rows = c->rows[kk] > 0 ? c->rows[kk] : default_rows(c->precision, nch);
+    const int strip = c->strip[kk] > 0 ? c->strip[kk] : default_strip(c->ny, c->nx, rows);
+    LaunchTimer timer(c, kk);
     switch (S.kind[n]) {
       case KIND_U: launch_shape<T, KIND_U, 1, MODE>(c, L, G, nch, c->kernel, rows, strip); break;
       case KIND_V: launch_shape<T, KIND_V, 1, MODE>(c, L, G, nch, c->kernel, rows, strip); break;
@@ -448,6 +466,86 @@ int forward_steps_impl(monc_tvd_ctx* c, double dtm, int nsteps) {
   return MONC_TVD_OK;
 }
 
+// Tile-shape autotuner: for each kernel kind of the forward step (u, v, w,
+// the scalar group) every compiled march shape (rows per block) and strip
+// length is timed with HIP events on the context's own fields, writing the
+// step's result into the spare buffers (the state is not advanced), and the
+// fastest shape per kind is kept.
+template <typename T>
+int autotune_impl(monc_tvd_ctx* c, double dtm, int reps, int* chosen) {
+  int rc;
+  const int nch = (c->nz + 63) / 64;
+  if (nch > 4 || !c->buffer_ok) return fail(MONC_TVD_ERR_ARG, "the march kernel does not run this context");
+  for (int fl : {MONC_TVD_U, MONC_TVD_V, MONC_TVD_W})
+    if ((rc = need(c, fl, MONC_TVD_PROG))) return rc;
+  std::vector<int> fields = advected_fields(c);
+  for (int fl : fields) {
+    if ((rc = need(c, fl, MONC_TVD_PROG))) return rc;
+    if (!c->spare[fl]) HIP_TRY(hipMalloc(&c->spare[fl], c->n_elem * c->elem_size));
+  }
+  const LaunchArgs<T> L = launch_args<T>(c, dtm);
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  static const int strips[] = {8, 16, 32, 64};
+  for (size_t n = 0; n < fields.size();) {
+    FieldSet<T> G{};
+    const int fl0 = fields[n];
+    const int kind = kind_of(fl0);
+    const bool pair = kind == KIND_SCALAR && n + 1 < fields.size();
+    G.nf = pair ? 2 : 1;
+    for (int m = 0; m < G.nf; m++) {
+      const int fl = fields[n + m];
+      G.kind[m] = kind;
+      G.F[m] = (const T*)slot(c, fl, MONC_TVD_PROG);
+      G.f[m] = G.F[m];
+      G.out[m] = (T*)c->spare[fl];
+      G.dt[m] = (T)dtm;
+    }
+    const int kk = kind == KIND_SCALAR ? 3 : kind;
+    float best = 1e30f;
+    int best_rows = 0, best_strip = 0;
+    for (int rows : MARCH_ROWS[nch - 1]) {
+      if (!rows) continue;
+      for (int strip : strips) {
+        float ms = 0.f;
+        for (int it = 0; it <= reps; it++) {
+          if (it == 1) HIP_TRY(hipEventRecord(a, c->stream));
+          switch (kind) {
+            case KIND_U: launch_shape<T, KIND_U, 1, MODE_STEP>(c, L, G, nch, MONC_TVD_KERNEL_MARCH, rows, strip); break;
+            case KIND_V: launch_shape<T, KIND_V, 1, MODE_STEP>(c, L, G, nch, MONC_TVD_KERNEL_MARCH, rows, strip); break;
+            case KIND_W: launch_shape<T, KIND_W, 1, MODE_STEP>(c, L, G, nch, MONC_TVD_KERNEL_MARCH, rows, strip); break;
+            default:
+              if (pair) launch_shape<T, KIND_SCALAR, 2, MODE_STEP>(c, L, G, nch, MONC_TVD_KERNEL_MARCH, rows, strip);
+              else launch_shape<T, KIND_SCALAR, 1, MODE_STEP>(c, L, G, nch, MONC_TVD_KERNEL_MARCH, rows, strip);
+          }
+          HIP_TRY(hipGetLastError());
+        }
+        HIP_TRY(hipEventRecord(b, c->stream));
+        HIP_TRY(hipEventSynchronize(b));
+        HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        if (ms < best) {
+          best = ms;
+          best_rows = rows;
+          best_strip = strip;
+        }
+      }
+    }
+    c->rows[kk] = best_rows;
+    c->strip[kk] = best_strip;
+    n += G.nf;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  c->kernel = MONC_TVD_KERNEL_MARCH;
+  if (chosen)
+    for (int q = 0; q < 4; q++) {
+      chosen[2 * q] = c->rows[q];
+      chosen[2 * q + 1] = c->strip[q];
+    }
+  return MONC_TVD_OK;
+}
+
 template <typename T>
 int fold_halo(monc_tvd_ctx* c, T* f) {
   Dims d = dims_of(c);
@@ -613,8 +711,7 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags,
   c->stream = nullptr;
   c->dmet = nullptr;
   c->kernel = MONC_TVD_KERNEL_MARCH;
-  c->rows = 0;
-  c->strip = 0;
+  for (int q = 0; q < 4; q++) c->rows[q] = c->strip[q] = 0;
   c->buffer_ok = n_elem * c->elem_size < ((size_t)1 << 32) - ((size_t)1 << 20);
   c->profiling = 0;
   c->diag_flags = 0;
@@ -691,8 +788,10 @@ int monc_tvd_set_tiling(monc_tvd_ctx* c, int kernel, int rows, int strip) {
     return fail(MONC_TVD_ERR_ARG, "no march shape with %d rows for %d-level columns", rows, c->nz);
   if (rows < 0) return fail(MONC_TVD_ERR_ARG, "rows %d < 0", rows);
   c->kernel = kernel;
-  c->rows = kernel == MONC_TVD_KERNEL_MARCH ? rows : 0;
-  c->strip = strip;
+  for (int q = 0; q < 4; q++) {
+    c->rows[q] = kernel == MONC_TVD_KERNEL_MARCH ? rows : 0;
+    c->strip[q] = strip;
+  }
   return MONC_TVD_OK;
 }
 
@@ -963,6 +1062,14 @@ int monc_tvd_get_diagnostic(monc_tvd_ctx* c, int field, int which, double* host)
     for (size_t q = 0; q < n; q++) host[q] = (double)tmp[q];
   }
   return MONC_TVD_OK;
+}
+
+int monc_tvd_autotune(monc_tvd_ctx* c, double dtm, int reps, int* chosen) {
+  if (int rc = check_ctx(c)) return rc;
+  if (reps < 1 || reps > 100) return fail(MONC_TVD_ERR_ARG, "reps %d out of range [1,100]", reps);
+  if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
+  return c->precision == MONC_TVD_FP64 ? autotune_impl<double>(c, dtm, reps, chosen)
+                                       : autotune_impl<float>(c, dtm, reps, chosen);
 }
 
 }  // extern "C"

@@ -267,35 +267,39 @@ __device__ __forceinline__ void mz_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
   }
 }
 
-template <typename T, int KIND, int NS, int MODE, int NCH, int R, int WPE>
-__global__ void __launch_bounds__(64 * NCH * R) __attribute__((amdgpu_waves_per_eu(WPE)))
-march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
-  constexpr int NK = 64 * NCH;
+// LDS of one block: the metric table of the field kind (NKMAX levels), the
+// north y faces of every wave (double-buffered by column parity) and the top z
+// face of the last level of every wave's sub-column.
+template <typename T, int NSL, int WAVES, int NKMAX>
+struct MarchLds {
+  T met[M_COUNT][NKMAX];
+  T yn[2][WAVES][NSL][64];
+  T zt[2][WAVES][NSL];
+};
+
+// One tile: cell rows j0 = 3 + ty*R .. j0+R-1 (R = WAVES / nch), all levels
+// (nch sub-columns of 64), x columns 3 + tx*strip onwards.  Wave wv takes
+// sub-column wv % nch of row wv / nch; waves past R*nch only take the
+// barriers.  Every wave of the block calls this with the same tile.
+template <typename T, int KIND, int NS, int MODE, int WAVES, int NKMAX, int NSL>
+__device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSet<T>& S, int nch, int ty, int tx,
+                                           int strip, MarchLds<T, NSL, WAVES, NKMAX>& lds) {
   constexpr bool IS_W = KIND == KIND_W;
   constexpr int kdof = IS_W ? 1 : 0;
-  __shared__ T met[M_COUNT][NK];
-  __shared__ T yn[2][R][NS][NK];
-  __shared__ T zt[2][R][NCH][NS];
+  const int R = WAVES / nch;
+  const int NK = 64 * nch;
   const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;
   const VMet m = IS_W ? L.mw : L.mc;
   const int tid = threadIdx.x + 64 * threadIdx.y;
-  for (int e = tid; e < M_COUNT * NK; e += 64 * NCH * R) {
+  for (int e = tid; e < M_COUNT * NK; e += 64 * WAVES) {
     const int mm = e / NK, kk = 1 + e % NK;
-    met[mm][kk - 1] = met_value<T>(mm, kk, nz, kdof, m);
+    lds.met[mm][kk - 1] = met_value<T>(mm, kk, nz, kdof, m);
   }
-  // tile of this block: consecutive blocks of one XCD (blocks are dealt to the
-  // 8 XCDs round-robin) take consecutive tiles, so that rows shared by
-  // neighbouring tiles meet in one L2
-  const int nb = (int)gridDim.x, bid = (int)blockIdx.x;
-  const int full = nb - nb % 8;
-  const int t = bid < full ? (bid % 8) * (full / 8) + bid / 8 : bid;
-  const int ty = t % tiles_y, tx = t / tiles_y;
-
   const int lane = threadIdx.x;
   const int wv = (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int qc = wv % NCH, r = wv / NCH;
+  const int qc = wv % nch, r = wv / nch;
   const int j = 3 + ty * R + r;
-  const bool active = j <= ny + 2;  // wave-uniform
+  const bool active = r < R && j <= ny + 2;  // wave-uniform
   const int k = 1 + 64 * qc + lane;
   const int kmax = IS_W ? nz - 1 : nz;
   const bool cell = k >= 2 && k <= kmax;
@@ -328,7 +332,7 @@ march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
   }
   __syncthreads();
   const int kl = (k <= NK ? k : NK) - 1;
-  const T rdz_k = met[M_RDZ][kl], rdz_kd = met[M_RDZKD][kl];
+  const T rdz_k = lds.met[M_RDZ][kl], rdz_kd = lds.met[M_RDZKD][kl];
 
   T fxl[NS];
 #pragma unroll
@@ -378,33 +382,29 @@ march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
       mx_face<T, NS>(F, g, lv, j, i, q.u_m, q.u_0, q.u_p, q.v_0, q.v_s, q.v_0p, q.v_sp, q.w_0, q.w_p, q.wm_0,
                      q.wm_p, rdz_k, rdz_kd, dt, cx, cy, fxr);
       T zf[NS];
-      mz_face<T, NS>(F, g, lv, j, i, q, edge, ztop, zbot, met[M_ZS_UP][kl], met[M_ZS_DN][kl], met[M_RDC_UP][kl],
-                     met[M_RDC_DN][kl], met[M_RDU_UP][kl], met[M_RDU_DN][kl], met[M_RDD][kl], met[M_DD2][kl], dt,
-                     cx, cy, zf);
+      mz_face<T, NS>(F, g, lv, j, i, q, edge, ztop, zbot, lds.met[M_ZS_UP][kl], lds.met[M_ZS_DN][kl],
+                     lds.met[M_RDC_UP][kl], lds.met[M_RDC_DN][kl], lds.met[M_RDU_UP][kl], lds.met[M_RDU_DN][kl],
+                     lds.met[M_RDD][kl], lds.met[M_DD2][kl], dt, cx, cy, zf);
 #pragma unroll
       for (int n = 0; n < NS; n++) fzt[n] = has_z ? zf[n] : T(0);
     }
 #pragma unroll
     for (int n = 0; n < NS; n++) {
-      yn[par][r][n][64 * qc + lane] = fyn[n];
+      lds.yn[par][wv][n][lane] = fyn[n];
       fzb[n] = __shfl_up(fzt[n], 1);
     }
-    if constexpr (NCH > 1) {
-      if (lane == 63) {
+    if (nch > 1 && lane == 63) {
 #pragma unroll
-        for (int n = 0; n < NS; n++) zt[par][r][qc][n] = fzt[n];
-      }
+      for (int n = 0; n < NS; n++) lds.zt[par][wv][n] = fzt[n];
     }
     __syncthreads();
-    if (r > 0) {
+    if (r > 0) {  // the south face is the north face of the row below: wave wv - nch
 #pragma unroll
-      for (int n = 0; n < NS; n++) fys[n] = yn[par][r - 1][n][64 * qc + lane];
+      for (int n = 0; n < NS; n++) fys[n] = lds.yn[par][wv - nch][n][lane];
     }
-    if constexpr (NCH > 1) {
-      if (lane == 0 && qc > 0) {
+    if (nch > 1 && lane == 0 && qc > 0) {
 #pragma unroll
-        for (int n = 0; n < NS; n++) fzb[n] = zt[par][r][qc - 1][n];
-      }
+      for (int n = 0; n < NS; n++) fzb[n] = lds.zt[par][wv - 1][n];
     }
     par ^= 1;
     if (active) {
@@ -414,7 +414,7 @@ march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
       const int jh = (j <= 4) ? j + ny : (j >= ny + 1 ? j - ny : 0);
       const int ih = (i <= 4) ? i + nx : (i >= nx + 1 ? i - nx : 0);
       const unsigned soj = g.so(jh ? jh : j, i), soi = g.so(j, ih ? ih : i), soji = g.so(jh ? jh : j, ih ? ih : i);
-      const T t1 = met[M_T1][kl], t2 = met[M_T2][kl];
+      const T t1 = lds.met[M_T1][kl], t2 = lds.met[M_T2][kl];
       const T wt = k < nz ? q.w_0 : T(0);
 #pragma unroll
       for (int n = 0; n < NS; n++) {
@@ -448,6 +448,21 @@ march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
 #pragma unroll
     for (int n = 0; n < NS; n++) fxl[n] = fxr[n];
   }
+  __syncthreads();  // the LDS is reused by the block's next tile
+}
+
+// Single-domain launch: a 1-D grid of tiles_y * tiles_x blocks of NCH*R waves.
+// Consecutive blocks of one XCD (blocks are dealt to the 8 XCDs round-robin)
+// take consecutive tiles, so that rows shared by neighbouring tiles meet in
+// one L2.
+template <typename T, int KIND, int NS, int MODE, int NCH, int R, int WPE>
+__global__ void __launch_bounds__(64 * NCH * R) __attribute__((amdgpu_waves_per_eu(WPE)))
+march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
+  __shared__ MarchLds<T, NS, NCH * R, 64 * NCH> lds;
+  const int nb = (int)gridDim.x, bid = (int)blockIdx.x;
+  const int full = nb - nb % 8;
+  const int t = bid < full ? (bid % 8) * (full / 8) + bid / 8 : bid;
+  march_tile<T, KIND, NS, MODE, NCH * R, 64 * NCH, NS>(L, S, NCH, t % tiles_y, t / tiles_y, strip, lds);
 }
 
 }  // namespace tvd

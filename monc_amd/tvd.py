@@ -34,7 +34,7 @@ EXPORTS = (
     "monc_tvd_forward_steps", "monc_tvd_last_error", "monc_tvd_set_tiling",
     "monc_tvd_set_profiling", "monc_tvd_kernel_times", "monc_tvd_weighted_sum",
     "monc_tvd_adjoint_step", "monc_tvd_adjoint_fold", "monc_tvd_tangent_steps", "monc_tvd_march_shapes",
-    "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic",
+    "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic", "monc_tvd_autotune",
 )
 DIAG_TENDENCY, DIAG_ZFACE = 1, 2
 DIAG_3D, DIAG_PROFILE, DIAG_TABS_3D, DIAG_TABS_PROFILE, DIAG_ZFACES = 0, 1, 2, 3, 4
@@ -76,6 +76,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_set_tiling.argtypes = [c_void_p, c_int, c_int, c_int]
     lib.monc_tvd_march_shapes.argtypes = [c_void_p, ctypes.POINTER(c_int), c_int]
     lib.monc_tvd_set_diagnostics.argtypes = [c_void_p, c_int, dp]
+    lib.monc_tvd_autotune.argtypes = [c_void_p, c_double, c_int, ctypes.POINTER(c_int)]
     lib.monc_tvd_get_diagnostic.argtypes = [c_void_p, c_int, c_int, dp]
     lib.monc_tvd_set_profiling.argtypes = [c_void_p, c_int]
     lib.monc_tvd_weighted_sum.argtypes = [c_void_p, c_int, c_int, dp, dp]
@@ -182,8 +183,15 @@ class TvdAdvection:
     def set_tiling(self, kernel: int = KERNEL_MARCH, rows: int = 0, strip: int = 0) -> None:
         """kernel KERNEL_MARCH (default; rows = cell rows per block, 0 the
         default shape), KERNEL_CELL or KERNEL_SPLIT; strip = x columns per
-        block (0: 16)."""
+        block (0: 32)."""
         _check(self._lib.monc_tvd_set_tiling(self._h, kernel, rows, strip))
+
+    def autotune(self, dtm: float, reps: int = 3) -> dict:
+        """Time every compiled march shape and strip per kernel kind on this
+        context's fields and keep the fastest; returns {kind: (rows, strip)}."""
+        out = (ctypes.c_int * 8)()
+        _check(self._lib.monc_tvd_autotune(self._h, dtm, reps, out))
+        return {k: (out[2 * i], out[2 * i + 1]) for i, k in enumerate(("u", "v", "w", "scalars"))}
 
     def march_shapes(self) -> list:
         """Cell rows per block of the march shapes compiled for this column height, default first."""

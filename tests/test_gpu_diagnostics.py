@@ -87,7 +87,7 @@ def test_zface_diagnostics(gpu, tiling, nz):
 
 
 def test_diagnostics_need_recording(gpu):
-    st = ss.make_state(8, 6, 6, 50.0, 300.0, seed=1)
+    st = ss.make_state(8, 6, 6, 50.0, 200.0, seed=1)
     t = tvd.TvdAdvection.initialise(st, precision=tvd.FP64)
     try:
         with pytest.raises(tvd.TvdError):

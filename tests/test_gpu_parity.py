@@ -44,8 +44,9 @@ def test_tiny_grid_stepping_modes(gpu, fs, ms, sc, anelastic, tiling):
 # every compiled march shape (cell rows per block) for column heights that
 # fill 1..4 sub-columns of 64 levels partially and exactly, with strips that
 # do and do not divide nx; the split kernel at the same heights
-MARCH_CASES = [(nz, rows) for nz in (4, 5, 63, 64) for rows in (8, 4)]
-MARCH_CASES += [(nz, rows) for nz in (65, 70, 127, 128, 129, 150, 191, 192) for rows in (4, 2)]
+MARCH_CASES = [(nz, rows) for nz in (4, 5, 63, 64) for rows in (12, 8, 4)]
+MARCH_CASES += [(nz, rows) for nz in (65, 70, 127, 128) for rows in (6, 4, 2)]
+MARCH_CASES += [(nz, rows) for nz in (129, 150, 191, 192) for rows in (4, 2)]
 MARCH_CASES += [(nz, rows) for nz in (193, 200, 256) for rows in (4, 2, 3, 1)]
 
 
@@ -173,3 +174,32 @@ def test_full_size_windows_and_conservation(gpu):
         tot = float((s.sum(axis=(0, 1))[1:] * g.rhon[1:] * g.dz[1:]).sum())
         mag = float((np.abs(s).sum(axis=(0, 1))[1:] * g.rhon[1:] * g.dz[1:]).sum())
         assert abs(tot) <= 1e-12 * mag, (n, tot, mag)
+
+
+@pytest.mark.parametrize("precision", [tvd.FP64, tvd.FP32])
+def test_autotune_keeps_state_and_parity(gpu, precision):
+    """The autotuner times every shape without advancing the state, picks a
+    compiled shape per kernel, and the tuned step still matches the oracle
+    (FP64 at 1e-12; FP32 to the FP32 bar of the precision tests)."""
+    st = ss.make_state(70, 24, 20, 50.0, 1500.0, seed=31, n_q=1, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    t = tvd.TvdAdvection.initialise(st, precision=precision)
+    try:
+        t.halo_fill()
+        before = [t.get_field(n, tvd.PROG) for n in range(5)]
+        chosen = t.autotune(dt, reps=1)
+        shapes = t.march_shapes()
+        assert set(chosen) == {"u", "v", "w", "scalars"}
+        for rows, strip in chosen.values():
+            assert rows in shapes and strip in (8, 16, 32, 64)
+        for n in range(5):
+            np.testing.assert_array_equal(t.get_field(n, tvd.PROG), before[n])
+        t.forward_steps(dt, 3)
+        got = [t.get_field(n, tvd.PROG) for n in range(5)]
+    finally:
+        t.close()
+    ref = oracle.forward_run(st, dt, 3)
+    tol = TOL if precision == tvd.FP64 else 2e-6
+    for n, (a, b) in enumerate(zip(got, ref)):
+        e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        assert e <= tol, (n, e)
