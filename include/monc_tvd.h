@@ -238,6 +238,60 @@ int monc_tvd_adjoint_fold(monc_tvd_ctx *ctx);
  */
 int monc_tvd_tangent_steps(monc_tvd_ctx *ctx, double dtm, int nsteps);
 
+/*
+ * Batched ensembles (no counterpart in the reference, which runs one domain
+ * per MPI job): n_members independent local domains of uneven sizes, each
+ * advecting u, v, w, th and q (field ids 0..4) with forward-Euler steps of its
+ * own dtm, as monc_tvd_forward_steps does for one context.  All members'
+ * fields live in one device pool per variant (the state and the adjoint);
+ * member m's field f starts at element offset(m) + f * field_stride(m) of the
+ * pool, in the (nz, ny+4, nx+4) layout above.  Members need nz <= 256 and
+ * every field under 4 GiB.  metrics holds, member after member, the eight
+ * FP64 vectors rdz, rdzn, dz, dzn, tzc1, tzc2, tzd1, tzd2 of length nz(m)
+ * each (the order of monc_tvd_create).
+ */
+typedef struct monc_tvd_ensemble monc_tvd_ensemble;
+int monc_tvd_ensemble_create(int n_members, const int *nz, const int *ny, const int *nx, const double *cx,
+                             const double *cy, const double *metrics, int precision, monc_tvd_ensemble **out);
+int monc_tvd_ensemble_destroy(monc_tvd_ensemble *ens);
+/* Pool layout: element offset and field stride of a member; member = -1
+ * gives the pool's total element count in *offset. */
+int monc_tvd_ensemble_layout(monc_tvd_ensemble *ens, int member, long long *offset, long long *field_stride);
+/* Host staging of one member's field (FP64 host array (nz, ny+4, nx+4));
+ * variant MONC_TVD_PROG (the current state) or MONC_TVD_ADJOINT. */
+int monc_tvd_ensemble_set_field(monc_tvd_ensemble *ens, int member, int field, int variant, const double *host);
+int monc_tvd_ensemble_get_field(monc_tvd_ensemble *ens, int member, int field, int variant, double *host);
+/* Device copy of a whole pool (the current state or the adjoint) to dst, or
+ * from src into it (exactly one non-null; a buffer of layout(-1) elements of
+ * the ensemble's precision).  Asynchronous. */
+int monc_tvd_ensemble_copy_pool(monc_tvd_ensemble *ens, int variant, void *dst, const void *src);
+/* Per-member dtm (n_members values); synchronises the stream. */
+int monc_tvd_ensemble_set_dt(monc_tvd_ensemble *ens, const double *dtm);
+int monc_tvd_ensemble_set_stream(monc_tvd_ensemble *ens, void *hip_stream);
+int monc_tvd_ensemble_synchronize(monc_tvd_ensemble *ens);
+/* Periodic halo fill of every field of every member (monc_tvd_halo_fill). */
+int monc_tvd_ensemble_halo_fill(monc_tvd_ensemble *ens);
+/* nsteps forward-Euler steps of every member.  Each step is ONE persistent
+ * launch over the march tiles of all members (u, v, w and the (th, q) pair),
+ * largest members first; pairs of steps replay a captured hipGraph unless
+ * graphs are switched off.  Asynchronous. */
+int monc_tvd_ensemble_forward_steps(monc_tvd_ensemble *ens, int nsteps);
+/* Adjoint of one step of every member (monc_tvd_adjoint_step), from the state
+ * in `state` (a device buffer in pool layout; NULL: the current state), on
+ * the adjoint pool. */
+int monc_tvd_ensemble_adjoint_step(monc_tvd_ensemble *ens, const void *state);
+/* Adjoint of the halo fill on the adjoint pool (monc_tvd_adjoint_fold). */
+int monc_tvd_ensemble_adjoint_fold(monc_tvd_ensemble *ens);
+/* Tile shape of the persistent step: waves per block (4, 8 or 12; at least
+ * the 64-level sub-columns of the tallest member), x columns per tile, and
+ * whether step pairs replay a captured graph. */
+int monc_tvd_ensemble_set_tiling(monc_tvd_ensemble *ens, int waves, int strip, int use_graph);
+/* Autotuner of the persistent step: times every compiled shape (waves 4, 8,
+ * 12 x strip 8, 16, 32, 64) over reps steps written to library scratch (the
+ * state is not advanced) and keeps the fastest; chosen (2 ints, may be null)
+ * receives waves and strip. */
+int monc_tvd_ensemble_autotune(monc_tvd_ensemble *ens, int reps, int *chosen);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char *monc_tvd_last_error(void);
 

@@ -14,6 +14,10 @@ with their periodic halo) as a `torch.autograd.Function`:
   bitwise the state the forward pass stepped from, so the adjoint follows the
   branches the forward pass took.
 
+The states of the last segment are kept as the forward pass computes them
+(in the segment buffers the backward pass uses), so the backward pass
+recomputes every segment but the last.
+
 Memory: ceil(n/k) checkpoints plus k segment states, each one state (all
 fields with halo) in the context precision.
 
@@ -55,6 +59,7 @@ class TvdRollout:
         for f in range(self.nfields):
             self.t.bind_field(f, tvd.ADJOINT, self.adj[f])
         self._seg: List[List[torch.Tensor]] = []  # recompute buffers, reused across segments
+        self._seg_tag = None  # the forward pass whose last segment the buffers hold
 
     # helpers -------------------------------------------------------------------
     def _bind_prog(self, fields: Sequence[torch.Tensor]) -> None:
@@ -102,12 +107,22 @@ class RolloutFn(torch.autograd.Function):
         eng.t.halo_fill()
         ckpts = []
         done = 0
+        tag = object()
         while done < nsteps:
-            ckpts.append((done, eng._snapshot()))
             n = min(eng.k, nsteps - done)
-            eng.t.forward_steps(dt, n)
+            if done + n < nsteps:
+                ckpts.append((done, eng._snapshot()))
+                eng.t.forward_steps(dt, n)
+            else:  # last segment: keep every state the backward pass needs
+                seg = eng._seg_buffers(n)
+                for m in range(n):
+                    for b, p in zip(seg[m], eng.prog):
+                        b.copy_(p)
+                    eng.t.forward_steps(dt, 1)
+                ckpts.append((done, None))
+                eng._seg_tag = tag
             done += n
-        ctx.eng, ctx.dt, ctx.nsteps, ctx.ckpts = eng, dt, nsteps, ckpts
+        ctx.eng, ctx.dt, ctx.nsteps, ctx.ckpts, ctx.tag = eng, dt, nsteps, ckpts, tag
         return tuple(eng._snapshot())
 
     @staticmethod
@@ -123,18 +138,22 @@ class RolloutFn(torch.autograd.Function):
         seg = eng._seg_buffers(eng.k)
         for start, ck in reversed(ctx.ckpts):
             n = min(eng.k, ctx.nsteps - start)
-            # recompute X_start .. X_{start+n-1} in the working buffers
-            eng._bind_prog(eng.prog)
-            eng._load(ck)
-            for m in range(n):
-                if m:
-                    eng.t.forward_steps(dt, 1)
-                for s, p in zip(seg[m], eng.prog):
-                    s.copy_(p)
+            if ck is None and eng._seg_tag is not ctx.tag:
+                raise RuntimeError("another rollout of this engine reused its segment buffers before this backward")
+            if ck is not None:
+                # recompute X_start .. X_{start+n-1} in the working buffers
+                eng._bind_prog(eng.prog)
+                eng._load(ck)
+                for m in range(n):
+                    if m:
+                        eng.t.forward_steps(dt, 1)
+                    for s, p in zip(seg[m], eng.prog):
+                        s.copy_(p)
             # adjoint sweep over the segment, newest state first
             for m in reversed(range(n)):
                 eng._bind_prog(seg[m])
                 eng.t.adjoint_step(dt)
         eng._bind_prog(eng.prog)
+        eng._seg_tag = None
         ctx.ckpts = None
         return (None, None, None) + tuple(a.clone() for a in eng.adj)

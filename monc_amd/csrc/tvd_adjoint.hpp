@@ -520,12 +520,13 @@ struct AdjArgs {
 
 // One thread per (k, j, i) over levels 1..nz, rows 2..ny+2, columns 2..nx+2:
 // the owner of the right x face, north y face and top z face of that point.
+// Block (bx, by, bz): 64 levels x 4 rows of one column.
 template <typename T, int KIND>
-__global__ void __launch_bounds__(256) adjoint_kernel(LaunchArgs<T> L, AdjArgs<T> A) {
+__device__ __forceinline__ void adjoint_block(const LaunchArgs<T>& L, const AdjArgs<T>& A, int bx, int by, int bz) {
   const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;
-  const int k = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 2 + blockIdx.y * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int i = 2 + blockIdx.z;
+  const int k = 1 + bx * 64 + (int)threadIdx.x;
+  const int j = 2 + by * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int i = 2 + bz;
   if (j > ny + 2 || k > nz) return;
   const Flows<T, KIND> fl(L);
   constexpr int DK = Flows<T, KIND>::DK, DJ = Flows<T, KIND>::DJ, DI = Flows<T, KIND>::DI;
@@ -574,13 +575,18 @@ __global__ void __launch_bounds__(256) adjoint_kernel(LaunchArgs<T> L, AdjArgs<T
   }
 }
 
+template <typename T, int KIND>
+__global__ void __launch_bounds__(256) adjoint_kernel(LaunchArgs<T> L, AdjArgs<T> A) {
+  adjoint_block<T, KIND>(L, A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+}
+
 // Adjoint of the periodic halo fill, in two passes so every target is
 // written by one thread: rows first (over all columns, corners included),
 // then columns over the interior rows.  Halo entries are zeroed.
 template <typename T>
-__global__ void __launch_bounds__(256) fold_rows_kernel(T* f, Dims d) {
-  const int k = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int h = blockIdx.y * 4 + threadIdx.y;  // 0..4*(nx+4)-1
+__device__ __forceinline__ void fold_rows_block(T* f, const Dims& d, int bx, int by) {
+  const int k = 1 + bx * 64 + (int)threadIdx.x;
+  const int h = by * 4 + (int)threadIdx.y;  // 0..4*(nx+4)-1
   if (k > d.nz || h >= 4 * (d.nx + 4)) return;
   const int r = h / (d.nx + 4), i = 1 + h % (d.nx + 4);
   const int j = r < 2 ? 1 + r : d.ny + 1 + r;
@@ -590,9 +596,9 @@ __global__ void __launch_bounds__(256) fold_rows_kernel(T* f, Dims d) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) fold_cols_kernel(T* f, Dims d) {
-  const int k = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int h = blockIdx.y * 4 + threadIdx.y;  // 0..4*ny-1
+__device__ __forceinline__ void fold_cols_block(T* f, const Dims& d, int bx, int by) {
+  const int k = 1 + bx * 64 + (int)threadIdx.x;
+  const int h = by * 4 + (int)threadIdx.y;  // 0..4*ny-1
   if (k > d.nz || h >= 4 * d.ny) return;
   const int c = h / d.ny, j = 3 + h % d.ny;
   const int i = c < 2 ? 1 + c : d.nx + 1 + c;
@@ -601,18 +607,15 @@ __global__ void __launch_bounds__(256) fold_cols_kernel(T* f, Dims d) {
   f[src] = T(0.0);
 }
 
-// Tangent-linear support: pack (value, tangent) pairs and split them again.
-__global__ void __launch_bounds__(256) pack_dual_kernel(Dual* out, const double* v, const double* dv, int64_t n) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t < n) out[t] = Dual(v[t], dv[t]);
+template <typename T>
+__global__ void __launch_bounds__(256) fold_rows_kernel(T* f, Dims d) {
+  fold_rows_block<T>(f, d, (int)blockIdx.x, (int)blockIdx.y);
 }
-__global__ void __launch_bounds__(256) unpack_dual_kernel(double* v, double* dv, const Dual* in, int64_t n) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t < n) {
-    const Dual x = in[t];
-    v[t] = x.v;
-    dv[t] = x.d;
-  }
+
+template <typename T>
+__global__ void __launch_bounds__(256) fold_cols_kernel(T* f, Dims d) {
+  fold_cols_block<T>(f, d, (int)blockIdx.x, (int)blockIdx.y);
 }
+
 
 }  // namespace tvd

@@ -264,10 +264,11 @@ __global__ void __launch_bounds__(64 * NCH * R) split_kernel(LaunchArgs<T> L, Fi
 // Periodic fill of the depth-2 x/y halo of one field, corners included.  One
 // thread per (k, halo column); the halo columns are the 4 rows j in {1,2,ny+3,
 // ny+4} over every i, then the 4 columns i in {1,2,nx+3,nx+4} over interior j.
+// Block (bx, by) of the fill: 64 levels x 4 halo columns.
 template <typename T>
-__global__ void __launch_bounds__(256) halo_kernel(T* f, Dims d) {
-  const int k = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int64_t h = (int64_t)blockIdx.y * 4 + threadIdx.y;
+__device__ __forceinline__ void halo_block(T* f, const Dims& d, int bx, int64_t by) {
+  const int k = 1 + bx * 64 + (int)threadIdx.x;
+  const int64_t h = by * 4 + threadIdx.y;
   const int nx4 = d.nx + 4;
   const int64_t nrow = 4 * (int64_t)nx4;
   const int64_t nh = nrow + 4 * (int64_t)d.ny;
@@ -284,6 +285,11 @@ __global__ void __launch_bounds__(256) halo_kernel(T* f, Dims d) {
     i = c < 2 ? 1 + c : d.nx + 1 + c;
   }
   f[at(d, k, j, i)] = f[at(d, k, wrap(j, d.ny), wrap(i, d.nx))];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) halo_kernel(T* f, Dims d) {
+  halo_block<T>(f, d, (int)blockIdx.x, (int64_t)blockIdx.y);
 }
 
 // FP64 weighted sum over the interior, sum_k w(k) sum_{j,i} f(k,j,i): one

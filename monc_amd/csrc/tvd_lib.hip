@@ -13,16 +13,18 @@
 
 #include "../../include/monc_tvd.h"
 #include "tvd_adjoint.hpp"
+#include "tvd_diag.hpp"
+#include "tvd_host.hpp"
 #include "tvd_kernels.hpp"
 #include "tvd_march.hpp"
-#include "tvd_diag.hpp"
 
 using namespace tvd;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
+namespace tvd {
 int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -32,15 +34,26 @@ int fail(int code, const char* fmt, ...) {
   g_err = buf;
   return code;
 }
+}  // namespace tvd
 
-#define HIP_TRY(call)                                                                   \
-  do {                                                                                  \
-    hipError_t e_ = (call);                                                             \
-    if (e_ != hipSuccess) return fail(MONC_TVD_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
-  } while (0)
+namespace {
 
-enum { MET_RDZ, MET_RDZN, MET_DZ, MET_DZN, MET_TZC1, MET_TZC2, MET_TZD1, MET_TZD2, MET_N };
 constexpr int NVAR = 5;  // PROG, ZFIELD, SOURCE, ADJOINT, TANGENT
+
+// Tangent-linear support: pack (value, tangent) pairs and split them again.
+__global__ void __launch_bounds__(256) pack_dual_kernel(Dual* out, const double* v, const double* dv, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) out[t] = Dual(v[t], dv[t]);
+}
+__global__ void __launch_bounds__(256) unpack_dual_kernel(double* v, double* dv, const Dual* in, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) {
+    const Dual x = in[t];
+    v[t] = x.v;
+    dv[t] = x.d;
+  }
+}
+
 
 }  // namespace
 
@@ -133,25 +146,7 @@ Dims dims_of(const monc_tvd_ctx* c) {
   return d;
 }
 
-VMet vmet_of(const monc_tvd_ctx* c, bool is_w) {
-  const double* m = c->dmet;
-  const int nz = c->nz;
-  VMet v;
-  if (is_w) {  // tvdadvection.F90:881-894, :898-904
-    v.rdz = m + MET_RDZN * nz;
-    v.rdzn = m + MET_RDZ * nz;
-    v.dzn = m + MET_DZ * nz;
-    v.t1 = m + MET_TZD1 * nz;
-    v.t2 = m + MET_TZD2 * nz;
-  } else {  // :739-757
-    v.rdz = m + MET_RDZ * nz;
-    v.rdzn = m + MET_RDZN * nz;
-    v.dzn = m + MET_DZN * nz;
-    v.t1 = m + MET_TZC1 * nz;
-    v.t2 = m + MET_TZC2 * nz;
-  }
-  return v;
-}
+VMet vmet_of(const monc_tvd_ctx* c, bool is_w) { return vmet_from(c->dmet, c->nz, is_w); }
 
 template <typename T>
 LaunchArgs<T> launch_args(monc_tvd_ctx* c, double dtm) {

@@ -249,3 +249,27 @@ def cfl_dt(state: State, cfl: float = 0.3) -> float:
     rdzn_next[:-1] = state.grid.rdzn[1:]
     wterm = float(np.max(wmax_k * rdzn_next))
     return cfl / (u * state.cx + v * state.cy + wterm)
+
+
+ENSEMBLE_NXY = (64, 96, 128, 192, 256)
+ENSEMBLE_NZ = (64, 96, 128)
+
+
+def ensemble_states(n_members: int = 48, seed: int = 0, dxy: float = 50.0, ztop: float = 2000.0,
+                    nxy=ENSEMBLE_NXY, nzs=ENSEMBLE_NZ) -> List[State]:
+    """Members of the batched-ensemble workload: sizes nx, ny drawn from `nxy`
+    and nz from `nzs`, mean u ~ U(0, 10) m/s and lapse rate ~ U(0.001, 0.006)
+    K/m, drawn in that order per member from default_rng(seed); member m's
+    fields are make_state(..., seed=seed+m) (dx = dy = dxy, z tanh-stretched to
+    ztop, one q field)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n_members):
+        nx = int(rng.choice(nxy))
+        ny = int(rng.choice(nxy))
+        nz = int(rng.choice(nzs))
+        u_mean = float(rng.uniform(0.0, 10.0))
+        lapse = float(rng.uniform(0.001, 0.006))
+        out.append(make_state(nz, ny, nx, dxy, ztop, seed=seed + m, n_q=1, u_mean=u_mean, lapse=lapse,
+                              with_zfields=False))
+    return out

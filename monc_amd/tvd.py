@@ -35,6 +35,11 @@ EXPORTS = (
     "monc_tvd_set_profiling", "monc_tvd_kernel_times", "monc_tvd_weighted_sum",
     "monc_tvd_adjoint_step", "monc_tvd_adjoint_fold", "monc_tvd_tangent_steps", "monc_tvd_march_shapes",
     "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic", "monc_tvd_autotune",
+    "monc_tvd_ensemble_create", "monc_tvd_ensemble_destroy", "monc_tvd_ensemble_layout",
+    "monc_tvd_ensemble_set_field", "monc_tvd_ensemble_get_field", "monc_tvd_ensemble_copy_pool",
+    "monc_tvd_ensemble_set_dt", "monc_tvd_ensemble_set_stream", "monc_tvd_ensemble_synchronize",
+    "monc_tvd_ensemble_halo_fill", "monc_tvd_ensemble_forward_steps", "monc_tvd_ensemble_adjoint_step",
+    "monc_tvd_ensemble_adjoint_fold", "monc_tvd_ensemble_set_tiling", "monc_tvd_ensemble_autotune",
 )
 DIAG_TENDENCY, DIAG_ZFACE = 1, 2
 DIAG_3D, DIAG_PROFILE, DIAG_TABS_3D, DIAG_TABS_PROFILE, DIAG_ZFACES = 0, 1, 2, 3, 4
@@ -89,6 +94,23 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_adjoint_step.argtypes = [c_void_p, c_double]
     lib.monc_tvd_adjoint_fold.argtypes = [c_void_p]
     lib.monc_tvd_tangent_steps.argtypes = [c_void_p, c_double, c_int]
+    ip = ctypes.POINTER(c_int)
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    lib.monc_tvd_ensemble_create.argtypes = [c_int, ip, ip, ip, dp, dp, dp, c_int, ctypes.POINTER(c_void_p)]
+    lib.monc_tvd_ensemble_destroy.argtypes = [c_void_p]
+    lib.monc_tvd_ensemble_layout.argtypes = [c_void_p, c_int, llp, llp]
+    lib.monc_tvd_ensemble_set_field.argtypes = [c_void_p, c_int, c_int, c_int, dp]
+    lib.monc_tvd_ensemble_get_field.argtypes = [c_void_p, c_int, c_int, c_int, dp]
+    lib.monc_tvd_ensemble_copy_pool.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
+    lib.monc_tvd_ensemble_set_dt.argtypes = [c_void_p, dp]
+    lib.monc_tvd_ensemble_set_stream.argtypes = [c_void_p, c_void_p]
+    lib.monc_tvd_ensemble_synchronize.argtypes = [c_void_p]
+    lib.monc_tvd_ensemble_halo_fill.argtypes = [c_void_p]
+    lib.monc_tvd_ensemble_forward_steps.argtypes = [c_void_p, c_int]
+    lib.monc_tvd_ensemble_adjoint_step.argtypes = [c_void_p, c_void_p]
+    lib.monc_tvd_ensemble_adjoint_fold.argtypes = [c_void_p]
+    lib.monc_tvd_ensemble_set_tiling.argtypes = [c_void_p, c_int, c_int, c_int]
+    lib.monc_tvd_ensemble_autotune.argtypes = [c_void_p, c_int, ip]
     lib.monc_tvd_last_error.argtypes = []
     lib.monc_tvd_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:

@@ -64,3 +64,48 @@ def test_null_context_is_an_argument_error():
     assert "null context" in lib.monc_tvd_last_error().decode()
     assert lib.monc_tvd_tendencies(None, 1.0, 1, 1, 1) == tvd.ERR_ARG
     assert lib.monc_tvd_destroy(None) == tvd.OK
+
+
+def _ens_create(sizes, prec=1, cx=0.02):
+    lib = tvd.load_library()
+    n = len(sizes)
+    nz = (ctypes.c_int * n)(*[s[0] for s in sizes])
+    ny = (ctypes.c_int * n)(*[s[1] for s in sizes])
+    nx = (ctypes.c_int * n)(*[s[2] for s in sizes])
+    c = np.full(max(n, 1), cx)
+    met = np.ones(8 * max(sum(s[0] for s in sizes), 1))
+    dp = ctypes.POINTER(ctypes.c_double)
+    h = ctypes.c_void_p()
+    rc = lib.monc_tvd_ensemble_create(n, nz, ny, nx, c.ctypes.data_as(dp), c.ctypes.data_as(dp),
+                                      met.ctypes.data_as(dp), prec, ctypes.byref(h))
+    return rc, lib.monc_tvd_last_error().decode(), h
+
+
+@pytest.mark.parametrize("kw,needle", [
+    (dict(sizes=[]), "n_members"),
+    (dict(sizes=[(16, 8, 8), (3, 8, 8)]), "too small"),
+    (dict(sizes=[(300, 8, 8)]), "above the ensemble"),
+    (dict(sizes=[(16, 8, 8)], prec=5), "precision"),
+    (dict(sizes=[(16, 8, 8)], cx=-1.0), "positive"),
+])
+def test_ensemble_create_rejects_invalid_arguments(kw, needle):
+    rc, msg, h = _ens_create(**kw)
+    assert rc == tvd.ERR_ARG
+    assert needle in msg
+    assert not h.value
+
+
+def test_null_ensemble_is_an_argument_error():
+    lib = tvd.load_library()
+    assert lib.monc_tvd_ensemble_forward_steps(None, 2) == tvd.ERR_ARG
+    assert "null ensemble" in lib.monc_tvd_last_error().decode()
+    assert lib.monc_tvd_ensemble_destroy(None) == tvd.OK
+
+
+def test_ensemble_workload_sizes():
+    from monc_amd import setup_state as ss
+    sts = ss.ensemble_states(6, seed=0, nxy=(8, 12), nzs=(16, 20), ztop=400.0)
+    assert len(sts) == 6
+    for st in sts:
+        assert st.nx in (8, 12) and st.ny in (8, 12) and st.nz in (16, 20)
+        assert len(st.fields) == 5
