@@ -200,6 +200,26 @@ int monc_tvd_tendencies(monc_tvd_ctx *ctx, double dtm, int field_stepping, int m
  */
 int monc_tvd_forward_steps(monc_tvd_ctx *ctx, double dtm, int nsteps);
 
+/*
+ * Device-resident MONC timesteps of an advection-only run, nsteps times: per
+ * step the sources are cleared (clearsourceterms), the TVD tendencies added
+ * with momentum and scalar stepping CENTRED (steppingdirection.F90:27-32:
+ * ultflx takes dt = 2*dtm) and field stepping FORWARD on the first step when
+ * first_forward != 0, CENTRED otherwise (:41-48), then stepfields
+ * (stepfields.F90:314-351, 522-627: u and v shifted by ugal and vgal, th and
+ * q time-smoothed with c1 = 1 - 2*tsmth, c2 = tsmth when centred) and
+ * swap_smooth (swapsmooth.F90:44-57: the classic swap after a forward step,
+ * the Robert filter after a centred one) on every field, with the periodic
+ * halo of the new fields and z-fields written by the same kernels.  Replaces
+ * the clearsourceterms / tvd_advection / stepfields / swap_smooth sequence of
+ * one timestep (global_config:86-91) when no other component adds sources.
+ * Needs the context created with all advect flags, the prognostics and, unless
+ * first_forward, the z-fields (time level t-1) with filled halos.
+ * Asynchronous on the context stream.
+ */
+int monc_tvd_leapfrog_steps(monc_tvd_ctx *ctx, double dtm, double tsmth, double ugal, double vgal, int nsteps,
+                            int first_forward);
+
 /* FP64 conservation sum of one field over the interior columns:
  * out = sum_k weights[k-1] * sum_{j,i} f(k,j,i), accumulated in FP64 on the
  * device whatever the context precision (weights: host, length nz; e.g.

@@ -6,7 +6,11 @@
 namespace tvd {
 
 enum FieldKind { KIND_U = 0, KIND_V = 1, KIND_W = 2, KIND_SCALAR = 3 };
-enum Mode { MODE_ACCUM = 0, MODE_STEP = 1 };
+// MODE_ACCUM: s += tendency (the drop-in timestep callback); MODE_STEP: the
+// fused forward-Euler step; MODE_LEAP: the fused MONC step of an
+// advection-only run (stepfields + swap_smooth: leapfrog with the Robert
+// filter, or the forward first step with the classic swap).
+enum Mode { MODE_ACCUM = 0, MODE_STEP = 1, MODE_LEAP = 2 };
 
 // Largest number of fields one launch advects (u, v, w, th and up to 12
 // scalars); the host splits longer scalar lists over several launches.
@@ -21,6 +25,9 @@ struct FieldSet {
   T* out[MAX_FIELDS];      // MODE_ACCUM: source field; MODE_STEP: next state
   T dt[MAX_FIELDS];        // ultflx dt of the field
   T* zf[MAX_FIELDS];       // MODE_ACCUM: flux_z copy of the field (tvd_dgs_terms), or null
+  const T* z[MAX_FIELDS];  // MODE_LEAP: current z-field (time level t-1)
+  T* out2[MAX_FIELDS];     // MODE_LEAP: next z-field
+  T gal[MAX_FIELDS];       // MODE_LEAP: galilean shift of the field (ugal, vgal; 0 otherwise)
 };
 
 template <typename T>
@@ -32,7 +39,48 @@ struct LaunchArgs {
   VMet mc;  // scalars, u, v
   VMet mw;  // w (swapped metric roles)
   T dtm, cx, cy;
+  // MODE_LEAP: Robert filter coefficient tsmth, c1 = 1 - 2 tsmth, 2 dtm (all
+  // formed in FP64), and whether this is the forward first step
+  T tsmth, c1, dtm2;
+  int forward;
 };
+
+// stepfields.F90:597-627 (with perform_timesmooth_for_field, :522-533, for
+// the scalars when centred) followed by swap_smooth (swapsmooth.F90) for one
+// point: the current field fc and z-field zc, the tendency s (0 where the
+// sources stay clear) -> the next field fn and z-field zn.
+//   forward (first step):  e = gal + fc (u, v) or fc; zs = e + dtm s;
+//                          classic swap: x = e + zs, fn = x*1 + e*(-1), zn = x - fn;
+//   centred, u and v:      sav = zc + gal, e = gal + fc, zs = gal + zc + 2dtm s,
+//                          fn = zs, zn = c1 e + tsmth (zs + sav - gal);
+//   centred, w:            sav = zc, zs = zc + 2dtm s, fn = zs, zn = c1 fc + tsmth (zs + sav);
+//   centred, scalars:      sm = c1 fc + tsmth zc, zs = zc + 2dtm s, fn = zs, zn = sm + tsmth zs.
+template <typename T, int KIND>
+__device__ __forceinline__ void leap_update(const LaunchArgs<T>& L, T gal, T fc, T zc, T s, T& fn, T& zn) {
+  constexpr bool FLOW = KIND == KIND_U || KIND == KIND_V;
+  if (L.forward) {
+    const T e = FLOW ? gal + fc : fc;
+    const T zs = e + L.dtm * s;
+    const T x = e + zs;
+    fn = x - e;
+    zn = x - fn;
+  } else if constexpr (KIND == KIND_SCALAR) {
+    const T sm = L.c1 * fc + L.tsmth * zc;
+    const T zs = zc + L.dtm2 * s;
+    fn = zs;
+    zn = sm + L.tsmth * zs;
+  } else if constexpr (KIND == KIND_W) {
+    const T zs = zc + L.dtm2 * s;
+    fn = zs;
+    zn = L.c1 * fc + L.tsmth * (zs + zc);
+  } else {
+    const T sav = zc + gal;
+    const T e = gal + fc;
+    const T zs = (gal + zc) + L.dtm2 * s;
+    fn = zs;
+    zn = L.c1 * e + L.tsmth * ((zs + sav) - gal);
+  }
+}
 
 // Wrapped index of a halo position for a depth-2 periodic halo:
 // 1 -> n+1, 2 -> n+2, n+3 -> 3, n+4 -> 4 (halocommunication.F90:1121-1168).

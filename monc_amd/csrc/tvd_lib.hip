@@ -67,7 +67,9 @@ struct monc_tvd_ctx {
   std::vector<void*> buf;      // [field*NVAR + variant]
   std::vector<char> owned;     // library-allocated?
   std::vector<void*> bound;    // caller pointer of a bound PROG buffer (restored after stepping)
+  std::vector<void*> bound_z;  // caller pointer of a bound ZFIELD buffer (restored after stepping)
   std::vector<void*> spare;    // library-owned second buffer of each prognostic (stepping)
+  std::vector<void*> spare_z;  // library-owned second buffer of each z-field (leapfrog stepping)
   std::vector<void*> adj_tmp;  // copy of the incoming adjoint of each field (adjoint step)
   std::vector<void*> dual[2];  // packed (value, tangent) state, double-buffered (tangent steps)
   hipStream_t stream;
@@ -160,6 +162,10 @@ LaunchArgs<T> launch_args(monc_tvd_ctx* c, double dtm) {
   L.dtm = (T)dtm;
   L.cx = (T)c->cx;
   L.cy = (T)c->cy;
+  L.tsmth = T(0);
+  L.c1 = T(1);
+  L.dtm2 = (T)(2.0 * dtm);
+  L.forward = 0;
   return L;
 }
 
@@ -224,28 +230,37 @@ bool march_rows_ok(int nch, int rows) {
 template <typename T, int KIND, int NS, int MODE>
 void launch_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int nch, int kernel, int rows,
                   int strip) {
-  if (kernel == MONC_TVD_KERNEL_SPLIT) {
+  if constexpr (MODE == MODE_LEAP) {  // compiled for the default shapes only
+    switch (nch * 100 + rows) {
+      case 108: launch_march<T, KIND, NS, MODE, 1, 8>(c, L, S, strip); break;
+      case 204: launch_march<T, KIND, NS, MODE, 2, 4>(c, L, S, strip); break;
+      case 202: launch_march<T, KIND, NS, MODE, 2, 2>(c, L, S, strip); break;
+      case 304: launch_march<T, KIND, NS, MODE, 3, 4>(c, L, S, strip); break;
+      case 302: launch_march<T, KIND, NS, MODE, 3, 2>(c, L, S, strip); break;
+      default: launch_march<T, KIND, NS, MODE, 4, 3>(c, L, S, strip); break;
+    }
+  } else if (kernel == MONC_TVD_KERNEL_SPLIT) {
     switch (nch) {
       case 1: launch_split<T, KIND, NS, MODE, 1>(c, L, S, strip); break;
       case 2: launch_split<T, KIND, NS, MODE, 2>(c, L, S, strip); break;
       case 3: launch_split<T, KIND, NS, MODE, 3>(c, L, S, strip); break;
       default: launch_split<T, KIND, NS, MODE, 4>(c, L, S, strip); break;
     }
-    return;
-  }
-  switch (nch * 100 + rows) {
-    case 112: launch_march<T, KIND, NS, MODE, 1, 12>(c, L, S, strip); break;
-    case 108: launch_march<T, KIND, NS, MODE, 1, 8>(c, L, S, strip); break;
-    case 104: launch_march<T, KIND, NS, MODE, 1, 4>(c, L, S, strip); break;
-    case 206: launch_march<T, KIND, NS, MODE, 2, 6>(c, L, S, strip); break;
-    case 204: launch_march<T, KIND, NS, MODE, 2, 4>(c, L, S, strip); break;
-    case 202: launch_march<T, KIND, NS, MODE, 2, 2>(c, L, S, strip); break;
-    case 304: launch_march<T, KIND, NS, MODE, 3, 4>(c, L, S, strip); break;
-    case 302: launch_march<T, KIND, NS, MODE, 3, 2>(c, L, S, strip); break;
-    case 404: launch_march<T, KIND, NS, MODE, 4, 4>(c, L, S, strip); break;
-    case 402: launch_march<T, KIND, NS, MODE, 4, 2>(c, L, S, strip); break;
-    case 403: launch_march<T, KIND, NS, MODE, 4, 3>(c, L, S, strip); break;
-    case 401: launch_march<T, KIND, NS, MODE, 4, 1>(c, L, S, strip); break;
+  } else {
+    switch (nch * 100 + rows) {
+      case 112: launch_march<T, KIND, NS, MODE, 1, 12>(c, L, S, strip); break;
+      case 108: launch_march<T, KIND, NS, MODE, 1, 8>(c, L, S, strip); break;
+      case 104: launch_march<T, KIND, NS, MODE, 1, 4>(c, L, S, strip); break;
+      case 206: launch_march<T, KIND, NS, MODE, 2, 6>(c, L, S, strip); break;
+      case 204: launch_march<T, KIND, NS, MODE, 2, 4>(c, L, S, strip); break;
+      case 202: launch_march<T, KIND, NS, MODE, 2, 2>(c, L, S, strip); break;
+      case 304: launch_march<T, KIND, NS, MODE, 3, 4>(c, L, S, strip); break;
+      case 302: launch_march<T, KIND, NS, MODE, 3, 2>(c, L, S, strip); break;
+      case 404: launch_march<T, KIND, NS, MODE, 4, 4>(c, L, S, strip); break;
+      case 402: launch_march<T, KIND, NS, MODE, 4, 2>(c, L, S, strip); break;
+      case 403: launch_march<T, KIND, NS, MODE, 4, 3>(c, L, S, strip); break;
+      case 401: launch_march<T, KIND, NS, MODE, 4, 1>(c, L, S, strip); break;
+    }
   }
 }
 
@@ -255,7 +270,11 @@ void launch_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
 template <typename T, int MODE>
 int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
   const int nch = (c->nz + 63) / 64;
-  if (c->kernel == MONC_TVD_KERNEL_CELL || nch > 4 || !c->buffer_ok) {
+  const bool march_fits = nch <= 4 && c->buffer_ok;
+  if constexpr (MODE == MODE_LEAP) {
+    if (!march_fits)
+      return fail(MONC_TVD_ERR_ARG, "leapfrog stepping needs columns of at most 256 levels and fields under 4 GiB");
+  } else if (c->kernel == MONC_TVD_KERNEL_CELL || !march_fits) {
     const int nk = MODE == MODE_STEP ? c->nz : c->nz - 1;
     dim3 block(64, 4, 1);
     dim3 grid((nk + 63) / 64, (c->ny + 3) / 4, c->nx);
@@ -275,20 +294,26 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
       G.out[m] = S.out[n + m];
       G.dt[m] = S.dt[n + m];
       G.zf[m] = S.zf[n + m];
+      G.z[m] = S.z[n + m];
+      G.out2[m] = S.out2[n + m];
+      G.gal[m] = S.gal[n + m];
     }
     if (c->kernel == MONC_TVD_KERNEL_SPLIT && G.zf[0])
       return fail(MONC_TVD_ERR_ARG, "z-face diagnostics need the march or cell kernel, not the split kernel");
     const int kk = pair || S.kind[n] == KIND_SCALAR ? 3 : S.kind[n];
-    const int rows = c->rows[kk] > 0 ? c->rows[kk] : default_rows(c->precision, nch);
-    const int strip = c->strip[kk] > 0 ? c->strip[kk] : default_strip(c->ny, c->nx, rows);
+    // the leapfrog step runs the march kernel in its default shape
+    const bool leap = MODE == MODE_LEAP;
+    const int kernel = leap ? MONC_TVD_KERNEL_MARCH : c->kernel;
+    const int rows = c->rows[kk] > 0 && !leap ? c->rows[kk] : default_rows(c->precision, nch);
+    const int strip = c->strip[kk] > 0 && !leap ? c->strip[kk] : default_strip(c->ny, c->nx, rows);
     LaunchTimer timer(c, kk);
     switch (S.kind[n]) {
-      case KIND_U: launch_shape<T, KIND_U, 1, MODE>(c, L, G, nch, c->kernel, rows, strip); break;
-      case KIND_V: launch_shape<T, KIND_V, 1, MODE>(c, L, G, nch, c->kernel, rows, strip); break;
-      case KIND_W: launch_shape<T, KIND_W, 1, MODE>(c, L, G, nch, c->kernel, rows, strip); break;
+      case KIND_U: launch_shape<T, KIND_U, 1, MODE>(c, L, G, nch, kernel, rows, strip); break;
+      case KIND_V: launch_shape<T, KIND_V, 1, MODE>(c, L, G, nch, kernel, rows, strip); break;
+      case KIND_W: launch_shape<T, KIND_W, 1, MODE>(c, L, G, nch, kernel, rows, strip); break;
       default:
-        if (pair) launch_shape<T, KIND_SCALAR, 2, MODE>(c, L, G, nch, c->kernel, rows, strip);
-        else launch_shape<T, KIND_SCALAR, 1, MODE>(c, L, G, nch, c->kernel, rows, strip);
+        if (pair) launch_shape<T, KIND_SCALAR, 2, MODE>(c, L, G, nch, kernel, rows, strip);
+        else launch_shape<T, KIND_SCALAR, 1, MODE>(c, L, G, nch, kernel, rows, strip);
     }
     HIP_TRY(hipGetLastError());
     n += G.nf;
@@ -456,6 +481,67 @@ int forward_steps_impl(monc_tvd_ctx* c, double dtm, int nsteps) {
       HIP_TRY(hipMemcpyAsync(b, slot(c, fl, MONC_TVD_PROG), c->n_elem * c->elem_size, hipMemcpyDeviceToDevice,
                              c->stream));
       std::swap(slot(c, fl, MONC_TVD_PROG), c->spare[fl]);
+    }
+  }
+  return MONC_TVD_OK;
+}
+
+// The MONC timestep of an advection-only run, resident: per step the TVD
+// tendencies from cleared sources (momentum and scalar stepping CENTRED, so
+// ultflx takes dt = 2 dtm, steppingdirection.F90:27-32; the advected field is
+// the prognostic on a forward step and the z-field otherwise,
+// tvdadvection.F90:739-749), then stepfields and swap_smooth fused into the
+// same pass (leap_update).  New fields and z-fields go to spare buffers with
+// their periodic halo images, and the roles swap after every step.
+template <typename T>
+int leapfrog_impl(monc_tvd_ctx* c, double dtm, double tsmth, double ugal, double vgal, int nsteps, int first_forward) {
+  int rc;
+  if (c->advect != (MONC_TVD_ADVECT_FLOW | MONC_TVD_ADVECT_THETA | MONC_TVD_ADVECT_Q))
+    return fail(MONC_TVD_ERR_ARG, "leapfrog stepping steps every field: create the context with all advect flags");
+  std::vector<int> fields = advected_fields(c);
+  for (int fl : fields) {
+    if ((rc = need(c, fl, MONC_TVD_PROG))) return rc;
+    if (!first_forward && (rc = need(c, fl, MONC_TVD_ZFIELD))) return rc;
+    if ((rc = ensure(c, fl, MONC_TVD_ZFIELD))) return rc;
+    if (!c->spare[fl]) HIP_TRY(hipMalloc(&c->spare[fl], c->n_elem * c->elem_size));
+    if (!c->spare_z[fl]) HIP_TRY(hipMalloc(&c->spare_z[fl], c->n_elem * c->elem_size));
+  }
+  for (int s = 0; s < nsteps; s++) {
+    const bool forward = first_forward && s == 0;
+    LaunchArgs<T> L = launch_args<T>(c, dtm);
+    L.tsmth = (T)tsmth;
+    L.c1 = (T)(1.0 - 2.0 * tsmth);
+    L.forward = forward ? 1 : 0;
+    for (size_t first = 0; first < fields.size(); first += MAX_FIELDS) {
+      FieldSet<T> S{};
+      S.nf = (int)std::min(fields.size() - first, (size_t)MAX_FIELDS);
+      for (int n = 0; n < S.nf; n++) {
+        const int fl = fields[first + n];
+        S.kind[n] = kind_of(fl);
+        S.f[n] = (const T*)slot(c, fl, MONC_TVD_PROG);
+        S.z[n] = (const T*)slot(c, fl, MONC_TVD_ZFIELD);
+        S.F[n] = forward ? S.f[n] : S.z[n];
+        S.out[n] = (T*)c->spare[fl];
+        S.out2[n] = (T*)c->spare_z[fl];
+        S.dt[n] = (T)(2.0 * dtm);
+        S.gal[n] = (T)(fl == MONC_TVD_U ? ugal : fl == MONC_TVD_V ? vgal : 0.0);
+      }
+      if ((rc = launch_set<T, MODE_LEAP>(c, L, S))) return rc;
+    }
+    for (int fl : fields) {
+      std::swap(slot(c, fl, MONC_TVD_PROG), c->spare[fl]);
+      std::swap(slot(c, fl, MONC_TVD_ZFIELD), c->spare_z[fl]);
+    }
+  }
+  // caller-bound buffers keep holding the state after the call
+  for (int fl : fields) {
+    for (int v : {MONC_TVD_PROG, MONC_TVD_ZFIELD}) {
+      void* b = v == MONC_TVD_PROG ? c->bound[fl] : c->bound_z[fl];
+      void*& sp = v == MONC_TVD_PROG ? c->spare[fl] : c->spare_z[fl];
+      if (b && slot(c, fl, v) != b) {
+        HIP_TRY(hipMemcpyAsync(b, slot(c, fl, v), c->n_elem * c->elem_size, hipMemcpyDeviceToDevice, c->stream));
+        std::swap(slot(c, fl, v), sp);
+      }
     }
   }
   return MONC_TVD_OK;
@@ -699,7 +785,9 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags,
   c->buf.assign((size_t)c->nfields * NVAR, nullptr);
   c->owned.assign((size_t)c->nfields * NVAR, 0);
   c->bound.assign((size_t)c->nfields, nullptr);
+  c->bound_z.assign((size_t)c->nfields, nullptr);
   c->spare.assign((size_t)c->nfields, nullptr);
+  c->spare_z.assign((size_t)c->nfields, nullptr);
   c->adj_tmp.assign((size_t)c->nfields, nullptr);
   c->dual[0].assign((size_t)c->nfields, nullptr);
   c->dual[1].assign((size_t)c->nfields, nullptr);
@@ -754,10 +842,15 @@ int monc_tvd_destroy(monc_tvd_ctx* c) {
       if (p != c->bound[field]) (void)hipFree(p);
       continue;
     }
+    if (variant == MONC_TVD_ZFIELD && c->bound_z[field]) {
+      if (p != c->bound_z[field]) (void)hipFree(p);
+      continue;
+    }
     if (c->owned[n]) (void)hipFree(p);
   }
   for (int f = 0; f < c->nfields; f++) {
     if (c->spare[f] && c->spare[f] != c->bound[f]) (void)hipFree(c->spare[f]);
+    if (c->spare_z[f] && c->spare_z[f] != c->bound_z[f]) (void)hipFree(c->spare_z[f]);
     if (c->adj_tmp[f]) (void)hipFree(c->adj_tmp[f]);
     for (int b = 0; b < 2; b++)
       if (c->dual[b][f]) (void)hipFree(c->dual[b][f]);
@@ -890,6 +983,7 @@ int monc_tvd_bind_field(monc_tvd_ctx* c, int field, int variant, void* dptr) {
   c->buf[n] = dptr;
   c->owned[n] = 0;
   if (variant == MONC_TVD_PROG) c->bound[field] = dptr;
+  if (variant == MONC_TVD_ZFIELD) c->bound_z[field] = dptr;
   return MONC_TVD_OK;
 }
 
@@ -984,6 +1078,16 @@ int monc_tvd_tangent_steps(monc_tvd_ctx* c, double dtm, int nsteps) {
   if (nsteps < 0) return fail(MONC_TVD_ERR_ARG, "nsteps %d < 0", nsteps);
   if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
   return tangent_steps_impl(c, dtm, nsteps);
+}
+
+int monc_tvd_leapfrog_steps(monc_tvd_ctx* c, double dtm, double tsmth, double ugal, double vgal, int nsteps,
+                            int first_forward) {
+  if (int rc = check_ctx(c)) return rc;
+  if (nsteps < 0) return fail(MONC_TVD_ERR_ARG, "nsteps %d < 0", nsteps);
+  if (!(dtm == dtm) || !(tsmth == tsmth) || !(ugal == ugal) || !(vgal == vgal))
+    return fail(MONC_TVD_ERR_ARG, "dtm, tsmth, ugal and vgal must not be NaN");
+  return c->precision == MONC_TVD_FP64 ? leapfrog_impl<double>(c, dtm, tsmth, ugal, vgal, nsteps, first_forward)
+                                       : leapfrog_impl<float>(c, dtm, tsmth, ugal, vgal, nsteps, first_forward);
 }
 
 int monc_tvd_forward_steps(monc_tvd_ctx* c, double dtm, int nsteps) {

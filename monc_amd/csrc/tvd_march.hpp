@@ -323,12 +323,16 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
   g.sjb = (unsigned)L.d.sj * es;
   g.sib = (unsigned)L.d.si * es;
   MFlow<T, KIND> U{rsrc(L.u), g}, V{rsrc(L.v), g}, W{rsrc(L.w), g};
-  __amdgpu_buffer_rsrc_t F[NS], Fo[NS], Ff[NS];
+  __amdgpu_buffer_rsrc_t F[NS], Fo[NS], Ff[NS], Fz[NS], Fo2[NS];
 #pragma unroll
   for (int n = 0; n < NS; n++) {
     F[n] = rsrc(S.F[n]);
     Fo[n] = rsrc(S.out[n]);
     Ff[n] = rsrc(S.f[n]);
+    if constexpr (MODE == MODE_LEAP) {
+      Fz[n] = rsrc(S.z[n]);
+      Fo2[n] = rsrc(S.out2[n]);
+    }
   }
   __syncthreads();
   const int kl = (k <= NK ? k : NK) - 1;
@@ -420,7 +424,7 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
       for (int n = 0; n < NS; n++) {
         T sv;
         const T s0 = MODE == MODE_ACCUM ? ldb<T>(Fo[n], lv.k, so) : T(0);
-        const T fcur = MODE == MODE_STEP ? ldb<T>(Ff[n], lv.k, so) : T(0);
+        const T fcur = MODE != MODE_ACCUM ? ldb<T>(Ff[n], lv.k, so) : T(0);
         if (top) {
           sv = (q.v_s * fys[n] - q.v_0 * fyn[n]) * cy;
           sv = sv + T(4.0) * t1 * q.wm_0 * fzb[n];
@@ -433,6 +437,27 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
         if constexpr (MODE == MODE_ACCUM) {
           if (cell) stb<T>(Fo[n], sv, lv.k, so);
           if (S.zf[n] && k <= nz) stb<T>(rsrc(S.zf[n]), k == 1 ? T(0) : fzb[n], lv.k, so);
+        } else if constexpr (MODE == MODE_LEAP) {
+          // cleared sources: 0 where the reference leaves s untouched
+          const T zcur = L.forward ? T(0) : ldb<T>(Fz[n], lv.k, so);
+          T fn, zn;
+          leap_update<T, KIND>(L, S.gal[n], fcur, zcur, cell ? sv : T(0), fn, zn);
+          if (k <= nz) {
+            stb<T>(Fo[n], fn, lv.k, so);
+            stb<T>(Fo2[n], zn, lv.k, so);
+            if (jh) {
+              stb<T>(Fo[n], fn, lv.k, soj);
+              stb<T>(Fo2[n], zn, lv.k, soj);
+            }
+            if (ih) {
+              stb<T>(Fo[n], fn, lv.k, soi);
+              stb<T>(Fo2[n], zn, lv.k, soi);
+            }
+            if (jh && ih) {
+              stb<T>(Fo[n], fn, lv.k, soji);
+              stb<T>(Fo2[n], zn, lv.k, soji);
+            }
+          }
         } else {
           // levels the reference leaves untouched (k = 1; k = nz for w) keep f
           const T val = cell ? fcur + L.dtm * sv : fcur;

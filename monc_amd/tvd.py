@@ -35,6 +35,7 @@ EXPORTS = (
     "monc_tvd_set_profiling", "monc_tvd_kernel_times", "monc_tvd_weighted_sum",
     "monc_tvd_adjoint_step", "monc_tvd_adjoint_fold", "monc_tvd_tangent_steps", "monc_tvd_march_shapes",
     "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic", "monc_tvd_autotune",
+    "monc_tvd_leapfrog_steps",
     "monc_tvd_ensemble_create", "monc_tvd_ensemble_destroy", "monc_tvd_ensemble_layout",
     "monc_tvd_ensemble_set_field", "monc_tvd_ensemble_get_field", "monc_tvd_ensemble_copy_pool",
     "monc_tvd_ensemble_set_dt", "monc_tvd_ensemble_set_stream", "monc_tvd_ensemble_synchronize",
@@ -91,6 +92,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_halo_fill.argtypes = [c_void_p]
     lib.monc_tvd_tendencies.argtypes = [c_void_p, c_double, c_int, c_int, c_int]
     lib.monc_tvd_forward_steps.argtypes = [c_void_p, c_double, c_int]
+    lib.monc_tvd_leapfrog_steps.argtypes = [c_void_p, c_double, c_double, c_double, c_double, c_int, c_int]
     lib.monc_tvd_adjoint_step.argtypes = [c_void_p, c_double]
     lib.monc_tvd_adjoint_fold.argtypes = [c_void_p]
     lib.monc_tvd_tangent_steps.argtypes = [c_void_p, c_double, c_int]
@@ -280,6 +282,13 @@ class TvdAdvection:
 
     def forward_steps(self, dtm: float, nsteps: int) -> None:
         _check(self._lib.monc_tvd_forward_steps(self._h, dtm, nsteps))
+
+    def leapfrog_steps(self, dtm: float, tsmth: float, nsteps: int, ugal: float = 0.0, vgal: float = 0.0,
+                       first_forward: bool = True) -> None:
+        """MONC timesteps of an advection-only run (clearsourceterms, TVD,
+        stepfields, swap_smooth), resident: the first one forward if asked,
+        then leapfrog with the Robert filter."""
+        _check(self._lib.monc_tvd_leapfrog_steps(self._h, dtm, tsmth, ugal, vgal, nsteps, 1 if first_forward else 0))
 
     def adjoint_step(self, dtm: float) -> None:
         _check(self._lib.monc_tvd_adjoint_step(self._h, dtm))

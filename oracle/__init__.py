@@ -21,6 +21,13 @@ Parity status per oracle function (the checks live in tests/test_oracle.py):
 * oracle_forward_step -- pinned by direct evaluation of field + dtm*s.
 * oracle_profile_total -- pinned by exact integer sums (every partial sum
   exactly representable) and by comparison with a pairwise sum on random data.
+* oracle_step_field, oracle_swap_classic, oracle_swap_robert and their
+  composition leapfrog_run (the MONC step of an advection-only run) -- pinned
+  by the textbook leapfrog with the Robert-Asselin filter
+  x(t) + nu*(x(t+1) - 2x(t) + x(t-1)) evaluated independently, the forward
+  first step f + dtm*s, the galilean identities (new field + gal, new z-field
+  + (1 - tsmth)*gal for u and v), exact swaps at s = 0, and scalar mass
+  conservation through both filters.
 """
 from __future__ import annotations
 
@@ -60,6 +67,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_horizontal_c.restype = cd
         L.oracle_forward_step.argtypes = [dp, dp, dp, ci, ci, ci, cd]
         L.oracle_profile_total.argtypes = [dp, ci, ci, ci, dp]
+        L.oracle_step_field.argtypes = [dp, dp, dp, dp, ci, ci, ci, ci, ci, cd, cd, ci, cd, cd]
+        L.oracle_swap_classic.argtypes = [dp, dp, ci, ci, ci]
+        L.oracle_swap_robert.argtypes = [dp, dp, dp, ci, cd, cd, ci, ci, ci]
         _lib = L
     return _lib
 
@@ -187,3 +197,77 @@ def tendency_diagnostics(state, dtm: float, field_stepping: int, momentum_steppi
         tabs = after[3][2:-2, 2:-2, :] * r - sources[3][2:-2, 2:-2, :] * r
         tabs_p = profile_total(tabs)
     return after, t3d, prof, tabs, tabs_p
+
+
+CENTRED, FORWARD = 0, 1  # state.F90:15
+
+
+def step_field(f, zf, s, dtm: float, direction: int, flow_field: bool = False, gal: float = 0.0,
+               timesmooth: bool = False, tsmth: float = 0.0, with_sav: bool = False):
+    """stepfields.F90:553-627 on copies: returns (field, zfield, sav or None);
+    s(1) is zeroed in a copy of s as the reference does in place."""
+    f = np.array(f, dtype=np.float64, copy=True)
+    zf = np.array(zf, dtype=np.float64, copy=True)
+    s = np.array(s, dtype=np.float64, copy=True)
+    sav = np.zeros_like(f) if with_sav else None
+    nx, ny, nz = f.shape[0] - 4, f.shape[1] - 4, f.shape[2]
+    c1, c2 = 1.0 - 2.0 * tsmth, tsmth
+    lib().oracle_step_field(_p(f), _p(zf), _p(s), _p(sav) if sav is not None else None, nz, ny, nx,
+                            1 if flow_field else 0, direction, dtm, gal, 1 if timesmooth else 0, c1, c2)
+    return f, zf, sav
+
+
+def swap_classic(f, zf):
+    f = np.array(f, dtype=np.float64, copy=True)
+    zf = np.array(zf, dtype=np.float64, copy=True)
+    nx, ny, nz = f.shape[0] - 4, f.shape[1] - 4, f.shape[2]
+    lib().oracle_swap_classic(_p(f), _p(zf), nz, ny, nx)
+    return f, zf
+
+
+def swap_robert(f, zf, sav, kind: int, tsmth: float, gal: float = 0.0):
+    """kind 0: u or v (galilean shift gal), 1: w, 2: scalar."""
+    f = np.array(f, dtype=np.float64, copy=True)
+    zf = np.array(zf, dtype=np.float64, copy=True)
+    nx, ny, nz = f.shape[0] - 4, f.shape[1] - 4, f.shape[2]
+    sv = np.ascontiguousarray(sav, dtype=np.float64) if sav is not None else None
+    lib().oracle_swap_robert(_p(f), _p(zf), _p(sv) if sv is not None else None, kind, tsmth, gal, nz, ny, nx)
+    return f, zf
+
+
+def leapfrog_run(state, dtm: float, tsmth: float, nsteps: int, ugal: float = 0.0, vgal: float = 0.0,
+                 first_forward: bool = True, zfields=None):
+    """nsteps MONC timesteps of an advection-only run, column semantics of
+    timestepper.F90:65-122: halo_swapper fills fields and z-fields, the
+    sources are cleared (clearsourceterms), tvd_advection adds the tendencies
+    with momentum and scalar stepping CENTRED (steppingdirection.F90:27-32) and
+    field stepping FORWARD on the first step when first_forward, CENTRED
+    otherwise (:41-48), stepfields steps every field (stepfields.F90:314-351:
+    u, v flow fields with ugal/vgal and sav, w with sav, th and q time-smoothed
+    when centred), and swap_smooth swaps (classic when forward, Robert filter
+    when centred; swapsmooth.F90:44-57).  Returns (fields, zfields)."""
+    fields = [halo_fill(np.array(f, dtype=np.float64, copy=True)) for f in state.fields]
+    zsrc = zfields if zfields is not None else (state.zfields or state.fields)
+    zfs = [halo_fill(np.array(f, dtype=np.float64, copy=True)) for f in zsrc]
+    for step in range(nsteps):
+        direction = FORWARD if (first_forward and step == 0) else CENTRED
+        fields = [halo_fill(f) for f in fields]
+        zfs = [halo_fill(z) for z in zfs]
+        srcs, _ = tendencies(state.nz, state.ny, state.nx, state.cx, state.cy, state.grid.metric_vectors(), dtm,
+                             direction, CENTRED, CENTRED, fields, zfs, None, 7)
+        new_f, new_z = [], []
+        for n, (f, z, s) in enumerate(zip(fields, zfs, srcs)):
+            flow = n in (0, 1)
+            gal = ugal if n == 0 else (vgal if n == 1 else 0.0)
+            scalar = n >= 3
+            fs, zs, sav = step_field(f, z, s, dtm, direction, flow_field=flow, gal=gal,
+                                     timesmooth=scalar and direction == CENTRED, tsmth=tsmth,
+                                     with_sav=not scalar)
+            if direction == FORWARD:
+                fs, zs = swap_classic(fs, zs)
+            else:
+                fs, zs = swap_robert(fs, zs, sav, 2 if scalar else (1 if n == 2 else 0), tsmth, gal)
+            new_f.append(fs)
+            new_z.append(zs)
+        fields, zfs = new_f, new_z
+    return [halo_fill(f) for f in fields], [halo_fill(z) for z in zfs]

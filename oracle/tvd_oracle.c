@@ -648,3 +648,79 @@ void oracle_profile_total(const double *t, int nz, int ny, int nx, double *out) 
     for (int j = 0; j < ny; j++)
       for (int k = 0; k < nz; k++) out[k] = out[k] + t[k + (size_t)nz * ((size_t)j + (size_t)ny * (size_t)i)];
 }
+
+/* stepfields.F90:553-584 step_column_in_slice and :597-627 step_field for
+ * one field over the interior columns: optional time smoothing
+ * (perform_timesmooth_for_field, :522-533: field = c1*field + c2*zfield),
+ * then sfield(1) = 0 and, per level, sav = zfield + gal (centred stepping,
+ * when sav is given), field = gal + field (flow fields), and
+ * zfield = field + dtm*s (forward) or gal + zfield + 2*dtm*s (centred), with
+ * gal applied only to flow fields (actual_gal, :614). */
+void oracle_step_field(double *f, double *zf, double *s, double *sav, int nz, int ny, int nx, int flow_field,
+                       int direction, double dtm, double gal, int do_timesmoothing, double c1, double c2) {
+  g_nz = nz;
+  g_ny = ny;
+  g_nx = nx;
+  const double dtm_x2 = 2.0 * dtm;
+  const double actual_gal = flow_field ? gal : 0.0;
+  for (int i = 3; i <= nx + 2; i++)
+    for (int j = 3; j <= ny + 2; j++) {
+      if (do_timesmoothing)
+        for (int k = 1; k <= nz; k++) A(f, k, j, i) = c1 * A(f, k, j, i) + c2 * A(zf, k, j, i);
+      A(s, 1, j, i) = 0.0;
+      for (int k = 1; k <= nz; k++) {
+        if (sav && direction == OR_CENTRED) A(sav, k, j, i) = A(zf, k, j, i) + actual_gal;
+        if (flow_field) A(f, k, j, i) = actual_gal + A(f, k, j, i);
+        if (direction == OR_FORWARD)
+          A(zf, k, j, i) = A(f, k, j, i) + dtm * A(s, k, j, i);
+        else
+          A(zf, k, j, i) = actual_gal + A(zf, k, j, i) + dtm_x2 * A(s, k, j, i);
+      }
+    }
+}
+
+/* swapsmooth.F90 swap_and_smooth_classic with old_smoother = .false.
+ * (c1 = 1, c2 = -1) for one field over the interior columns: existing =
+ * field + zfield; field = existing*c1 + field*c2; zfield = existing - field. */
+void oracle_swap_classic(double *f, double *zf, int nz, int ny, int nx) {
+  g_nz = nz;
+  g_ny = ny;
+  g_nx = nx;
+  const double c1 = 1.0, c2 = -1.0;
+  for (int i = 3; i <= nx + 2; i++)
+    for (int j = 3; j <= ny + 2; j++)
+      for (int k = 1; k <= nz; k++) {
+        const double existing = A(f, k, j, i) + A(zf, k, j, i);
+        A(f, k, j, i) = existing * c1 + A(f, k, j, i) * c2;
+        A(zf, k, j, i) = existing - A(f, k, j, i);
+      }
+}
+
+/* swapsmooth.F90 swap_and_smooth_robert_filter for one field over the interior
+ * columns, c1 = 1 - 2*tsmth, c2 = tsmth.  kind 0 (u, v): field <- zfield,
+ * zfield <- c1*existing + c2*(field + sav - gal); kind 1 (w): the same without
+ * gal; kind 2 (th, q, tracers): zfield <- field + tsmth*zfield, field <- the
+ * old zfield. */
+void oracle_swap_robert(double *f, double *zf, const double *sav, int kind, double tsmth, double gal, int nz,
+                        int ny, int nx) {
+  g_nz = nz;
+  g_ny = ny;
+  g_nx = nx;
+  const double c1 = 1.0 - 2.0 * tsmth, c2 = tsmth;
+  for (int i = 3; i <= nx + 2; i++)
+    for (int j = 3; j <= ny + 2; j++)
+      for (int k = 1; k <= nz; k++) {
+        if (kind == 2) {
+          const double existing = A(zf, k, j, i);
+          A(zf, k, j, i) = A(f, k, j, i) + tsmth * existing;
+          A(f, k, j, i) = existing;
+        } else {
+          const double existing = A(f, k, j, i);
+          A(f, k, j, i) = A(zf, k, j, i);
+          if (kind == 0)
+            A(zf, k, j, i) = c1 * existing + c2 * (A(f, k, j, i) + A(sav, k, j, i) - gal);
+          else
+            A(zf, k, j, i) = c1 * existing + c2 * (A(f, k, j, i) + A(sav, k, j, i));
+        }
+      }
+}

@@ -492,3 +492,101 @@ def test_profile_total_pinned():
     t = np.zeros((3, 4, 2))
     t[2, 3, 1] = 1.0
     assert oracle.profile_total(t).tolist() == [0.0, 1.0]
+
+
+# --------------------------------------------------------------------------
+# The MONC step around the path: stepfields + swap_smooth (leapfrog_run)
+def _rand_field(st, rng, scale=1.0, offset=0.0):
+    a = offset + scale * rng.standard_normal(st.fields[0].shape)
+    return ss.halo_fill(a)
+
+
+@pytest.mark.parametrize("kind", ["scalar", "w", "u"])
+def test_leapfrog_robert_asselin_textbook(kind):
+    """Centred step + Robert filter equals x(t+1) = x(t-1) + 2 dtm s and
+    filtered x(t) = x(t) + nu (x(t+1) - 2 x(t) + x(t-1)) (Robert 1966,
+    Asselin 1972), written here independently of the reference's split of
+    the filter between stepfields and swap_smooth."""
+    st = state(nz=10, ny=6, nx=7, seed=21)
+    rng = np.random.default_rng(3)
+    x_t, x_tm1, s = _rand_field(st, rng, 1.0, 3.0), _rand_field(st, rng, 1.0, 3.0), _rand_field(st, rng, 0.2)
+    dtm, nu = 0.7, 0.06
+    scalar = kind == "scalar"
+    f, z, sav = oracle.step_field(x_t, x_tm1, s, dtm, oracle.CENTRED, flow_field=(kind == "u"), gal=0.0,
+                                  timesmooth=scalar, tsmth=nu, with_sav=not scalar)
+    f, z = oracle.swap_robert(f, z, sav, {"scalar": 2, "w": 1, "u": 0}[kind], nu, 0.0)
+    s0 = s.copy()
+    s0[:, :, 0] = 0.0  # stepfields zeroes s at k = 1
+    x_tp1 = x_tm1 + 2.0 * dtm * s0
+    filt = x_t + nu * (x_tp1 - 2.0 * x_t + x_tm1)
+    np.testing.assert_allclose(inner(f), inner(x_tp1), rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(inner(z), inner(filt), rtol=1e-14, atol=1e-13)
+    # halo columns are not touched
+    assert np.array_equal(f[0:2], x_t[0:2]) and np.array_equal(z[0:2], x_tm1[0:2])
+
+
+def test_leapfrog_galilean_shift():
+    """u with gal g: new field = new field(g=0) + g, new z-field = new z-field(g=0)
+    + (1 - nu) g (the field carries +g through stepfields and the filter
+    removes it from the t+1 term once)."""
+    st = state(nz=10, ny=6, nx=7, seed=22)
+    rng = np.random.default_rng(4)
+    x_t, x_tm1, s = _rand_field(st, rng, 1.0, 5.0), _rand_field(st, rng, 1.0, 5.0), _rand_field(st, rng, 0.2)
+    dtm, nu, g = 0.4, 0.1, 2.5
+    out = {}
+    for gal in (0.0, g):
+        f, z, sav = oracle.step_field(x_t, x_tm1, s, dtm, oracle.CENTRED, flow_field=True, gal=gal, with_sav=True)
+        out[gal] = oracle.swap_robert(f, z, sav, 0, nu, gal)
+    np.testing.assert_allclose(inner(out[g][0]), inner(out[0.0][0]) + g, rtol=1e-14, atol=1e-13)
+    np.testing.assert_allclose(inner(out[g][1]), inner(out[0.0][1]) + (1.0 - nu) * g, rtol=1e-14, atol=1e-13)
+
+
+def test_forward_first_step_swaps_classically():
+    """Forward step + classic swap: field = f + dtm s, z-field = f (to rounding)."""
+    st = state(nz=10, ny=6, nx=7, seed=23)
+    rng = np.random.default_rng(5)
+    x, zjunk, s = _rand_field(st, rng, 1.0, 3.0), _rand_field(st, rng), _rand_field(st, rng, 0.2)
+    f, z, _ = oracle.step_field(x, zjunk, s, 0.9, oracle.FORWARD)
+    f, z = oracle.swap_classic(f, z)
+    s0 = s.copy()
+    s0[:, :, 0] = 0.0
+    np.testing.assert_allclose(inner(f), inner(x + 0.9 * s0), rtol=1e-15, atol=1e-14)
+    np.testing.assert_allclose(inner(z), inner(x), rtol=1e-15, atol=1e-14)
+
+
+def test_leapfrog_zero_flow_is_an_exact_swap():
+    """u = v = w = 0: s = 0, so the new field is the old z-field bitwise and
+    the scalars' new z-field is c1 f + c2 zf + nu zf."""
+    st = state(nz=10, ny=6, nx=7, seed=24)
+    for n in range(3):
+        st.fields[n][:] = 0.0
+        st.zfields[n][:] = 0.0
+    nu = 0.05
+    f, z = oracle.leapfrog_run(st, 0.8, nu, 1, first_forward=False)
+    for n in (3, 4):
+        assert np.array_equal(inner(f[n]), inner(st.zfields[n]))
+        np.testing.assert_allclose(inner(z[n]), inner((1 - 2 * nu) * st.fields[n] + nu * st.zfields[n]
+                                                      + nu * st.zfields[n]), rtol=1e-15)
+
+
+@pytest.mark.parametrize("anel", [None, 3000.0])
+def test_leapfrog_run_conserves_scalar_mass(anel):
+    """With z-fields equal to the fields at the start, the rho_n dz weighted
+    scalar mass of both time levels is conserved by forward + leapfrog steps
+    with the Robert filter (every update is a combination of conserved
+    quantities)."""
+    st = state(nz=12, ny=7, nx=9, seed=25, anel=anel)
+    st.zfields = [f.copy() for f in st.fields]
+    dt = ss.cfl_dt(st, 0.3)
+    g = st.grid
+    w = g.rhon * g.dz
+    w[0] = 0.0
+
+    def mass(a):
+        return float((inner(a).sum(axis=(0, 1)) * w).sum())
+
+    f, z = oracle.leapfrog_run(st, dt, 0.06, 4)
+    for n in (3, 4):
+        m0 = mass(st.fields[n])
+        assert abs(mass(f[n]) - m0) <= 1e-12 * abs(m0), n
+        assert abs(mass(z[n]) - m0) <= 1e-12 * abs(m0), n
