@@ -13,9 +13,14 @@ reported beside it.
 Output: one JSON line.  value = interior cell-updates per second (all five
 fields) of the FP64 form, wall clock over the timed steps; roofline = the
 dominant kernel's algorithmic bytes per launch over its average duration
-(HIP events recorded by the library around every launch on its stream);
+(HIP events recorded by the library around every launch on its stream), with
+`traffic` the HBM bytes rocprofv3 measured for that kernel at that tile shape
+(profiles/march_kernel_traffic_512x512x256.json; null for another shape);
 cpu_baseline = the C restatement (oracle) timed on a bounded sample on this
-host's CPU.
+host's CPU.  Beside it: the FP32 form, the FP32 adjoint step on the same
+domain, the MONC leapfrog timestep (FP64), and the batched ensemble (48 FP32
+members of seeded sizes 64..256 x 64..256 x 64..128, one persistent launch per
+step, graph-replayed).
 """
 from __future__ import annotations
 
@@ -83,6 +88,114 @@ def roofline(kt: dict, cells: int, es: int):
     return kind, avg_ms, bytes_launch, achieved
 
 
+def measured_traffic(precision: str, kind: str, nz: int, shape):
+    """HBM bytes per launch that rocprofv3 measured for this kernel and tile
+    shape, or None."""
+    p = os.path.join(ROOT, "profiles", "march_kernel_traffic_512x512x256.json")
+    if nz != 256 or not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        t = json.load(fh)
+    if shape is None or list(shape) != [t["shape"]["rows"], t["shape"]["strip"]]:
+        return None
+    return t["bytes_per_launch"].get(precision, {}).get(kind)
+
+
+def adjoint_form(state, dt: float, steps: int):
+    """FP32 adjoint step (monc_tvd_adjoint_step) on the bench domain: ms per
+    step against the FP32 forward step on the same context."""
+    import numpy as np
+    import torch
+
+    from monc_amd import tvd
+
+    t = tvd.TvdAdvection.initialise(state, precision=tvd.FP32)
+    try:
+        t.set_stream(torch.cuda.current_stream().cuda_stream)
+        t.halo_fill()
+        rng = np.random.default_rng(1)
+        for f in range(len(state.fields)):
+            lam = np.zeros(t.field_shape())
+            lam[2:-2, 2:-2, :] = rng.standard_normal((state.nx, state.ny, state.nz))
+            t.set_field(f, tvd.ADJOINT, lam)
+        t.adjoint_step(dt)
+        t.forward_steps(dt, 1)
+        t.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t.adjoint_step(dt)
+        t.synchronize()
+        t1 = time.perf_counter()
+        t.forward_steps(dt, steps)
+        t.synchronize()
+        t2 = time.perf_counter()
+        cells = state.nx * state.ny * state.nz
+        return {"ms_per_step": round((t1 - t0) / steps * 1e3, 4),
+                "value": round(cells * steps / (t1 - t0) / 1e9, 4), "unit": "Gcell-updates/s",
+                "over_forward_step": round((t1 - t0) / (t2 - t1), 3),
+                "bytes_per_cell_update": 80}
+    finally:
+        t.close()
+
+
+def leapfrog_form(state, dt: float, steps: int):
+    """FP64 MONC timestep (monc_tvd_leapfrog_steps: TVD + stepfields +
+    swap_smooth, Robert filter tsmth = 0.06 as MONC's default)."""
+    from monc_amd import tvd
+
+    t = tvd.TvdAdvection.initialise(state, precision=tvd.FP64)
+    try:
+        t.halo_fill()
+        t.leapfrog_steps(dt, 0.06, 2)
+        t.synchronize()
+        t0 = time.perf_counter()
+        t.leapfrog_steps(dt, 0.06, steps, first_forward=False)
+        t.synchronize()
+        el = time.perf_counter() - t0
+        cells = state.nx * state.ny * state.nz
+        return {"ms_per_step": round(el / steps * 1e3, 4), "value": round(cells * steps / el / 1e9, 4),
+                "unit": "Gcell-updates/s", "bytes_per_cell_update": 160, "dtype": "f64"}
+    finally:
+        t.close()
+
+
+def ensemble_form(steps: int):
+    """48 FP32 members of seeded uneven sizes: forward steps by one persistent
+    launch per step over all members' tiles, with and without graph replay,
+    tile shape from the library's autotuner."""
+    import torch
+
+    from monc_amd import setup_state as ss
+    from monc_amd import tvd
+    from monc_amd.ensemble import TvdEnsemble
+
+    sts = ss.ensemble_states(48, seed=0)
+    dts = [ss.cfl_dt(st, 0.3) for st in sts]
+    cells = sum(st.nx * st.ny * st.nz for st in sts)
+    e = TvdEnsemble.from_states(sts, dts, precision=tvd.FP32)
+    del sts
+    try:
+        e.set_stream(torch.cuda.current_stream().cuda_stream)
+        e.halo_fill()
+        shape = e.autotune(reps=2)
+        out = {"members": 48, "cells": cells, "shape": {"waves": shape[0], "strip": shape[1]}}
+        for graph in (True, False):
+            e.set_tiling(shape[0], shape[1], graph)
+            e.forward_steps(2)
+            e.synchronize()
+            t0 = time.perf_counter()
+            e.forward_steps(steps)
+            e.synchronize()
+            el = time.perf_counter() - t0
+            out["graph" if graph else "direct"] = {"ms_per_step": round(el / steps * 1e3, 4),
+                                                   "value": round(cells * steps / el / 1e9, 4)}
+        out["unit"] = "Gcell-updates/s"
+        out["bytes_per_cell_update"] = 40
+        return out
+    finally:
+        e.close()
+
+
 def cpu_baseline(nz: int, seconds_hint: float = 12.0):
     """The oracle (single-threaded C restatement) on a bounded sample of the
     same workload: one forward step of a 288x288 column block at full depth."""
@@ -117,6 +230,8 @@ def main() -> None:
     ap.add_argument("--no-tune", action="store_true", help="skip the tile-shape autotuner (untimed)")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the adjoint, leapfrog and ensemble measurements")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed work, run three 512 MiB device-to-device copies: the known-size "
                          "reference that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
@@ -146,6 +261,8 @@ def main() -> None:
     ms_step = wall64 / args.steps * 1e3
     value = cells * args.steps / wall64 / 1e9
     kind, kavg, kbytes, achieved = roofline(kt64, cells, 8)
+    traffic = measured_traffic("fp64", kind, nz, (tuned64 or {}).get(kind) if tuned64 else
+                               ([3, 32] if args.kernel == 0 and args.rows == 0 else None))
     step_kernel_ms = sum(v[0] for k, v in kt64.items() if k in KIND_FIELDS) / args.steps
     out = {
         "metric": METRIC,
@@ -165,7 +282,7 @@ def main() -> None:
                                "z tanh-stretched to 5 km, fields u,v,w,th,q, CFL 0.3",
                    "nx": nx, "ny": ny, "nz": nz, "fields": 5, "dt_s": dt},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"{KERNEL_NAMES[args.kernel]} ({kind})", "kernel_ms": round(kavg, 4),
                      "algorithmic_bytes_per_launch": kbytes,
                      "step": {"kernel_ms": round(step_kernel_ms, 4), "bytes_per_cell_update": 80,
@@ -186,6 +303,13 @@ def main() -> None:
                        "step_achieved_GBs": round(40 * cells / (step32 * 1e-3) / 1e9, 2),
                        "speedup_vs_fp64": round(v32 / value, 3)}
         out["tile_shapes"]["fp32"] = tuned32
+    if not args.no_extra:
+        log("bench: FP32 adjoint step")
+        out["adjoint_fp32"] = adjoint_form(st, dt, max(2, args.steps // 2))
+        log("bench: FP64 leapfrog timestep")
+        out["leapfrog_fp64"] = leapfrog_form(st, dt, max(2, args.steps // 2))
+        log("bench: batched ensemble (48 FP32 members)")
+        out["ensemble_fp32"] = ensemble_form(max(4, args.steps // 2))
     if args.calibrate_copy:
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)

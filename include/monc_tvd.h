@@ -201,6 +201,17 @@ int monc_tvd_tendencies(monc_tvd_ctx *ctx, double dtm, int field_stepping, int m
 int monc_tvd_forward_steps(monc_tvd_ctx *ctx, double dtm, int nsteps);
 
 /*
+ * One step of monc_tvd_forward_steps from the state in `in` into `out`:
+ * arrays of one device pointer per field (u, v, w, th, q(:)), each a buffer
+ * of the context's precision and field shape with filled halos; out must not
+ * alias in.  Only the advected fields are written (with their periodic halo);
+ * the context's own fields are not touched.  No counterpart in the reference:
+ * it lets a caller keep every state of a rollout (the recompute of a
+ * checkpointed adjoint) without copies.  Asynchronous.
+ */
+int monc_tvd_forward_step_into(monc_tvd_ctx *ctx, double dtm, void *const *in, void *const *out);
+
+/*
  * Device-resident MONC timesteps of an advection-only run, nsteps times: per
  * step the sources are cleared (clearsourceterms), the TVD tendencies added
  * with momentum and scalar stepping CENTRED (steppingdirection.F90:27-32:

@@ -58,6 +58,14 @@ class TvdRollout:
         self._bind_prog(self.prog)
         for f in range(self.nfields):
             self.t.bind_field(f, tvd.ADJOINT, self.adj[f])
+        adv = set()
+        if advect & tvd.ADVECT_FLOW:
+            adv |= {0, 1, 2}
+        if advect & tvd.ADVECT_THETA:
+            adv.add(3)
+        if advect & tvd.ADVECT_Q:
+            adv |= set(range(4, self.nfields))
+        self._fixed = [f for f in range(self.nfields) if f not in adv]  # fields the step leaves as they are
         self._seg: List[List[torch.Tensor]] = []  # recompute buffers, reused across segments
         self._seg_tag = None  # the forward pass whose last segment the buffers hold
 
@@ -72,6 +80,12 @@ class TvdRollout:
     def _load(self, fields: Sequence[torch.Tensor]) -> None:
         for p, f in zip(self.prog, fields):
             p.copy_(f)
+
+    def _step_into(self, dt: float, src: Sequence[torch.Tensor], dst: Sequence[torch.Tensor]) -> None:
+        """dst = one forward step of src (monc_tvd_forward_step_into)."""
+        self.t.forward_step_into(dt, src, dst)
+        for f in self._fixed:
+            dst[f].copy_(src[f])
 
     def _snapshot(self) -> List[torch.Tensor]:
         return [p.clone() for p in self.prog]
@@ -115,10 +129,11 @@ class RolloutFn(torch.autograd.Function):
                 eng.t.forward_steps(dt, n)
             else:  # last segment: keep every state the backward pass needs
                 seg = eng._seg_buffers(n)
-                for m in range(n):
-                    for b, p in zip(seg[m], eng.prog):
-                        b.copy_(p)
-                    eng.t.forward_steps(dt, 1)
+                for b, p in zip(seg[0], eng.prog):
+                    b.copy_(p)
+                for m in range(1, n):
+                    eng._step_into(dt, seg[m - 1], seg[m])
+                eng._step_into(dt, seg[n - 1], eng.prog)
                 ckpts.append((done, None))
                 eng._seg_tag = tag
             done += n
@@ -142,13 +157,10 @@ class RolloutFn(torch.autograd.Function):
                 raise RuntimeError("another rollout of this engine reused its segment buffers before this backward")
             if ck is not None:
                 # recompute X_start .. X_{start+n-1} in the working buffers
-                eng._bind_prog(eng.prog)
-                eng._load(ck)
-                for m in range(n):
-                    if m:
-                        eng.t.forward_steps(dt, 1)
-                    for s, p in zip(seg[m], eng.prog):
-                        s.copy_(p)
+                for s, c in zip(seg[0], ck):
+                    s.copy_(c)
+                for m in range(1, n):
+                    eng._step_into(dt, seg[m - 1], seg[m])
             # adjoint sweep over the segment, newest state first
             for m in reversed(range(n)):
                 eng._bind_prog(seg[m])

@@ -607,14 +607,37 @@ __device__ __forceinline__ void fold_cols_block(T* f, const Dims& d, int bx, int
   f[src] = T(0.0);
 }
 
+// Up to MAX_FIELDS fields of one domain, one launch for all (blockIdx.z or
+// blockIdx.y selects the field).
 template <typename T>
-__global__ void __launch_bounds__(256) fold_rows_kernel(T* f, Dims d) {
-  fold_rows_block<T>(f, d, (int)blockIdx.x, (int)blockIdx.y);
+struct FieldPtrs {
+  T* f[MAX_FIELDS];
+  const T* src[MAX_FIELDS];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) fold_rows_kernel(FieldPtrs<T> P, Dims d) {
+  fold_rows_block<T>(P.f[blockIdx.z], d, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) fold_cols_kernel(T* f, Dims d) {
-  fold_cols_block<T>(f, d, (int)blockIdx.x, (int)blockIdx.y);
+__global__ void __launch_bounds__(256) fold_cols_kernel(FieldPtrs<T> P, Dims d) {
+  fold_cols_block<T>(P.f[blockIdx.z], d, (int)blockIdx.x, (int)blockIdx.y);
+}
+
+// f[n] = src[n] for every field n (blockIdx.y), `words` 32-bit words each, in
+// 16-byte pieces (both buffers 16-byte aligned) and a word tail.
+template <typename T>
+__global__ void __launch_bounds__(256) copy_fields_kernel(FieldPtrs<T> P, int64_t words) {
+  const uint4* s4 = (const uint4*)P.src[blockIdx.y];
+  uint4* d4 = (uint4*)P.f[blockIdx.y];
+  const int64_t n4 = words / 4;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n4; t += stride) d4[t] = s4[t];
+  const unsigned* s1 = (const unsigned*)P.src[blockIdx.y];
+  unsigned* d1 = (unsigned*)P.f[blockIdx.y];
+  const int64_t t = 4 * n4 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < words) d1[t] = s1[t];
 }
 
 

@@ -154,7 +154,9 @@ __host__ __device__ inline long long op_blocks_per_field(int op, int nz, int ny,
     case OP_HALO: m2 = (4 * (nx + 4) + 4 * ny + 3) / 4; break;
     case OP_FOLD_ROWS: m2 = nx + 4; break;
     case OP_FOLD_COLS: m2 = ny; break;
-    default: m2 = (ny + 1 + 3) / 4 * (nx + 1); break;
+    default:  // one work item per x column: it loops over the column's blocks
+      if (n2) *n2 = nx + 1;
+      return nx + 1;
   }
   if (n2) *n2 = m2;
   return (long long)nbx * m2;
@@ -167,12 +169,13 @@ __global__ void __launch_bounds__(256) ens_grid_kernel(const EnsMember* mem, con
   for (long long b = blockIdx.x; b < total; b += gridDim.x) {
     const int m = member_of(prefix, nmem, b);
     const EnsMember& M = mem[m];
-    int n2;
-    const long long per = op_blocks_per_field(OP, M.d.nz, M.d.ny, M.d.nx, &n2);
-    long long r = b - prefix[m];
-    const int f = (int)(r / per);
-    r -= (long long)f * per;
-    const int bx = (int)(r % ((M.d.nz + 63) / 64)), by = (int)(r / ((M.d.nz + 63) / 64));
+    // per-member counts fit in 32 bits: no 64-bit division per block
+    const int per = (int)op_blocks_per_field(OP, M.d.nz, M.d.ny, M.d.nx, nullptr);
+    int r = (int)(b - prefix[m]);
+    const int f = r / per;
+    r -= f * per;
+    const int nbx = (M.d.nz + 63) / 64;
+    const int by = r / nbx, bx = r - by * nbx;
     T* fp = pool + M.off + (long long)f * M.fstride;
     if constexpr (OP == OP_HALO) halo_block<T>(fp, M.d, bx, by);
     else if constexpr (OP == OP_FOLD_ROWS) fold_rows_block<T>(fp, M.d, bx, by);
@@ -180,7 +183,9 @@ __global__ void __launch_bounds__(256) ens_grid_kernel(const EnsMember* mem, con
   }
 }
 
-// Adjoint of one forward step of every member (adjoint_kernel per field).
+// Adjoint of one forward step of every member (adjoint_kernel per field):
+// one work item is one x column (bz) of one field of one member, whose
+// 64-level x 4-row blocks the block visits in turn.
 template <typename T>
 __global__ void __launch_bounds__(256) ens_adjoint_kernel(const EnsMember* mem, const double* met,
                                                           const long long* prefix, int nmem, const T* state,
@@ -189,14 +194,9 @@ __global__ void __launch_bounds__(256) ens_adjoint_kernel(const EnsMember* mem, 
   for (long long b = blockIdx.x; b < total; b += gridDim.x) {
     const int m = member_of(prefix, nmem, b);
     const EnsMember& M = mem[m];
-    const long long per = op_blocks_per_field(OP_ADJOINT, M.d.nz, M.d.ny, M.d.nx, nullptr);
-    long long r = b - prefix[m];
-    const int f = (int)(r / per);
-    r -= (long long)f * per;
+    const int r = (int)(b - prefix[m]);
+    const int f = r / (M.d.nx + 1), bz = r - f * (M.d.nx + 1);
     const int nbx = (M.d.nz + 63) / 64, nby = (M.d.ny + 1 + 3) / 4;
-    const int bx = (int)(r % nbx);
-    r /= nbx;
-    const int by = (int)(r % nby), bz = (int)(r / nby);
     const LaunchArgs<T> L = member_args<T>(M, state, met);
     AdjArgs<T> A;
     const long long fo = M.off + (long long)f * M.fstride;
@@ -207,12 +207,15 @@ __global__ void __launch_bounds__(256) ens_adjoint_kernel(const EnsMember* mem, 
     A.lv = lam + M.off + M.fstride;
     A.lw = lam + M.off + 2 * M.fstride;
     A.dtm = (T)M.dtm;
-    switch (f) {
-      case 0: adjoint_block<T, KIND_U>(L, A, bx, by, bz); break;
-      case 1: adjoint_block<T, KIND_V>(L, A, bx, by, bz); break;
-      case 2: adjoint_block<T, KIND_W>(L, A, bx, by, bz); break;
-      default: adjoint_block<T, KIND_SCALAR>(L, A, bx, by, bz); break;
-    }
+    for (int by = 0; by < nby; by++)
+      for (int bx = 0; bx < nbx; bx++) {
+        switch (f) {
+          case 0: adjoint_block<T, KIND_U>(L, A, bx, by, bz); break;
+          case 1: adjoint_block<T, KIND_V>(L, A, bx, by, bz); break;
+          case 2: adjoint_block<T, KIND_W>(L, A, bx, by, bz); break;
+          default: adjoint_block<T, KIND_SCALAR>(L, A, bx, by, bz); break;
+        }
+      }
   }
 }
 

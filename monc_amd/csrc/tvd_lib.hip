@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
@@ -486,6 +487,34 @@ int forward_steps_impl(monc_tvd_ctx* c, double dtm, int nsteps) {
   return MONC_TVD_OK;
 }
 
+// One forward step from caller buffers to caller buffers (the recompute of
+// a checkpointed rollout keeps every state without copies).
+template <typename T>
+int forward_into_impl(monc_tvd_ctx* c, double dtm, void* const* in, void* const* out) {
+  std::vector<int> fields = advected_fields(c);
+  for (int f = 0; f < c->nfields; f++)
+    if (!in[f] || !out[f]) return fail(MONC_TVD_ERR_ARG, "null buffer for field %d", f);
+  LaunchArgs<T> L = launch_args<T>(c, dtm);
+  L.u = (const T*)in[MONC_TVD_U];
+  L.v = (const T*)in[MONC_TVD_V];
+  L.w = (const T*)in[MONC_TVD_W];
+  for (size_t first = 0; first < fields.size(); first += MAX_FIELDS) {
+    FieldSet<T> S{};
+    S.nf = (int)std::min(fields.size() - first, (size_t)MAX_FIELDS);
+    for (int n = 0; n < S.nf; n++) {
+      const int fl = fields[first + n];
+      if (in[fl] == out[fl]) return fail(MONC_TVD_ERR_ARG, "field %d: the step cannot run in place", fl);
+      S.kind[n] = kind_of(fl);
+      S.F[n] = (const T*)in[fl];
+      S.f[n] = S.F[n];
+      S.out[n] = (T*)out[fl];
+      S.dt[n] = (T)dtm;
+    }
+    if (int rc = launch_set<T, MODE_STEP>(c, L, S)) return rc;
+  }
+  return MONC_TVD_OK;
+}
+
 // The MONC timestep of an advection-only run, resident: per step the TVD
 // tendencies from cleared sources (momentum and scalar stepping CENTRED, so
 // ultflx takes dt = 2 dtm, steppingdirection.F90:27-32; the advected field is
@@ -627,15 +656,35 @@ int autotune_impl(monc_tvd_ctx* c, double dtm, int reps, int* chosen) {
   return MONC_TVD_OK;
 }
 
+// Adjoint of the periodic fill on every allocated adjoint field: rows of all
+// fields in one launch, then columns.
 template <typename T>
-int fold_halo(monc_tvd_ctx* c, T* f) {
+int fold_halo(monc_tvd_ctx* c) {
+  FieldPtrs<T> P{};
+  int nf = 0;
+  for (int f = 0; f < c->nfields; f++) {
+    void* p = slot(c, f, MONC_TVD_ADJOINT);
+    if (!p) continue;
+    if (nf == MAX_FIELDS) {  // more fields than one launch holds
+      Dims d = dims_of(c);
+      dim3 block(64, 4, 1);
+      dim3 g1((c->nz + 63) / 64, (unsigned)((4 * (c->nx + 4) + 3) / 4), nf);
+      hipLaunchKernelGGL((fold_rows_kernel<T>), g1, block, 0, c->stream, P, d);
+      dim3 g2((c->nz + 63) / 64, (unsigned)((4 * c->ny + 3) / 4), nf);
+      hipLaunchKernelGGL((fold_cols_kernel<T>), g2, block, 0, c->stream, P, d);
+      HIP_TRY(hipGetLastError());
+      nf = 0;
+    }
+    P.f[nf++] = (T*)p;
+  }
+  if (!nf) return MONC_TVD_OK;
   Dims d = dims_of(c);
   dim3 block(64, 4, 1);
-  dim3 g1((c->nz + 63) / 64, (unsigned)((4 * (c->nx + 4) + 3) / 4), 1);
-  hipLaunchKernelGGL((fold_rows_kernel<T>), g1, block, 0, c->stream, f, d);
+  dim3 g1((c->nz + 63) / 64, (unsigned)((4 * (c->nx + 4) + 3) / 4), nf);
+  hipLaunchKernelGGL((fold_rows_kernel<T>), g1, block, 0, c->stream, P, d);
   HIP_TRY(hipGetLastError());
-  dim3 g2((c->nz + 63) / 64, (unsigned)((4 * c->ny + 3) / 4), 1);
-  hipLaunchKernelGGL((fold_cols_kernel<T>), g2, block, 0, c->stream, f, d);
+  dim3 g2((c->nz + 63) / 64, (unsigned)((4 * c->ny + 3) / 4), nf);
+  hipLaunchKernelGGL((fold_cols_kernel<T>), g2, block, 0, c->stream, P, d);
   HIP_TRY(hipGetLastError());
   return MONC_TVD_OK;
 }
@@ -654,9 +703,27 @@ int adjoint_step_impl(monc_tvd_ctx* c, double dtm) {
   }
   // the incoming adjoint is read while the accumulators (which start from it:
   // the identity part of X' = X + dtm T(X)) receive the transposed Jacobian
+  const int64_t words = (int64_t)(c->n_elem * c->elem_size / 4);
+  bool aligned = true;
   for (int fl : fields)
-    HIP_TRY(hipMemcpyAsync(c->adj_tmp[fl], slot(c, fl, MONC_TVD_ADJOINT), c->n_elem * c->elem_size,
-                           hipMemcpyDeviceToDevice, c->stream));
+    aligned = aligned && ((uintptr_t)slot(c, fl, MONC_TVD_ADJOINT) % 16 == 0);
+  for (size_t first = 0; first < fields.size(); first += MAX_FIELDS) {
+    const int nf = (int)std::min(fields.size() - first, (size_t)MAX_FIELDS);
+    if (!aligned) {
+      for (int n = 0; n < nf; n++)
+        HIP_TRY(hipMemcpyAsync(c->adj_tmp[fields[first + n]], slot(c, fields[first + n], MONC_TVD_ADJOINT),
+                               c->n_elem * c->elem_size, hipMemcpyDeviceToDevice, c->stream));
+      continue;
+    }
+    FieldPtrs<T> P{};
+    for (int n = 0; n < nf; n++) {
+      P.f[n] = (T*)c->adj_tmp[fields[first + n]];
+      P.src[n] = (const T*)slot(c, fields[first + n], MONC_TVD_ADJOINT);
+    }
+    const unsigned nb = (unsigned)std::min<int64_t>((words / 4 + 255) / 256 + 1, 4096);
+    hipLaunchKernelGGL((copy_fields_kernel<T>), dim3(nb, nf, 1), dim3(256), 0, c->stream, P, words);
+    HIP_TRY(hipGetLastError());
+  }
   const LaunchArgs<T> L = launch_args<T>(c, dtm);
   dim3 block(64, 4, 1);
   dim3 grid((c->nz + 63) / 64, (c->ny + 1 + 3) / 4, c->nx + 1);
@@ -678,11 +745,7 @@ int adjoint_step_impl(monc_tvd_ctx* c, double dtm) {
     }
     HIP_TRY(hipGetLastError());
   }
-  for (int f = 0; f < c->nfields; f++) {
-    void* p = slot(c, f, MONC_TVD_ADJOINT);
-    if (p && (rc = fold_halo<T>(c, (T*)p))) return rc;
-  }
-  return MONC_TVD_OK;
+  return fold_halo<T>(c);
 }
 
 int tangent_steps_impl(monc_tvd_ctx* c, double dtm, int nsteps) {
@@ -1063,13 +1126,7 @@ int monc_tvd_adjoint_step(monc_tvd_ctx* c, double dtm) {
 
 int monc_tvd_adjoint_fold(monc_tvd_ctx* c) {
   if (int rc = check_ctx(c)) return rc;
-  for (int f = 0; f < c->nfields; f++) {
-    void* p = slot(c, f, MONC_TVD_ADJOINT);
-    if (!p) continue;
-    const int rc = c->precision == MONC_TVD_FP64 ? fold_halo<double>(c, (double*)p) : fold_halo<float>(c, (float*)p);
-    if (rc) return rc;
-  }
-  return MONC_TVD_OK;
+  return c->precision == MONC_TVD_FP64 ? fold_halo<double>(c) : fold_halo<float>(c);
 }
 
 int monc_tvd_tangent_steps(monc_tvd_ctx* c, double dtm, int nsteps) {
@@ -1078,6 +1135,14 @@ int monc_tvd_tangent_steps(monc_tvd_ctx* c, double dtm, int nsteps) {
   if (nsteps < 0) return fail(MONC_TVD_ERR_ARG, "nsteps %d < 0", nsteps);
   if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
   return tangent_steps_impl(c, dtm, nsteps);
+}
+
+int monc_tvd_forward_step_into(monc_tvd_ctx* c, double dtm, void* const* in, void* const* out) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!in || !out) return fail(MONC_TVD_ERR_ARG, "null buffer array");
+  if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
+  return c->precision == MONC_TVD_FP64 ? forward_into_impl<double>(c, dtm, in, out)
+                                       : forward_into_impl<float>(c, dtm, in, out);
 }
 
 int monc_tvd_leapfrog_steps(monc_tvd_ctx* c, double dtm, double tsmth, double ugal, double vgal, int nsteps,

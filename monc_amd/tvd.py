@@ -35,7 +35,7 @@ EXPORTS = (
     "monc_tvd_set_profiling", "monc_tvd_kernel_times", "monc_tvd_weighted_sum",
     "monc_tvd_adjoint_step", "monc_tvd_adjoint_fold", "monc_tvd_tangent_steps", "monc_tvd_march_shapes",
     "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic", "monc_tvd_autotune",
-    "monc_tvd_leapfrog_steps",
+    "monc_tvd_leapfrog_steps", "monc_tvd_forward_step_into",
     "monc_tvd_ensemble_create", "monc_tvd_ensemble_destroy", "monc_tvd_ensemble_layout",
     "monc_tvd_ensemble_set_field", "monc_tvd_ensemble_get_field", "monc_tvd_ensemble_copy_pool",
     "monc_tvd_ensemble_set_dt", "monc_tvd_ensemble_set_stream", "monc_tvd_ensemble_synchronize",
@@ -92,6 +92,8 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_halo_fill.argtypes = [c_void_p]
     lib.monc_tvd_tendencies.argtypes = [c_void_p, c_double, c_int, c_int, c_int]
     lib.monc_tvd_forward_steps.argtypes = [c_void_p, c_double, c_int]
+    lib.monc_tvd_forward_step_into.argtypes = [c_void_p, c_double, ctypes.POINTER(c_void_p),
+                                               ctypes.POINTER(c_void_p)]
     lib.monc_tvd_leapfrog_steps.argtypes = [c_void_p, c_double, c_double, c_double, c_double, c_int, c_int]
     lib.monc_tvd_adjoint_step.argtypes = [c_void_p, c_double]
     lib.monc_tvd_adjoint_fold.argtypes = [c_void_p]
@@ -282,6 +284,16 @@ class TvdAdvection:
 
     def forward_steps(self, dtm: float, nsteps: int) -> None:
         _check(self._lib.monc_tvd_forward_steps(self._h, dtm, nsteps))
+
+    def forward_step_into(self, dtm: float, src, dst) -> None:
+        """One forward step from the torch tensors `src` (one per field) into
+        `dst`, without touching the context's own fields."""
+        n = len(src)
+        if n != len(dst) or n != 4 + self.n_q:
+            raise ValueError("one source and one destination tensor per field")
+        a = (ctypes.c_void_p * n)(*[t.data_ptr() for t in src])
+        b = (ctypes.c_void_p * n)(*[t.data_ptr() for t in dst])
+        _check(self._lib.monc_tvd_forward_step_into(self._h, dtm, a, b))
 
     def leapfrog_steps(self, dtm: float, tsmth: float, nsteps: int, ugal: float = 0.0, vgal: float = 0.0,
                        first_forward: bool = True) -> None:
