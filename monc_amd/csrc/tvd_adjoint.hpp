@@ -43,10 +43,25 @@ struct Acc {
   __device__ __forceinline__ void add(int k, int j, int i, T v) const { atomic_add(a + at(d, k, j, i), v); }
 };
 
+// Adjoint of one target field accumulated in LDS: a ring of 4 x columns
+// (slot = i mod 4) x `rows` rows (from j0 - 1) x ADJ_KB levels (from k0 - 1),
+// with LDS atomics.  Every contribution of the face owners at (k, j, i) lands
+// within levels k-1..k+2, rows j-1..j+2 and columns i-1..i+2 (the stencils of
+// the three faces and the dual-grid interpolation of the flows).
+constexpr int ADJ_KB = 67;
+template <typename T>
+struct LdsAcc {
+  T* s;
+  int k0, j0, rows;
+  __device__ __forceinline__ void add(int k, int j, int i, T v) const {
+    if (v != T(0)) unsafeAtomicAdd(&s[((i & 3) * rows + (j - j0 + 1)) * ADJ_KB + (k - k0 + 1)], v);
+  }
+};
+
 // Adjoint of Flow<T,DK,DJ,DI>: the dual-grid interpolation scatters to both points.
-template <typename T, int DK, int DJ, int DI>
+template <typename T, int DK, int DJ, int DI, class ACC = Acc<T>>
 struct AccFlow {
-  Acc<T> a;
+  ACC a;
   int nz;
   __device__ __forceinline__ void add(int k, int j, int i, T v) const {
     if constexpr (DK == 0 && DJ == 0 && DI == 0) {
@@ -150,8 +165,8 @@ __device__ __forceinline__ T courant_adj(T vel, T ch, T dt, T geom, T lhcfl, T l
 
 // Right x face of column (j,i) at level k: value and reverse.  Returns the
 // face value (the divergence adjoint of the flow coefficient needs it).
-template <typename T, class FU, class FV, class FW, class AU, class AV, class AW>
-__device__ T face_x_adj(int k, int j, int i, const Adv<T, 1>& F, const Acc<T>& aF, const FU& U, const FV& V,
+template <typename T, class AF, class FU, class FV, class FW, class AU, class AV, class AW>
+__device__ T face_x_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, const FU& U, const FV& V,
                         const FW& W, const AU& aU, const AV& aV, const AW& aW, const VMet& m, int kdof, T dt, T cx,
                         T cy, int nz, T lam) {
   const int kp1 = k + 1 < nz ? k + 1 : nz;
@@ -241,8 +256,8 @@ __device__ T face_x_adj(int k, int j, int i, const Adv<T, 1>& F, const Acc<T>& a
 }
 
 // North y face of row j at column i, level k: value and reverse.
-template <typename T, class FU, class FV, class FW, class AU, class AV, class AW>
-__device__ T face_y_adj(int k, int j, int i, const Adv<T, 1>& F, const Acc<T>& aF, const FU& U, const FV& V,
+template <typename T, class AF, class FU, class FV, class FW, class AU, class AV, class AW>
+__device__ T face_y_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, const FU& U, const FV& V,
                         const FW& W, const AU& aU, const AV& aV, const AW& aW, const VMet& m, int kdof, T dt, T cx,
                         T cy, int nz, T lam) {
   const int kp1 = k + 1 < nz ? k + 1 : nz;
@@ -327,8 +342,8 @@ __device__ T face_y_adj(int k, int j, int i, const Adv<T, 1>& F, const Acc<T>& a
 }
 
 // Reverse of the z faces' transverse terms (ZGrad) for one field.
-template <typename T, class FU, class FV, class AU, class AV>
-__device__ __forceinline__ void zgrad_adj(int k, int j, int i, int kk, T fc, const Adv<T, 1>& F, const Acc<T>& aF,
+template <typename T, class AF, class FU, class FV, class AU, class AV>
+__device__ __forceinline__ void zgrad_adj(int k, int j, int i, int kk, T fc, const Adv<T, 1>& F, const AF& aF,
                                           const FU& U, const FV& V, const AU& aU, const AV& aV, T dt, T cx, T cy,
                                           T lg1, T lg2, T& lfc) {
   const T v00 = V(k, j - 1, i), v01 = V(k, j, i), v10 = V(k + 1, j - 1, i), v11 = V(k + 1, j, i);
@@ -360,8 +375,8 @@ __device__ __forceinline__ void zgrad_adj(int k, int j, int i, int kk, T fc, con
 }
 
 // Top face of level k (flux_z(k+1)), 1 <= k <= nz-1: value and reverse.
-template <typename T, class FU, class FV, class FW, class AU, class AV, class AW>
-__device__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F, const Acc<T>& aF, const FU& U, const FV& V,
+template <typename T, class AF, class FU, class FV, class FW, class AU, class AV, class AW>
+__device__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, const FU& U, const FV& V,
                         const FW& W, const AU& aU, const AV& aV, const AW& aW, const VMet& m, int kdof, int kmin, T dt,
                         T cx, T cy, int nz, T lam) {
   if (k == 1 && kmin > 1) return T(0.0);
@@ -518,20 +533,16 @@ struct AdjArgs {
   T dtm;
 };
 
-// One thread per (k, j, i) over levels 1..nz, rows 2..ny+2, columns 2..nx+2:
-// the owner of the right x face, north y face and top z face of that point.
-// Block (bx, by, bz): 64 levels x 4 rows of one column.
-template <typename T, int KIND>
-__device__ __forceinline__ void adjoint_block(const LaunchArgs<T>& L, const AdjArgs<T>& A, int bx, int by, int bz) {
+// The face owner (k, j, i), 1 <= k <= nz, 2 <= j <= ny+2, 2 <= i <= nx+2: the
+// right x face, north y face and top z face of that point, their adjoints
+// scattered into the accumulators aF (advected field) and aU, aV, aW (flows,
+// through the dual-grid interpolation of the field kind).
+template <typename T, int KIND, class AF, class AU, class AV, class AW>
+__device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjArgs<T>& A, int k, int j, int i,
+                                              const AF& aF, const AU& aU, const AV& aV, const AW& aW) {
   const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;
-  const int k = 1 + bx * 64 + (int)threadIdx.x;
-  const int j = 2 + by * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int i = 2 + bz;
   if (j > ny + 2 || k > nz) return;
   const Flows<T, KIND> fl(L);
-  constexpr int DK = Flows<T, KIND>::DK, DJ = Flows<T, KIND>::DJ, DI = Flows<T, KIND>::DI;
-  const AccFlow<T, DK, DJ, DI> aU{{A.lu, L.d}, nz}, aV{{A.lv, L.d}, nz}, aW{{A.lw, L.d}, nz};
-  const Acc<T> aF{A.lf, L.d};
   Adv<T, 1> F{};
   F.a[0] = Arr<T>(A.F, L.d);
   const VMet m = KIND == KIND_W ? L.mw : L.mc;
@@ -575,9 +586,101 @@ __device__ __forceinline__ void adjoint_block(const LaunchArgs<T>& L, const AdjA
   }
 }
 
+// One thread per face owner with global atomics (the reference form of the
+// adjoint scatter; the march form below accumulates in LDS first).
+template <typename T, int KIND>
+__device__ __forceinline__ void adjoint_block(const LaunchArgs<T>& L, const AdjArgs<T>& A, int bx, int by, int bz) {
+  constexpr int DK = Flows<T, KIND>::DK, DJ = Flows<T, KIND>::DJ, DI = Flows<T, KIND>::DI;
+  const int nz = L.d.nz;
+  const AccFlow<T, DK, DJ, DI> aU{{A.lu, L.d}, nz}, aV{{A.lv, L.d}, nz}, aW{{A.lw, L.d}, nz};
+  const Acc<T> aF{A.lf, L.d};
+  const int k = 1 + bx * 64 + (int)threadIdx.x;
+  const int j = 2 + by * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
+  adjoint_point<T, KIND>(L, A, k, j, 2 + bz, aF, aU, aV, aW);
+}
+
+// LDS of the march adjoint: the rings of the four target fields (advected
+// field, u, v, w flows) for R owner rows.
+template <typename T, int R>
+struct AdjLds {
+  T ring[4][4][R + 3][ADJ_KB];
+};
+
+// March adjoint tile: 64 levels (band `band`) x R owner rows (from row 2 +
+// ty*R) x the x strip of owner columns from 2 + tx*strip.  Each column of
+// owners scatters into the LDS rings with LDS atomics; a column leaves the
+// ring (x-1 after owners x) once no later owner of the tile can reach it, and
+// is added to the global adjoints with one atomic per non-zero value.  Device-
+// scope atomics are carried out beyond the L2, so the ring cuts the atomics
+// that reach memory from ~70 per face owner to ~(strip+3)(R+3)/(strip R) per
+// target.  Every thread of the block calls this with the same tile.
+template <typename T, int KIND, int R>
+__device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjArgs<T>& A, int band, int ty, int tx,
+                                             int strip, AdjLds<T, R>& lds) {
+  constexpr int DK = Flows<T, KIND>::DK, DJ = Flows<T, KIND>::DJ, DI = Flows<T, KIND>::DI;
+  constexpr int ROWS = R + 3, PER = ROWS * ADJ_KB, NT = 64 * R;
+  const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;
+  const int lane = threadIdx.x, r = (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int tid = lane + 64 * r;
+  const int k0 = 1 + 64 * band, j0 = 2 + ty * R;
+  const int i_begin = 2 + tx * strip, i_end = min(i_begin + strip, nx + 3);
+  // An owner's face adjoints are proportional to the incoming adjoint at
+  // (k,j,i), (k,j,i+1), (k,j+1,i) and (k+1,j,i): a tile that reads only zeros
+  // there adds nothing (the adjoint of a localised loss is zero far from it).
+  {
+    const int k = k0 + lane, j = j0 + r;
+    int any = 0;
+    if (j <= ny + 2 && k <= nz)
+      for (int i = i_begin; i < i_end && !any; i++) {
+        any = A.lin[at(L.d, k, j, i)] != T(0) || A.lin[at(L.d, k, j, i + 1)] != T(0) ||
+              A.lin[at(L.d, k, j + 1, i)] != T(0) || (k < nz && A.lin[at(L.d, k + 1, j, i)] != T(0));
+      }
+    if (!__syncthreads_or(any)) return;
+  }
+  T* flat = &lds.ring[0][0][0][0];
+  for (int e = tid; e < 4 * 4 * PER; e += NT) flat[e] = T(0);
+  __syncthreads();
+  const LdsAcc<T> aF{&lds.ring[0][0][0][0], k0, j0, ROWS};
+  const AccFlow<T, DK, DJ, DI, LdsAcc<T>> aU{{&lds.ring[1][0][0][0], k0, j0, ROWS}, nz},
+      aV{{&lds.ring[2][0][0][0], k0, j0, ROWS}, nz}, aW{{&lds.ring[3][0][0][0], k0, j0, ROWS}, nz};
+  T* const dst[4] = {A.lf, A.lu, A.lv, A.lw};
+  // add ring column c (all four targets) to the global adjoints and clear it
+  auto flush = [&](int c) {
+    if (c < 1 || c > nx + 4) return;
+    for (int e = tid; e < 4 * PER; e += NT) {
+      const int t = e / PER, rem = e - t * PER;
+      const int rr = rem / ADJ_KB, lev = rem - rr * ADJ_KB;
+      T* p = &lds.ring[t][c & 3][rr][lev];
+      const T v = *p;
+      const int kk = k0 - 1 + lev, jj = j0 - 1 + rr;
+      if (v != T(0) && kk >= 1 && kk <= nz && jj >= 1 && jj <= ny + 4) unsafeAtomicAdd(dst[t] + at(L.d, kk, jj, c), v);
+      *p = T(0);
+    }
+  };
+  for (int i = i_begin; i < i_end; i++) {
+    adjoint_point<T, KIND>(L, A, k0 + lane, j0 + r, i, aF, aU, aV, aW);
+    __syncthreads();
+    flush(i - 1);
+    __syncthreads();
+  }
+  // the last three columns the tile reached (distinct ring slots)
+  flush(i_end - 1);
+  flush(i_end);
+  flush(i_end + 1);
+  __syncthreads();
+}
+
 template <typename T, int KIND>
 __global__ void __launch_bounds__(256) adjoint_kernel(LaunchArgs<T> L, AdjArgs<T> A) {
   adjoint_block<T, KIND>(L, A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+}
+
+// March adjoint: grid (64-level bands, owner-row tiles, x strips).
+constexpr int ADJ_ROWS = 4;
+template <typename T, int KIND>
+__global__ void __launch_bounds__(64 * ADJ_ROWS) adjoint_march_kernel(LaunchArgs<T> L, AdjArgs<T> A, int strip) {
+  __shared__ AdjLds<T, ADJ_ROWS> lds;
+  adjoint_tile<T, KIND, ADJ_ROWS>(L, A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, strip, lds);
 }
 
 // Adjoint of the periodic halo fill, in two passes so every target is

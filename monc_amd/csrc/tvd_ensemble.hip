@@ -42,6 +42,7 @@ constexpr int ENS_NKMAX = 256;   // tallest column: 4 sub-columns of 64 levels
 constexpr int ENS_ALIGN = 64;    // field alignment in elements (256 B in FP32)
 constexpr int ENS_WAVES[3] = {4, 8, 12};
 constexpr int ENS_STRIPS[4] = {8, 16, 32, 64};
+constexpr int ENS_ADJ_STRIP = 16;  // owner columns per march adjoint tile
 
 // One member as the device sees it.
 struct EnsMember {
@@ -154,9 +155,9 @@ __host__ __device__ inline long long op_blocks_per_field(int op, int nz, int ny,
     case OP_HALO: m2 = (4 * (nx + 4) + 4 * ny + 3) / 4; break;
     case OP_FOLD_ROWS: m2 = nx + 4; break;
     case OP_FOLD_COLS: m2 = ny; break;
-    default:  // one work item per x column: it loops over the column's blocks
-      if (n2) *n2 = nx + 1;
-      return nx + 1;
+    default:  // one work item per march adjoint tile (64 levels x 4 owner rows x 16 owner columns)
+      m2 = (ny + 1 + ADJ_ROWS - 1) / ADJ_ROWS * ((nx + 1 + ENS_ADJ_STRIP - 1) / ENS_ADJ_STRIP);
+      break;
   }
   if (n2) *n2 = m2;
   return (long long)nbx * m2;
@@ -183,20 +184,26 @@ __global__ void __launch_bounds__(256) ens_grid_kernel(const EnsMember* mem, con
   }
 }
 
-// Adjoint of one forward step of every member (adjoint_kernel per field):
-// one work item is one x column (bz) of one field of one member, whose
-// 64-level x 4-row blocks the block visits in turn.
+// Adjoint of one forward step of every member: one work item is one march
+// adjoint tile (adjoint_tile: 64 levels x 4 owner rows x 16 owner columns,
+// LDS accumulation) of one field of one member.
 template <typename T>
-__global__ void __launch_bounds__(256) ens_adjoint_kernel(const EnsMember* mem, const double* met,
-                                                          const long long* prefix, int nmem, const T* state,
-                                                          const T* lin, T* lam) {
+__global__ void __launch_bounds__(64 * ADJ_ROWS) ens_adjoint_kernel(const EnsMember* mem, const double* met,
+                                                                    const long long* prefix, int nmem,
+                                                                    const T* state, const T* lin, T* lam) {
+  __shared__ AdjLds<T, ADJ_ROWS> lds;
   const long long total = prefix[nmem];
   for (long long b = blockIdx.x; b < total; b += gridDim.x) {
     const int m = member_of(prefix, nmem, b);
     const EnsMember& M = mem[m];
-    const int r = (int)(b - prefix[m]);
-    const int f = r / (M.d.nx + 1), bz = r - f * (M.d.nx + 1);
-    const int nbx = (M.d.nz + 63) / 64, nby = (M.d.ny + 1 + 3) / 4;
+    const int nbx = (M.d.nz + 63) / 64, nty = (M.d.ny + 1 + ADJ_ROWS - 1) / ADJ_ROWS;
+    const int per = (int)op_blocks_per_field(OP_ADJOINT, M.d.nz, M.d.ny, M.d.nx, nullptr);
+    int r = (int)(b - prefix[m]);
+    const int f = r / per;
+    r -= f * per;
+    const int band = r % nbx;
+    r /= nbx;
+    const int ty = r % nty, tx = r / nty;
     const LaunchArgs<T> L = member_args<T>(M, state, met);
     AdjArgs<T> A;
     const long long fo = M.off + (long long)f * M.fstride;
@@ -207,15 +214,12 @@ __global__ void __launch_bounds__(256) ens_adjoint_kernel(const EnsMember* mem, 
     A.lv = lam + M.off + M.fstride;
     A.lw = lam + M.off + 2 * M.fstride;
     A.dtm = (T)M.dtm;
-    for (int by = 0; by < nby; by++)
-      for (int bx = 0; bx < nbx; bx++) {
-        switch (f) {
-          case 0: adjoint_block<T, KIND_U>(L, A, bx, by, bz); break;
-          case 1: adjoint_block<T, KIND_V>(L, A, bx, by, bz); break;
-          case 2: adjoint_block<T, KIND_W>(L, A, bx, by, bz); break;
-          default: adjoint_block<T, KIND_SCALAR>(L, A, bx, by, bz); break;
-        }
-      }
+    switch (f) {
+      case 0: adjoint_tile<T, KIND_U, ADJ_ROWS>(L, A, band, ty, tx, ENS_ADJ_STRIP, lds); break;
+      case 1: adjoint_tile<T, KIND_V, ADJ_ROWS>(L, A, band, ty, tx, ENS_ADJ_STRIP, lds); break;
+      case 2: adjoint_tile<T, KIND_W, ADJ_ROWS>(L, A, band, ty, tx, ENS_ADJ_STRIP, lds); break;
+      default: adjoint_tile<T, KIND_SCALAR, ADJ_ROWS>(L, A, band, ty, tx, ENS_ADJ_STRIP, lds); break;
+    }
   }
 }
 
@@ -398,7 +402,8 @@ int fold_impl(Ens* e) {
 template <typename T>
 int adjoint_impl(Ens* e, const void* state) {
   HIP_TRY(hipMemcpyAsync(e->adj_in, e->adj, (size_t)e->pool_elems * e->es, hipMemcpyDeviceToDevice, e->stream));
-  hipLaunchKernelGGL(ens_adjoint_kernel<T>, dim3(grid_for(e, e->nblocks[OP_ADJOINT])), dim3(64, 4, 1), 0, e->stream,
+  hipLaunchKernelGGL(ens_adjoint_kernel<T>, dim3(grid_for(e, e->nblocks[OP_ADJOINT])), dim3(64, ADJ_ROWS, 1), 0,
+                     e->stream,
                      e->dmem, e->dmet, e->dprefix[OP_ADJOINT], e->nmem, (const T*)(state ? state : e->pool[e->cur]),
                      (const T*)e->adj_in, (T*)e->adj);
   HIP_TRY(hipGetLastError());

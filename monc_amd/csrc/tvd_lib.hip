@@ -725,8 +725,10 @@ int adjoint_step_impl(monc_tvd_ctx* c, double dtm) {
     HIP_TRY(hipGetLastError());
   }
   const LaunchArgs<T> L = launch_args<T>(c, dtm);
-  dim3 block(64, 4, 1);
-  dim3 grid((c->nz + 63) / 64, (c->ny + 1 + 3) / 4, c->nx + 1);
+  // march adjoint: 64-level bands x 4 owner rows x strips of 16 owner columns
+  constexpr int ADJ_STRIP = 16;
+  dim3 block(64, ADJ_ROWS, 1);
+  dim3 grid((c->nz + 63) / 64, (c->ny + 1 + ADJ_ROWS - 1) / ADJ_ROWS, (c->nx + 1 + ADJ_STRIP - 1) / ADJ_STRIP);
   for (int fl : fields) {
     AdjArgs<T> A;
     A.F = (const T*)slot(c, fl, MONC_TVD_PROG);
@@ -738,10 +740,18 @@ int adjoint_step_impl(monc_tvd_ctx* c, double dtm) {
     A.dtm = (T)dtm;
     LaunchTimer timer(c, kind_of(fl));
     switch (kind_of(fl)) {
-      case KIND_U: hipLaunchKernelGGL((adjoint_kernel<T, KIND_U>), grid, block, 0, c->stream, L, A); break;
-      case KIND_V: hipLaunchKernelGGL((adjoint_kernel<T, KIND_V>), grid, block, 0, c->stream, L, A); break;
-      case KIND_W: hipLaunchKernelGGL((adjoint_kernel<T, KIND_W>), grid, block, 0, c->stream, L, A); break;
-      default: hipLaunchKernelGGL((adjoint_kernel<T, KIND_SCALAR>), grid, block, 0, c->stream, L, A); break;
+      case KIND_U:
+        hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_U>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
+        break;
+      case KIND_V:
+        hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_V>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
+        break;
+      case KIND_W:
+        hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_W>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
+        break;
+      default:
+        hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_SCALAR>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
+        break;
     }
     HIP_TRY(hipGetLastError());
   }

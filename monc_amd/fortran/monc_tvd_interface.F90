@@ -64,6 +64,21 @@ module monc_tvd_interface_mod
       integer(c_int), value :: nsteps
     end function monc_tvd_forward_steps
 
+    integer(c_int) function monc_tvd_leapfrog_steps(ctx, dtm, tsmth, ugal, vgal, nsteps, first_forward) &
+        bind(C, name="monc_tvd_leapfrog_steps")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dtm, tsmth, ugal, vgal
+      integer(c_int), value :: nsteps, first_forward
+    end function monc_tvd_leapfrog_steps
+
+    integer(c_int) function monc_tvd_bind_field(ctx, field, variant, device_ptr) bind(C, name="monc_tvd_bind_field")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field, variant
+      type(c_ptr), value :: device_ptr
+    end function monc_tvd_bind_field
+
     integer(c_int) function monc_tvd_synchronize(ctx) bind(C, name="monc_tvd_synchronize")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
