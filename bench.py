@@ -232,6 +232,8 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the adjoint, leapfrog and ensemble measurements")
+    ap.add_argument("--only", choices=["adjoint", "leapfrog", "ensemble"], default=None,
+                    help="run only this side measurement (for profiling) and print its JSON object")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed work, run three 512 MiB device-to-device copies: the known-size "
                          "reference that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
@@ -254,6 +256,14 @@ def main() -> None:
     st = ss.make_state(nz, ny, nx, 25.0, 5000.0, seed=0, n_q=1, with_zfields=False)
     dt = ss.cfl_dt(st, 0.3)
     cells = nx * ny * nz
+
+    if args.only:
+        log(f"bench: {args.only} only")
+        side = {"adjoint": lambda: adjoint_form(st, dt, max(2, args.steps // 2)),
+                "leapfrog": lambda: leapfrog_form(st, dt, max(2, args.steps // 2)),
+                "ensemble": lambda: ensemble_form(max(4, args.steps // 2))}[args.only]()
+        print(json.dumps({args.only: side}), flush=True)
+        return
 
     log("bench: FP64 form")
     tune = not args.no_tune and args.kernel == 0 and args.rows == 0
