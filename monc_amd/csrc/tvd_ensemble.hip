@@ -96,7 +96,7 @@ __device__ __forceinline__ void ens_tile(const EnsMember& M, const double* met, 
 // another's arithmetic across the per-column barrier.
 template <int ES, int WAVES>
 constexpr int ens_wpe() {
-  return ES == 4 ? 6 : 3;
+  return ES == 4 ? 5 : 3;
 }
 
 // The persistent forward step: tiles are taken from ctr[0] until the list is

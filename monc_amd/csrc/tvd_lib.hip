@@ -189,14 +189,15 @@ void launch_split(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
 // waves per SIMD the compiler keeps registers for, lets two or more blocks
 // share a CU, so that one block's loads overlap another's arithmetic across
 // the per-column barrier.
-template <int ES, int NCH, int R>
+template <int ES, int NCH, int R, int NS>
 constexpr int march_wpe() {
-  return ES == 4 ? (NCH * R >= 16 ? 8 : 6) : (NCH * R >= 16 ? 4 : 3);
+  // the FP32 scalar pair carries two x windows: 5 waves (96 registers)
+  return ES == 4 ? (NCH * R >= 16 ? 8 : (NS == 2 ? 5 : 6)) : (NCH * R >= 16 ? 4 : 3);
 }
 
 template <typename T, int KIND, int NS, int MODE, int NCH, int R>
 void launch_march(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int strip) {
-  constexpr int WPE = march_wpe<(int)sizeof(T), NCH, R>();
+  constexpr int WPE = march_wpe<(int)sizeof(T), NCH, R, NS>();
   const int tiles_y = (c->ny + R - 1) / R, tiles_x = (c->nx + strip - 1) / strip;
   dim3 block(64, NCH * R, 1);
   hipLaunchKernelGGL((march_kernel<T, KIND, NS, MODE, NCH, R, WPE>), dim3(tiles_y * tiles_x), block, 0, c->stream,

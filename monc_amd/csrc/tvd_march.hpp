@@ -128,11 +128,13 @@ struct FlowsAt {
   T uk_m, uk_0, vk_s, vk_0, wk_0;   // U(k+1) at (j,i-1) (j,i); V(k+1) at (j-1,i) (j,i); W(k+1) at (j,i)
 };
 
-// Right x face of column (j, i) at the lane's level (ultimateflux.F90:102-165).
+// Right x face of column (j, i) at the lane's level (ultimateflux.F90:102-165);
+// xw[n] holds F(k, j, i-1 .. i+2) of field n (the x window of the march).
 template <typename T, int NS>
 __device__ __forceinline__ void mx_face(const __amdgpu_buffer_rsrc_t (&F)[NS], const Geo& g, const Levels& lv, int j,
-                                        int i, T u_m, T u_0, T u_p, T v_0, T v_s, T v_0p, T v_sp, T w_0, T w_p,
-                                        T wm_0, T wm_p, T rdz_k, T rdz_kd, T dt, T cx, T cy, T out[NS]) {
+                                        int i, const T (&xw)[NS][4], T u_m, T u_0, T u_p, T v_0, T v_s, T v_0p,
+                                        T v_sp, T w_0, T w_p, T wm_0, T wm_p, T rdz_k, T rdz_kd, T dt, T cx, T cy,
+                                        T out[NS]) {
   const bool upos = u_0 > T(0.0);
   T s = rdz_k * (pos(upos ? w_0 : w_p) + negabs(upos ? wm_0 : wm_p));
   s = s + cy * (pos(upos ? v_0 : v_0p) + negabs(upos ? v_s : v_sp));
@@ -148,14 +150,13 @@ __device__ __forceinline__ void mx_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
   const T cyt = fabs(sy) * cy * dt;
   const HLim<T> lim(u_0, cx, dt, s);
   const bool ap = sign_pos(u_0);
-  const unsigned s_m = g.so(j, i - 1), s_0 = g.so(j, i), s_1 = g.so(j, i + 1), s_2 = g.so(j, i + 2);
+  const unsigned s_0 = g.so(j, i), s_1 = g.so(j, i + 1);
   const unsigned sj0 = g.so(j - 1, i), sj1 = g.so(j - 1, i + 1);
 #pragma unroll
   for (int n = 0; n < NS; n++) {
     // every load is unconditional and only values are selected: a load whose
     // scalar offset were chosen per lane would have to be serialised
-    const T fa = ldb<T>(F[n], lv.k, s_0), fb = ldb<T>(F[n], lv.k, s_1);
-    const T fm = ldb<T>(F[n], lv.k, s_m), f2 = ldb<T>(F[n], lv.k, s_2);
+    const T fm = xw[n][0], fa = xw[n][1], fb = xw[n][2], f2 = xw[n][3];
     const T fu = ap ? fm : f2;
     const T fc = ap ? fa : fb;
     const T fd = ap ? fb : fa;
@@ -174,8 +175,8 @@ __device__ __forceinline__ void mx_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
 // u1/u1m = U at (row j, i), (row j, i-1), (row j+1, i), (row j+1, i-1).
 template <typename T, int NS>
 __device__ __forceinline__ void my_face(const __amdgpu_buffer_rsrc_t (&F)[NS], const Geo& g, const Levels& lv, int j,
-                                        int i, T vk, T vn, T vs, T w0, T w0m, T w1, T w1m, T u0, T u0m, T u1, T u1m,
-                                        T rdz_k, T rdz_kd, T dt, T cx, T cy, T out[NS]) {
+                                        int i, const T (&fc0)[NS], T vk, T vn, T vs, T w0, T w0m, T w1, T w1m, T u0,
+                                        T u0m, T u1, T u1m, T rdz_k, T rdz_kd, T dt, T cx, T cy, T out[NS]) {
   const bool vpos = sign_pos(vk);
   T s = rdz_k * (pos(vpos ? w0 : w1) + negabs(vpos ? w0m : w1m));
   s = s + cy * (pos(vpos ? vk : vn) + negabs(vpos ? vs : vk));
@@ -196,7 +197,7 @@ __device__ __forceinline__ void my_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
   const unsigned s_im = g.so(j, i - 1);
 #pragma unroll
   for (int n = 0; n < NS; n++) {
-    const T fa = ldb<T>(F[n], lv.k, s_0), fb = ldb<T>(F[n], lv.k, s_1);
+    const T fa = fc0[n], fb = ldb<T>(F[n], lv.k, s_1);
     const T fs = ldb<T>(F[n], lv.k, s_s), f2 = ldb<T>(F[n], lv.k, s_2);
     const T fu = vpos ? fs : f2;
     const T fc = vpos ? fa : fb;
@@ -214,7 +215,8 @@ __device__ __forceinline__ void my_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
 // the top and has no inflow from above), `bot` marks k = 1 (no level below).
 template <typename T, int NS>
 __device__ __forceinline__ void mz_face(const __amdgpu_buffer_rsrc_t (&F)[NS], const Geo& g, const Levels& lv, int j,
-                                        int i, const FlowsAt<T>& q, bool edge, bool top, bool bot, T zs_up, T zs_dn,
+                                        int i, const T (&fc0)[NS], const FlowsAt<T>& q, bool edge, bool top, bool bot,
+                                        T zs_up, T zs_dn,
                                         T rdc_up, T rdc_dn, T rdu_up, T rdu_dn, T rdd, T dd2, T dt, T cx, T cy,
                                         T out[NS]) {
   const T wk = q.w_0;
@@ -246,7 +248,7 @@ __device__ __forceinline__ void mz_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
   const unsigned s_0 = g.so(j, i), s_jm = g.so(j - 1, i), s_im = g.so(j, i - 1);
 #pragma unroll
   for (int n = 0; n < NS; n++) {
-    const T fk = ldb<T>(F[n], lv.k, s_0), fk1 = ldb<T>(F[n], lv.kp1, s_0);
+    const T fk = fc0[n], fk1 = ldb<T>(F[n], lv.kp1, s_0);
     const T fkm = ldb<T>(F[n], lv.km1, s_0), fk2 = ldb<T>(F[n], lv.kp2, s_0);
     const T fdn = top ? T(2.0) * fk1 - fk : fk2;
     const T fu = up ? fkm : fdn;
@@ -338,15 +340,45 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
   const int kl = (k <= NK ? k : NK) - 1;
   const T rdz_k = lds.met[M_RDZ][kl], rdz_kd = lds.met[M_RDZKD][kl];
 
+  // The march carries, from column i to column i+1, the x window of the
+  // advected fields at (k, j): xw[n] = F(i-1 .. i+2), one new load per column
+  // (the centre value also serves the y and z faces and the step's base), and
+  // the flows the two columns share (U at i-1, i; U(j+1) at i-1; V at i and
+  // (j-1, i); W(k), W(k-1) at i; U(k+1) at i-1): each value is loaded once.
+  // The kernels are bound by the vector-memory address path, so loads saved
+  // are time saved.  The FP64 scalar pair has no registers to carry them at
+  // 3 waves per SIMD and reloads them every column (WIN = false).
+  constexpr bool WIN = !(sizeof(T) == 8 && NS == 2);
   T fxl[NS];
+  T xw[NS][4];
+  T c_um = T(0), c_u0 = T(0), c_u1m = T(0), c_v0 = T(0), c_vs = T(0), c_w0 = T(0), c_wm0 = T(0), c_ukm = T(0);
 #pragma unroll
-  for (int n = 0; n < NS; n++) fxl[n] = T(0);
+  for (int n = 0; n < NS; n++) fxl[n] = xw[n][0] = xw[n][1] = xw[n][2] = xw[n][3] = T(0);
   if (active) {
     const int i = i_begin - 1;
-    mx_face<T, NS>(F, g, lv, j, i, U(lv.k, lv.kp1, j, i - 1), U(lv.k, lv.kp1, j, i), U(lv.k, lv.kp1, j, i + 1),
-                   V(lv.k, lv.kp1, j, i), V(lv.k, lv.kp1, j - 1, i), V(lv.k, lv.kp1, j, i + 1),
-                   V(lv.k, lv.kp1, j - 1, i + 1), W(lv.k, lv.kp1, j, i), W(lv.k, lv.kp1, j, i + 1),
-                   W(lv.km1, lv.k, j, i), W(lv.km1, lv.k, j, i + 1), rdz_k, rdz_kd, dt, cx, cy, fxl);
+#pragma unroll
+    for (int n = 0; n < NS; n++)
+#pragma unroll
+      for (int d = 0; d < 4; d++) xw[n][d] = ldb<T>(F[n], lv.k, g.so(j, i - 1 + d));
+    const T um = U(lv.k, lv.kp1, j, i - 1);
+    c_um = U(lv.k, lv.kp1, j, i);
+    c_u0 = U(lv.k, lv.kp1, j, i + 1);
+    c_u1m = U(lv.k, lv.kp1, j + 1, i);
+    const T v0 = V(lv.k, lv.kp1, j, i), vs = V(lv.k, lv.kp1, j - 1, i);
+    c_v0 = V(lv.k, lv.kp1, j, i + 1);
+    c_vs = V(lv.k, lv.kp1, j - 1, i + 1);
+    const T w0 = W(lv.k, lv.kp1, j, i), wm0 = W(lv.km1, lv.k, j, i);
+    c_w0 = W(lv.k, lv.kp1, j, i + 1);
+    c_wm0 = W(lv.km1, lv.k, j, i + 1);
+    c_ukm = U(lv.kp1, lv.kp2, j, i);
+    mx_face<T, NS>(F, g, lv, j, i, xw, um, c_um, c_u0, v0, vs, c_v0, c_vs, w0, c_w0, wm0, c_wm0, rdz_k, rdz_kd, dt,
+                   cx, cy, fxl);
+#pragma unroll
+    for (int n = 0; n < NS; n++) {  // window of column i_begin: F(i_begin-1 .. i_begin+1), F(i_begin+2) below
+      xw[n][0] = xw[n][1];
+      xw[n][1] = xw[n][2];
+      xw[n][2] = xw[n][3];
+    }
   }
   int par = 0;
   for (int i = i_begin; i < i_end; i++) {
@@ -354,39 +386,62 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
 #pragma unroll
     for (int n = 0; n < NS; n++) fxr[n] = fyn[n] = fys[n] = fzt[n] = T(0);
     FlowsAt<T> q;
+    T fc0[NS];
     if (active) {
-      q.u_m = U(lv.k, lv.kp1, j, i - 1);
-      q.u_0 = U(lv.k, lv.kp1, j, i);
+      if constexpr (!WIN) {  // reload what the window would carry
+#pragma unroll
+        for (int n = 0; n < NS; n++)
+#pragma unroll
+          for (int d = 0; d < 3; d++) xw[n][d] = ldb<T>(F[n], lv.k, g.so(j, i - 1 + d));
+        c_um = U(lv.k, lv.kp1, j, i - 1);
+        c_u0 = U(lv.k, lv.kp1, j, i);
+        c_u1m = U(lv.k, lv.kp1, j + 1, i - 1);
+        c_v0 = V(lv.k, lv.kp1, j, i);
+        c_vs = V(lv.k, lv.kp1, j - 1, i);
+        c_w0 = W(lv.k, lv.kp1, j, i);
+        c_wm0 = W(lv.km1, lv.k, j, i);
+        c_ukm = U(lv.kp1, lv.kp2, j, i - 1);
+      }
+#pragma unroll
+      for (int n = 0; n < NS; n++) {
+        xw[n][3] = ldb<T>(F[n], lv.k, g.so(j, i + 2));
+        fc0[n] = xw[n][1];
+      }
+      q.u_m = c_um;
+      q.u_0 = c_u0;
       q.u_p = U(lv.k, lv.kp1, j, i + 1);
-      q.u1_m = U(lv.k, lv.kp1, j + 1, i - 1);
+      q.u1_m = c_u1m;
       q.u1_0 = U(lv.k, lv.kp1, j + 1, i);
-      q.v_0 = V(lv.k, lv.kp1, j, i);
-      q.v_s = V(lv.k, lv.kp1, j - 1, i);
+      q.v_0 = c_v0;
+      q.v_s = c_vs;
       q.v_0p = V(lv.k, lv.kp1, j, i + 1);
       q.v_sp = V(lv.k, lv.kp1, j - 1, i + 1);
       q.v_n = V(lv.k, lv.kp1, j + 1, i);
-      q.w_0 = W(lv.k, lv.kp1, j, i);
+      q.w_0 = c_w0;
       q.w_p = W(lv.k, lv.kp1, j, i + 1);
       q.w_n = W(lv.k, lv.kp1, j + 1, i);
-      q.wm_0 = W(lv.km1, lv.k, j, i);
+      q.wm_0 = c_wm0;
       q.wm_p = W(lv.km1, lv.k, j, i + 1);
       q.wm_n = W(lv.km1, lv.k, j + 1, i);
-      q.uk_m = U(lv.kp1, lv.kp2, j, i - 1);
+      q.uk_m = c_ukm;
       q.uk_0 = U(lv.kp1, lv.kp2, j, i);
       q.vk_s = V(lv.kp1, lv.kp2, j - 1, i);
       q.vk_0 = V(lv.kp1, lv.kp2, j, i);
       q.wk_0 = W(lv.kp1, lv.kp2, j, i);
-      my_face<T, NS>(F, g, lv, j, i, q.v_0, q.v_n, q.v_s, q.w_0, q.wm_0, q.w_n, q.wm_n, q.u_0, q.u_m, q.u1_0,
+      my_face<T, NS>(F, g, lv, j, i, fc0, q.v_0, q.v_n, q.v_s, q.w_0, q.wm_0, q.w_n, q.wm_n, q.u_0, q.u_m, q.u1_0,
                      q.u1_m, rdz_k, rdz_kd, dt, cx, cy, fyn);
       if (r == 0) {  // south face of the block's first row
-        my_face<T, NS>(F, g, lv, j - 1, i, q.v_s, q.v_0, V(lv.k, lv.kp1, j - 2, i), W(lv.k, lv.kp1, j - 1, i),
+        T fs0[NS];
+#pragma unroll
+        for (int n = 0; n < NS; n++) fs0[n] = ldb<T>(F[n], lv.k, g.so(j - 1, i));
+        my_face<T, NS>(F, g, lv, j - 1, i, fs0, q.v_s, q.v_0, V(lv.k, lv.kp1, j - 2, i), W(lv.k, lv.kp1, j - 1, i),
                        W(lv.km1, lv.k, j - 1, i), q.w_0, q.wm_0, U(lv.k, lv.kp1, j - 1, i),
                        U(lv.k, lv.kp1, j - 1, i - 1), q.u_0, q.u_m, rdz_k, rdz_kd, dt, cx, cy, fys);
       }
-      mx_face<T, NS>(F, g, lv, j, i, q.u_m, q.u_0, q.u_p, q.v_0, q.v_s, q.v_0p, q.v_sp, q.w_0, q.w_p, q.wm_0,
+      mx_face<T, NS>(F, g, lv, j, i, xw, q.u_m, q.u_0, q.u_p, q.v_0, q.v_s, q.v_0p, q.v_sp, q.w_0, q.w_p, q.wm_0,
                      q.wm_p, rdz_k, rdz_kd, dt, cx, cy, fxr);
       T zf[NS];
-      mz_face<T, NS>(F, g, lv, j, i, q, edge, ztop, zbot, lds.met[M_ZS_UP][kl], lds.met[M_ZS_DN][kl],
+      mz_face<T, NS>(F, g, lv, j, i, fc0, q, edge, ztop, zbot, lds.met[M_ZS_UP][kl], lds.met[M_ZS_DN][kl],
                      lds.met[M_RDC_UP][kl], lds.met[M_RDC_DN][kl], lds.met[M_RDU_UP][kl], lds.met[M_RDU_DN][kl],
                      lds.met[M_RDD][kl], lds.met[M_DD2][kl], dt, cx, cy, zf);
 #pragma unroll
@@ -424,7 +479,9 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
       for (int n = 0; n < NS; n++) {
         T sv;
         const T s0 = MODE == MODE_ACCUM ? ldb<T>(Fo[n], lv.k, so) : T(0);
-        const T fcur = MODE != MODE_ACCUM ? ldb<T>(Ff[n], lv.k, so) : T(0);
+        T fcur = T(0);
+        if constexpr (MODE == MODE_STEP) fcur = S.f[n] == S.F[n] ? fc0[n] : ldb<T>(Ff[n], lv.k, so);
+        else if constexpr (MODE == MODE_LEAP) fcur = ldb<T>(Ff[n], lv.k, so);
         if (top) {
           sv = (q.v_s * fys[n] - q.v_0 * fyn[n]) * cy;
           sv = sv + T(4.0) * t1 * q.wm_0 * fzb[n];
@@ -471,7 +528,22 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
       }
     }
 #pragma unroll
-    for (int n = 0; n < NS; n++) fxl[n] = fxr[n];
+    for (int n = 0; n < NS; n++) {
+      fxl[n] = fxr[n];
+      xw[n][0] = xw[n][1];
+      xw[n][1] = xw[n][2];
+      xw[n][2] = xw[n][3];
+    }
+    if (active) {
+      c_um = q.u_0;
+      c_u0 = q.u_p;
+      c_u1m = q.u1_0;
+      c_v0 = q.v_0p;
+      c_vs = q.v_sp;
+      c_w0 = q.w_p;
+      c_wm0 = q.wm_p;
+      c_ukm = q.uk_0;
+    }
   }
   __syncthreads();  // the LDS is reused by the block's next tile
 }
