@@ -627,14 +627,21 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjAr
   // An owner's face adjoints are proportional to the incoming adjoint at
   // (k,j,i), (k,j,i+1), (k,j+1,i) and (k+1,j,i): a tile that reads only zeros
   // there adds nothing (the adjoint of a localised loss is zero far from it).
+  // The check covers the box k..k+1, j..j+1, i_begin..i_end around the
+  // tile's owners (a superset): each lane reads its own (k, j) over the
+  // columns, the last row in range also row j+1 and lane 63 level k+1, all
+  // loads independent of each other.
   {
     const int k = k0 + lane, j = j0 + r;
     int any = 0;
-    if (j <= ny + 2 && k <= nz)
-      for (int i = i_begin; i < i_end && !any; i++) {
-        any = A.lin[at(L.d, k, j, i)] != T(0) || A.lin[at(L.d, k, j, i + 1)] != T(0) ||
-              A.lin[at(L.d, k, j + 1, i)] != T(0) || (k < nz && A.lin[at(L.d, k + 1, j, i)] != T(0));
+    if (j <= ny + 2 && k <= nz) {
+      const bool row_up = r == R - 1 || j == ny + 2, lev_up = lane == 63 && k < nz;
+      for (int i = i_begin; i <= i_end; i++) {
+        any |= A.lin[at(L.d, k, j, i)] != T(0);
+        if (row_up) any |= A.lin[at(L.d, k, j + 1, i)] != T(0);
+        if (lev_up) any |= A.lin[at(L.d, k + 1, j, i)] != T(0);
       }
+    }
     if (!__syncthreads_or(any)) return;
   }
   T* flat = &lds.ring[0][0][0][0];

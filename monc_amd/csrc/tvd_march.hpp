@@ -282,8 +282,9 @@ struct MarchLds {
 // One tile: cell rows j0 = 3 + ty*R .. j0+R-1 (R = WAVES / nch), all levels
 // (nch sub-columns of 64), x columns 3 + tx*strip onwards.  Wave wv takes
 // sub-column wv % nch of row wv / nch; waves past R*nch only take the
-// barriers.  Every wave of the block calls this with the same tile.
-template <typename T, int KIND, int NS, int MODE, int WAVES, int NKMAX, int NSL>
+// barriers.  Every wave of the block calls this with the same tile.  WIN:
+// carry the x windows from column to column (below) or reload them.
+template <typename T, int KIND, int NS, int MODE, int WAVES, int NKMAX, int NSL, bool WIN>
 __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSet<T>& S, int nch, int ty, int tx,
                                            int strip, MarchLds<T, NSL, WAVES, NKMAX>& lds) {
   constexpr bool IS_W = KIND == KIND_W;
@@ -346,9 +347,7 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
   // the flows the two columns share (U at i-1, i; U(j+1) at i-1; V at i and
   // (j-1, i); W(k), W(k-1) at i; U(k+1) at i-1): each value is loaded once.
   // The kernels are bound by the vector-memory address path, so loads saved
-  // are time saved.  The FP64 scalar pair has no registers to carry them at
-  // 3 waves per SIMD and reloads them every column (WIN = false).
-  constexpr bool WIN = !(sizeof(T) == 8 && NS == 2);
+  // are time saved.  With WIN = false every column reloads them.
   T fxl[NS];
   T xw[NS][4];
   T c_um = T(0), c_u0 = T(0), c_u1m = T(0), c_v0 = T(0), c_vs = T(0), c_w0 = T(0), c_wm0 = T(0), c_ukm = T(0);
@@ -559,7 +558,10 @@ march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
   const int nb = (int)gridDim.x, bid = (int)blockIdx.x;
   const int full = nb - nb % 8;
   const int t = bid < full ? (bid % 8) * (full / 8) + bid / 8 : bid;
-  march_tile<T, KIND, NS, MODE, NCH * R, 64 * NCH, NS>(L, S, NCH, t % tiles_y, t / tiles_y, strip, lds);
+  // The scalar pair reloads its x windows: two windows leave FP64 no
+  // registers at 3 waves per SIMD and cost FP32 a wave per SIMD, which
+  // measured slower than the loads they save.
+  march_tile<T, KIND, NS, MODE, NCH * R, 64 * NCH, NS, NS != 2>(L, S, NCH, t % tiles_y, t / tiles_y, strip, lds);
 }
 
 }  // namespace tvd
