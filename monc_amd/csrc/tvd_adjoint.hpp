@@ -41,6 +41,7 @@ struct Acc {
   T* a;
   Dims d;
   __device__ __forceinline__ void add(int k, int j, int i, T v) const { atomic_add(a + at(d, k, j, i), v); }
+  __device__ __forceinline__ void dyn(int k, int j, int i, T v) const { add(k, j, i, v); }
 };
 
 // Adjoint of one target field accumulated in LDS: a ring of 4 x columns
@@ -56,26 +57,126 @@ struct LdsAcc {
   __device__ __forceinline__ void add(int k, int j, int i, T v) const {
     if (v != T(0)) unsafeAtomicAdd(&s[((i & 3) * rows + (j - j0 + 1)) * ADJ_KB + (k - k0 + 1)], v);
   }
+  __device__ __forceinline__ void dyn(int k, int j, int i, T v) const { add(k, j, i, v); }
 };
+
+// One face owner's scatter gathered in registers before it reaches the LDS
+// rings.  LDS floating-point atomics issue at a small fraction of the rate of
+// plain LDS traffic (measured on MI355X: ~190 CU cycles per 64-lane ds_add_f32
+// against ~12 for a read and a write), and an owner adds ~110 times.  Slot
+// (t, dk, dj, di) holds the owner's total for target t at (k+dk, j+dj, i+di),
+// dk, dj, di in -1..2.  The offsets are compile-time constants at the call
+// sites of add(), so the slots no call reaches stay constant zero and vanish.
+// After each face the level offsets are folded onto each lane's own level
+// with wave shifts (lane L of a wave holds level k0+L), leaving one LDS atomic
+// per (t, dj, di) and the band's edge levels.  dyn() takes an offset chosen at
+// run time (an upwind pick) straight to the ring.
+template <typename T>
+struct Slots {
+  T v[4][4][4][4];  // [target][dk+1][dj+1][di+1]
+};
+
+template <typename T, int TGT>
+struct SlotAcc {
+  Slots<T>* s;
+  const LdsAcc<T>* ring;
+  int k, j, i;
+  // (kk, jj, ii) a fixed offset from the owner (k, j, i)
+  __device__ __forceinline__ void add(int kk, int jj, int ii, T v) const {
+    const int dk = kk - k, dj = jj - j, di = ii - i;
+    if (dk < -1 || dk > 2 || dj < -1 || dj > 2 || di < -1 || di > 2) {  // none by construction
+      ring[TGT].add(kk, jj, ii, v);
+      return;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          if (dk == a - 1 && dj == b - 1 && di == c - 1) s->v[TGT][a][b][c] += v;
+  }
+  __device__ __forceinline__ void dyn(int kk, int jj, int ii, T v) const { ring[TGT].add(kk, jj, ii, v); }
+};
+
+// Wave shifts by one lane (DPP wave_shr:1 / wave_shl:1); a lane shifted in
+// from outside the wave reads 0.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x) { return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, true); }
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_shift(T x) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, dpp_i<CTRL>(__builtin_bit_cast(int, x)));
+  } else {
+    const long long b = __builtin_bit_cast(long long, x);
+    const int lo = dpp_i<CTRL>((int)b), hi = dpp_i<CTRL>((int)(b >> 32));
+    return __builtin_bit_cast(T, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+  }
+}
+// lane L receives lane L-1's value / lane L+1's value
+template <typename T>
+__device__ __forceinline__ T from_below(T x) { return dpp_shift<0x138>(x); }
+template <typename T>
+__device__ __forceinline__ T from_above(T x) { return dpp_shift<0x130>(x); }
+// any lane of the wave non-zero (uniform; folds to false for a constant zero)
+template <typename T>
+__device__ __forceinline__ bool any_nz(T x) { return __builtin_amdgcn_ballot_w64(x != T(0)) != 0; }
+
+// Adds the slots of every lane of the wave (lane L = level k0+L of row j,
+// column i) to the rings.  Called by all 64 lanes together.
+template <typename T>
+__device__ __forceinline__ void flush_slots(const Slots<T>& s, const LdsAcc<T> (&ring)[4], int k, int j, int i,
+                                            int lane) {
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const T vm = s.v[t][0][b][c], v0 = s.v[t][1][b][c], v1 = s.v[t][2][b][c], v2 = s.v[t][3][b][c];
+        const bool hm = any_nz(vm), h1 = any_nz(v1), h2 = any_nz(v2);
+        const int jj = j + b - 1, ii = i + c - 1;
+        // level k0+L collects dk = 0 of lane L, +1 of lane L-1, +2 of lane L-2
+        // and -1 of lane L+1; `up` at lane L is what lanes L and L-1 send to
+        // level k0+L+1
+        T up = v1;
+        if (h2) up += from_below(v2);
+        T tot = v0;
+        if (h1 || h2) tot += from_below(up);
+        if (hm) tot += from_above(vm);
+        ring[t].add(k, jj, ii, tot);
+        // the band's edges: level k0-1 from lane 0, k0+64 and k0+65 from lanes 62 and 63
+        if (hm) ring[t].add(k - 1, jj, ii, lane == 0 ? vm : T(0));
+        if (h1 || h2) ring[t].add(k + 1, jj, ii, lane == 63 ? up : T(0));
+        if (h2) ring[t].add(k + 2, jj, ii, lane == 63 ? v2 : T(0));
+      }
+}
 
 // Adjoint of Flow<T,DK,DJ,DI>: the dual-grid interpolation scatters to both points.
 template <typename T, int DK, int DJ, int DI, class ACC = Acc<T>>
 struct AccFlow {
   ACC a;
   int nz;
-  __device__ __forceinline__ void add(int k, int j, int i, T v) const {
+  __device__ __forceinline__ void add(int k, int j, int i, T v) const { put<false>(k, j, i, v); }
+  __device__ __forceinline__ void dyn(int k, int j, int i, T v) const { put<true>(k, j, i, v); }
+  template <bool DYN>
+  __device__ __forceinline__ void put(int k, int j, int i, T v) const {
+    auto to = [&](int kk, int jj, int ii, T x) {
+      if constexpr (DYN) a.dyn(kk, jj, ii, x);
+      else a.add(kk, jj, ii, x);
+    };
     if constexpr (DK == 0 && DJ == 0 && DI == 0) {
-      a.add(k, j, i, v);
+      to(k, j, i, v);
     } else if constexpr (DK == 1) {
       if (k < nz) {
-        a.add(k, j, i, T(0.5) * v);
-        a.add(k + 1, j, i, T(0.5) * v);
+        to(k, j, i, T(0.5) * v);
+        to(k + 1, j, i, T(0.5) * v);
       } else {
-        a.add(k, j, i, v);
+        to(k, j, i, v);
       }
     } else {
-      a.add(k, j, i, T(0.5) * v);
-      a.add(k, j + DJ, i + DI, T(0.5) * v);
+      to(k, j, i, T(0.5) * v);
+      to(k, j + DJ, i + DI, T(0.5) * v);
     }
   }
 };
@@ -166,7 +267,7 @@ __device__ __forceinline__ T courant_adj(T vel, T ch, T dt, T geom, T lhcfl, T l
 // Right x face of column (j,i) at level k: value and reverse.  Returns the
 // face value (the divergence adjoint of the flow coefficient needs it).
 template <typename T, class AF, class FU, class FV, class FW, class AU, class AV, class AW>
-__device__ T face_x_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, const FU& U, const FV& V,
+__device__ __forceinline__ T face_x_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, const FU& U, const FV& V,
                         const FW& W, const AU& aU, const AV& aV, const AW& aW, const VMet& m, int kdof, T dt, T cx,
                         T cy, int nz, T lam) {
   const int kp1 = k + 1 < nz ? k + 1 : nz;
@@ -218,12 +319,12 @@ __device__ T face_x_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, c
   aF.add(k, j, i + 1, lfc * an + r.fd * ap);
   aF.add(k, j, i - 1, r.fu * ap);
   aF.add(k, j, i + 2, r.fu * an);
-  aF.add(ksl, j, i, ltmp * ap);
-  aF.add(ksu, j, i, -(ltmp * ap));
-  aF.add(ksl, j, i + 1, ltmp * an);
-  aF.add(ksu, j, i + 1, -(ltmp * an));
-  aF.add(k, js, i, lt * ap);
-  aF.add(k, js, i + 1, lt * an);
+  aF.dyn(ksl, j, i, ltmp * ap);
+  aF.dyn(ksu, j, i, -(ltmp * ap));
+  aF.dyn(ksl, j, i + 1, ltmp * an);
+  aF.dyn(ksu, j, i + 1, -(ltmp * an));
+  aF.dyn(k, js, i, lt * ap);
+  aF.dyn(k, js, i + 1, lt * an);
   // flows
   const T lS = -(lim.hrcfl * lim.hrcfl) * r.hrcfl;
   T luk = courant_adj(uk, cx, dt, T(1.0), r.hcfl, r.cflcoef);
@@ -257,7 +358,7 @@ __device__ T face_x_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, c
 
 // North y face of row j at column i, level k: value and reverse.
 template <typename T, class AF, class FU, class FV, class FW, class AU, class AV, class AW>
-__device__ T face_y_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, const FU& U, const FV& V,
+__device__ __forceinline__ T face_y_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, const FU& U, const FV& V,
                         const FW& W, const AU& aU, const AV& aV, const AW& aW, const VMet& m, int kdof, T dt, T cx,
                         T cy, int nz, T lam) {
   const int kp1 = k + 1 < nz ? k + 1 : nz;
@@ -308,10 +409,10 @@ __device__ T face_y_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, c
   aF.add(k, j + 1, i, lfc * an + r.fd * ap);
   aF.add(k, j - 1, i, r.fu * ap);
   aF.add(k, j + 2, i, r.fu * an);
-  aF.add(ksl, jj, i, ldiff);
-  aF.add(ksu, jj, i, -ldiff);
-  aF.add(k, jj, i + 1, ltx * (T(1.0) - posuf));
-  aF.add(k, jj, i - 1, ltx * posuf);
+  aF.dyn(ksl, jj, i, ldiff);
+  aF.dyn(ksu, jj, i, -ldiff);
+  aF.dyn(k, jj, i + 1, ltx * (T(1.0) - posuf));
+  aF.dyn(k, jj, i - 1, ltx * posuf);
   const T lS = -(lim.hrcfl * lim.hrcfl) * r.hrcfl;
   T lvk = courant_adj(vk, cy, dt, T(1.0), r.hcfl, r.cflcoef);
   const T lsz = T(0.125) * (lcz * sgn_abs(sz) * rzk * dt);
@@ -359,9 +460,9 @@ __device__ __forceinline__ void zgrad_adj(int k, int j, int i, int kk, T fc, con
   const T tx = fe * (T(1.0) - posuf) + fw * posuf;
   // g1 = cyt*(fc - fsj), g2 = cxt*(fc - tx)
   lfc = lfc + lg1 * cyt + lg2 * cxt;
-  aF.add(kk, js, i, -(lg1 * cyt));
-  aF.add(kk, j, i + 1, -(lg2 * cxt) * (T(1.0) - posuf));
-  aF.add(kk, j, i - 1, -(lg2 * cxt) * posuf);
+  aF.dyn(kk, js, i, -(lg1 * cyt));
+  aF.dyn(kk, j, i + 1, -(lg2 * cxt) * (T(1.0) - posuf));
+  aF.dyn(kk, j, i - 1, -(lg2 * cxt) * posuf);
   const T lsy = T(0.125) * (lg1 * (fc - fsj) * sgn_abs(sy) * cy * dt);
   const T lsx = T(0.125) * (lg2 * (fc - tx) * sgn_abs(sx) * cx * dt);
   aV.add(k, j - 1, i, lsy);
@@ -376,7 +477,7 @@ __device__ __forceinline__ void zgrad_adj(int k, int j, int i, int kk, T fc, con
 
 // Top face of level k (flux_z(k+1)), 1 <= k <= nz-1: value and reverse.
 template <typename T, class AF, class FU, class FV, class FW, class AU, class AV, class AW>
-__device__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, const FU& U, const FV& V,
+__device__ __forceinline__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, const FU& U, const FV& V,
                         const FW& W, const AU& aU, const AV& aV, const AW& aW, const VMet& m, int kdof, int kmin, T dt,
                         T cx, T cy, int nz, T lam) {
   if (k == 1 && kmin > 1) return T(0.0);
@@ -469,17 +570,17 @@ __device__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, c
   if (mode == 0) {
     aF.add(k - 1, j, i, r.fu * ap);
     aF.add(k + 2, j, i, r.fu * an);
-    aF.add(kk, j, i, lfc);
+    aF.dyn(kk, j, i, lfc);
     aF.add(k + 1, j, i, r.fd * ap);
     aF.add(k, j, i, r.fd * an);
   } else if (bottom) {
     if (mode == 1) {  // fu = fc = F(1), fd = F(2)
-      aF.add(1, j, i, r.fu + lfc);
-      aF.add(2, j, i, r.fd);
+      aF.dyn(1, j, i, r.fu + lfc);
+      aF.dyn(2, j, i, r.fd);
     } else {  // fu = F(3), fc = F(2), fd = F(1)
-      aF.add(3, j, i, r.fu);
-      aF.add(2, j, i, lfc);
-      aF.add(1, j, i, r.fd);
+      aF.dyn(3, j, i, r.fu);
+      aF.dyn(2, j, i, lfc);
+      aF.dyn(1, j, i, r.fd);
     }
   } else {
     if (mode == 1) {  // fu = F(k-1), fc = F(k), fd = F(k+1)
@@ -498,19 +599,19 @@ __device__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, c
   const T lS = -(lim.hrcfl * lim.hrcfl) * r.hrcfl;
   const T lS3 = lS * dt;
   const T lB = lS3 * cy, lC = lS3 * cx;
-  aV.add(kv, j, i, lB * dpos(vb0));
-  aV.add(kv, j - 1, i, lB * dnegabs(vb1));
-  aU.add(kv, j, i, lC * dpos(ub0));
-  aU.add(kv, j, i - 1, lC * dnegabs(ub1));
+  aV.dyn(kv, j, i, lB * dpos(vb0));
+  aV.dyn(kv, j - 1, i, lB * dnegabs(vb1));
+  aU.dyn(kv, j, i, lC * dpos(ub0));
+  aU.dyn(kv, j, i - 1, lC * dnegabs(ub1));
   const T lW = lS3;  // adjoint of the W part a0 of S
   if (mode == 0) {
     const T wa = W(kw0, j, i), wb = W(kw1, j, i);
-    aW.add(kw0, j, i, lW * rzw * dpos(wa));
-    aW.add(kw1, j, i, lW * rzw * dnegabs(wb));
+    aW.dyn(kw0, j, i, lW * rzw * dpos(wa));
+    aW.dyn(kw1, j, i, lW * rzw * dnegabs(wb));
   } else if (bottom && mode == 1) {  // a0 = wk * rzw
     lwk = lwk + lW * rzw;
   } else if (bottom) {  // a0 = rzw * (pos(W(2)) - wk)
-    aW.add(2, j, i, lW * rzw * dpos(W(2, j, i)));
+    aW.dyn(2, j, i, lW * rzw * dpos(W(2, j, i)));
     lwk = lwk - lW * rzw;
   } else if (mode == 1) {  // a0 = rzw * (wk + negabs(W(k-1)))
     lwk = lwk + lW * rzw;
@@ -536,12 +637,15 @@ struct AdjArgs {
 // The face owner (k, j, i), 1 <= k <= nz, 2 <= j <= ny+2, 2 <= i <= nx+2: the
 // right x face, north y face and top z face of that point, their adjoints
 // scattered into the accumulators aF (advected field) and aU, aV, aW (flows,
-// through the dual-grid interpolation of the field kind).
-template <typename T, int KIND, class AF, class AU, class AV, class AW>
+// through the dual-grid interpolation of the field kind).  `after_face()` runs
+// after each face in control flow that every lane of the wave reaches (lanes
+// past nz take no face); register accumulators are emptied there.
+template <typename T, int KIND, class AF, class AU, class AV, class AW, class FL>
 __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjArgs<T>& A, int k, int j, int i,
-                                              const AF& aF, const AU& aU, const AV& aV, const AW& aW) {
+                                              const AF& aF, const AU& aU, const AV& aV, const AW& aW,
+                                              const FL& after_face) {
   const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;
-  if (j > ny + 2 || k > nz) return;
+  if (j > ny + 2) return;  // uniform over the wave
   const Flows<T, KIND> fl(L);
   Adv<T, 1> F{};
   F.a[0] = Arr<T>(A.F, L.d);
@@ -567,6 +671,7 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
     const T fx = face_x_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, dt, L.cx, L.cy, nz, uf * L.cx * d);
     aU.add(k, j, i, fx * L.cx * d);
   }
+  after_face();
   // y face (k, j+1/2, i)
   if (col_in && k >= 2 && k <= kmax) {
     const T d = A.dtm * (lraw(k, j + 1, i) - lself);
@@ -574,6 +679,7 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
     const T fy = face_y_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, dt, L.cx, L.cy, nz, vf * L.cy * d);
     aV.add(k, j, i, fy * L.cy * d);
   }
+  after_face();
   // z face at level k (flux_z(k+1)): s(k) -= 4 W Fz t2(k), s(k+1) += 4 W Fz t1(k+1);
   // t1(k+1) l(k+1) - t2(k) l(k) = t1(k+1) (l(k+1) - l(k)) + (t1(k+1) - t2(k)) l(k),
   // the metric difference formed in FP64
@@ -584,6 +690,7 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
     const T fz = face_z_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, kmin, dt, L.cx, L.cy, nz, wf * e);
     aW.add(k, j, i, fz * e);
   }
+  after_face();
 }
 
 // One thread per face owner with global atomics (the reference form of the
@@ -596,7 +703,7 @@ __device__ __forceinline__ void adjoint_block(const LaunchArgs<T>& L, const AdjA
   const Acc<T> aF{A.lf, L.d};
   const int k = 1 + bx * 64 + (int)threadIdx.x;
   const int j = 2 + by * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
-  adjoint_point<T, KIND>(L, A, k, j, 2 + bz, aF, aU, aV, aW);
+  if (k <= nz) adjoint_point<T, KIND>(L, A, k, j, 2 + bz, aF, aU, aV, aW, [] {});
 }
 
 // LDS of the march adjoint: the rings of the four target fields (advected
@@ -647,9 +754,10 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjAr
   T* flat = &lds.ring[0][0][0][0];
   for (int e = tid; e < 4 * 4 * PER; e += NT) flat[e] = T(0);
   __syncthreads();
-  const LdsAcc<T> aF{&lds.ring[0][0][0][0], k0, j0, ROWS};
-  const AccFlow<T, DK, DJ, DI, LdsAcc<T>> aU{{&lds.ring[1][0][0][0], k0, j0, ROWS}, nz},
-      aV{{&lds.ring[2][0][0][0], k0, j0, ROWS}, nz}, aW{{&lds.ring[3][0][0][0], k0, j0, ROWS}, nz};
+  const LdsAcc<T> ring[4] = {{&lds.ring[0][0][0][0], k0, j0, ROWS},
+                             {&lds.ring[1][0][0][0], k0, j0, ROWS},
+                             {&lds.ring[2][0][0][0], k0, j0, ROWS},
+                             {&lds.ring[3][0][0][0], k0, j0, ROWS}};
   T* const dst[4] = {A.lf, A.lu, A.lv, A.lw};
   // add ring column c (all four targets) to the global adjoints and clear it
   auto flush = [&](int c) {
@@ -665,7 +773,18 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjAr
     }
   };
   for (int i = i_begin; i < i_end; i++) {
-    adjoint_point<T, KIND>(L, A, k0 + lane, j0 + r, i, aF, aU, aV, aW);
+    {
+      const int k = k0 + lane, j = j0 + r;
+      Slots<T> sl{};
+      const SlotAcc<T, 0> aF{&sl, ring, k, j, i};
+      const AccFlow<T, DK, DJ, DI, SlotAcc<T, 1>> aU{{&sl, ring, k, j, i}, nz};
+      const AccFlow<T, DK, DJ, DI, SlotAcc<T, 2>> aV{{&sl, ring, k, j, i}, nz};
+      const AccFlow<T, DK, DJ, DI, SlotAcc<T, 3>> aW{{&sl, ring, k, j, i}, nz};
+      adjoint_point<T, KIND>(L, A, k, j, i, aF, aU, aV, aW, [&] {
+        flush_slots(sl, ring, k, j, i, lane);
+        sl = Slots<T>{};
+      });
+    }
     __syncthreads();
     flush(i - 1);
     __syncthreads();
