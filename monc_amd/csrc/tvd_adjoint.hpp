@@ -99,25 +99,6 @@ struct SlotAcc {
   __device__ __forceinline__ void dyn(int kk, int jj, int ii, T v) const { ring[TGT].add(kk, jj, ii, v); }
 };
 
-// Wave shifts by one lane (DPP wave_shr:1 / wave_shl:1); a lane shifted in
-// from outside the wave reads 0.
-template <int CTRL>
-__device__ __forceinline__ int dpp_i(int x) { return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, true); }
-template <int CTRL, typename T>
-__device__ __forceinline__ T dpp_shift(T x) {
-  if constexpr (sizeof(T) == 4) {
-    return __builtin_bit_cast(T, dpp_i<CTRL>(__builtin_bit_cast(int, x)));
-  } else {
-    const long long b = __builtin_bit_cast(long long, x);
-    const int lo = dpp_i<CTRL>((int)b), hi = dpp_i<CTRL>((int)(b >> 32));
-    return __builtin_bit_cast(T, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-  }
-}
-// lane L receives lane L-1's value / lane L+1's value
-template <typename T>
-__device__ __forceinline__ T from_below(T x) { return dpp_shift<0x138>(x); }
-template <typename T>
-__device__ __forceinline__ T from_above(T x) { return dpp_shift<0x130>(x); }
 // any lane of the wave non-zero (uniform; folds to false for a constant zero)
 template <typename T>
 __device__ __forceinline__ bool any_nz(T x) { return __builtin_amdgcn_ballot_w64(x != T(0)) != 0; }

@@ -89,9 +89,10 @@ __device__ __forceinline__ void ens_tile(const EnsMember& M, const double* met, 
     S.zf[n] = nullptr;
   }
   // one kernel holds every field kind, so its register budget is set by the
-  // largest; in FP32 the scalar pair's windows fit within it
-  constexpr bool WIN = !(sizeof(T) == 8 && NS == 2);
-  march_tile<T, KIND, NS, MODE_STEP, WAVES, ENS_NKMAX, 2, WIN>(L, S, M.nch, ty, tx, strip, lds);
+  // largest; in FP32 the scalar pair's windows fit within it, the registers
+  // of vertical neighbours by wave shifts do not (measured slower)
+  constexpr bool WIN = !(sizeof(T) == 8 && NS == 2), VD = false;
+  march_tile<T, KIND, NS, MODE_STEP, WAVES, ENS_NKMAX, 2, WIN, VD>(L, S, M.nch, ty, tx, strip, lds);
 }
 
 // Waves per SIMD the compiler keeps registers for (as the single-domain

@@ -45,6 +45,37 @@ __device__ __forceinline__ T ldb(__amdgpu_buffer_rsrc_t r, unsigned voff, unsign
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
+// Wave shifts by one lane (DPP wave_shr:1 / wave_shl:1): from_below gives
+// lane L the value of lane L-1, from_above that of lane L+1; a lane shifted in
+// from outside the wave reads 0.  Every lane of the wave must take part.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x) { return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, true); }
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_shift(T x) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, dpp_i<CTRL>(__builtin_bit_cast(int, x)));
+  } else {
+    const long long b = __builtin_bit_cast(long long, x);
+    const int lo = dpp_i<CTRL>((int)b), hi = dpp_i<CTRL>((int)(b >> 32));
+    return __builtin_bit_cast(T, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+  }
+}
+template <typename T>
+__device__ __forceinline__ T from_below(T x) { return dpp_shift<0x138>(x); }
+template <typename T>
+__device__ __forceinline__ T from_above(T x) { return dpp_shift<0x130>(x); }
+// lane `l`'s value, for every lane (wave-uniform)
+template <typename T>
+__device__ __forceinline__ T lane_value(T x, int l) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+  } else {
+    const long long b = __builtin_bit_cast(long long, x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(T, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+  }
+}
+
 // gfx9 raw-buffer descriptor word 3 (32-bit data, no swizzle)
 constexpr int BUFFER_RSRC_WORD3 = 0x00020000;
 

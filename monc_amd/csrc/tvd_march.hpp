@@ -74,7 +74,41 @@ struct MFlow {
 // Per-lane byte offsets of the levels a lane reads, each clamped to [1, nz].
 struct Levels {
   unsigned k, km1, kp1, kp0, kp2;  // k, k-1, k+1, (k+1 clamped) - 1, k+2
+  unsigned edge;                   // the wave's edge levels: k0-1 (lane 0), k0+64 (lane 1), k0+65 (lane 2)
 };
+
+// The advected field around the lane's level k, clamped to [1, nz] as the
+// loads clamp: m1 = F(k-1), p1 = F(k+1), p2 = F(k+2), p0 = F(min(k+1, nz)-1),
+// taken from the lane's own F(k) by wave shifts instead of loads (lane L of a
+// wave holds level k0+L); the levels beyond the wave (k0-1, k0+64, k0+65)
+// come from one load per column.  Lanes past nz hold F(nz) and compute faces
+// nothing uses.
+template <typename T>
+struct VEdge {
+  T lo, hi1, hi2;
+};
+template <typename T>
+struct VNb {
+  T m1, p1, p2, p0;
+};
+
+template <typename T>
+__device__ __forceinline__ VEdge<T> vedge(__amdgpu_buffer_rsrc_t r, const Levels& lv, unsigned so) {
+  const T v = ldb<T>(r, lv.edge, so);
+  return {lane_value(v, 0), lane_value(v, 1), lane_value(v, 2)};
+}
+
+template <typename T>
+__device__ __forceinline__ VNb<T> vnb(T x, const VEdge<T>& e, int lane, int k, int nz) {
+  VNb<T> v;
+  const T a = from_above(x), b = from_below(x);
+  v.p1 = k >= nz ? x : (lane == 63 ? e.hi1 : a);
+  v.m1 = k == 1 ? x : (lane == 0 ? e.lo : b);
+  const T a2 = from_above(v.p1);
+  v.p2 = lane == 63 ? e.hi2 : a2;
+  v.p0 = k >= nz ? v.m1 : x;
+  return v;
+}
 
 // Metric factors per level, tabulated in LDS for the field kind (for w the
 // reference passes (rdzn, rdz, dz) in the roles of (rdz, rdzn, dzn), kdof = 1;
@@ -129,28 +163,27 @@ struct FlowsAt {
 };
 
 // Right x face of column (j, i) at the lane's level (ultimateflux.F90:102-165);
-// xw[n] holds F(k, j, i-1 .. i+2) of field n (the x window of the march).
-template <typename T, int NS>
+// xw[n] holds F(k, j, i-1 .. i+2) of field n (the x window of the march); with
+// VD the vertical neighbours at columns i and i+1 come from c0 and c1, else
+// they are loaded.
+template <typename T, int NS, bool VD>
 __device__ __forceinline__ void mx_face(const __amdgpu_buffer_rsrc_t (&F)[NS], const Geo& g, const Levels& lv, int j,
-                                        int i, const T (&xw)[NS][4], T u_m, T u_0, T u_p, T v_0, T v_s, T v_0p,
-                                        T v_sp, T w_0, T w_p, T wm_0, T wm_p, T rdz_k, T rdz_kd, T dt, T cx, T cy,
-                                        T out[NS]) {
+                                        int i, const T (&xw)[NS][4], const VNb<T> (&c0)[NS], const VNb<T> (&c1)[NS],
+                                        T u_m, T u_0, T u_p, T v_0, T v_s, T v_0p, T v_sp, T w_0, T w_p, T wm_0,
+                                        T wm_p, T rdz_k, T rdz_kd, T dt, T cx, T cy, T out[NS]) {
   const bool upos = u_0 > T(0.0);
   T s = rdz_k * (pos(upos ? w_0 : w_p) + negabs(upos ? wm_0 : wm_p));
   s = s + cy * (pos(upos ? v_0 : v_0p) + negabs(upos ? v_s : v_sp));
   s = s + cx * (upos ? u_0 + negabs(u_m) : pos(u_p) - u_0);
   s = s * dt;
   const T sz = T(0.125) * (w_0 + wm_0 + w_p + wm_p);
-  const bool pz = sign_pos(sz);
-  const unsigned osl = pz ? lv.k : lv.kp0;   // ksl
-  const unsigned osu = pz ? lv.km1 : lv.kp1;  // ksu
+  const bool pz = sign_pos(sz);  // ksl = k, ksu = k-1; else ksl = (k+1 clamped) - 1, ksu = k+1
   const T cz = fabs(sz) * rdz_kd * dt;
   const T sy = T(0.125) * (v_0p + v_0 + v_sp + v_s);
   const unsigned ojs = lv.k + (sign_pos(sy) ? 0u : 2u * g.sjb);  // row j-1 or j+1, from row j-1
   const T cyt = fabs(sy) * cy * dt;
   const HLim<T> lim(u_0, cx, dt, s);
   const bool ap = sign_pos(u_0);
-  const unsigned s_0 = g.so(j, i), s_1 = g.so(j, i + 1);
   const unsigned sj0 = g.so(j - 1, i), sj1 = g.so(j - 1, i + 1);
 #pragma unroll
   for (int n = 0; n < NS; n++) {
@@ -160,8 +193,16 @@ __device__ __forceinline__ void mx_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
     const T fu = ap ? fm : f2;
     const T fc = ap ? fa : fb;
     const T fd = ap ? fb : fa;
-    const T d0 = ldb<T>(F[n], osl, s_0) - ldb<T>(F[n], osu, s_0);
-    const T d1 = ldb<T>(F[n], osl, s_1) - ldb<T>(F[n], osu, s_1);
+    T d0, d1;
+    if constexpr (VD) {
+      d0 = pz ? fa - c0[n].m1 : c0[n].p0 - c0[n].p1;
+      d1 = pz ? fb - c1[n].m1 : c1[n].p0 - c1[n].p1;
+    } else {
+      const unsigned osl = pz ? lv.k : lv.kp0, osu = pz ? lv.km1 : lv.kp1;
+      const unsigned s_0 = g.so(j, i), s_1 = g.so(j, i + 1);
+      d0 = ldb<T>(F[n], osl, s_0) - ldb<T>(F[n], osu, s_0);
+      d1 = ldb<T>(F[n], osl, s_1) - ldb<T>(F[n], osu, s_1);
+    }
     const T g1 = cz * (ap ? d0 : d1);
     const T t0 = ldb<T>(F[n], ojs, sj0), t1 = ldb<T>(F[n], ojs, sj1);
     const T t = ap ? t0 : t1;
@@ -213,9 +254,11 @@ __device__ __forceinline__ void my_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
 // k = nz-1, whose upwind test is W >= 0 (the middle one takes the sign, so they
 // differ for W = -0), `top` marks k = nz-1 (W < 0 extrapolates linearly above
 // the top and has no inflow from above), `bot` marks k = 1 (no level below).
-template <typename T, int NS>
+// With VD the levels k-1, k+1, k+2 come from c0, else they are loaded.
+template <typename T, int NS, bool VD>
 __device__ __forceinline__ void mz_face(const __amdgpu_buffer_rsrc_t (&F)[NS], const Geo& g, const Levels& lv, int j,
-                                        int i, const T (&fc0)[NS], const FlowsAt<T>& q, bool edge, bool top, bool bot,
+                                        int i, const T (&fc0)[NS], const VNb<T> (&c0)[NS], const FlowsAt<T>& q,
+                                        bool edge, bool top, bool bot,
                                         T zs_up, T zs_dn,
                                         T rdc_up, T rdc_dn, T rdu_up, T rdu_dn, T rdd, T dd2, T dt, T cx, T cy,
                                         T out[NS]) {
@@ -245,11 +288,21 @@ __device__ __forceinline__ void mz_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
   const T hrcfl = recip(s + T(1.e-30));
   const T hcfl = T(0.5) * cflmod;
   const T cflcoef = r6 * (T(1.0) - cflmod * cflmod) * (dd2 * rdc);
-  const unsigned s_0 = g.so(j, i), s_jm = g.so(j - 1, i), s_im = g.so(j, i - 1);
+  const unsigned s_jm = g.so(j - 1, i), s_im = g.so(j, i - 1);
 #pragma unroll
   for (int n = 0; n < NS; n++) {
-    const T fk = fc0[n], fk1 = ldb<T>(F[n], lv.kp1, s_0);
-    const T fkm = ldb<T>(F[n], lv.km1, s_0), fk2 = ldb<T>(F[n], lv.kp2, s_0);
+    const T fk = fc0[n];
+    T fk1, fkm, fk2;
+    if constexpr (VD) {
+      fk1 = c0[n].p1;
+      fkm = c0[n].m1;
+      fk2 = c0[n].p2;
+    } else {
+      const unsigned s_0 = g.so(j, i);
+      fk1 = ldb<T>(F[n], lv.kp1, s_0);
+      fkm = ldb<T>(F[n], lv.km1, s_0);
+      fk2 = ldb<T>(F[n], lv.kp2, s_0);
+    }
     const T fdn = top ? T(2.0) * fk1 - fk : fk2;
     const T fu = up ? fkm : fdn;
     const T fc = up ? fk : fk1;
@@ -283,8 +336,9 @@ struct MarchLds {
 // (nch sub-columns of 64), x columns 3 + tx*strip onwards.  Wave wv takes
 // sub-column wv % nch of row wv / nch; waves past R*nch only take the
 // barriers.  Every wave of the block calls this with the same tile.  WIN:
-// carry the x windows from column to column (below) or reload them.
-template <typename T, int KIND, int NS, int MODE, int WAVES, int NKMAX, int NSL, bool WIN>
+// carry the x windows from column to column (below) or reload them; VD: take
+// the advected field's vertical neighbours by wave shifts (VNb) or load them.
+template <typename T, int KIND, int NS, int MODE, int WAVES, int NKMAX, int NSL, bool WIN, bool VD>
 __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSet<T>& S, int nch, int ty, int tx,
                                            int strip, MarchLds<T, NSL, WAVES, NKMAX>& lds) {
   constexpr bool IS_W = KIND == KIND_W;
@@ -322,6 +376,7 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
   lv.kp1 = off(k + 1);
   lv.kp0 = off((k + 1 > nz ? nz : k + 1) - 1);
   lv.kp2 = off(k + 2);
+  lv.edge = off(lane == 1 ? k - lane + 64 : (lane == 2 ? k - lane + 65 : k - lane - 1));
   Geo g;
   g.sjb = (unsigned)L.d.sj * es;
   g.sib = (unsigned)L.d.si * es;
@@ -350,6 +405,7 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
   // are time saved.  With WIN = false every column reloads them.
   T fxl[NS];
   T xw[NS][4];
+  VEdge<T> e0[NS], e1[NS];  // the wave's edge levels of F at (j, i) and (j, i+1)
   T c_um = T(0), c_u0 = T(0), c_u1m = T(0), c_v0 = T(0), c_vs = T(0), c_w0 = T(0), c_wm0 = T(0), c_ukm = T(0);
 #pragma unroll
   for (int n = 0; n < NS; n++) fxl[n] = xw[n][0] = xw[n][1] = xw[n][2] = xw[n][3] = T(0);
@@ -370,8 +426,19 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
     c_w0 = W(lv.k, lv.kp1, j, i + 1);
     c_wm0 = W(lv.km1, lv.k, j, i + 1);
     c_ukm = U(lv.kp1, lv.kp2, j, i);
-    mx_face<T, NS>(F, g, lv, j, i, xw, um, c_um, c_u0, v0, vs, c_v0, c_vs, w0, c_w0, wm0, c_wm0, rdz_k, rdz_kd, dt,
-                   cx, cy, fxl);
+    VNb<T> c0[NS], c1[NS];
+    if constexpr (VD) {
+#pragma unroll
+      for (int n = 0; n < NS; n++) {
+        e0[n] = vedge<T>(F[n], lv, g.so(j, i));
+        e1[n] = vedge<T>(F[n], lv, g.so(j, i + 1));
+        c0[n] = vnb(xw[n][1], e0[n], lane, k, nz);
+        c1[n] = vnb(xw[n][2], e1[n], lane, k, nz);
+        e0[n] = e1[n];
+      }
+    }
+    mx_face<T, NS, VD>(F, g, lv, j, i, xw, c0, c1, um, c_um, c_u0, v0, vs, c_v0, c_vs, w0, c_w0, wm0, c_wm0, rdz_k,
+                   rdz_kd, dt, cx, cy, fxl);
 #pragma unroll
     for (int n = 0; n < NS; n++) {  // window of column i_begin: F(i_begin-1 .. i_begin+1), F(i_begin+2) below
       xw[n][0] = xw[n][1];
@@ -401,10 +468,17 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
         c_wm0 = W(lv.km1, lv.k, j, i);
         c_ukm = U(lv.kp1, lv.kp2, j, i - 1);
       }
+      VNb<T> c0[NS], c1[NS];
 #pragma unroll
       for (int n = 0; n < NS; n++) {
         xw[n][3] = ldb<T>(F[n], lv.k, g.so(j, i + 2));
         fc0[n] = xw[n][1];
+        if constexpr (VD) {
+          if constexpr (!WIN) e0[n] = vedge<T>(F[n], lv, g.so(j, i));
+          e1[n] = vedge<T>(F[n], lv, g.so(j, i + 1));
+          c0[n] = vnb(xw[n][1], e0[n], lane, k, nz);
+          c1[n] = vnb(xw[n][2], e1[n], lane, k, nz);
+        }
       }
       q.u_m = c_um;
       q.u_0 = c_u0;
@@ -437,10 +511,10 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
                        W(lv.km1, lv.k, j - 1, i), q.w_0, q.wm_0, U(lv.k, lv.kp1, j - 1, i),
                        U(lv.k, lv.kp1, j - 1, i - 1), q.u_0, q.u_m, rdz_k, rdz_kd, dt, cx, cy, fys);
       }
-      mx_face<T, NS>(F, g, lv, j, i, xw, q.u_m, q.u_0, q.u_p, q.v_0, q.v_s, q.v_0p, q.v_sp, q.w_0, q.w_p, q.wm_0,
-                     q.wm_p, rdz_k, rdz_kd, dt, cx, cy, fxr);
+      mx_face<T, NS, VD>(F, g, lv, j, i, xw, c0, c1, q.u_m, q.u_0, q.u_p, q.v_0, q.v_s, q.v_0p, q.v_sp, q.w_0, q.w_p,
+                     q.wm_0, q.wm_p, rdz_k, rdz_kd, dt, cx, cy, fxr);
       T zf[NS];
-      mz_face<T, NS>(F, g, lv, j, i, fc0, q, edge, ztop, zbot, lds.met[M_ZS_UP][kl], lds.met[M_ZS_DN][kl],
+      mz_face<T, NS, VD>(F, g, lv, j, i, fc0, c0, q, edge, ztop, zbot, lds.met[M_ZS_UP][kl], lds.met[M_ZS_DN][kl],
                      lds.met[M_RDC_UP][kl], lds.met[M_RDC_DN][kl], lds.met[M_RDU_UP][kl], lds.met[M_RDU_DN][kl],
                      lds.met[M_RDD][kl], lds.met[M_DD2][kl], dt, cx, cy, zf);
 #pragma unroll
@@ -532,6 +606,7 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
       xw[n][0] = xw[n][1];
       xw[n][1] = xw[n][2];
       xw[n][2] = xw[n][3];
+      if constexpr (VD) e0[n] = e1[n];
     }
     if (active) {
       c_um = q.u_0;
@@ -560,8 +635,12 @@ march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
   const int t = bid < full ? (bid % 8) * (full / 8) + bid / 8 : bid;
   // The scalar pair reloads its x windows: two windows leave FP64 no
   // registers at 3 waves per SIMD and cost FP32 a wave per SIMD, which
-  // measured slower than the loads they save.
-  march_tile<T, KIND, NS, MODE, NCH * R, 64 * NCH, NS, NS != 2>(L, S, NCH, t % tiles_y, t / tiles_y, strip, lds);
+  // measured slower than the loads they save.  Vertical neighbours by wave
+  // shifts pay in FP32 and for FP64 w; the other FP64 kernels, with 3 waves
+  // per SIMD and two shifts per value, run faster loading them.
+  constexpr bool VD = sizeof(T) == 4 || (KIND == KIND_W && NS == 1);
+  march_tile<T, KIND, NS, MODE, NCH * R, 64 * NCH, NS, NS != 2, VD>(L, S, NCH, t % tiles_y, t / tiles_y, strip,
+                                                                   lds);
 }
 
 }  // namespace tvd
