@@ -636,9 +636,10 @@ march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
   // The scalar pair reloads its x windows: two windows leave FP64 no
   // registers at 3 waves per SIMD and cost FP32 a wave per SIMD, which
   // measured slower than the loads they save.  Vertical neighbours by wave
-  // shifts pay in FP32 and for FP64 w; the other FP64 kernels, with 3 waves
-  // per SIMD and two shifts per value, run faster loading them.
-  constexpr bool VD = sizeof(T) == 4 || (KIND == KIND_W && NS == 1);
+  // shifts pay for the w kernel (3% in FP64, 2% in FP32); the FP64 u, v and
+  // scalar kernels, at 3 waves per SIMD and two shifts per value, run faster
+  // loading them, and in FP32 the gain depends on the tile shape.
+  constexpr bool VD = KIND == KIND_W && NS == 1;
   march_tile<T, KIND, NS, MODE, NCH * R, 64 * NCH, NS, NS != 2, VD>(L, S, NCH, t % tiles_y, t / tiles_y, strip,
                                                                    lds);
 }
