@@ -159,6 +159,43 @@ def leapfrog_form(state, dt: float, steps: int):
         t.close()
 
 
+def thadv_form(state, steps: int):
+    """th_advection's source terms (monc_thadv_tendencies: reference-state
+    vertical advection and mean baroclinicity, every column) on the FP64
+    state: one launch reads sth, u, v, w and writes sth, 40 B per cell
+    (halo columns included), priced against HBM with HIP events."""
+    import numpy as np
+
+    from monc_amd import tvd
+
+    t = tvd.TvdAdvection.initialise(state, precision=tvd.FP64)
+    try:
+        nz = state.nz
+        t.set_field(tvd.TH, tvd.SOURCE, np.zeros(t.field_shape()))
+        thref = 290.0 + 0.004 * state.grid.zn
+        dth = np.zeros(nz)
+        dth[:-1] = thref[1:] - thref[:-1]
+        kw = dict(dthref=dth, thref=thref, baroclinicity=True, thref0=300.0, fcoriol=1.0e-4, g=9.81,
+                  rate_x=2.0e-3, rate_y=-1.0e-3)
+        t.thadv_tendencies(True, **kw)
+        t.synchronize()
+        t.set_profiling(True)
+        t.kernel_times(reset=True)
+        for _ in range(steps):
+            t.thadv_tendencies(True, **kw)
+        t.synchronize()
+        # timed under the library's whole-field kind (halo fill, th_advection)
+        ms, n = t.kernel_times(reset=True).get("halo", (0.0, 0))
+        t.set_profiling(False)
+        per = ms / max(n, 1)
+        nbytes = 40.0 * (state.nx + 4) * (state.ny + 4) * (nz - 1)
+        return {"kernel": "thadv_kernel<double>", "launches": n, "kernel_ms": round(per, 4),
+                "bytes_per_launch": nbytes, "achieved_GBs": round(nbytes / per / 1e6, 2),
+                "frac_hbm": round(nbytes / per / 1e6 / 8000.0, 4), "dtype": "f64"}
+    finally:
+        t.close()
+
+
 def ensemble_form(steps: int):
     """48 FP32 members of seeded uneven sizes: forward steps by one persistent
     launch per step over all members' tiles, with and without graph replay,
@@ -232,7 +269,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the adjoint, leapfrog and ensemble measurements")
-    ap.add_argument("--only", choices=["adjoint", "leapfrog", "ensemble"], default=None,
+    ap.add_argument("--only", choices=["adjoint", "leapfrog", "ensemble", "thadv"], default=None,
                     help="run only this side measurement (for profiling) and print its JSON object")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed work, run three 512 MiB device-to-device copies: the known-size "
@@ -261,7 +298,8 @@ def main() -> None:
         log(f"bench: {args.only} only")
         side = {"adjoint": lambda: adjoint_form(st, dt, max(2, args.steps // 2)),
                 "leapfrog": lambda: leapfrog_form(st, dt, max(2, args.steps // 2)),
-                "ensemble": lambda: ensemble_form(max(4, args.steps // 2))}[args.only]()
+                "ensemble": lambda: ensemble_form(max(4, args.steps // 2)),
+                "thadv": lambda: thadv_form(st, max(4, args.steps))}[args.only]()
         print(json.dumps({args.only: side}), flush=True)
         return
 
@@ -318,6 +356,8 @@ def main() -> None:
         out["adjoint_fp32"] = adjoint_form(st, dt, max(2, args.steps // 2))
         log("bench: FP64 leapfrog timestep")
         out["leapfrog_fp64"] = leapfrog_form(st, dt, max(2, args.steps // 2))
+        log("bench: th_advection")
+        out["thadv_fp64"] = thadv_form(st, max(4, args.steps))
         log("bench: batched ensemble (48 FP32 members)")
         out["ensemble_fp32"] = ensemble_form(max(4, args.steps // 2))
     if args.calibrate_copy:

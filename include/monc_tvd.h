@@ -161,8 +161,9 @@ int monc_tvd_get_diagnostic(monc_tvd_ctx *ctx, int field, int which, double *hos
  * With profiling on, every launch is bracketed by HIP events on the context
  * stream; monc_tvd_kernel_times synchronises, folds the events into per-kind
  * totals and returns the first n of them: milliseconds and launch counts for
- * kinds 0 = u, 1 = v, 2 = w, 3 = scalars, 4 = halo fill, 5 = one-cell-per-
- * thread kernel (all fields).  reset != 0 clears the totals afterwards. */
+ * kinds 0 = u, 1 = v, 2 = w, 3 = scalars, 4 = whole-field kernels (halo
+ * fill, th_advection), 5 = one-cell-per-thread kernel (all fields).
+ * reset != 0 clears the totals afterwards. */
 #define MONC_TVD_NKERNEL_KINDS 6
 int monc_tvd_set_profiling(monc_tvd_ctx *ctx, int on);
 int monc_tvd_kernel_times(monc_tvd_ctx *ctx, double *ms, long long *launches, int n, int reset);
@@ -260,6 +261,28 @@ int monc_tvd_adjoint_step(monc_tvd_ctx *ctx, double dtm);
  * monc_tvd_adjoint_step.  Asynchronous.
  */
 int monc_tvd_adjoint_fold(monc_tvd_ctx *ctx);
+
+/*
+ * The sibling component th_advection on the same context: replaces
+ * components/thadvection/src/thadvection.F90:124-211 (timestep_callback's
+ * vertical_advection_of_reference_state and advection_of_mean_baroclinicity)
+ * with the factors of its initialisation_callback (:60-77).  Adds to the th
+ * source field (MONC_TVD_TH, MONC_TVD_SOURCE) at every column, halo columns
+ * included (the component does not skip them), levels 2..nz:
+ *   anelastic != 0:     sth -= tzc1(k)*2*dthref(k-1)*w(k-1) + tzc2(k)*2*dthref(k)*w(k)
+ *   baroclinicity != 0: sth += thref(k)*(fcoriol/g)*((v+vgal)*rate_x - (u+ugal)*rate_y)  (anelastic)
+ *                       sth += (v+vgal)*mfx - (u+ugal)*mfy, mf = rate*thref0*(fcoriol/g)  (otherwise)
+ * from the prognostics u, v, w and the context's tzc1, tzc2.  dthref and
+ * thref (length nz: vertical%dthref, vertical%thref) are read only when the
+ * term needs them.  baroclinicity is the reference's
+ * l_advect_mean_baroclinicity (baroclinicity_use_geostrophic_shear with a
+ * passive q); rate_x, rate_y are rate_change_geostrophic_wind_x/_y; g is
+ * science_constants G.  The tendency diagnostics of the component are not
+ * produced.  Launches on the library stream.
+ */
+int monc_thadv_tendencies(monc_tvd_ctx *ctx, int anelastic, const double *dthref, int baroclinicity,
+                          const double *thref, double thref0, double fcoriol, double g, double rate_x,
+                          double rate_y, double ugal, double vgal);
 
 /*
  * Tangent-linear model of monc_tvd_forward_steps (FP64 contexts only; no

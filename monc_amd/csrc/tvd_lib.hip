@@ -96,6 +96,7 @@ struct monc_tvd_ctx {
   double* tabs_prof;
   int tabs_have;
   double* diag_partial;        // ny*nz FP64 scratch of the level totals
+  double* thadv_tab;           // device, 2*nz: dthref, thref of th_advection (allocated on first use)
 };
 
 namespace {
@@ -347,6 +348,63 @@ std::vector<int> advected_fields(const monc_tvd_ctx* c) {
   if (c->advect & MONC_TVD_ADVECT_Q)
     for (int n = 0; n < c->n_q; n++) f.push_back(MONC_TVD_Q(n));
   return f;
+}
+
+// th_advection (thadvection.F90:124-211): at every column the registry hands
+// it -- halo columns included, as the component does not skip them -- and
+// levels k = 2..nz, the vertical advection of the reference state
+//   sth -= tzc1(k)*2*dthref(k-1)*w(k-1) + tzc2(k)*2*dthref(k)*w(k)
+// (anelastic only, :165-179) and then the advection of the mean
+// baroclinicity (:186-211)
+//   anelastic:  sth += thref(k)*(fcoriol/G) * ((v+vgal)*rate_x - (u+ugal)*rate_y)
+//   otherwise:  sth += (v+vgal)*mfx - (u+ugal)*mfy,  mf = rate*thref0*(fcoriol/G)
+// with the factors of initialisation_callback (:60-77) formed on the host in
+// FP64.  One thread per (level, column), z fastest.
+template <typename T>
+__global__ void __launch_bounds__(256) thadv_kernel(T* sth, const T* u, const T* v, const T* w, const double* tzc1,
+                                                    const double* tzc2, const double* dthref, const double* thref,
+                                                    Dims d, int anelastic, int baro, double fcog, double rate_x,
+                                                    double rate_y, double mfx, double mfy, T ugal, T vgal) {
+  const int k = 2 + (int)blockIdx.x * 64 + (int)threadIdx.x;
+  const long long col = (long long)blockIdx.y * 4 + threadIdx.y;
+  if (k > d.nz || col >= (long long)(d.ny + 4) * (d.nx + 4)) return;
+  const int j = 1 + (int)(col % (d.ny + 4)), i = 1 + (int)(col / (d.ny + 4));
+  const int64_t o = at(d, k, j, i);
+  T s = sth[o];
+  if (anelastic) {
+    const T s1 = (T)(tzc1[k - 1] * 2.0 * dthref[k - 2]), s2 = (T)(tzc2[k - 1] * 2.0 * dthref[k - 1]);
+    s = s - (s1 * w[o - 1] + s2 * w[o]);
+  }
+  if (baro) {
+    const T vv = v[o] + vgal, uu = u[o] + ugal;
+    if (anelastic) s = s + (T)(thref[k - 1] * fcog) * (vv * (T)rate_x - uu * (T)rate_y);
+    else s = s + (vv * (T)mfx - uu * (T)mfy);
+  }
+  sth[o] = s;
+}
+
+template <typename T>
+int thadv_impl(monc_tvd_ctx* c, int anelastic, const double* dthref, int baro, const double* thref, double thref0,
+               double fcoriol, double g, double rate_x, double rate_y, double ugal, double vgal) {
+  const int nz = c->nz;
+  if (!c->thadv_tab) HIP_TRY(hipMalloc(&c->thadv_tab, 2 * (size_t)nz * sizeof(double)));
+  if (anelastic)
+    HIP_TRY(hipMemcpyAsync(c->thadv_tab, dthref, nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (baro && anelastic)
+    HIP_TRY(hipMemcpyAsync(c->thadv_tab + nz, thref, nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const double fcog = fcoriol / g;
+  const double mfx = rate_x * thref0 * fcog, mfy = rate_y * thref0 * fcog;
+  const Dims d = dims_of(c);
+  const long long ncol = (long long)(c->ny + 4) * (c->nx + 4);
+  dim3 block(64, 4, 1), grid((nz - 1 + 63) / 64, (unsigned)((ncol + 3) / 4), 1);
+  LaunchTimer timer(c, 4);
+  hipLaunchKernelGGL((thadv_kernel<T>), grid, block, 0, c->stream, (T*)slot(c, MONC_TVD_TH, MONC_TVD_SOURCE),
+                     (const T*)slot(c, MONC_TVD_U, MONC_TVD_PROG), (const T*)slot(c, MONC_TVD_V, MONC_TVD_PROG),
+                     (const T*)slot(c, MONC_TVD_W, MONC_TVD_PROG), c->dmet + MET_TZC1 * nz, c->dmet + MET_TZC2 * nz,
+                     c->thadv_tab, c->thadv_tab + nz, d, anelastic, baro, fcog, rate_x, rate_y, mfx, mfy, (T)ugal,
+                     (T)vgal);
+  HIP_TRY(hipGetLastError());
+  return MONC_TVD_OK;
 }
 
 int need(monc_tvd_ctx* c, int field, int variant) {
@@ -936,7 +994,7 @@ int monc_tvd_destroy(monc_tvd_ctx* c) {
     for (void* p : {c->diag_before[f], c->diag_3d[f], c->diag_zf[f], (void*)c->diag_prof[f]})
       if (p) (void)hipFree(p);
   }
-  for (void* p : {(void*)c->rprefrcp, c->tabs_3d, (void*)c->tabs_prof, (void*)c->diag_partial})
+  for (void* p : {(void*)c->rprefrcp, c->tabs_3d, (void*)c->tabs_prof, (void*)c->diag_partial, (void*)c->thadv_tab})
     if (p) (void)hipFree(p);
   if (c->dmet) (void)hipFree(c->dmet);
   delete c;
@@ -1141,6 +1199,29 @@ int monc_tvd_adjoint_step(monc_tvd_ctx* c, double dtm) {
 int monc_tvd_adjoint_fold(monc_tvd_ctx* c) {
   if (int rc = check_ctx(c)) return rc;
   return c->precision == MONC_TVD_FP64 ? fold_halo<double>(c) : fold_halo<float>(c);
+}
+
+int monc_thadv_tendencies(monc_tvd_ctx* c, int anelastic, const double* dthref, int baroclinicity,
+                          const double* thref, double thref0, double fcoriol, double g, double rate_x, double rate_y,
+                          double ugal, double vgal) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!anelastic && !baroclinicity) return MONC_TVD_OK;
+  if (anelastic && !dthref) return fail(MONC_TVD_ERR_ARG, "anelastic th_advection needs dthref");
+  if (baroclinicity && anelastic && !thref)
+    return fail(MONC_TVD_ERR_ARG, "anelastic advection of the mean baroclinicity needs thref");
+  if (baroclinicity && !(g != 0.0)) return fail(MONC_TVD_ERR_ARG, "g must be non-zero");
+  if (int rc = need(c, MONC_TVD_TH, MONC_TVD_SOURCE)) return rc;
+  if (anelastic)
+    if (int rc = need(c, MONC_TVD_W, MONC_TVD_PROG)) return rc;
+  if (baroclinicity) {
+    if (int rc = need(c, MONC_TVD_U, MONC_TVD_PROG)) return rc;
+    if (int rc = need(c, MONC_TVD_V, MONC_TVD_PROG)) return rc;
+  }
+  return c->precision == MONC_TVD_FP64
+             ? thadv_impl<double>(c, anelastic, dthref, baroclinicity, thref, thref0, fcoriol, g, rate_x, rate_y,
+                                  ugal, vgal)
+             : thadv_impl<float>(c, anelastic, dthref, baroclinicity, thref, thref0, fcoriol, g, rate_x, rate_y,
+                                 ugal, vgal);
 }
 
 int monc_tvd_tangent_steps(monc_tvd_ctx* c, double dtm, int nsteps) {

@@ -72,6 +72,16 @@ module monc_tvd_interface_mod
       integer(c_int), value :: nsteps, first_forward
     end function monc_tvd_leapfrog_steps
 
+    ! th_advection's timestep callback (thadvection.F90:124-211) on the same context
+    integer(c_int) function monc_thadv_tendencies(ctx, anelastic, dthref, baroclinicity, thref, thref0, fcoriol, &
+        g, rate_x, rate_y, ugal, vgal) bind(C, name="monc_thadv_tendencies")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: anelastic, baroclinicity
+      real(c_double), intent(in) :: dthref(*), thref(*)
+      real(c_double), value :: thref0, fcoriol, g, rate_x, rate_y, ugal, vgal
+    end function monc_thadv_tendencies
+
     integer(c_int) function monc_tvd_bind_field(ctx, field, variant, device_ptr) bind(C, name="monc_tvd_bind_field")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -35,7 +35,7 @@ EXPORTS = (
     "monc_tvd_set_profiling", "monc_tvd_kernel_times", "monc_tvd_weighted_sum",
     "monc_tvd_adjoint_step", "monc_tvd_adjoint_fold", "monc_tvd_tangent_steps", "monc_tvd_march_shapes",
     "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic", "monc_tvd_autotune",
-    "monc_tvd_leapfrog_steps", "monc_tvd_forward_step_into",
+    "monc_tvd_leapfrog_steps", "monc_tvd_forward_step_into", "monc_thadv_tendencies",
     "monc_tvd_ensemble_create", "monc_tvd_ensemble_destroy", "monc_tvd_ensemble_layout",
     "monc_tvd_ensemble_set_field", "monc_tvd_ensemble_get_field", "monc_tvd_ensemble_copy_pool",
     "monc_tvd_ensemble_set_dt", "monc_tvd_ensemble_set_stream", "monc_tvd_ensemble_synchronize",
@@ -97,6 +97,8 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_leapfrog_steps.argtypes = [c_void_p, c_double, c_double, c_double, c_double, c_int, c_int]
     lib.monc_tvd_adjoint_step.argtypes = [c_void_p, c_double]
     lib.monc_tvd_adjoint_fold.argtypes = [c_void_p]
+    dp = ctypes.POINTER(c_double)
+    lib.monc_thadv_tendencies.argtypes = [c_void_p, c_int, dp, c_int, dp] + [c_double] * 7
     lib.monc_tvd_tangent_steps.argtypes = [c_void_p, c_double, c_int]
     ip = ctypes.POINTER(c_int)
     llp = ctypes.POINTER(ctypes.c_longlong)
@@ -307,6 +309,25 @@ class TvdAdvection:
 
     def adjoint_fold(self) -> None:
         _check(self._lib.monc_tvd_adjoint_fold(self._h))
+
+    def thadv_tendencies(self, anelastic: bool, dthref=None, baroclinicity: bool = False, thref=None,
+                         thref0: float = 0.0, fcoriol: float = 0.0, g: float = 9.81, rate_x: float = 0.0,
+                         rate_y: float = 0.0, ugal: float = 0.0, vgal: float = 0.0) -> None:
+        """th_advection's source terms added to the th source field
+        (thadvection.F90:124-211); dthref, thref: length-nz FP64 vectors."""
+        def vec(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.shape != (self.nz,):
+                raise ValueError(f"expected a vector of length nz={self.nz}, got {a.shape}")
+            return a
+        dth, thr = vec(dthref), vec(thref)
+        dp = ctypes.POINTER(ctypes.c_double)
+        _check(self._lib.monc_thadv_tendencies(
+            self._h, int(bool(anelastic)), dth.ctypes.data_as(dp) if dth is not None else None,
+            int(bool(baroclinicity)), thr.ctypes.data_as(dp) if thr is not None else None,
+            thref0, fcoriol, g, rate_x, rate_y, ugal, vgal))
 
     def tangent_steps(self, dtm: float, nsteps: int) -> None:
         _check(self._lib.monc_tvd_tangent_steps(self._h, dtm, nsteps))

@@ -28,6 +28,14 @@ Parity status per oracle function (the checks live in tests/test_oracle.py):
   first step f + dtm*s, the galilean identities (new field + gal, new z-field
   + (1 - tsmth)*gal for u and v), exact swaps at s = 0, and scalar mass
   conservation through both filters.
+* oracle_thadv (th_advection, the first sibling plugin) -- pinned by the
+  centred vertical advection of a reference profile on a uniform grid
+  (-w (thref(k+1) - thref(k-1)) / (2 dz), exact for a quadratic thref), the
+  mass-weighted split of each interface's transport on an anelastic grid
+  (each of the two cells takes rho(k) w(k) dthref(k) / 2 per unit mass-depth),
+  the thermal-wind form of the mean baroclinicity term (-u.grad(thbar) with
+  grad(thbar) = (thref0 f / g) (dVg/dz, -dUg/dz)), galilean pairing, and the
+  untouched level 1 and halo-inclusive columns.
 """
 from __future__ import annotations
 
@@ -70,6 +78,7 @@ def lib() -> ctypes.CDLL:
         L.oracle_step_field.argtypes = [dp, dp, dp, dp, ci, ci, ci, ci, ci, cd, cd, ci, cd, cd]
         L.oracle_swap_classic.argtypes = [dp, dp, ci, ci, ci]
         L.oracle_swap_robert.argtypes = [dp, dp, dp, ci, cd, cd, ci, ci, ci]
+        L.oracle_thadv.argtypes = [dp, dp, dp, dp, dp, dp, dp, dp, ci, ci, ci, ci, ci] + [cd] * 7
         _lib = L
     return _lib
 
@@ -233,6 +242,24 @@ def swap_robert(f, zf, sav, kind: int, tsmth: float, gal: float = 0.0):
     sv = np.ascontiguousarray(sav, dtype=np.float64) if sav is not None else None
     lib().oracle_swap_robert(_p(f), _p(zf), _p(sv) if sv is not None else None, kind, tsmth, gal, nz, ny, nx)
     return f, zf
+
+
+def thadv(sth, u, v, w, tzc1, tzc2, dthref=None, thref=None, anelastic: bool = True,
+          baroclinicity: bool = False, thref0: float = 0.0, fcoriol: float = 0.0, g: float = 9.81,
+          rate_x: float = 0.0, rate_y: float = 0.0, ugal: float = 0.0, vgal: float = 0.0):
+    """th_advection (thadvection.F90:124-211) on a copy of the th source
+    field sth, from the prognostics u, v, w; returns the new source field."""
+    out = np.array(sth, dtype=np.float64, copy=True)
+    nx, ny, nz = out.shape[0] - 4, out.shape[1] - 4, out.shape[2]
+
+    def vec(a):
+        return np.ascontiguousarray(a if a is not None else np.zeros(nz), dtype=np.float64)
+
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w)]
+    vs = [vec(a) for a in (tzc1, tzc2, dthref, thref)]
+    lib().oracle_thadv(_p(out), *[_p(a) for a in arrs], *[_p(a) for a in vs], nz, ny, nx, int(bool(anelastic)),
+                       int(bool(baroclinicity)), thref0, fcoriol, g, rate_x, rate_y, ugal, vgal)
+    return out
 
 
 def leapfrog_run(state, dtm: float, tsmth: float, nsteps: int, ugal: float = 0.0, vgal: float = 0.0,

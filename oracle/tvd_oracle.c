@@ -724,3 +724,39 @@ void oracle_swap_robert(double *f, double *zf, const double *sav, int kind, doub
         }
       }
 }
+
+/* thadvection.F90:124-211 at every column (1..nx+4, 1..ny+4: the registry
+ * calls the component for halo columns too and it does not skip them),
+ * levels 2..nz, with the factors of initialisation_callback (:60-77):
+ * fcoriol_over_G = fcoriol/G, multiplicative_factor = rate*thref0*fcoriol_over_G.
+ * vertical_advection_of_reference_state (:165-179, anelastic only), then
+ * advection_of_mean_baroclinicity (:186-211, when baroclinicity is set: the
+ * reference's l_advect_mean_baroclinicity). */
+void oracle_thadv(double *sth, const double *u, const double *v, const double *w, const double *tzc1,
+                  const double *tzc2, const double *dthref, const double *thref, int nz, int ny, int nx,
+                  int anelastic, int baroclinicity, double thref0, double fcoriol, double g, double rate_x,
+                  double rate_y, double ugal, double vgal) {
+  g_nz = nz;
+  g_ny = ny;
+  g_nx = nx;
+  const double fcoriol_over_g = fcoriol / g;
+  const double mfx = rate_x * thref0 * fcoriol_over_g, mfy = rate_y * thref0 * fcoriol_over_g;
+  for (int i = 1; i <= nx + 4; i++)
+    for (int j = 1; j <= ny + 4; j++) {
+      if (anelastic)
+        for (int k = 2; k <= nz; k++) {
+          const double sctmp1 = tzc1[k - 1] * 2.0 * dthref[k - 2];
+          const double sctmp2 = tzc2[k - 1] * 2.0 * dthref[k - 1];
+          A(sth, k, j, i) = A(sth, k, j, i) - (sctmp1 * A(w, k - 1, j, i) + sctmp2 * A(w, k, j, i));
+        }
+      if (baroclinicity) {
+        for (int k = 2; k <= nz; k++) {
+          if (anelastic)
+            A(sth, k, j, i) = A(sth, k, j, i) + thref[k - 1] * fcoriol_over_g *
+                                                    ((A(v, k, j, i) + vgal) * rate_x - (A(u, k, j, i) + ugal) * rate_y);
+          else
+            A(sth, k, j, i) = A(sth, k, j, i) + ((A(v, k, j, i) + vgal) * mfx - (A(u, k, j, i) + ugal) * mfy);
+        }
+      }
+    }
+}
