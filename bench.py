@@ -196,6 +196,40 @@ def thadv_form(state, steps: int):
         t.close()
 
 
+def pw_form(state, steps: int):
+    """pw_advection's centred tendencies (monc_pw_tendencies, flow, th and q)
+    on the FP64 state: one launch reads u, v, w, th, q and sq and writes su,
+    sv, sw, sth and sq, 88 B per interior cell, priced against HBM with HIP
+    events."""
+    import numpy as np
+
+    from monc_amd import tvd
+
+    t = tvd.TvdAdvection.initialise(state, precision=tvd.FP64)
+    try:
+        for n in range(len(state.fields)):
+            t.set_field(n, tvd.SOURCE, np.zeros(t.field_shape()))
+        tc = dict(tcx=0.25 * state.cx, tcy=0.25 * state.cy)
+        t.pw_tendencies(**tc)
+        t.synchronize()
+        t.set_profiling(True)
+        t.kernel_times(reset=True)
+        for _ in range(steps):
+            t.pw_tendencies(**tc)
+        t.synchronize()
+        # timed under the library's whole-field kind
+        ms, n = t.kernel_times(reset=True).get("halo", (0.0, 0))
+        t.set_profiling(False)
+        per = ms / max(n, 1)
+        nbytes = 88.0 * state.nx * state.ny * (state.nz - 1)
+        return {"kernel": "pw_kernel<double>", "launches": n, "kernel_ms": round(per, 4),
+                "bytes_per_launch": nbytes, "achieved_GBs": round(nbytes / per / 1e6, 2),
+                "frac_hbm": round(nbytes / per / 1e6 / 8000.0, 4),
+                "gcell_per_s": round(state.nx * state.ny * state.nz / per / 1e6, 3), "dtype": "f64"}
+    finally:
+        t.close()
+
+
 def ensemble_form(steps: int):
     """48 FP32 members of seeded uneven sizes: forward steps by one persistent
     launch per step over all members' tiles, with and without graph replay,
@@ -269,7 +303,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the adjoint, leapfrog and ensemble measurements")
-    ap.add_argument("--only", choices=["adjoint", "leapfrog", "ensemble", "thadv"], default=None,
+    ap.add_argument("--only", choices=["adjoint", "leapfrog", "ensemble", "thadv", "pw"], default=None,
                     help="run only this side measurement (for profiling) and print its JSON object")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed work, run three 512 MiB device-to-device copies: the known-size "
@@ -299,7 +333,8 @@ def main() -> None:
         side = {"adjoint": lambda: adjoint_form(st, dt, max(2, args.steps // 2)),
                 "leapfrog": lambda: leapfrog_form(st, dt, max(2, args.steps // 2)),
                 "ensemble": lambda: ensemble_form(max(4, args.steps // 2)),
-                "thadv": lambda: thadv_form(st, max(4, args.steps))}[args.only]()
+                "thadv": lambda: thadv_form(st, max(4, args.steps)),
+                "pw": lambda: pw_form(st, max(4, args.steps))}[args.only]()
         print(json.dumps({args.only: side}), flush=True)
         return
 
@@ -358,6 +393,8 @@ def main() -> None:
         out["leapfrog_fp64"] = leapfrog_form(st, dt, max(2, args.steps // 2))
         log("bench: th_advection")
         out["thadv_fp64"] = thadv_form(st, max(4, args.steps))
+        log("bench: pw_advection")
+        out["pw_fp64"] = pw_form(st, max(4, args.steps))
         log("bench: batched ensemble (48 FP32 members)")
         out["ensemble_fp32"] = ensemble_form(max(4, args.steps // 2))
     if args.calibrate_copy:

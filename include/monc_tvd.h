@@ -285,6 +285,24 @@ int monc_thadv_tendencies(monc_tvd_ctx *ctx, int anelastic, const double *dthref
                           double rate_y, double ugal, double vgal);
 
 /*
+ * The sibling component pw_advection on the same context: replaces
+ * components/pwadvection/src/pwadvection.F90:258-610 (timestep_callback with
+ * advect_flow_fields, advect_th_field, advect_q_field; U_ACTIVE, V_ACTIVE,
+ * W_ACTIVE, l_toplevel = .true.), the centred second-order scheme, over the
+ * interior columns (halo columns are skipped, :307) and levels 2..nz.  As the
+ * reference does, it overwrites the sources of u, v, w (advect_flow) and of
+ * th (advect_th) and adds to those of the q fields (advect_q); level 1, the
+ * halo columns and sw at k = nz are left untouched.  Reads the prognostics
+ * u, v, w (and th, q when advected), the context's cx, cy and tzc1, tzc2,
+ * tzd1, tzd2; tcx, tcy are the reference's horizontal%tcx = 0.25/dx and
+ * tcy = 0.25/dy (gridmanager.F90:2638-2641).  The flags are the reference's
+ * advect_flow/advect_th/advect_q (the field's advection option "pw" or
+ * "any").  The tendency diagnostics of the component are not produced.
+ * Launches on the library stream.
+ */
+int monc_pw_tendencies(monc_tvd_ctx *ctx, int advect_flow, int advect_th, int advect_q, double tcx, double tcy);
+
+/*
  * Tangent-linear model of monc_tvd_forward_steps (FP64 contexts only; no
  * counterpart in the reference).  Advances the state (MONC_TVD_PROG) and its
  * tangent (MONC_TVD_TANGENT, periodic halo filled) together nsteps times:

@@ -407,6 +407,117 @@ int thadv_impl(monc_tvd_ctx* c, int anelastic, const double* dthref, int baro, c
   return MONC_TVD_OK;
 }
 
+// pw_advection (pwadvection.F90:258-610): the centred second-order scheme,
+// one thread per interior cell (halo columns are skipped, :307), levels
+// 2..nz, all three directions active.  su, sv, sw and sth are overwritten
+// (the reference assigns them), sq(n) accumulates; l_toplevel = .true.
+// (:21): at k = nz su, sv and the scalars drop the W(nz) term and sw is left
+// alone.  tcx = 0.25/dx, tcy = 0.25/dy; cx, cy = 1/dx, 1/dy.  Expressions
+// keep the reference's operation order.
+template <typename T>
+struct PwArgs {
+  const T* f[MAX_FIELDS];
+  T* s[MAX_FIELDS];
+  const double *tzc1, *tzc2, *tzd1, *tzd2;
+  double cx, cy, tcx, tcy;
+  int nf, flow, th, q;
+};
+
+template <typename T>
+__device__ __forceinline__ T pw_scalar_cell(const T* f, const T* u, const T* v, const T* w, T s, const Dims& d,
+                                            int k, int j, int i, T cxh, T cyh, T t1, T t2, bool top) {
+  auto F = [&](const T* a, int kk, int jj, int ii) { return a[at(d, kk, jj, ii)]; };
+  s = s + cxh * (F(u, k, j, i - 1) * F(f, k, j, i - 1) - F(u, k, j, i) * F(f, k, j, i + 1));
+  s = s + cyh * (F(v, k, j - 1, i) * F(f, k, j - 1, i) - F(v, k, j, i) * F(f, k, j + 1, i));
+  if (!top) s = s + T(2.0) * (t1 * F(w, k - 1, j, i) * F(f, k - 1, j, i) - t2 * F(w, k, j, i) * F(f, k + 1, j, i));
+  else s = s + t1 * T(2.0) * F(w, k - 1, j, i) * F(f, k - 1, j, i);
+  return s;
+}
+
+// A block is 64 levels x 4 rows of one x column.  (An XCD-aware 1-D order
+// with x fastest and 8 rows per block fetched 21% less from HBM but ran 5%
+// slower at 512x512x256.)
+template <typename T>
+__global__ void __launch_bounds__(256) pw_kernel(PwArgs<T> P, Dims d) {
+  const int k = 2 + (int)blockIdx.x * 64 + (int)threadIdx.x;
+  const int j = 3 + (int)blockIdx.y * 4 + (int)threadIdx.y;
+  const int i = 3 + (int)blockIdx.z;
+  if (k > d.nz || j > d.ny + 2) return;
+  const bool top = k == d.nz;
+  const T *u = P.f[0], *v = P.f[1], *w = P.f[2];
+  auto F = [&](const T* a, int kk, int jj, int ii) { return a[at(d, kk, jj, ii)]; };
+  const T t1 = (T)P.tzc1[k - 1], t2 = (T)P.tzc2[k - 1];
+  const T tcx = (T)P.tcx, tcy = (T)P.tcy;
+  const int64_t o = at(d, k, j, i);
+  if (P.flow) {
+    T a = tcx * (F(u, k, j, i - 1) * (F(u, k, j, i) + F(u, k, j, i - 1)) -
+                 F(u, k, j, i + 1) * (F(u, k, j, i) + F(u, k, j, i + 1)));
+    a = a + tcy * (F(u, k, j - 1, i) * (F(v, k, j - 1, i) + F(v, k, j - 1, i + 1)) -
+                   F(u, k, j + 1, i) * (F(v, k, j, i) + F(v, k, j, i + 1)));
+    if (!top)
+      a = a + (t1 * F(u, k - 1, j, i) * (F(w, k - 1, j, i) + F(w, k - 1, j, i + 1)) -
+               t2 * F(u, k + 1, j, i) * (F(w, k, j, i) + F(w, k, j, i + 1)));
+    else
+      a = a + t1 * F(u, k - 1, j, i) * (F(w, k - 1, j, i) + F(w, k - 1, j, i + 1));
+    P.s[0][o] = a;
+    T b = tcy * (F(v, k, j - 1, i) * (F(v, k, j, i) + F(v, k, j - 1, i)) -
+                 F(v, k, j + 1, i) * (F(v, k, j, i) + F(v, k, j + 1, i)));
+    b = b + tcx * (F(v, k, j, i - 1) * (F(u, k, j, i - 1) + F(u, k, j + 1, i - 1)) -
+                   F(v, k, j, i + 1) * (F(u, k, j, i) + F(u, k, j + 1, i)));
+    if (!top)
+      b = b + (t1 * F(v, k - 1, j, i) * (F(w, k - 1, j, i) + F(w, k - 1, j + 1, i)) -
+               t2 * F(v, k + 1, j, i) * (F(w, k, j, i) + F(w, k, j + 1, i)));
+    else
+      b = b + t1 * F(v, k - 1, j, i) * (F(w, k - 1, j, i) + F(w, k - 1, j + 1, i));
+    P.s[1][o] = b;
+    if (!top) {
+      const T d1 = (T)P.tzd1[k - 1], d2 = (T)P.tzd2[k - 1];
+      T c = (d1 * F(w, k - 1, j, i) * (F(w, k, j, i) + F(w, k - 1, j, i)) -
+             d2 * F(w, k + 1, j, i) * (F(w, k, j, i) + F(w, k + 1, j, i)));
+      c = c + tcx * (F(w, k, j, i - 1) * (F(u, k, j, i - 1) + F(u, k + 1, j, i - 1)) -
+                     F(w, k, j, i + 1) * (F(u, k, j, i) + F(u, k + 1, j, i)));
+      c = c + tcy * (F(w, k, j - 1, i) * (F(v, k, j - 1, i) + F(v, k + 1, j - 1, i)) -
+                     F(w, k, j + 1, i) * (F(v, k, j, i) + F(v, k + 1, j, i)));
+      P.s[2][o] = c;
+    }
+  }
+  const T cxh = (T)(P.cx * 0.5), cyh = (T)(P.cy * 0.5);
+  if (P.th) {
+    // the reference assigns the x term to sth, then adds y and z (:407, :437)
+    P.s[3][o] = pw_scalar_cell<T>(P.f[3], u, v, w, T(0), d, k, j, i, cxh, cyh, t1, t2, top);
+  }
+  if (P.q)
+    for (int n = 4; n < P.nf; n++)
+      P.s[n][o] = pw_scalar_cell<T>(P.f[n], u, v, w, P.s[n][o], d, k, j, i, cxh, cyh, t1, t2, top);
+}
+
+template <typename T>
+int pw_impl(monc_tvd_ctx* c, int flow, int th, int q, double tcx, double tcy) {
+  PwArgs<T> P{};
+  P.nf = c->nfields;
+  for (int n = 0; n < c->nfields; n++) {
+    P.f[n] = (const T*)slot(c, n, MONC_TVD_PROG);
+    P.s[n] = (T*)slot(c, n, MONC_TVD_SOURCE);
+  }
+  P.tzc1 = c->dmet + MET_TZC1 * c->nz;
+  P.tzc2 = c->dmet + MET_TZC2 * c->nz;
+  P.tzd1 = c->dmet + MET_TZD1 * c->nz;
+  P.tzd2 = c->dmet + MET_TZD2 * c->nz;
+  P.cx = c->cx;
+  P.cy = c->cy;
+  P.tcx = tcx;
+  P.tcy = tcy;
+  P.flow = flow;
+  P.th = th;
+  P.q = q;
+  const Dims d = dims_of(c);
+  dim3 block(64, 4, 1), grid((c->nz - 1 + 63) / 64, (c->ny + 3) / 4, c->nx);
+  LaunchTimer timer(c, 4);
+  hipLaunchKernelGGL((pw_kernel<T>), grid, block, 0, c->stream, P, d);
+  HIP_TRY(hipGetLastError());
+  return MONC_TVD_OK;
+}
+
 int need(monc_tvd_ctx* c, int field, int variant) {
   if (!slot(c, field, variant))
     return fail(MONC_TVD_ERR_STATE, "field %d variant %d is needed but was never set or bound", field, variant);
@@ -1222,6 +1333,27 @@ int monc_thadv_tendencies(monc_tvd_ctx* c, int anelastic, const double* dthref, 
                                   ugal, vgal)
              : thadv_impl<float>(c, anelastic, dthref, baroclinicity, thref, thref0, fcoriol, g, rate_x, rate_y,
                                  ugal, vgal);
+}
+
+int monc_pw_tendencies(monc_tvd_ctx* c, int advect_flow, int advect_th, int advect_q, double tcx, double tcy) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!advect_flow && !advect_th && !advect_q) return MONC_TVD_OK;
+  if (c->nfields > MAX_FIELDS) return fail(MONC_TVD_ERR_ARG, "%d fields above the kernel's %d", c->nfields, MAX_FIELDS);
+  if (!(tcx == tcx) || !(tcy == tcy)) return fail(MONC_TVD_ERR_ARG, "tcx, tcy must not be NaN");
+  for (int f = MONC_TVD_U; f <= MONC_TVD_W; f++)
+    if (int rc = need(c, f, MONC_TVD_PROG)) return rc;
+  if (advect_flow)
+    for (int f = MONC_TVD_U; f <= MONC_TVD_W; f++)
+      if (int rc = need(c, f, MONC_TVD_SOURCE)) return rc;
+  if (advect_th)
+    for (int v : {MONC_TVD_PROG, MONC_TVD_SOURCE})
+      if (int rc = need(c, MONC_TVD_TH, v)) return rc;
+  if (advect_q)
+    for (int n = 0; n < c->n_q; n++)
+      for (int v : {MONC_TVD_PROG, MONC_TVD_SOURCE})
+        if (int rc = need(c, MONC_TVD_Q(n), v)) return rc;
+  return c->precision == MONC_TVD_FP64 ? pw_impl<double>(c, advect_flow, advect_th, advect_q, tcx, tcy)
+                                       : pw_impl<float>(c, advect_flow, advect_th, advect_q, tcx, tcy);
 }
 
 int monc_tvd_tangent_steps(monc_tvd_ctx* c, double dtm, int nsteps) {

@@ -82,6 +82,15 @@ module monc_tvd_interface_mod
       real(c_double), value :: thref0, fcoriol, g, rate_x, rate_y, ugal, vgal
     end function monc_thadv_tendencies
 
+    ! pw_advection's timestep callback (pwadvection.F90:258-610) on the same context
+    integer(c_int) function monc_pw_tendencies(ctx, advect_flow, advect_th, advect_q, tcx, tcy) &
+        bind(C, name="monc_pw_tendencies")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: advect_flow, advect_th, advect_q
+      real(c_double), value :: tcx, tcy
+    end function monc_pw_tendencies
+
     integer(c_int) function monc_tvd_bind_field(ctx, field, variant, device_ptr) bind(C, name="monc_tvd_bind_field")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

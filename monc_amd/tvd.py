@@ -36,6 +36,7 @@ EXPORTS = (
     "monc_tvd_adjoint_step", "monc_tvd_adjoint_fold", "monc_tvd_tangent_steps", "monc_tvd_march_shapes",
     "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic", "monc_tvd_autotune",
     "monc_tvd_leapfrog_steps", "monc_tvd_forward_step_into", "monc_thadv_tendencies",
+    "monc_pw_tendencies",
     "monc_tvd_ensemble_create", "monc_tvd_ensemble_destroy", "monc_tvd_ensemble_layout",
     "monc_tvd_ensemble_set_field", "monc_tvd_ensemble_get_field", "monc_tvd_ensemble_copy_pool",
     "monc_tvd_ensemble_set_dt", "monc_tvd_ensemble_set_stream", "monc_tvd_ensemble_synchronize",
@@ -99,6 +100,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_adjoint_fold.argtypes = [c_void_p]
     dp = ctypes.POINTER(c_double)
     lib.monc_thadv_tendencies.argtypes = [c_void_p, c_int, dp, c_int, dp] + [c_double] * 7
+    lib.monc_pw_tendencies.argtypes = [c_void_p, c_int, c_int, c_int, c_double, c_double]
     lib.monc_tvd_tangent_steps.argtypes = [c_void_p, c_double, c_int]
     ip = ctypes.POINTER(c_int)
     llp = ctypes.POINTER(ctypes.c_longlong)
@@ -328,6 +330,13 @@ class TvdAdvection:
             self._h, int(bool(anelastic)), dth.ctypes.data_as(dp) if dth is not None else None,
             int(bool(baroclinicity)), thr.ctypes.data_as(dp) if thr is not None else None,
             thref0, fcoriol, g, rate_x, rate_y, ugal, vgal))
+
+    def pw_tendencies(self, advect_flow: bool = True, advect_th: bool = True, advect_q: bool = True, *,
+                      tcx: float, tcy: float) -> None:
+        """pw_advection's centred tendencies (pwadvection.F90:258-610): su, sv,
+        sw and sth overwritten, sq added to, interior columns."""
+        _check(self._lib.monc_pw_tendencies(self._h, int(bool(advect_flow)), int(bool(advect_th)),
+                                            int(bool(advect_q)), tcx, tcy))
 
     def tangent_steps(self, dtm: float, nsteps: int) -> None:
         _check(self._lib.monc_tvd_tangent_steps(self._h, dtm, nsteps))

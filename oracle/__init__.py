@@ -36,6 +36,14 @@ Parity status per oracle function (the checks live in tests/test_oracle.py):
   the thermal-wind form of the mean baroclinicity term (-u.grad(thbar) with
   grad(thbar) = (thref0 f / g) (dVg/dz, -dUg/dz)), galilean pairing, and the
   untouched level 1 and halo-inclusive columns.
+* oracle_pw (pw_advection, the second sibling plugin) -- pinned against the
+  TVD oracle: for constant advected fields (scalars, and u, v advected by
+  themselves) the centred scheme's tendency is half of TVD's, whose constant
+  field case is pinned above (both reduce to c times the discrete flow
+  divergence); by exact centred advection of linear fields under uniform
+  flow (-U dq/dx, -V dq/dy, -W dq/dz, -V du/dy, -U dv/dx, -U dw/dx); and by
+  the reference's overwrite of su, sv, sw, sth against the accumulation into
+  sq, untouched level 1, sw left alone at the top and halo columns skipped.
 """
 from __future__ import annotations
 
@@ -79,6 +87,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_swap_classic.argtypes = [dp, dp, ci, ci, ci]
         L.oracle_swap_robert.argtypes = [dp, dp, dp, ci, cd, cd, ci, ci, ci]
         L.oracle_thadv.argtypes = [dp, dp, dp, dp, dp, dp, dp, dp, ci, ci, ci, ci, ci] + [cd] * 7
+        pp = ctypes.POINTER(dp)
+        L.oracle_pw.argtypes = [dp, dp, dp, dp, pp, dp, dp, dp, dp, pp, ci, dp, dp, dp, dp, cd, cd, cd, cd, ci, ci,
+                                ci, ci, ci, ci]
         _lib = L
     return _lib
 
@@ -260,6 +271,25 @@ def thadv(sth, u, v, w, tzc1, tzc2, dthref=None, thref=None, anelastic: bool = T
     lib().oracle_thadv(_p(out), *[_p(a) for a in arrs], *[_p(a) for a in vs], nz, ny, nx, int(bool(anelastic)),
                        int(bool(baroclinicity)), thref0, fcoriol, g, rate_x, rate_y, ugal, vgal)
     return out
+
+
+def pw(fields, sources, grid, cx: float, cy: float, tcx: float, tcy: float, advect_flow: bool = True,
+       advect_th: bool = True, advect_q: bool = True):
+    """pw_advection (pwadvection.F90:258-610) on copies of the source fields
+    [su, sv, sw, sth, sq(1..)] from the prognostics [u, v, w, th, q(1..)];
+    returns the new source fields."""
+    f = [np.ascontiguousarray(a, dtype=np.float64) for a in fields]
+    s = [np.array(a, dtype=np.float64, copy=True) for a in sources]
+    nx, ny, nz = f[0].shape[0] - 4, f[0].shape[1] - 4, f[0].shape[2]
+    n_q = len(f) - 4
+    dp = ctypes.POINTER(ctypes.c_double)
+    qs = (dp * max(n_q, 1))(*[_p(a) for a in f[4:]])
+    sqs = (dp * max(n_q, 1))(*[_p(a) for a in s[4:]])
+    m = [np.ascontiguousarray(getattr(grid, n), dtype=np.float64) for n in ("tzc1", "tzc2", "tzd1", "tzd2")]
+    lib().oracle_pw(_p(s[0]), _p(s[1]), _p(s[2]), _p(s[3]), sqs, _p(f[0]), _p(f[1]), _p(f[2]), _p(f[3]), qs, n_q,
+                    *[_p(a) for a in m], cx, cy, tcx, tcy, nz, ny, nx, int(bool(advect_flow)), int(bool(advect_th)),
+                    int(bool(advect_q)))
+    return s
 
 
 def leapfrog_run(state, dtm: float, tsmth: float, nsteps: int, ugal: float = 0.0, vgal: float = 0.0,

@@ -760,3 +760,79 @@ void oracle_thadv(double *sth, const double *u, const double *v, const double *w
       }
     }
 }
+
+/* pwadvection.F90:258-610 over the interior columns (halo columns return at
+ * :307), all three directions active (U_ACTIVE, V_ACTIVE, W_ACTIVE), with
+ * l_toplevel = .true. (:21).  Flow fields (advect_flow_fields, :466-610):
+ * su, sv, sw are overwritten (k = 2..nz-1; su and sv also at k = nz without
+ * the W(nz) term; sw not at k = nz).  th (advect_th_field, :397-460):
+ * overwritten too (the reference's own `sth = cx*...` at :407 and :437).
+ * q fields (advect_q_field, :327-391): added to sq(n).  tcx = 0.25/dx,
+ * tcy = 0.25/dy (gridmanager.F90:2638-2641); cx, cy the model's 1/dx, 1/dy. */
+static void pw_scalar(double *s, const double *f, const double *u, const double *v, const double *w,
+                      const double *tzc1, const double *tzc2, double cx, double cy, int nz, int j, int i,
+                      int overwrite) {
+  for (int k = 2; k <= nz; k++) {
+    double x = cx * 0.5 * (A(u, k, j, i - 1) * A(f, k, j, i - 1) - A(u, k, j, i) * A(f, k, j, i + 1));
+    double acc = overwrite ? x : A(s, k, j, i) + x;
+    acc = acc + cy * 0.5 * (A(v, k, j - 1, i) * A(f, k, j - 1, i) - A(v, k, j, i) * A(f, k, j + 1, i));
+    if (k < nz)
+      acc = acc + 2.0 * (tzc1[k - 1] * A(w, k - 1, j, i) * A(f, k - 1, j, i) -
+                         tzc2[k - 1] * A(w, k, j, i) * A(f, k + 1, j, i));
+    else
+      acc = acc + tzc1[k - 1] * 2.0 * A(w, k - 1, j, i) * A(f, k - 1, j, i);
+    A(s, k, j, i) = acc;
+  }
+}
+
+void oracle_pw(double *su, double *sv, double *sw, double *sth, double *const *sq, const double *u,
+               const double *v, const double *w, const double *th, const double *const *q, int n_q,
+               const double *tzc1, const double *tzc2, const double *tzd1, const double *tzd2, double cx,
+               double cy, double tcx, double tcy, int nz, int ny, int nx, int advect_flow, int advect_th,
+               int advect_q) {
+  g_nz = nz;
+  g_ny = ny;
+  g_nx = nx;
+  for (int i = 3; i <= nx + 2; i++)
+    for (int j = 3; j <= ny + 2; j++) {
+      if (advect_flow) {
+        for (int k = 2; k <= nz; k++) {
+          const int top = k == nz;
+          double a = tcx * (A(u, k, j, i - 1) * (A(u, k, j, i) + A(u, k, j, i - 1)) -
+                            A(u, k, j, i + 1) * (A(u, k, j, i) + A(u, k, j, i + 1)));
+          a = a + tcy * (A(u, k, j - 1, i) * (A(v, k, j - 1, i) + A(v, k, j - 1, i + 1)) -
+                         A(u, k, j + 1, i) * (A(v, k, j, i) + A(v, k, j, i + 1)));
+          if (!top)
+            a = a + (tzc1[k - 1] * A(u, k - 1, j, i) * (A(w, k - 1, j, i) + A(w, k - 1, j, i + 1)) -
+                     tzc2[k - 1] * A(u, k + 1, j, i) * (A(w, k, j, i) + A(w, k, j, i + 1)));
+          else
+            a = a + tzc1[k - 1] * A(u, k - 1, j, i) * (A(w, k - 1, j, i) + A(w, k - 1, j, i + 1));
+          A(su, k, j, i) = a;
+
+          double b = tcy * (A(v, k, j - 1, i) * (A(v, k, j, i) + A(v, k, j - 1, i)) -
+                            A(v, k, j + 1, i) * (A(v, k, j, i) + A(v, k, j + 1, i)));
+          b = b + tcx * (A(v, k, j, i - 1) * (A(u, k, j, i - 1) + A(u, k, j + 1, i - 1)) -
+                         A(v, k, j, i + 1) * (A(u, k, j, i) + A(u, k, j + 1, i)));
+          if (!top)
+            b = b + (tzc1[k - 1] * A(v, k - 1, j, i) * (A(w, k - 1, j, i) + A(w, k - 1, j + 1, i)) -
+                     tzc2[k - 1] * A(v, k + 1, j, i) * (A(w, k, j, i) + A(w, k, j + 1, i)));
+          else
+            b = b + tzc1[k - 1] * A(v, k - 1, j, i) * (A(w, k - 1, j, i) + A(w, k - 1, j + 1, i));
+          A(sv, k, j, i) = b;
+
+          if (!top) {
+            double c = (tzd1[k - 1] * A(w, k - 1, j, i) * (A(w, k, j, i) + A(w, k - 1, j, i)) -
+                        tzd2[k - 1] * A(w, k + 1, j, i) * (A(w, k, j, i) + A(w, k + 1, j, i)));
+            c = c + tcx * (A(w, k, j, i - 1) * (A(u, k, j, i - 1) + A(u, k + 1, j, i - 1)) -
+                           A(w, k, j, i + 1) * (A(u, k, j, i) + A(u, k + 1, j, i)));
+            c = c + tcy * (A(w, k, j - 1, i) * (A(v, k, j - 1, i) + A(v, k + 1, j - 1, i)) -
+                           A(w, k, j + 1, i) * (A(v, k, j, i) + A(v, k + 1, j, i)));
+            A(sw, k, j, i) = c;
+          }
+        }
+      }
+      if (advect_th) pw_scalar(sth, th, u, v, w, tzc1, tzc2, cx, cy, nz, j, i, 1);
+      if (advect_q)
+        for (int n = 0; n < n_q; n++) pw_scalar(sq[n], q[n], u, v, w, tzc1, tzc2, cx, cy, nz, j, i, 0);
+    }
+}

@@ -15,7 +15,7 @@ HEADER = Path(__file__).resolve().parents[1] / "include" / "monc_tvd.h"
 def declared_functions():
     text = HEADER.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(monc_(?:tvd|thadv)_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(monc_(?:tvd|thadv|pw)_[a-z_]+)\s*\(", text)))
 
 
 def test_header_declares_the_binding_set():
