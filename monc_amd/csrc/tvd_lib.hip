@@ -434,9 +434,10 @@ __device__ __forceinline__ T pw_scalar_cell(const T* f, const T* u, const T* v, 
   return s;
 }
 
-// A block is 64 levels x 4 rows of one x column.  (An XCD-aware 1-D order
-// with x fastest and 8 rows per block fetched 21% less from HBM but ran 5%
-// slower at 512x512x256.)
+// A block is 64 levels x 4 rows of one x column.  Measured alternatives at
+// 512x512x256 FP64: an XCD-aware 1-D order with x fastest and 8 rows per
+// block (HBM fetch -21%, 5% slower) and blocks marching 16 x columns (fetch
+// +48%, 4% slower).
 template <typename T>
 __global__ void __launch_bounds__(256) pw_kernel(PwArgs<T> P, Dims d) {
   const int k = 2 + (int)blockIdx.x * 64 + (int)threadIdx.x;
