@@ -18,9 +18,10 @@ dominant kernel's algorithmic bytes per launch over its average duration
 (profiles/march_kernel_traffic_512x512x256.json; null for another shape);
 cpu_baseline = the C restatement (oracle) timed on a bounded sample on this
 host's CPU.  Beside it: the FP32 form, the FP32 adjoint step on the same
-domain, the MONC leapfrog timestep (FP64), and the batched ensemble (48 FP32
-members of seeded sizes 64..256 x 64..256 x 64..128, one persistent launch per
-step, graph-replayed).
+domain, the MONC leapfrog timestep (FP64), the sibling components
+th_advection and pw_advection on the same FP64 state (kernel time and HBM
+fraction), and the batched ensemble (48 FP32 members of seeded sizes
+64..256 x 64..256 x 64..128, one persistent launch per step, graph-replayed).
 """
 from __future__ import annotations
 

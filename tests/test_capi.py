@@ -63,6 +63,10 @@ def test_null_context_is_an_argument_error():
     assert lib.monc_tvd_halo_fill(None) == tvd.ERR_ARG
     assert "null context" in lib.monc_tvd_last_error().decode()
     assert lib.monc_tvd_tendencies(None, 1.0, 1, 1, 1) == tvd.ERR_ARG
+    assert lib.monc_thadv_tendencies(None, 1, None, 0, None, 300.0, 1e-4, 9.81, 0.0, 0.0, 0.0, 0.0) == tvd.ERR_ARG
+    assert "null context" in lib.monc_tvd_last_error().decode()
+    assert lib.monc_pw_tendencies(None, 1, 1, 1, 0.005, 0.005) == tvd.ERR_ARG
+    assert "null context" in lib.monc_tvd_last_error().decode()
     assert lib.monc_tvd_destroy(None) == tvd.OK
 
 
