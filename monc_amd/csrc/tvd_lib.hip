@@ -359,16 +359,19 @@ std::vector<int> advected_fields(const monc_tvd_ctx* c) {
 //   anelastic:  sth += thref(k)*(fcoriol/G) * ((v+vgal)*rate_x - (u+ugal)*rate_y)
 //   otherwise:  sth += (v+vgal)*mfx - (u+ugal)*mfy,  mf = rate*thref0*(fcoriol/G)
 // with the factors of initialisation_callback (:60-77) formed on the host in
-// FP64.  One thread per (level, column), z fastest.
+// FP64.  One thread per (level, column), z fastest; a wave covers levels
+// 1..64 (65..128, ...) so that its loads and stores fill whole cache lines
+// (level 1 idles).
 template <typename T>
 __global__ void __launch_bounds__(256) thadv_kernel(T* sth, const T* u, const T* v, const T* w, const double* tzc1,
                                                     const double* tzc2, const double* dthref, const double* thref,
                                                     Dims d, int anelastic, int baro, double fcog, double rate_x,
                                                     double rate_y, double mfx, double mfy, T ugal, T vgal) {
-  const int k = 2 + (int)blockIdx.x * 64 + (int)threadIdx.x;
-  const long long col = (long long)blockIdx.y * 4 + threadIdx.y;
-  if (k > d.nz || col >= (long long)(d.ny + 4) * (d.nx + 4)) return;
-  const int j = 1 + (int)(col % (d.ny + 4)), i = 1 + (int)(col / (d.ny + 4));
+  const int k = 1 + (int)blockIdx.x * 64 + (int)threadIdx.x;
+  const int col = (int)blockIdx.y * 4 + (int)threadIdx.y;  // < 2^31, checked on the host
+  if (k < 2 || k > d.nz || col >= (d.ny + 4) * (d.nx + 4)) return;
+  const int jb = d.ny + 4, i0 = col / jb;
+  const int j = 1 + (col - i0 * jb), i = 1 + i0;
   const int64_t o = at(d, k, j, i);
   T s = sth[o];
   if (anelastic) {
@@ -396,7 +399,8 @@ int thadv_impl(monc_tvd_ctx* c, int anelastic, const double* dthref, int baro, c
   const double mfx = rate_x * thref0 * fcog, mfy = rate_y * thref0 * fcog;
   const Dims d = dims_of(c);
   const long long ncol = (long long)(c->ny + 4) * (c->nx + 4);
-  dim3 block(64, 4, 1), grid((nz - 1 + 63) / 64, (unsigned)((ncol + 3) / 4), 1);
+  if (ncol > 0x7ffffff0LL) return fail(MONC_TVD_ERR_ARG, "too many columns for th_advection's grid");
+  dim3 block(64, 4, 1), grid((nz + 63) / 64, (unsigned)((ncol + 3) / 4), 1);
   LaunchTimer timer(c, 4);
   hipLaunchKernelGGL((thadv_kernel<T>), grid, block, 0, c->stream, (T*)slot(c, MONC_TVD_TH, MONC_TVD_SOURCE),
                      (const T*)slot(c, MONC_TVD_U, MONC_TVD_PROG), (const T*)slot(c, MONC_TVD_V, MONC_TVD_PROG),
@@ -434,16 +438,18 @@ __device__ __forceinline__ T pw_scalar_cell(const T* f, const T* u, const T* v, 
   return s;
 }
 
-// A block is 64 levels x 4 rows of one x column.  Measured alternatives at
+// A block is 64 levels x 4 rows of one x column; a wave covers levels 1..64
+// (65..128, ...) so that its loads and stores fill whole cache lines (level 1
+// idles).  Measured alternatives at
 // 512x512x256 FP64: an XCD-aware 1-D order with x fastest and 8 rows per
 // block (HBM fetch -21%, 5% slower) and blocks marching 16 x columns (fetch
 // +48%, 4% slower).
 template <typename T>
 __global__ void __launch_bounds__(256) pw_kernel(PwArgs<T> P, Dims d) {
-  const int k = 2 + (int)blockIdx.x * 64 + (int)threadIdx.x;
+  const int k = 1 + (int)blockIdx.x * 64 + (int)threadIdx.x;
   const int j = 3 + (int)blockIdx.y * 4 + (int)threadIdx.y;
   const int i = 3 + (int)blockIdx.z;
-  if (k > d.nz || j > d.ny + 2) return;
+  if (k < 2 || k > d.nz || j > d.ny + 2) return;
   const bool top = k == d.nz;
   const T *u = P.f[0], *v = P.f[1], *w = P.f[2];
   auto F = [&](const T* a, int kk, int jj, int ii) { return a[at(d, kk, jj, ii)]; };
@@ -512,7 +518,7 @@ int pw_impl(monc_tvd_ctx* c, int flow, int th, int q, double tcx, double tcy) {
   P.th = th;
   P.q = q;
   const Dims d = dims_of(c);
-  dim3 block(64, 4, 1), grid((c->nz - 1 + 63) / 64, (c->ny + 3) / 4, c->nx);
+  dim3 block(64, 4, 1), grid((c->nz + 63) / 64, (c->ny + 3) / 4, c->nx);
   LaunchTimer timer(c, 4);
   hipLaunchKernelGGL((pw_kernel<T>), grid, block, 0, c->stream, P, d);
   HIP_TRY(hipGetLastError());
