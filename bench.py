@@ -89,15 +89,22 @@ def roofline(kt: dict, cells: int, es: int):
     return kind, avg_ms, bytes_launch, achieved
 
 
+def default_shape(precision: str, kind: str):
+    """The library's default march tile (rows, x columns) for this workload
+    (256 levels over 512 x 512 columns)."""
+    return [2, 64] if precision == "fp64" and kind == "scalars" else [3, 32]
+
+
 def measured_traffic(precision: str, kind: str, nz: int, shape):
-    """HBM bytes per launch that rocprofv3 measured for this kernel and tile
-    shape, or None."""
+    """HBM bytes per launch that rocprofv3 measured for this kernel at the
+    tile shape it was profiled in, or None for another shape."""
     p = os.path.join(ROOT, "profiles", "march_kernel_traffic_512x512x256.json")
     if nz != 256 or not os.path.exists(p):
         return None
     with open(p) as fh:
         t = json.load(fh)
-    if shape is None or list(shape) != [t["shape"]["rows"], t["shape"]["strip"]]:
+    prof = t.get("shapes", {}).get(precision, {}).get(kind)
+    if shape is None or prof is None or list(shape) != list(prof):
         return None
     return t["bytes_per_launch"].get(precision, {}).get(kind)
 
@@ -346,7 +353,7 @@ def main() -> None:
     value = cells * args.steps / wall64 / 1e9
     kind, kavg, kbytes, achieved = roofline(kt64, cells, 8)
     traffic = measured_traffic("fp64", kind, nz, (tuned64 or {}).get(kind) if tuned64 else
-                               ([3, 32] if args.kernel == 0 and args.rows == 0 else None))
+                               (default_shape("fp64", kind) if args.kernel == 0 and args.rows == 0 else None))
     step_kernel_ms = sum(v[0] for k, v in kt64.items() if k in KIND_FIELDS) / args.steps
     out = {
         "metric": METRIC,

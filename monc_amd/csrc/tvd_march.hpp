@@ -622,6 +622,14 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
   __syncthreads();  // the LDS is reused by the block's next tile
 }
 
+// The FP64 scalar pair in tiles of at most 2 rows of 256 levels runs at 2
+// waves per SIMD, with registers to carry its x windows (3.03 against 3.18 ms
+// per launch at 512x512x256 in 2 x 64 tiles).
+template <typename T, int NS, int NCH, int R>
+constexpr bool pair_wide() {
+  return sizeof(T) == 8 && NS == 2 && NCH == 4 && R <= 2;
+}
+
 // Single-domain launch: a 1-D grid of tiles_y * tiles_x blocks of NCH*R waves.
 // Consecutive blocks of one XCD (blocks are dealt to the 8 XCDs round-robin)
 // take consecutive tiles, so that rows shared by neighbouring tiles meet in
@@ -635,13 +643,13 @@ march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
   const int t = bid < full ? (bid % 8) * (full / 8) + bid / 8 : bid;
   // The scalar pair reloads its x windows: two windows leave FP64 no
   // registers at 3 waves per SIMD and cost FP32 a wave per SIMD, which
-  // measured slower than the loads they save.  Vertical neighbours by wave
+  // measured slower than the loads they save (pair_wide shapes excepted).  Vertical neighbours by wave
   // shifts pay for the w kernel (3% in FP64, 2% in FP32); the FP64 u, v and
   // scalar kernels, at 3 waves per SIMD and two shifts per value, run faster
   // loading them, and in FP32 the gain depends on the tile shape.
   constexpr bool VD = KIND == KIND_W && NS == 1;
-  march_tile<T, KIND, NS, MODE, NCH * R, 64 * NCH, NS, NS != 2, VD>(L, S, NCH, t % tiles_y, t / tiles_y, strip,
-                                                                   lds);
+  constexpr bool WIN = NS != 2 || pair_wide<T, NS, NCH, R>();
+  march_tile<T, KIND, NS, MODE, NCH * R, 64 * NCH, NS, WIN, VD>(L, S, NCH, t % tiles_y, t / tiles_y, strip, lds);
 }
 
 }  // namespace tvd
