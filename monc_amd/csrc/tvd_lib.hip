@@ -432,75 +432,81 @@ struct PwArgs {
   int nf, flow, th, q;
 };
 
-template <typename T>
-__device__ __forceinline__ T pw_scalar_cell(const T* f, const T* u, const T* v, const T* w, T s, const Dims& d,
-                                            int k, int j, int i, T cxh, T cyh, T t1, T t2, bool top) {
-  auto F = [&](const T* a, int kk, int jj, int ii) { return a[at(d, kk, jj, ii)]; };
-  s = s + cxh * (F(u, k, j, i - 1) * F(f, k, j, i - 1) - F(u, k, j, i) * F(f, k, j, i + 1));
-  s = s + cyh * (F(v, k, j - 1, i) * F(f, k, j - 1, i) - F(v, k, j, i) * F(f, k, j + 1, i));
-  if (!top) s = s + T(2.0) * (t1 * F(w, k - 1, j, i) * F(f, k - 1, j, i) - t2 * F(w, k, j, i) * F(f, k + 1, j, i));
-  else s = s + t1 * T(2.0) * F(w, k - 1, j, i) * F(f, k - 1, j, i);
-  return s;
-}
-
-// A block is 64 levels x 4 rows of one x column; a wave covers levels 1..64
-// (65..128, ...) so that its loads and stores fill whole cache lines (level 1
-// idles).  Measured alternatives at
-// 512x512x256 FP64: an XCD-aware 1-D order with x fastest and 8 rows per
-// block (HBM fetch -21%, 5% slower) and blocks marching 16 x columns (fetch
-// +48%, 4% slower).
+// A block is 64 levels x 4 rows; each thread marches a strip of PW_STRIP x
+// columns of its (level, row) and carries the flows two columns share in
+// registers (u, v, w at i-1..i+1; v(j-1), w(k-1) at i..i+1; u(j+1), u(k+1)
+// at i-1..i), so a column loads 17 flow values instead of ~33.  Waves cover
+// levels 1..64, 65..128, ... so that accesses fill whole cache lines (level 1
+// idles).  Every expression keeps the reference's operation order.
+constexpr int PW_STRIP = 16;
 template <typename T>
 __global__ void __launch_bounds__(256) pw_kernel(PwArgs<T> P, Dims d) {
   const int k = 1 + (int)blockIdx.x * 64 + (int)threadIdx.x;
   const int j = 3 + (int)blockIdx.y * 4 + (int)threadIdx.y;
-  const int i = 3 + (int)blockIdx.z;
   if (k < 2 || k > d.nz || j > d.ny + 2) return;
+  const int i0 = 3 + (int)blockIdx.z * PW_STRIP, i1 = min(i0 + PW_STRIP, d.nx + 3);
   const bool top = k == d.nz;
+  const int kp = top ? k : k + 1;  // level k+1 (only read below the top; in bounds there too)
   const T *u = P.f[0], *v = P.f[1], *w = P.f[2];
   auto F = [&](const T* a, int kk, int jj, int ii) { return a[at(d, kk, jj, ii)]; };
   const T t1 = (T)P.tzc1[k - 1], t2 = (T)P.tzc2[k - 1];
+  const T d1 = top ? T(0) : (T)P.tzd1[k - 1], d2 = top ? T(0) : (T)P.tzd2[k - 1];
   const T tcx = (T)P.tcx, tcy = (T)P.tcy;
-  const int64_t o = at(d, k, j, i);
-  if (P.flow) {
-    T a = tcx * (F(u, k, j, i - 1) * (F(u, k, j, i) + F(u, k, j, i - 1)) -
-                 F(u, k, j, i + 1) * (F(u, k, j, i) + F(u, k, j, i + 1)));
-    a = a + tcy * (F(u, k, j - 1, i) * (F(v, k, j - 1, i) + F(v, k, j - 1, i + 1)) -
-                   F(u, k, j + 1, i) * (F(v, k, j, i) + F(v, k, j, i + 1)));
-    if (!top)
-      a = a + (t1 * F(u, k - 1, j, i) * (F(w, k - 1, j, i) + F(w, k - 1, j, i + 1)) -
-               t2 * F(u, k + 1, j, i) * (F(w, k, j, i) + F(w, k, j, i + 1)));
-    else
-      a = a + t1 * F(u, k - 1, j, i) * (F(w, k - 1, j, i) + F(w, k - 1, j, i + 1));
-    P.s[0][o] = a;
-    T b = tcy * (F(v, k, j - 1, i) * (F(v, k, j, i) + F(v, k, j - 1, i)) -
-                 F(v, k, j + 1, i) * (F(v, k, j, i) + F(v, k, j + 1, i)));
-    b = b + tcx * (F(v, k, j, i - 1) * (F(u, k, j, i - 1) + F(u, k, j + 1, i - 1)) -
-                   F(v, k, j, i + 1) * (F(u, k, j, i) + F(u, k, j + 1, i)));
-    if (!top)
-      b = b + (t1 * F(v, k - 1, j, i) * (F(w, k - 1, j, i) + F(w, k - 1, j + 1, i)) -
-               t2 * F(v, k + 1, j, i) * (F(w, k, j, i) + F(w, k, j + 1, i)));
-    else
-      b = b + t1 * F(v, k - 1, j, i) * (F(w, k - 1, j, i) + F(w, k - 1, j + 1, i));
-    P.s[1][o] = b;
-    if (!top) {
-      const T d1 = (T)P.tzd1[k - 1], d2 = (T)P.tzd2[k - 1];
-      T c = (d1 * F(w, k - 1, j, i) * (F(w, k, j, i) + F(w, k - 1, j, i)) -
-             d2 * F(w, k + 1, j, i) * (F(w, k, j, i) + F(w, k + 1, j, i)));
-      c = c + tcx * (F(w, k, j, i - 1) * (F(u, k, j, i - 1) + F(u, k + 1, j, i - 1)) -
-                     F(w, k, j, i + 1) * (F(u, k, j, i) + F(u, k + 1, j, i)));
-      c = c + tcy * (F(w, k, j - 1, i) * (F(v, k, j - 1, i) + F(v, k + 1, j - 1, i)) -
-                     F(w, k, j + 1, i) * (F(v, k, j, i) + F(v, k + 1, j, i)));
-      P.s[2][o] = c;
-    }
-  }
   const T cxh = (T)(P.cx * 0.5), cyh = (T)(P.cy * 0.5);
-  if (P.th) {
-    // the reference assigns the x term to sth, then adds y and z (:407, :437)
-    P.s[3][o] = pw_scalar_cell<T>(P.f[3], u, v, w, T(0), d, k, j, i, cxh, cyh, t1, t2, top);
+  T u_m = F(u, k, j, i0 - 1), u_0 = F(u, k, j, i0);
+  T v_m = F(v, k, j, i0 - 1), v_0 = F(v, k, j, i0);
+  T w_m = F(w, k, j, i0 - 1), w_0 = F(w, k, j, i0);
+  T vs_0 = F(v, k, j - 1, i0), wd_0 = F(w, k - 1, j, i0);
+  T un_m = F(u, k, j + 1, i0 - 1), uk_m = F(u, kp, j, i0 - 1);
+  for (int i = i0; i < i1; i++) {
+    const T u_p = F(u, k, j, i + 1), v_p = F(v, k, j, i + 1), w_p = F(w, k, j, i + 1);
+    const T vs_p = F(v, k, j - 1, i + 1), wd_p = F(w, k - 1, j, i + 1);
+    const T un_0 = F(u, k, j + 1, i), uk_0 = F(u, kp, j, i);
+    const int64_t o = at(d, k, j, i);
+    if (P.flow) {
+      const T us = F(u, k, j - 1, i), ud = F(u, k - 1, j, i);
+      const T vn = F(v, k, j + 1, i), vd = F(v, k - 1, j, i), vk = F(v, kp, j, i), vsk = F(v, kp, j - 1, i);
+      const T wn = F(w, k, j + 1, i), wnd = F(w, k - 1, j + 1, i), wk = F(w, kp, j, i), ws = F(w, k, j - 1, i);
+      T a = tcx * (u_m * (u_0 + u_m) - u_p * (u_0 + u_p));
+      a = a + tcy * (us * (vs_0 + vs_p) - un_0 * (v_0 + v_p));
+      if (!top) a = a + (t1 * ud * (wd_0 + wd_p) - t2 * uk_0 * (w_0 + w_p));
+      else a = a + t1 * ud * (wd_0 + wd_p);
+      P.s[0][o] = a;
+      T b = tcy * (vs_0 * (v_0 + vs_0) - vn * (v_0 + vn));
+      b = b + tcx * (v_m * (u_m + un_m) - v_p * (u_0 + un_0));
+      if (!top) b = b + (t1 * vd * (wd_0 + wnd) - t2 * vk * (w_0 + wn));
+      else b = b + t1 * vd * (wd_0 + wnd);
+      P.s[1][o] = b;
+      if (!top) {
+        T c = (d1 * wd_0 * (w_0 + wd_0) - d2 * wk * (w_0 + wk));
+        c = c + tcx * (w_m * (u_m + uk_m) - w_p * (u_0 + uk_0));
+        c = c + tcy * (ws * (vs_0 + vsk) - wn * (v_0 + vk));
+        P.s[2][o] = c;
+      }
+    }
+    // scalars: sth is assigned (the reference's own `sth = cx*...`, :407 and
+    // :437), sq(n) accumulates
+    for (int n = 3; n < P.nf; n++) {
+      if (n == 3 ? !P.th : !P.q) continue;
+      const T* f = P.f[n];
+      T s = n == 3 ? T(0) : P.s[n][o];
+      s = s + cxh * (u_m * F(f, k, j, i - 1) - u_0 * F(f, k, j, i + 1));
+      s = s + cyh * (vs_0 * F(f, k, j - 1, i) - v_0 * F(f, k, j + 1, i));
+      if (!top) s = s + T(2.0) * (t1 * wd_0 * F(f, k - 1, j, i) - t2 * w_0 * F(f, kp, j, i));
+      else s = s + t1 * T(2.0) * wd_0 * F(f, k - 1, j, i);
+      P.s[n][o] = s;
+    }
+    u_m = u_0;
+    u_0 = u_p;
+    v_m = v_0;
+    v_0 = v_p;
+    w_m = w_0;
+    w_0 = w_p;
+    vs_0 = vs_p;
+    wd_0 = wd_p;
+    un_m = un_0;
+    uk_m = uk_0;
   }
-  if (P.q)
-    for (int n = 4; n < P.nf; n++)
-      P.s[n][o] = pw_scalar_cell<T>(P.f[n], u, v, w, P.s[n][o], d, k, j, i, cxh, cyh, t1, t2, top);
 }
 
 template <typename T>
@@ -523,7 +529,7 @@ int pw_impl(monc_tvd_ctx* c, int flow, int th, int q, double tcx, double tcy) {
   P.th = th;
   P.q = q;
   const Dims d = dims_of(c);
-  dim3 block(64, 4, 1), grid((c->nz + 63) / 64, (c->ny + 3) / 4, c->nx);
+  dim3 block(64, 4, 1), grid((c->nz + 63) / 64, (c->ny + 3) / 4, (c->nx + PW_STRIP - 1) / PW_STRIP);
   LaunchTimer timer(c, 4);
   hipLaunchKernelGGL((pw_kernel<T>), grid, block, 0, c->stream, P, d);
   HIP_TRY(hipGetLastError());
