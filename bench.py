@@ -46,7 +46,7 @@ def log(msg: str) -> None:
 # (element size es): u, v and w kernels read u, v, w and write their field;
 # the scalar kernel reads u, v, w and its two scalars and writes both.
 KIND_FIELDS = {"u": 4, "v": 4, "w": 4, "scalars": 7, "cell": 10}
-KERNEL_NAMES = {0: "march_kernel", 1: "cell_kernel", 2: "split_kernel"}
+KERNEL_NAMES = {0: "march_kernel", 1: "cell_kernel"}
 
 
 def run_form(state, precision: int, steps: int, warmup: int, dt: float, kernel: int = 0, rows: int = 0,
@@ -304,7 +304,7 @@ def main() -> None:
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--nz", type=int, default=256)
-    ap.add_argument("--kernel", type=int, default=0, help="0 march (default), 1 cell, 2 split")
+    ap.add_argument("--kernel", type=int, default=0, help="0 march (default), 1 cell")
     ap.add_argument("--rows", type=int, default=0, help="march kernel cell rows per block (0: default)")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile-shape autotuner (untimed)")
     ap.add_argument("--no-fp32", action="store_true")

@@ -42,19 +42,25 @@ extern "C" {
 #define MONC_TVD_FP64 0
 #define MONC_TVD_FP32 1
 
-/* advect flags: tvdadvection.F90:349-352 (advection_flow_fields,
- * advection_theta_field, advection_q_fields selectors resolved to tvd) */
+/* advect flags: the selectors of tvdadvection.F90:349-352 resolved to tvd
+ * (advect_flow, advect_th, advect_q, advect_tracer).  The reference sets
+ * advect_tracer from the *theta* option (advection_theta_field, :352): a
+ * caller passes MONC_TVD_ADVECT_TRACER exactly when it passes
+ * MONC_TVD_ADVECT_THETA for the same configuration. */
 #define MONC_TVD_ADVECT_FLOW 1
 #define MONC_TVD_ADVECT_THETA 2
 #define MONC_TVD_ADVECT_Q 4
+#define MONC_TVD_ADVECT_TRACER 8
 
-/* field ids: u, v, w, th, then the n_q scalar fields (q(1..) then tracers,
- * tvdadvection.F90:650-695 advects both with advect_scalar_field) */
+/* field ids: u, v, w, th, then the n_q q fields q(1..n_q) and the n_tracers
+ * tracers tracer(1..n_tracers) (state.F90:86-92; all advected by
+ * advect_scalar_field, tvdadvection.F90:650-695) */
 #define MONC_TVD_U 0
 #define MONC_TVD_V 1
 #define MONC_TVD_W 2
 #define MONC_TVD_TH 3
 #define MONC_TVD_Q(n) (4 + (n))
+#define MONC_TVD_TRACER(n_q, n) (4 + (n_q) + (n))
 
 /* field variants: the prognostic, its z-field (time level t-1, the advected
  * field under centred stepping) and its source (tendency) field,
@@ -73,14 +79,26 @@ typedef struct monc_tvd_ctx monc_tvd_ctx;
  * Replaces tvdadvection.F90:286-481 initialisation_callback (plus the metric
  * vectors of components/gridmanager/src/gridmanager.F90:2048-2067, 2449-2456
  * and cx = 1/dx, cy = 1/dy of :2624-2633).  nz, ny, nx are the local interior
- * sizes; halo must be 2 (global_config halo_depth); nz >= 4, ny, nx >= 4.
- * Copies the FP64 metric vectors to the device and allocates nothing else:
- * fields are allocated by monc_tvd_set_field or supplied by monc_tvd_bind_field.
+ * sizes -- MONC's local_grid%size(Z_INDEX), size(Y_INDEX), size(X_INDEX)
+ * (grids.F90:14: Z_INDEX = 1, Y_INDEX = 2, X_INDEX = 3) -- halo must be 2
+ * (global_config halo_depth); nz >= 4, ny, nx >= 4.  n_q is
+ * number_q_fields and n_tracers is n_tracers of model_state_type
+ * (state.F90:126).  Every field starts active.  Copies the FP64 metric
+ * vectors to the device and allocates nothing else: fields are allocated by
+ * monc_tvd_set_field or supplied by monc_tvd_bind_field.
  */
-int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags, double cx, double cy,
-                    const double *rdz, const double *rdzn, const double *dz, const double *dzn,
+int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int n_tracers, int advect_flags, double cx,
+                    double cy, const double *rdz, const double *rdzn, const double *dz, const double *dzn,
                     const double *tzc1, const double *tzc2, const double *tzd1, const double *tzd2,
                     int precision, monc_tvd_ctx **out);
+
+/* The `active` flag of a prognostic (prognostic_field_type%active,
+ * prognostics.F90:13-20) for th and the q fields: the reference advects th
+ * only when th%active (tvdadvection.F90:706) and q(i) only when q(i)%active
+ * (:661); the flow fields and tracers are not gated (:603-646, :684-694), so
+ * other field ids are rejected.  An inactive field is left untouched by
+ * monc_tvd_tendencies and the stepping entry points. */
+int monc_tvd_set_field_active(monc_tvd_ctx *ctx, int field, int active);
 
 /* Replaces tvdadvection.F90:485-524 finalisation_callback: frees the context
  * and every buffer the library allocated (bound buffers are not freed). */
@@ -102,12 +120,9 @@ int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **
  * default for the column height; otherwise one of the shapes listed by
  * monc_tvd_march_shapes), strip = x columns per block (0: 32).
  * MONC_TVD_KERNEL_CELL: one thread per cell, every face recomputed (any
- * column height; the fallback beyond 256 levels or 4 GiB per field).
- * MONC_TVD_KERNEL_SPLIT: one wave per 64 levels of a row, both y faces of a
- * row computed by it (the earlier default, kept for comparison). */
+ * column height; the fallback beyond 256 levels or 4 GiB per field). */
 #define MONC_TVD_KERNEL_MARCH 0
 #define MONC_TVD_KERNEL_CELL 1
-#define MONC_TVD_KERNEL_SPLIT 2
 int monc_tvd_set_tiling(monc_tvd_ctx *ctx, int kernel, int rows, int strip);
 /* Tile-shape autotuner (no counterpart in the reference).  For each kernel
  * of the forward step (u, v, w, the scalar group) it times every compiled
@@ -137,8 +152,8 @@ int monc_tvd_march_shapes(monc_tvd_ctx *ctx, int *rows, int n);
  *   MONC_TVD_DIAG_ZFACE     every call records each advected field's flux_z
  *       per interior column: flux_z(k), the face between levels k-1 and k
  *       (0 at k = 1), in the field layout.
- * The march and cell kernels record z faces; the split kernel does not
- * (a call that needs it fails).  flags = 0 turns recording off. */
+ * Tracers have neither (the reference records none for them).  flags = 0
+ * turns recording off. */
 #define MONC_TVD_DIAG_TENDENCY 1
 #define MONC_TVD_DIAG_ZFACE 2
 int monc_tvd_set_diagnostics(monc_tvd_ctx *ctx, int flags, const double *rprefrcp);
@@ -363,6 +378,12 @@ int monc_tvd_ensemble_set_tiling(monc_tvd_ensemble *ens, int waves, int strip, i
  * state is not advanced) and keeps the fastest; chosen (2 ints, may be null)
  * receives waves and strip. */
 int monc_tvd_ensemble_autotune(monc_tvd_ensemble *ens, int reps, int *chosen);
+
+/* Test hook (no counterpart in the reference): the library's next n device
+ * allocations fail with hipErrorOutOfMemory without reaching the HIP runtime,
+ * so callers can exercise their MONC_TVD_ERR_HIP paths; n <= 0 clears it.
+ * Returns the count that was pending. */
+int monc_tvd_test_fail_allocations(int n);
 
 /* Message of the last failure on the calling thread ("" if none). */
 const char *monc_tvd_last_error(void);

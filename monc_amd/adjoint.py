@@ -49,8 +49,11 @@ class TvdRollout:
         self.precision = precision
         self.dtype = torch.float64 if precision == tvd.FP64 else torch.float32
         self.k = checkpoint_every
+        n_tr = getattr(state, "n_tracers", 0)
+        if n_tr and advect & tvd.ADVECT_THETA:
+            advect |= tvd.ADVECT_TRACER  # the reference advects tracers under the theta option
         self.t = tvd.TvdAdvection(state.nz, state.ny, state.nx, state.cx, state.cy, state.grid.metric_vectors(),
-                                  n_q=state.n_q, advect=advect, precision=precision)
+                                  n_q=state.n_q, advect=advect, precision=precision, n_tracers=n_tr)
         self.shape = self.t.field_shape()
         dev = torch.device("cuda", torch.cuda.current_device())
         self.prog = [torch.zeros(self.shape, dtype=self.dtype, device=dev) for _ in range(self.nfields)]
@@ -64,7 +67,9 @@ class TvdRollout:
         if advect & tvd.ADVECT_THETA:
             adv.add(3)
         if advect & tvd.ADVECT_Q:
-            adv |= set(range(4, self.nfields))
+            adv |= set(range(4, 4 + state.n_q))
+        if advect & tvd.ADVECT_TRACER:
+            adv |= set(range(4 + state.n_q, self.nfields))
         self._fixed = [f for f in range(self.nfields) if f not in adv]  # fields the step leaves as they are
         self._seg: List[List[torch.Tensor]] = []  # recompute buffers, reused across segments
         self._seg_tag = None  # the forward pass whose last segment the buffers hold

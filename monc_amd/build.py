@@ -78,12 +78,12 @@ def build_library(extra_flags: list[str] | None = None) -> Path:
 
 
 def build_fortran() -> Path | None:
-    """Compile the Fortran interface module and the drop-in driver; None if no amdflang."""
+    """Compile the Fortran interface module, the shim core and the drop-in driver; None if no amdflang."""
     if not shutil.which(FLANG) and not Path(FLANG).exists():
         return None
     out = FORTRAN / "_build"
     out.mkdir(exist_ok=True)
-    srcs = [FORTRAN / "monc_tvd_interface.F90", FORTRAN / "tvd_dropin_driver.F90"]
+    srcs = [FORTRAN / "monc_tvd_interface.F90", FORTRAN / "tvd_shim_core.F90", FORTRAN / "tvd_dropin_driver.F90"]
     objs = []
     for s in srcs:
         o = out / (s.stem + ".o")

@@ -674,19 +674,6 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
   after_face();
 }
 
-// One thread per face owner with global atomics (the reference form of the
-// adjoint scatter; the march form below accumulates in LDS first).
-template <typename T, int KIND>
-__device__ __forceinline__ void adjoint_block(const LaunchArgs<T>& L, const AdjArgs<T>& A, int bx, int by, int bz) {
-  constexpr int DK = Flows<T, KIND>::DK, DJ = Flows<T, KIND>::DJ, DI = Flows<T, KIND>::DI;
-  const int nz = L.d.nz;
-  const AccFlow<T, DK, DJ, DI> aU{{A.lu, L.d}, nz}, aV{{A.lv, L.d}, nz}, aW{{A.lw, L.d}, nz};
-  const Acc<T> aF{A.lf, L.d};
-  const int k = 1 + bx * 64 + (int)threadIdx.x;
-  const int j = 2 + by * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
-  if (k <= nz) adjoint_point<T, KIND>(L, A, k, j, 2 + bz, aF, aU, aV, aW, [] {});
-}
-
 // LDS of the march adjoint: the rings of the four target fields (advected
 // field, u, v, w flows) for R owner rows.
 template <typename T, int R>
@@ -775,11 +762,6 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjAr
   flush(i_end);
   flush(i_end + 1);
   __syncthreads();
-}
-
-template <typename T, int KIND>
-__global__ void __launch_bounds__(256) adjoint_kernel(LaunchArgs<T> L, AdjArgs<T> A) {
-  adjoint_block<T, KIND>(L, A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
 }
 
 // March adjoint: grid (64-level bands, owner-row tiles, x strips).

@@ -306,7 +306,7 @@ int build_tiles(Ens* e, int waves, int strip) {
   }
   if (e->dtiles) HIP_TRY(hipFree(e->dtiles));
   e->dtiles = nullptr;
-  HIP_TRY(hipMalloc(&e->dtiles, t.size() * sizeof(int4)));
+  HIP_TRY(dev_alloc(&e->dtiles, t.size() * sizeof(int4)));
   HIP_TRY(hipMemcpy(e->dtiles, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
   e->ntiles = (int)t.size();
   e->waves = waves;
@@ -502,21 +502,21 @@ int monc_tvd_ensemble_create(int n_members, const int* nz, const int* ny, const 
   if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking);
   const size_t bytes = (size_t)off * es;
   for (void** p : {&e->pool[0], &e->pool[1], &e->adj, &e->adj_in})
-    if (he == hipSuccess) he = hipMalloc(p, bytes);
+    if (he == hipSuccess) he = dev_alloc(p, bytes);
   for (void* p : {e->pool[0], e->pool[1], e->adj, e->adj_in})
     if (he == hipSuccess) he = hipMemset(p, 0, bytes);
-  if (he == hipSuccess) he = hipMalloc(&e->dmem, sizeof(EnsMember) * n_members);
+  if (he == hipSuccess) he = dev_alloc(&e->dmem, sizeof(EnsMember) * n_members);
   if (he == hipSuccess) he = hipMemcpy(e->dmem, mem.data(), sizeof(EnsMember) * n_members, hipMemcpyHostToDevice);
-  if (he == hipSuccess) he = hipMalloc(&e->dmet, sizeof(double) * met);
+  if (he == hipSuccess) he = dev_alloc(&e->dmet, sizeof(double) * met);
   if (he == hipSuccess) he = hipMemcpy(e->dmet, metrics, sizeof(double) * met, hipMemcpyHostToDevice);
-  if (he == hipSuccess) he = hipMalloc(&e->dctr, 2 * sizeof(unsigned));
+  if (he == hipSuccess) he = dev_alloc(&e->dctr, 2 * sizeof(unsigned));
   if (he == hipSuccess) he = hipMemset(e->dctr, 0, 2 * sizeof(unsigned));
   for (int op = 0; op < OP_COUNT && he == hipSuccess; op++) {
     std::vector<long long> pre(n_members + 1, 0);
     for (int m = 0; m < n_members; m++)
       pre[m + 1] = pre[m] + ENS_FIELDS * op_blocks_per_field(op, nz[m], ny[m], nx[m], nullptr);
     e->nblocks[op] = pre[n_members];
-    he = hipMalloc(&e->dprefix[op], sizeof(long long) * (n_members + 1));
+    he = dev_alloc(&e->dprefix[op], sizeof(long long) * (n_members + 1));
     if (he == hipSuccess)
       he = hipMemcpy(e->dprefix[op], pre.data(), sizeof(long long) * (n_members + 1), hipMemcpyHostToDevice);
   }
@@ -568,7 +568,7 @@ int monc_tvd_ensemble_set_field(monc_tvd_ensemble* e, int member, int field, int
     return MONC_TVD_OK;
   }
   double* tmp = nullptr;
-  HIP_TRY(hipMalloc(&tmp, n * sizeof(double)));
+  HIP_TRY(dev_alloc(&tmp, n * sizeof(double)));
   hipError_t he = hipMemcpy(tmp, host, n * sizeof(double), hipMemcpyHostToDevice);
   if (he == hipSuccess) {
     hipLaunchKernelGGL(to_prec_kernel<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, (float*)dst,
@@ -594,7 +594,7 @@ int monc_tvd_ensemble_get_field(monc_tvd_ensemble* e, int member, int field, int
     return MONC_TVD_OK;
   }
   double* tmp = nullptr;
-  HIP_TRY(hipMalloc(&tmp, n * sizeof(double)));
+  HIP_TRY(dev_alloc(&tmp, n * sizeof(double)));
   hipLaunchKernelGGL(to_double_kernel<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, tmp,
                      (const float*)src, n);
   hipError_t he = hipGetLastError();

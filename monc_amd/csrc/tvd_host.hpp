@@ -13,6 +13,21 @@ namespace tvd {
 __attribute__((visibility("hidden"))) int fail(int code, const char* fmt, ...)
     __attribute__((format(printf, 2, 3)));
 
+// Every device allocation of the library goes through dev_alloc, so that the
+// test hook monc_tvd_test_fail_allocations can make the next n of them report
+// hipErrorOutOfMemory without reaching the runtime (error paths are tested
+// without exhausting a GPU).
+__attribute__((visibility("hidden"))) extern int g_fail_allocs;
+template <typename P>
+inline hipError_t dev_alloc(P** p, size_t bytes) {
+  if (__atomic_load_n(&g_fail_allocs, __ATOMIC_RELAXED) > 0 &&
+      __atomic_sub_fetch(&g_fail_allocs, 1, __ATOMIC_RELAXED) >= 0) {
+    *p = nullptr;
+    return hipErrorOutOfMemory;
+  }
+  return hipMalloc((void**)p, bytes);
+}
+
 #define HIP_TRY(call)                                                                              \
   do {                                                                                             \
     hipError_t e_ = (call);                                                                        \

@@ -227,88 +227,6 @@ __global__ void __launch_bounds__(256) cell_kernel(LaunchArgs<T> L, FieldSet<T> 
 }
 
 // ---------------------------------------------------------------------------
-// Split wave kernel: the NCH sub-levels (64 levels each) of a row go to NCH
-// waves of one block, R rows per block, so a wave keeps one sub-level's
-// registers (occupancy of the single-sub-level wave kernel for columns of up
-// to 256 levels).  The z face crossing a sub-level boundary (the top face of
-// the last level of sub-level q, the bottom face of the first of q+1) passes
-// through LDS: one barrier per column, double-buffered by column parity so a
-// slot is rewritten only after every wave has passed the next barrier.
-// Rows past ny+2 (the last block when R does not divide ny) skip all loads
-// but keep taking the barriers.
-template <typename T, int KIND, int NS, int MODE, int NCH, int R>
-__global__ void __launch_bounds__(64 * NCH * R) split_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip) {
-  __shared__ T ztop[2][R][NCH][NS];
-  const Flows<T, KIND> fl(L);
-  const VMet m = KIND == KIND_W ? L.mw : L.mc;
-  const int nz = L.d.nz, ny = L.d.ny;
-  const int kdof = KIND == KIND_W ? 1 : 0;
-  const int kmin = KIND == KIND_W ? 1 : 2;
-  const int kmax = KIND == KIND_W ? nz - 1 : nz;
-  const T dt = S.dt[0];
-  Adv<T, NS> F{};
-#pragma unroll
-  for (int n = 0; n < NS; n++) F.a[n] = Arr<T>(S.F[n], L.d);
-  const int lane = threadIdx.x;
-  const int wv = (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int q = wv % NCH, r = wv / NCH;
-  const int j = 3 + (int)blockIdx.x * R + r;
-  const bool active = j <= ny + 2;  // wave-uniform
-  const int k = 1 + 64 * q + lane;
-  const bool cell = active && k >= 2 && k <= kmax;
-  const bool top = KIND != KIND_W && k == nz;
-  const int i_begin = 3 + (int)blockIdx.y * strip;
-  const int i_end = min(i_begin + strip, L.d.nx + 3);
-  T fxl[NS];
-#pragma unroll
-  for (int n = 0; n < NS; n++) fxl[n] = T(0);
-  if (cell) face_x(k, j, i_begin - 1, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fxl);
-  int par = 0;
-  for (int i = i_begin; i < i_end; i++) {
-    T fxr[NS], fyn[NS], fys[NS], fzt[NS], fzb[NS];
-#pragma unroll
-    for (int n = 0; n < NS; n++) fxr[n] = fyn[n] = fys[n] = fzt[n] = T(0);
-    if (cell) {
-      face_y(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fyn);
-      face_y(k, j - 1, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fys);
-      face_x(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, dt, L.cx, L.cy, nz, fxr);
-    }
-    if (active && k <= nz - 1) face_z(k, j, i, F, fl.U, fl.V, fl.W, m, kdof, kmin, dt, L.cx, L.cy, nz, fzt);
-#pragma unroll
-    for (int n = 0; n < NS; n++) fzb[n] = __shfl_up(fzt[n], 1);
-    if constexpr (NCH > 1) {
-      if (lane == 63) {
-#pragma unroll
-        for (int n = 0; n < NS; n++) ztop[par][r][q][n] = fzt[n];
-      }
-      __syncthreads();
-      if (lane == 0 && q > 0) {
-#pragma unroll
-        for (int n = 0; n < NS; n++) fzb[n] = ztop[par][r][q - 1][n];
-      }
-      par ^= 1;
-    }
-    if (cell) {
-      const DivW<T> dw(fl, m, k, j, i, nz);
-      const int64_t p = at(L.d, k, j, i);
-#pragma unroll
-      for (int n = 0; n < NS; n++) {
-        if (MODE == MODE_ACCUM)
-          S.out[n][p] = dw.apply(top, S.out[n][p], L.cx, L.cy, fxl[n], fxr[n], fys[n], fyn[n], fzb[n], fzt[n]);
-        else
-          store_step(L.d, S.out[n], S.f[n], L.dtm,
-                     dw.apply(top, T(0), L.cx, L.cy, fxl[n], fxr[n], fys[n], fyn[n], fzb[n], fzt[n]), k, j, i);
-      }
-    } else if (MODE == MODE_STEP && active && k <= nz) {
-#pragma unroll
-      for (int n = 0; n < NS; n++) store_step(L.d, S.out[n], S.f[n], L.dtm, T(0), k, j, i);
-    }
-#pragma unroll
-    for (int n = 0; n < NS; n++) fxl[n] = fxr[n];
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Periodic fill of the depth-2 x/y halo of one field, corners included.  One
 // thread per (k, halo column); the halo columns are the 4 rows j in {1,2,ny+3,
 // ny+4} over every i, then the 4 columns i in {1,2,nx+3,nx+4} over interior j.
@@ -340,25 +258,48 @@ __global__ void __launch_bounds__(256) halo_kernel(T* f, Dims d) {
   halo_block<T>(f, d, (int)blockIdx.x, (int64_t)blockIdx.y);
 }
 
-// FP64 weighted sum over the interior, sum_k w(k) sum_{j,i} f(k,j,i): one
-// partial per block (lanes along z, 4 rows, one column), reduced by the host
-// in a fixed order so the result does not depend on scheduling.
-template <typename T>
-__global__ void __launch_bounds__(256) wsum_kernel(const T* f, Dims d, const double* w, double* partial) {
-  __shared__ double red[256];
-  const int j = 3 + blockIdx.y * 4 + threadIdx.y;
-  const int i = 3 + blockIdx.z;
-  double acc = 0.0;
-  if (j <= d.ny + 2)
-    for (int k = 1 + (int)threadIdx.x; k <= d.nz; k += 64) acc += w[k - 1] * (double)f[at(d, k, j, i)];
-  const int t = threadIdx.x + 64 * threadIdx.y;
+// FP64 weighted sum over the interior, sum_k w(k) sum_{j,i} f(k,j,i), in two
+// stages whose order of additions is fixed by the grid alone (the result does
+// not depend on scheduling): WSUM_BLOCKS blocks of 64 levels x 4 rows each take
+// the column groups (4 rows x 1 column) b, b + WSUM_BLOCKS, ... and write one
+// partial; one block of 256 threads then adds the partials.  Scratch: nz
+// weights, then WSUM_BLOCKS partials, then the result.
+constexpr int WSUM_BLOCKS = 1024;
+
+__device__ __forceinline__ double block_sum_256(double acc, double* red) {
+  const int t = (int)(threadIdx.x + blockDim.x * threadIdx.y);
   red[t] = acc;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (t < s) red[t] += red[t + s];
     __syncthreads();
   }
-  if (t == 0) partial[blockIdx.y + gridDim.y * (size_t)blockIdx.z] = red[0];
+  return red[0];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) wsum_partial_kernel(const T* f, Dims d, const double* w, double* partial) {
+  __shared__ double red[256];
+  const int groups_y = (d.ny + 3) / 4;
+  const int64_t ngroups = (int64_t)groups_y * d.nx;
+  double acc = 0.0;
+  for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const int i = 3 + (int)(g / groups_y);
+    const int j = 3 + (int)(g % groups_y) * 4 + (int)threadIdx.y;
+    if (j <= d.ny + 2)
+      for (int k = 1 + (int)threadIdx.x; k <= d.nz; k += 64) acc += w[k - 1] * (double)f[at(d, k, j, i)];
+  }
+  const double s = block_sum_256(acc, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = s;
+}
+
+template <typename P>
+__global__ void __launch_bounds__(256) wsum_final_kernel(const P* partial, int n, P* out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int q = (int)threadIdx.x; q < n; q += 256) acc += partial[q];
+  const double s = block_sum_256(acc, red);
+  if (threadIdx.x == 0) out[0] = s;
 }
 
 }  // namespace tvd

@@ -4,6 +4,7 @@
 // failures through a status code and a thread-local message.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -26,6 +27,8 @@ thread_local std::string g_err;
 }  // namespace
 
 namespace tvd {
+int g_fail_allocs = 0;
+
 int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -59,7 +62,8 @@ __global__ void __launch_bounds__(256) unpack_dual_kernel(double* v, double* dv,
 }  // namespace
 
 struct monc_tvd_ctx {
-  int nz, ny, nx, n_q, advect, precision;
+  int nz, ny, nx, n_q, n_tracers, advect, precision;
+  std::vector<char> active;    // per field: the reference's %active (th and q fields are gated by it)
   double cx, cy;
   size_t n_elem;     // elements per field, nz*(ny+4)*(nx+4)
   size_t elem_size;  // 8 or 4
@@ -74,7 +78,7 @@ struct monc_tvd_ctx {
   std::vector<void*> adj_tmp;  // copy of the incoming adjoint of each field (adjoint step)
   std::vector<void*> dual[2];  // packed (value, tangent) state, double-buffered (tangent steps)
   hipStream_t stream;
-  int kernel;                  // MONC_TVD_KERNEL_MARCH, _CELL or _SPLIT
+  int kernel;                  // MONC_TVD_KERNEL_MARCH or _CELL
   int rows[4], strip[4];       // march tile shape per kind (u, v, w, scalars): cell rows and x columns
                                // per block (0: defaults)
   int buffer_ok;               // fields small enough for 32-bit buffer offsets
@@ -97,6 +101,7 @@ struct monc_tvd_ctx {
   int tabs_have;
   double* diag_partial;        // ny*nz FP64 scratch of the level totals
   double* thadv_tab;           // device, 2*nz: dthref, thref of th_advection (allocated on first use)
+  double* wsum_scratch;        // device, weights + partials + result of monc_tvd_weighted_sum (first use)
 };
 
 namespace {
@@ -134,7 +139,7 @@ void*& slot(monc_tvd_ctx* c, int field, int variant) { return c->buf[(size_t)fie
 int ensure(monc_tvd_ctx* c, int field, int variant) {
   void*& p = slot(c, field, variant);
   if (p) return MONC_TVD_OK;
-  HIP_TRY(hipMalloc(&p, c->n_elem * c->elem_size));
+  HIP_TRY(dev_alloc(&p, c->n_elem * c->elem_size));
   HIP_TRY(hipMemsetAsync(p, 0, c->n_elem * c->elem_size, c->stream));
   c->owned[(size_t)field * NVAR + variant] = 1;
   return MONC_TVD_OK;
@@ -173,16 +178,6 @@ LaunchArgs<T> launch_args(monc_tvd_ctx* c, double dtm) {
 
 int kind_of(int field) {
   return field == MONC_TVD_U ? KIND_U : field == MONC_TVD_V ? KIND_V : field == MONC_TVD_W ? KIND_W : KIND_SCALAR;
-}
-
-// Split wave kernel (kernel 2): NCH waves per row, R rows per block (256
-// threads, or 192 for three sub-levels).
-template <typename T, int KIND, int NS, int MODE, int NCH>
-void launch_split(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int strip) {
-  constexpr int R = NCH == 1 ? 4 : NCH == 2 ? 2 : 1;
-  dim3 block(64, NCH * R, 1);
-  dim3 grid((c->ny + R - 1) / R, (c->nx + strip - 1) / strip, 1);
-  hipLaunchKernelGGL((split_kernel<T, KIND, NS, MODE, NCH, R>), grid, block, 0, c->stream, L, S, strip);
 }
 
 // March kernel (kernel 0): NCH waves per row, R cell rows per block, on a 1-D
@@ -243,13 +238,6 @@ void launch_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
       case 302: launch_march<T, KIND, NS, MODE, 3, 2>(c, L, S, strip); break;
       default: launch_march<T, KIND, NS, MODE, 4, 3>(c, L, S, strip); break;
     }
-  } else if (kernel == MONC_TVD_KERNEL_SPLIT) {
-    switch (nch) {
-      case 1: launch_split<T, KIND, NS, MODE, 1>(c, L, S, strip); break;
-      case 2: launch_split<T, KIND, NS, MODE, 2>(c, L, S, strip); break;
-      case 3: launch_split<T, KIND, NS, MODE, 3>(c, L, S, strip); break;
-      default: launch_split<T, KIND, NS, MODE, 4>(c, L, S, strip); break;
-    }
   } else {
     switch (nch * 100 + rows) {
       case 112: launch_march<T, KIND, NS, MODE, 1, 12>(c, L, S, strip); break;
@@ -302,8 +290,6 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
       G.out2[m] = S.out2[n + m];
       G.gal[m] = S.gal[n + m];
     }
-    if (c->kernel == MONC_TVD_KERNEL_SPLIT && G.zf[0])
-      return fail(MONC_TVD_ERR_ARG, "z-face diagnostics need the march or cell kernel, not the split kernel");
     const int kk = pair || S.kind[n] == KIND_SCALAR ? 3 : S.kind[n];
     // the leapfrog step runs the march kernel in its default shape
     const bool leap = MODE == MODE_LEAP;
@@ -340,8 +326,10 @@ int launch_halo(monc_tvd_ctx* c, T* f) {
   return MONC_TVD_OK;
 }
 
-// Fields advected this call, in the reference's order u, v, w, th, q(:)
-// (tvdadvection.F90:591-594).
+// Fields advected this call, in the reference's order u, v, w, th, q(:),
+// tracers (tvdadvection.F90:591-594): th only when active (:706), each q field
+// only when active (:661), the flow fields and tracers always (:603-646,
+// :684-694).
 std::vector<int> advected_fields(const monc_tvd_ctx* c) {
   std::vector<int> f;
   if (c->advect & MONC_TVD_ADVECT_FLOW) {
@@ -349,11 +337,18 @@ std::vector<int> advected_fields(const monc_tvd_ctx* c) {
     f.push_back(MONC_TVD_V);
     f.push_back(MONC_TVD_W);
   }
-  if (c->advect & MONC_TVD_ADVECT_THETA) f.push_back(MONC_TVD_TH);
+  if ((c->advect & MONC_TVD_ADVECT_THETA) && c->active[MONC_TVD_TH]) f.push_back(MONC_TVD_TH);
   if (c->advect & MONC_TVD_ADVECT_Q)
-    for (int n = 0; n < c->n_q; n++) f.push_back(MONC_TVD_Q(n));
+    for (int n = 0; n < c->n_q; n++)
+      if (c->active[MONC_TVD_Q(n)]) f.push_back(MONC_TVD_Q(n));
+  if (c->advect & MONC_TVD_ADVECT_TRACER)
+    for (int n = 0; n < c->n_tracers; n++) f.push_back(MONC_TVD_TRACER(c->n_q, n));
   return f;
 }
+
+// Tracers have no tendency diagnostics and no tvd_dgs_terms face copies in
+// the reference (tvdadvection.F90:679-695, 1064-1188).
+bool is_tracer(const monc_tvd_ctx* c, int field) { return field >= 4 + c->n_q; }
 
 // th_advection (thadvection.F90:124-211): at every column the registry hands
 // it -- halo columns included, as the component does not skip them -- and
@@ -395,7 +390,7 @@ template <typename T>
 int thadv_impl(monc_tvd_ctx* c, int anelastic, const double* dthref, int baro, const double* thref, double thref0,
                double fcoriol, double g, double rate_x, double rate_y, double ugal, double vgal) {
   const int nz = c->nz;
-  if (!c->thadv_tab) HIP_TRY(hipMalloc(&c->thadv_tab, 2 * (size_t)nz * sizeof(double)));
+  if (!c->thadv_tab) HIP_TRY(dev_alloc(&c->thadv_tab, 2 * (size_t)nz * sizeof(double)));
   if (anelastic)
     HIP_TRY(hipMemcpyAsync(c->thadv_tab, dthref, nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
   if (baro && anelastic)
@@ -512,8 +507,8 @@ __global__ void __launch_bounds__(256) pw_kernel(PwArgs<T> P, Dims d) {
 template <typename T>
 int pw_impl(monc_tvd_ctx* c, int flow, int th, int q, double tcx, double tcy) {
   PwArgs<T> P{};
-  P.nf = c->nfields;
-  for (int n = 0; n < c->nfields; n++) {
+  P.nf = 4 + c->n_q;  // pw_advection's q loop covers number_q_fields (pwadvection.F90:333), no tracers
+  for (int n = 0; n < P.nf; n++) {
     P.f[n] = (const T*)slot(c, n, MONC_TVD_PROG);
     P.s[n] = (T*)slot(c, n, MONC_TVD_SOURCE);
   }
@@ -526,7 +521,7 @@ int pw_impl(monc_tvd_ctx* c, int flow, int th, int q, double tcx, double tcy) {
   P.tcx = tcx;
   P.tcy = tcy;
   P.flow = flow;
-  P.th = th;
+  P.th = th && c->active[MONC_TVD_TH];  // advect_th_field: only when th%active (pwadvection.F90:403)
   P.q = q;
   const Dims d = dims_of(c);
   dim3 block(64, 4, 1), grid((c->nz + 63) / 64, (c->ny + 3) / 4, (c->nx + PW_STRIP - 1) / PW_STRIP);
@@ -548,13 +543,14 @@ int diag_before_call(monc_tvd_ctx* c, const std::vector<int>& fields) {
   if (!c->diag_flags) return MONC_TVD_OK;
   const size_t bytes = c->n_elem * c->elem_size;
   for (int fl : fields) {
+    if (is_tracer(c, fl)) continue;
     if (c->diag_flags & MONC_TVD_DIAG_TENDENCY) {
-      if (!c->diag_before[fl]) HIP_TRY(hipMalloc(&c->diag_before[fl], bytes));
+      if (!c->diag_before[fl]) HIP_TRY(dev_alloc(&c->diag_before[fl], bytes));
       HIP_TRY(hipMemcpyAsync(c->diag_before[fl], slot(c, fl, MONC_TVD_SOURCE), bytes, hipMemcpyDeviceToDevice,
                              c->stream));
     }
     if ((c->diag_flags & MONC_TVD_DIAG_ZFACE) && !c->diag_zf[fl]) {
-      HIP_TRY(hipMalloc(&c->diag_zf[fl], bytes));
+      HIP_TRY(dev_alloc(&c->diag_zf[fl], bytes));
       HIP_TRY(hipMemsetAsync(c->diag_zf[fl], 0, bytes, c->stream));
     }
   }
@@ -581,18 +577,19 @@ template <typename T>
 int diag_after_call(monc_tvd_ctx* c, const std::vector<int>& fields) {
   if (!(c->diag_flags & MONC_TVD_DIAG_TENDENCY)) return MONC_TVD_OK;
   const size_t n3d = (size_t)c->nz * c->ny * c->nx;
-  if (!c->diag_partial) HIP_TRY(hipMalloc(&c->diag_partial, sizeof(double) * (size_t)c->nz * c->ny));
+  if (!c->diag_partial) HIP_TRY(dev_alloc(&c->diag_partial, sizeof(double) * (size_t)c->nz * c->ny));
   int rc;
   for (int fl : fields) {
-    if (!c->diag_3d[fl]) HIP_TRY(hipMalloc(&c->diag_3d[fl], n3d * sizeof(T)));
-    if (!c->diag_prof[fl]) HIP_TRY(hipMalloc(&c->diag_prof[fl], sizeof(double) * c->nz));
+    if (is_tracer(c, fl)) continue;
+    if (!c->diag_3d[fl]) HIP_TRY(dev_alloc(&c->diag_3d[fl], n3d * sizeof(T)));
+    if (!c->diag_prof[fl]) HIP_TRY(dev_alloc(&c->diag_prof[fl], sizeof(double) * c->nz));
     if ((rc = tend_and_profile<T>(c, (const T*)slot(c, fl, MONC_TVD_SOURCE), (const T*)c->diag_before[fl], nullptr,
                                   (T*)c->diag_3d[fl], c->diag_prof[fl])))
       return rc;
     c->diag_have[fl] = 1;
     if (fl == MONC_TVD_TH && c->rprefrcp) {
-      if (!c->tabs_3d) HIP_TRY(hipMalloc(&c->tabs_3d, n3d * sizeof(T)));
-      if (!c->tabs_prof) HIP_TRY(hipMalloc(&c->tabs_prof, sizeof(double) * c->nz));
+      if (!c->tabs_3d) HIP_TRY(dev_alloc(&c->tabs_3d, n3d * sizeof(T)));
+      if (!c->tabs_prof) HIP_TRY(dev_alloc(&c->tabs_prof, sizeof(double) * c->nz));
       if ((rc = tend_and_profile<T>(c, (const T*)slot(c, fl, MONC_TVD_SOURCE), (const T*)c->diag_before[fl],
                                     c->rprefrcp, (T*)c->tabs_3d, c->tabs_prof)))
         return rc;
@@ -626,7 +623,7 @@ int tendencies_impl(monc_tvd_ctx* c, double dtm, int field_stepping, int momentu
       S.f[n] = (const T*)slot(c, fl, MONC_TVD_PROG);
       S.out[n] = (T*)slot(c, fl, MONC_TVD_SOURCE);
       S.dt[n] = (T)(fl <= MONC_TVD_W ? dt_mom : dt_sca);
-      S.zf[n] = (c->diag_flags & MONC_TVD_DIAG_ZFACE) ? (T*)c->diag_zf[fl] : nullptr;
+      S.zf[n] = (c->diag_flags & MONC_TVD_DIAG_ZFACE) && !is_tracer(c, fl) ? (T*)c->diag_zf[fl] : nullptr;
     }
     if ((rc = launch_set<T, MODE_ACCUM>(c, L, S))) return rc;
   }
@@ -641,7 +638,7 @@ int forward_steps_impl(monc_tvd_ctx* c, double dtm, int nsteps) {
   std::vector<int> fields = advected_fields(c);
   for (int fl : fields) {
     if ((rc = need(c, fl, MONC_TVD_PROG))) return rc;
-    if (!c->spare[fl]) HIP_TRY(hipMalloc(&c->spare[fl], c->n_elem * c->elem_size));
+    if (!c->spare[fl]) HIP_TRY(dev_alloc(&c->spare[fl], c->n_elem * c->elem_size));
   }
   // the flows are read by every field's update, so no field may be updated in
   // place: each step writes the spare buffers, then the roles swap.
@@ -681,6 +678,14 @@ int forward_into_impl(monc_tvd_ctx* c, double dtm, void* const* in, void* const*
   std::vector<int> fields = advected_fields(c);
   for (int f = 0; f < c->nfields; f++)
     if (!in[f] || !out[f]) return fail(MONC_TVD_ERR_ARG, "null buffer for field %d", f);
+  // every written buffer must be distinct from every buffer the step reads
+  // (the flows u, v, w are read by every field's update) and from the others
+  for (int fl : fields) {
+    for (int g = 0; g < c->nfields; g++)
+      if (out[fl] == in[g]) return fail(MONC_TVD_ERR_ARG, "output of field %d aliases the input of field %d", fl, g);
+    for (int g : fields)
+      if (g != fl && out[fl] == out[g]) return fail(MONC_TVD_ERR_ARG, "fields %d and %d share an output", fl, g);
+  }
   LaunchArgs<T> L = launch_args<T>(c, dtm);
   L.u = (const T*)in[MONC_TVD_U];
   L.v = (const T*)in[MONC_TVD_V];
@@ -712,15 +717,18 @@ int forward_into_impl(monc_tvd_ctx* c, double dtm, void* const* in, void* const*
 template <typename T>
 int leapfrog_impl(monc_tvd_ctx* c, double dtm, double tsmth, double ugal, double vgal, int nsteps, int first_forward) {
   int rc;
-  if (c->advect != (MONC_TVD_ADVECT_FLOW | MONC_TVD_ADVECT_THETA | MONC_TVD_ADVECT_Q))
-    return fail(MONC_TVD_ERR_ARG, "leapfrog stepping steps every field: create the context with all advect flags");
+  const int all = MONC_TVD_ADVECT_FLOW | MONC_TVD_ADVECT_THETA | MONC_TVD_ADVECT_Q |
+                  (c->n_tracers ? MONC_TVD_ADVECT_TRACER : 0);
+  if ((c->advect & all) != all || (int)advected_fields(c).size() != c->nfields)
+    return fail(MONC_TVD_ERR_ARG,
+                "leapfrog stepping steps every field: create the context with all advect flags and every field active");
   std::vector<int> fields = advected_fields(c);
   for (int fl : fields) {
     if ((rc = need(c, fl, MONC_TVD_PROG))) return rc;
     if (!first_forward && (rc = need(c, fl, MONC_TVD_ZFIELD))) return rc;
     if ((rc = ensure(c, fl, MONC_TVD_ZFIELD))) return rc;
-    if (!c->spare[fl]) HIP_TRY(hipMalloc(&c->spare[fl], c->n_elem * c->elem_size));
-    if (!c->spare_z[fl]) HIP_TRY(hipMalloc(&c->spare_z[fl], c->n_elem * c->elem_size));
+    if (!c->spare[fl]) HIP_TRY(dev_alloc(&c->spare[fl], c->n_elem * c->elem_size));
+    if (!c->spare_z[fl]) HIP_TRY(dev_alloc(&c->spare_z[fl], c->n_elem * c->elem_size));
   }
   for (int s = 0; s < nsteps; s++) {
     const bool forward = first_forward && s == 0;
@@ -778,7 +786,7 @@ int autotune_impl(monc_tvd_ctx* c, double dtm, int reps, int* chosen) {
   std::vector<int> fields = advected_fields(c);
   for (int fl : fields) {
     if ((rc = need(c, fl, MONC_TVD_PROG))) return rc;
-    if (!c->spare[fl]) HIP_TRY(hipMalloc(&c->spare[fl], c->n_elem * c->elem_size));
+    if (!c->spare[fl]) HIP_TRY(dev_alloc(&c->spare[fl], c->n_elem * c->elem_size));
   }
   const LaunchArgs<T> L = launch_args<T>(c, dtm);
   hipEvent_t a, b;
@@ -886,7 +894,7 @@ int adjoint_step_impl(monc_tvd_ctx* c, double dtm) {
   }
   for (int fl : fields) {
     if ((rc = need(c, fl, MONC_TVD_PROG)) || (rc = need(c, fl, MONC_TVD_ADJOINT))) return rc;
-    if (!c->adj_tmp[fl]) HIP_TRY(hipMalloc(&c->adj_tmp[fl], c->n_elem * c->elem_size));
+    if (!c->adj_tmp[fl]) HIP_TRY(dev_alloc(&c->adj_tmp[fl], c->n_elem * c->elem_size));
   }
   // the incoming adjoint is read while the accumulators (which start from it:
   // the identity part of X' = X + dtm T(X)) receive the transposed Jacobian
@@ -960,7 +968,7 @@ int tangent_steps_impl(monc_tvd_ctx* c, double dtm, int nsteps) {
   for (int fl : state) {
     if ((rc = need(c, fl, MONC_TVD_PROG)) || (rc = need(c, fl, MONC_TVD_TANGENT))) return rc;
     for (int b = 0; b < 2; b++)
-      if (!c->dual[b][fl]) HIP_TRY(hipMalloc(&c->dual[b][fl], c->n_elem * sizeof(Dual)));
+      if (!c->dual[b][fl]) HIP_TRY(dev_alloc(&c->dual[b][fl], c->n_elem * sizeof(Dual)));
     hipLaunchKernelGGL(pack_dual_kernel, dim3(nb), dim3(256), 0, c->stream, (Dual*)c->dual[0][fl],
                        (const double*)slot(c, fl, MONC_TVD_PROG), (const double*)slot(c, fl, MONC_TVD_TANGENT), n);
     HIP_TRY(hipGetLastError());
@@ -1008,15 +1016,46 @@ int tangent_steps_impl(monc_tvd_ctx* c, double dtm, int nsteps) {
   return MONC_TVD_OK;
 }
 
+// Weighted sum of one field through wsum_partial_kernel / wsum_final_kernel
+// on `scratch` (wsum_scratch_len(nz) doubles: weights, partials, result);
+// synchronises the stream.
+size_t wsum_scratch_len(int nz) { return (size_t)nz + WSUM_BLOCKS + 1; }
+
+int weighted_sum_on(monc_tvd_ctx* c, const void* field, const double* weights, double* scratch, double* out) {
+  double* dw = scratch;
+  double* dpart = scratch + c->nz;
+  double* dres = dpart + WSUM_BLOCKS;
+  const int64_t ngroups = (int64_t)((c->ny + 3) / 4) * c->nx;
+  const int nb = (int)std::min<int64_t>(ngroups, WSUM_BLOCKS);
+  HIP_TRY(hipMemcpyAsync(dw, weights, sizeof(double) * c->nz, hipMemcpyHostToDevice, c->stream));
+  if (c->precision == MONC_TVD_FP64)
+    hipLaunchKernelGGL((wsum_partial_kernel<double>), dim3(nb), dim3(64, 4, 1), 0, c->stream, (const double*)field,
+                       dims_of(c), (const double*)dw, dpart);
+  else
+    hipLaunchKernelGGL((wsum_partial_kernel<float>), dim3(nb), dim3(64, 4, 1), 0, c->stream, (const float*)field,
+                       dims_of(c), (const double*)dw, dpart);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL((wsum_final_kernel<double>), dim3(1), dim3(256), 0, c->stream, (const double*)dpart, nb, dres);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, dres, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MONC_TVD_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* monc_tvd_last_error(void) { return g_err.c_str(); }
 
-int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags, double cx, double cy,
-                    const double* rdz, const double* rdzn, const double* dz, const double* dzn, const double* tzc1,
-                    const double* tzc2, const double* tzd1, const double* tzd2, int precision, monc_tvd_ctx** out) {
+int monc_tvd_test_fail_allocations(int n) {
+  return __atomic_exchange_n(&g_fail_allocs, n > 0 ? n : 0, __ATOMIC_RELAXED);
+}
+
+int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int n_tracers, int advect_flags, double cx,
+                    double cy, const double* rdz, const double* rdzn, const double* dz, const double* dzn,
+                    const double* tzc1, const double* tzc2, const double* tzd1, const double* tzd2, int precision,
+                    monc_tvd_ctx** out) {
   g_err.clear();
   if (!out) return fail(MONC_TVD_ERR_ARG, "null output pointer");
   *out = nullptr;
@@ -1024,10 +1063,12 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags,
   if (nz < 4 || ny < 4 || nx < 4)
     return fail(MONC_TVD_ERR_ARG, "local grid %dx%dx%d too small (need nz, ny, nx >= 4)", nz, ny, nx);
   if (n_q < 0 || n_q > 64) return fail(MONC_TVD_ERR_ARG, "n_q %d out of range [0,64]", n_q);
+  if (n_tracers < 0 || n_tracers > 64) return fail(MONC_TVD_ERR_ARG, "n_tracers %d out of range [0,64]", n_tracers);
   if (precision != MONC_TVD_FP64 && precision != MONC_TVD_FP32)
     return fail(MONC_TVD_ERR_ARG, "unknown precision %d", precision);
-  if (advect_flags & ~7) return fail(MONC_TVD_ERR_ARG, "unknown advect flags 0x%x", advect_flags);
-  if (!(cx > 0.0) || !(cy > 0.0)) return fail(MONC_TVD_ERR_ARG, "cx, cy must be positive");
+  if (advect_flags & ~15) return fail(MONC_TVD_ERR_ARG, "unknown advect flags 0x%x", advect_flags);
+  if (!(cx > 0.0) || !(cy > 0.0) || !std::isfinite(cx) || !std::isfinite(cy))
+    return fail(MONC_TVD_ERR_ARG, "cx, cy must be positive and finite");
   const double* mets[MET_N] = {rdz, rdzn, dz, dzn, tzc1, tzc2, tzd1, tzd2};
   for (int m = 0; m < MET_N; m++)
     if (!mets[m]) return fail(MONC_TVD_ERR_ARG, "metric vector %d is null", m);
@@ -1039,13 +1080,15 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags,
   c->ny = ny;
   c->nx = nx;
   c->n_q = n_q;
+  c->n_tracers = n_tracers;
   c->advect = advect_flags;
   c->precision = precision;
   c->cx = cx;
   c->cy = cy;
   c->n_elem = n_elem;
   c->elem_size = precision == MONC_TVD_FP64 ? 8 : 4;
-  c->nfields = 4 + n_q;
+  c->nfields = 4 + n_q + n_tracers;
+  c->active.assign((size_t)c->nfields, 1);
   c->buf.assign((size_t)c->nfields * NVAR, nullptr);
   c->owned.assign((size_t)c->nfields * NVAR, 0);
   c->bound.assign((size_t)c->nfields, nullptr);
@@ -1078,7 +1121,7 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags,
   }
   std::vector<double> h((size_t)MET_N * nz);
   for (int m = 0; m < MET_N; m++) std::memcpy(&h[(size_t)m * nz], mets[m], sizeof(double) * nz);
-  hipError_t e = hipMalloc(&c->dmet, h.size() * sizeof(double));
+  hipError_t e = dev_alloc(&c->dmet, h.size() * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(c->dmet, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     if (c->dmet) (void)hipFree(c->dmet);
@@ -1086,6 +1129,14 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int advect_flags,
     return fail(MONC_TVD_ERR_HIP, "metric upload: %s", hipGetErrorString(e));
   }
   *out = c;
+  return MONC_TVD_OK;
+}
+
+int monc_tvd_set_field_active(monc_tvd_ctx* c, int field, int active) {
+  if (int rc = check_ctx(c)) return rc;
+  if (field != MONC_TVD_TH && (field < MONC_TVD_Q(0) || field >= MONC_TVD_Q(c->n_q)))
+    return fail(MONC_TVD_ERR_ARG, "field %d has no active flag: only th and the q fields are gated", field);
+  c->active[field] = active ? 1 : 0;
   return MONC_TVD_OK;
 }
 
@@ -1123,7 +1174,8 @@ int monc_tvd_destroy(monc_tvd_ctx* c) {
     for (void* p : {c->diag_before[f], c->diag_3d[f], c->diag_zf[f], (void*)c->diag_prof[f]})
       if (p) (void)hipFree(p);
   }
-  for (void* p : {(void*)c->rprefrcp, c->tabs_3d, (void*)c->tabs_prof, (void*)c->diag_partial, (void*)c->thadv_tab})
+  for (void* p : {(void*)c->rprefrcp, c->tabs_3d, (void*)c->tabs_prof, (void*)c->diag_partial, (void*)c->thadv_tab,
+                  (void*)c->wsum_scratch})
     if (p) (void)hipFree(p);
   if (c->dmet) (void)hipFree(c->dmet);
   delete c;
@@ -1132,8 +1184,8 @@ int monc_tvd_destroy(monc_tvd_ctx* c) {
 
 int monc_tvd_set_tiling(monc_tvd_ctx* c, int kernel, int rows, int strip) {
   if (int rc = check_ctx(c)) return rc;
-  if (kernel != MONC_TVD_KERNEL_MARCH && kernel != MONC_TVD_KERNEL_CELL && kernel != MONC_TVD_KERNEL_SPLIT)
-    return fail(MONC_TVD_ERR_ARG, "kernel %d is not 0 (march), 1 (cell) or 2 (split)", kernel);
+  if (kernel != MONC_TVD_KERNEL_MARCH && kernel != MONC_TVD_KERNEL_CELL)
+    return fail(MONC_TVD_ERR_ARG, "kernel %d is not 0 (march) or 1 (cell)", kernel);
   if (strip < 0 || strip > 4096) return fail(MONC_TVD_ERR_ARG, "strip %d out of range [0,4096]", strip);
   const int nch = (c->nz + 63) / 64;
   if (kernel == MONC_TVD_KERNEL_MARCH && rows != 0 && !march_rows_ok(nch, rows))
@@ -1166,32 +1218,9 @@ int monc_tvd_weighted_sum(monc_tvd_ctx* c, int field, int variant, const double*
   if (int rc = check_fv(c, field, variant)) return rc;
   if (!weights || !out) return fail(MONC_TVD_ERR_ARG, "null weights or output");
   if (int rc = need(c, field, variant)) return rc;
-  const int gy = (c->ny + 3) / 4, gz = c->nx;
-  const size_t np = (size_t)gy * gz;
-  double *dw = nullptr, *dp = nullptr;
-  HIP_TRY(hipMalloc(&dw, sizeof(double) * c->nz));
-  hipError_t e = hipMalloc(&dp, sizeof(double) * np);
-  if (e == hipSuccess) e = hipMemcpyAsync(dw, weights, sizeof(double) * c->nz, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) {
-    dim3 block(64, 4, 1), grid(1, gy, gz);
-    if (c->precision == MONC_TVD_FP64)
-      hipLaunchKernelGGL((wsum_kernel<double>), grid, block, 0, c->stream, (const double*)slot(c, field, variant),
-                         dims_of(c), dw, dp);
-    else
-      hipLaunchKernelGGL((wsum_kernel<float>), grid, block, 0, c->stream, (const float*)slot(c, field, variant),
-                         dims_of(c), dw, dp);
-    e = hipGetLastError();
-  }
-  std::vector<double> h(np);
-  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dp, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (dw) (void)hipFree(dw);
-  if (dp) (void)hipFree(dp);
-  if (e != hipSuccess) return fail(MONC_TVD_ERR_HIP, "weighted sum: %s", hipGetErrorString(e));
-  double acc = 0.0;
-  for (size_t n = 0; n < np; n++) acc += h[n];
-  *out = acc;
-  return MONC_TVD_OK;
+  // scratch of the context, allocated on the first call and kept
+  if (!c->wsum_scratch) HIP_TRY(dev_alloc(&c->wsum_scratch, sizeof(double) * wsum_scratch_len(c->nz)));
+  return weighted_sum_on(c, slot(c, field, variant), weights, c->wsum_scratch, out);
 }
 
 int monc_tvd_set_profiling(monc_tvd_ctx* c, int on) {
@@ -1313,7 +1342,7 @@ int monc_tvd_tendencies(monc_tvd_ctx* c, double dtm, int field_stepping, int mom
   for (int s : {field_stepping, momentum_stepping, scalar_stepping})
     if (s != MONC_TVD_CENTRED_STEPPING && s != MONC_TVD_FORWARD_STEPPING)
       return fail(MONC_TVD_ERR_ARG, "stepping flag %d is neither CENTRED (0) nor FORWARD (1)", s);
-  if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
+  if (!std::isfinite(dtm)) return fail(MONC_TVD_ERR_ARG, "dtm must be finite");
   return c->precision == MONC_TVD_FP64
              ? tendencies_impl<double>(c, dtm, field_stepping, momentum_stepping, scalar_stepping)
              : tendencies_impl<float>(c, dtm, field_stepping, momentum_stepping, scalar_stepping);
@@ -1321,7 +1350,7 @@ int monc_tvd_tendencies(monc_tvd_ctx* c, double dtm, int field_stepping, int mom
 
 int monc_tvd_adjoint_step(monc_tvd_ctx* c, double dtm) {
   if (int rc = check_ctx(c)) return rc;
-  if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
+  if (!std::isfinite(dtm)) return fail(MONC_TVD_ERR_ARG, "dtm must be finite");
   return c->precision == MONC_TVD_FP64 ? adjoint_step_impl<double>(c, dtm) : adjoint_step_impl<float>(c, dtm);
 }
 
@@ -1356,14 +1385,14 @@ int monc_thadv_tendencies(monc_tvd_ctx* c, int anelastic, const double* dthref, 
 int monc_pw_tendencies(monc_tvd_ctx* c, int advect_flow, int advect_th, int advect_q, double tcx, double tcy) {
   if (int rc = check_ctx(c)) return rc;
   if (!advect_flow && !advect_th && !advect_q) return MONC_TVD_OK;
-  if (c->nfields > MAX_FIELDS) return fail(MONC_TVD_ERR_ARG, "%d fields above the kernel's %d", c->nfields, MAX_FIELDS);
+  if (4 + c->n_q > MAX_FIELDS) return fail(MONC_TVD_ERR_ARG, "%d fields above the kernel's %d", 4 + c->n_q, MAX_FIELDS);
   if (!(tcx == tcx) || !(tcy == tcy)) return fail(MONC_TVD_ERR_ARG, "tcx, tcy must not be NaN");
   for (int f = MONC_TVD_U; f <= MONC_TVD_W; f++)
     if (int rc = need(c, f, MONC_TVD_PROG)) return rc;
   if (advect_flow)
     for (int f = MONC_TVD_U; f <= MONC_TVD_W; f++)
       if (int rc = need(c, f, MONC_TVD_SOURCE)) return rc;
-  if (advect_th)
+  if (advect_th && c->active[MONC_TVD_TH])
     for (int v : {MONC_TVD_PROG, MONC_TVD_SOURCE})
       if (int rc = need(c, MONC_TVD_TH, v)) return rc;
   if (advect_q)
@@ -1378,14 +1407,14 @@ int monc_tvd_tangent_steps(monc_tvd_ctx* c, double dtm, int nsteps) {
   if (int rc = check_ctx(c)) return rc;
   if (c->precision != MONC_TVD_FP64) return fail(MONC_TVD_ERR_ARG, "the tangent-linear model runs in FP64 only");
   if (nsteps < 0) return fail(MONC_TVD_ERR_ARG, "nsteps %d < 0", nsteps);
-  if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
+  if (!std::isfinite(dtm)) return fail(MONC_TVD_ERR_ARG, "dtm must be finite");
   return tangent_steps_impl(c, dtm, nsteps);
 }
 
 int monc_tvd_forward_step_into(monc_tvd_ctx* c, double dtm, void* const* in, void* const* out) {
   if (int rc = check_ctx(c)) return rc;
   if (!in || !out) return fail(MONC_TVD_ERR_ARG, "null buffer array");
-  if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
+  if (!std::isfinite(dtm)) return fail(MONC_TVD_ERR_ARG, "dtm must be finite");
   return c->precision == MONC_TVD_FP64 ? forward_into_impl<double>(c, dtm, in, out)
                                        : forward_into_impl<float>(c, dtm, in, out);
 }
@@ -1394,8 +1423,8 @@ int monc_tvd_leapfrog_steps(monc_tvd_ctx* c, double dtm, double tsmth, double ug
                             int first_forward) {
   if (int rc = check_ctx(c)) return rc;
   if (nsteps < 0) return fail(MONC_TVD_ERR_ARG, "nsteps %d < 0", nsteps);
-  if (!(dtm == dtm) || !(tsmth == tsmth) || !(ugal == ugal) || !(vgal == vgal))
-    return fail(MONC_TVD_ERR_ARG, "dtm, tsmth, ugal and vgal must not be NaN");
+  if (!std::isfinite(dtm) || !std::isfinite(tsmth) || !std::isfinite(ugal) || !std::isfinite(vgal))
+    return fail(MONC_TVD_ERR_ARG, "dtm, tsmth, ugal and vgal must be finite");
   return c->precision == MONC_TVD_FP64 ? leapfrog_impl<double>(c, dtm, tsmth, ugal, vgal, nsteps, first_forward)
                                        : leapfrog_impl<float>(c, dtm, tsmth, ugal, vgal, nsteps, first_forward);
 }
@@ -1403,7 +1432,7 @@ int monc_tvd_leapfrog_steps(monc_tvd_ctx* c, double dtm, double tsmth, double ug
 int monc_tvd_forward_steps(monc_tvd_ctx* c, double dtm, int nsteps) {
   if (int rc = check_ctx(c)) return rc;
   if (nsteps < 0) return fail(MONC_TVD_ERR_ARG, "nsteps %d < 0", nsteps);
-  if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
+  if (!std::isfinite(dtm)) return fail(MONC_TVD_ERR_ARG, "dtm must be finite");
   return c->precision == MONC_TVD_FP64 ? forward_steps_impl<double>(c, dtm, nsteps)
                                        : forward_steps_impl<float>(c, dtm, nsteps);
 }
@@ -1413,7 +1442,7 @@ int monc_tvd_set_diagnostics(monc_tvd_ctx* c, int flags, const double* rprefrcp)
   if (flags & ~(MONC_TVD_DIAG_TENDENCY | MONC_TVD_DIAG_ZFACE))
     return fail(MONC_TVD_ERR_ARG, "unknown diagnostics flags 0x%x", flags);
   if (rprefrcp) {
-    if (!c->rprefrcp) HIP_TRY(hipMalloc(&c->rprefrcp, sizeof(double) * c->nz));
+    if (!c->rprefrcp) HIP_TRY(dev_alloc(&c->rprefrcp, sizeof(double) * c->nz));
     HIP_TRY(hipMemcpyAsync(c->rprefrcp, rprefrcp, sizeof(double) * c->nz, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
   }
@@ -1476,7 +1505,7 @@ int monc_tvd_get_diagnostic(monc_tvd_ctx* c, int field, int which, double* host)
 int monc_tvd_autotune(monc_tvd_ctx* c, double dtm, int reps, int* chosen) {
   if (int rc = check_ctx(c)) return rc;
   if (reps < 1 || reps > 100) return fail(MONC_TVD_ERR_ARG, "reps %d out of range [1,100]", reps);
-  if (!(dtm == dtm)) return fail(MONC_TVD_ERR_ARG, "dtm is NaN");
+  if (!std::isfinite(dtm)) return fail(MONC_TVD_ERR_ARG, "dtm must be finite");
   return c->precision == MONC_TVD_FP64 ? autotune_impl<double>(c, dtm, reps, chosen)
                                        : autotune_impl<float>(c, dtm, reps, chosen);
 }

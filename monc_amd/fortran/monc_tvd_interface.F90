@@ -11,19 +11,26 @@ module monc_tvd_interface_mod
   integer(c_int), parameter :: MONC_TVD_OK = 0
   integer(c_int), parameter :: MONC_TVD_CENTRED_STEPPING = 0, MONC_TVD_FORWARD_STEPPING = 1
   integer(c_int), parameter :: MONC_TVD_FP64 = 0, MONC_TVD_FP32 = 1
-  integer(c_int), parameter :: MONC_TVD_ADVECT_FLOW = 1, MONC_TVD_ADVECT_THETA = 2, MONC_TVD_ADVECT_Q = 4
+  integer(c_int), parameter :: MONC_TVD_ADVECT_FLOW = 1, MONC_TVD_ADVECT_THETA = 2, MONC_TVD_ADVECT_Q = 4, &
+       MONC_TVD_ADVECT_TRACER = 8
   integer(c_int), parameter :: MONC_TVD_U = 0, MONC_TVD_V = 1, MONC_TVD_W = 2, MONC_TVD_TH = 3
   integer(c_int), parameter :: MONC_TVD_PROG = 0, MONC_TVD_ZFIELD = 1, MONC_TVD_SOURCE = 2
 
   interface
-    integer(c_int) function monc_tvd_create(nz, ny, nx, halo, n_q, advect_flags, cx, cy, rdz, rdzn, dz, dzn, &
-         tzc1, tzc2, tzd1, tzd2, precision, ctx) bind(C, name="monc_tvd_create")
+    integer(c_int) function monc_tvd_create(nz, ny, nx, halo, n_q, n_tracers, advect_flags, cx, cy, rdz, rdzn, &
+         dz, dzn, tzc1, tzc2, tzd1, tzd2, precision, ctx) bind(C, name="monc_tvd_create")
       import :: c_int, c_double, c_ptr
-      integer(c_int), value :: nz, ny, nx, halo, n_q, advect_flags, precision
+      integer(c_int), value :: nz, ny, nx, halo, n_q, n_tracers, advect_flags, precision
       real(c_double), value :: cx, cy
       real(c_double), dimension(*), intent(in) :: rdz, rdzn, dz, dzn, tzc1, tzc2, tzd1, tzd2
       type(c_ptr), intent(out) :: ctx
     end function monc_tvd_create
+
+    integer(c_int) function monc_tvd_set_field_active(ctx, field, active) bind(C, name="monc_tvd_set_field_active")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field, active
+    end function monc_tvd_set_field_active
 
     integer(c_int) function monc_tvd_destroy(ctx) bind(C, name="monc_tvd_destroy")
       import :: c_int, c_ptr

@@ -1,20 +1,28 @@
 !> Fortran host driver of the drop-in path: reads one MONC local state from a
-!! raw file, runs the TVD timestep through the C ABI exactly as the component's
-!! timestep callback would (halo swap, then tendencies into zeroed source
-!! fields), and writes the source fields back out.
+!! raw file, creates the context through the shim core (tvd_shim_create, as
+!! the component's initialisation callback does), runs the TVD timestep
+!! through the C ABI as the component's timestep callback would (halo swap,
+!! then tendencies into zeroed source fields), and writes the source fields
+!! back out.
 !!
-!! Input file (stream, native endian): 8 int32 (nz, ny, nx, n_q, field_stepping,
-!! momentum_stepping, scalar_stepping, precision), 3 float64 (cx, cy, dtm), the 8
-!! metric vectors (rdz, rdzn, dz, dzn, tzc1, tzc2, tzd1, tzd2; nz each), then
-!! the 4+n_q prognostics and the 4+n_q z-fields, each (nz, ny+4, nx+4) float64.
-!! Output file: the 4+n_q source fields in the same layout.
+!! Input file (stream, native endian): 13 int32 (local_grid%size in Z_INDEX,
+!! Y_INDEX, X_INDEX order, number_q_fields, n_tracers, field_stepping,
+!! momentum_stepping, scalar_stepping, precision, advect_flow, advect_th,
+!! advect_q, th%active), number_q_fields int32 (q(n)%active), 3 float64 (cx,
+!! cy, dtm), the 8 metric vectors (rdz, rdzn, dz, dzn, tzc1, tzc2, tzd1, tzd2;
+!! nz each), then the prognostics and the z-fields of u, v, w, th, q(:),
+!! tracer(:), each (nz, ny+4, nx+4) float64.
+!! Output file: the source fields in the same layout and order.
 program tvd_dropin_driver
   use iso_c_binding, only : c_int, c_double, c_ptr
   use monc_tvd_interface_mod
+  use tvd_shim_core_mod
   implicit none
 
   character(len=4096) :: in_path, out_path
-  integer(c_int) :: hdr(8), nz, ny, nx, n_q, fs, ms, ss, prec, rc, n
+  integer(c_int) :: hdr(13), fs, ms, ss, prec, n
+  integer :: local_size(3), nq, ntr, nf
+  integer(c_int), allocatable :: qact(:)
   real(c_double) :: rhdr(3)
   real(c_double), allocatable :: met(:,:), fld(:,:,:), zero(:,:,:)
   type(c_ptr) :: ctx
@@ -24,21 +32,26 @@ program tvd_dropin_driver
   call get_command_argument(2, out_path)
   open(newunit=u_in, file=trim(in_path), access="stream", form="unformatted", status="old", action="read")
   read(u_in) hdr
+  local_size = hdr(1:3)
+  nq = hdr(4); ntr = hdr(5)
+  fs = hdr(6); ms = hdr(7); ss = hdr(8); prec = hdr(9)
+  allocate(qact(nq))
+  if (nq > 0) read(u_in) qact
   read(u_in) rhdr
-  nz = hdr(1); ny = hdr(2); nx = hdr(3); n_q = hdr(4)
-  fs = hdr(5); ms = hdr(6); ss = hdr(7); prec = hdr(8)
-  allocate(met(nz, 8), fld(nz, ny+4, nx+4), zero(nz, ny+4, nx+4))
+  nf = 4 + nq + ntr
+  allocate(met(local_size(Z_INDEX), 8), fld(local_size(Z_INDEX), local_size(Y_INDEX)+4, local_size(X_INDEX)+4))
+  allocate(zero(local_size(Z_INDEX), local_size(Y_INDEX)+4, local_size(X_INDEX)+4))
   read(u_in) met
   zero = 0.0_c_double
 
-  rc = monc_tvd_create(nz, ny, nx, 2_c_int, n_q, MONC_TVD_ADVECT_FLOW + MONC_TVD_ADVECT_THETA + MONC_TVD_ADVECT_Q, &
-       rhdr(1), rhdr(2), met(:,1), met(:,2), met(:,3), met(:,4), met(:,5), met(:,6), met(:,7), met(:,8), prec, ctx)
-  call check(rc, "monc_tvd_create")
-  do n = 0, 3 + n_q
+  call check(tvd_shim_create(local_size, nq, ntr, hdr(10) /= 0, hdr(11) /= 0, hdr(12) /= 0, hdr(13) /= 0, &
+       qact /= 0, rhdr(1), rhdr(2), met(:,1), met(:,2), met(:,3), met(:,4), met(:,5), met(:,6), met(:,7), &
+       met(:,8), prec, ctx), "tvd_shim_create")
+  do n = 0, nf - 1
     read(u_in) fld
     call check(monc_tvd_set_field(ctx, n, MONC_TVD_PROG, fld), "set prognostic")
   end do
-  do n = 0, 3 + n_q
+  do n = 0, nf - 1
     read(u_in) fld
     call check(monc_tvd_set_field(ctx, n, MONC_TVD_ZFIELD, fld), "set z-field")
     ! clearsourceterms zeroes every source field before the dynamics group
@@ -50,7 +63,7 @@ program tvd_dropin_driver
   call check(monc_tvd_tendencies(ctx, rhdr(3), fs, ms, ss), "tendencies")
 
   open(newunit=u_out, file=trim(out_path), access="stream", form="unformatted", status="replace", action="write")
-  do n = 0, 3 + n_q
+  do n = 0, nf - 1
     call check(monc_tvd_get_field(ctx, n, MONC_TVD_SOURCE, fld), "get source")
     write(u_out) fld
   end do

@@ -137,7 +137,8 @@ def empty_field(nz: int, ny: int, nx: int, dtype=np.float64) -> np.ndarray:
 
 @dataclass
 class State:
-    """Fields of one local domain: ids 0..2 = u, v, w; 3 = th; 4.. = q fields."""
+    """Fields of one local domain: ids 0..2 = u, v, w; 3 = th; then the n_q
+    q fields and, after them, the n_tracers tracers (state.F90:86-92)."""
     nz: int
     ny: int
     nx: int
@@ -146,10 +147,11 @@ class State:
     grid: VerticalGrid
     fields: List[np.ndarray]
     zfields: List[np.ndarray] = field(default_factory=list)
+    n_tracers: int = 0
 
     @property
     def n_q(self) -> int:
-        return len(self.fields) - 4
+        return len(self.fields) - 4 - self.n_tracers
 
     def shape(self):
         return (self.nx + 4, self.ny + 4, self.nz)
@@ -189,7 +191,7 @@ def nondivergent_flow(nz: int, ny: int, nx: int, dxy: float, z: np.ndarray, rng:
 
 def make_state(nz: int, ny: int, nx: int, dxy: float, ztop: float, seed: int, n_q: int = 1,
                anelastic_scale: Optional[float] = None, u_mean: float = 5.0, lapse: float = 0.003,
-               with_zfields: bool = True, flow: str = "noise") -> State:
+               with_zfields: bool = True, flow: str = "noise", n_tracers: int = 0) -> State:
     """Seeded synthetic state: u = u_mean + N(0,1), v = N(0,1), w = N(0,0.5) with
     w = 0 at k = 1 and nz, th = 300 + lapse*zn + U(-0.1,0.1),
     q = 0.01*exp(-zn/2000) + U(0,1e-4) (extra q fields reuse the q recipe).
@@ -198,7 +200,9 @@ def make_state(nz: int, ny: int, nx: int, dxy: float, ztop: float, seed: int, n_
     centred stepping) are the same fields plus a second seeded perturbation.
     flow = "nondivergent" replaces u, v, w (after the same draws) by the
     discretely divergence-free flow of `nondivergent_flow`, drawn next from
-    the same generator (dx = dy = dxy)."""
+    the same generator (dx = dy = dxy).  n_tracers tracers follow the q fields:
+    tracer n = exp(-((zn - z_n)/500)^2) + U(0, 0.01) (a blob level z_n spread
+    over the column), drawn after everything above."""
     z = tanh_stretched_z(nz, ztop)
     g = vertical_grid(z, anelastic_scale)
     rng = np.random.default_rng(seed)
@@ -221,7 +225,9 @@ def make_state(nz: int, ny: int, nx: int, dxy: float, ztop: float, seed: int, n_
         u, v, w = nondivergent_flow(nz, ny, nx, dxy, z, rng, u_mean=u_mean)
     elif flow != "noise":
         raise ValueError(f"unknown flow kind {flow!r}")
-    fields = [interior(a) for a in [u, v, w, th, *qs]]
+    trs = [np.exp(-((zn - (n + 1) * ztop / (n_tracers + 1)) / 500.0) ** 2) + rng.uniform(0.0, 0.01, shp)
+           for n in range(max(n_tracers, 0))]
+    fields = [interior(a) for a in [u, v, w, th, *qs, *trs]]
     zfields: List[np.ndarray] = []
     if with_zfields:
         rng2 = np.random.default_rng(seed + 7919)
@@ -234,7 +240,8 @@ def make_state(nz: int, ny: int, nx: int, dxy: float, ztop: float, seed: int, n_
                 zf[:, :, -1] = 0.0
             zfields.append(halo_fill(zf))
     c = horizontal_c(dxy)
-    return State(nz=nz, ny=ny, nx=nx, cx=c, cy=c, grid=g, fields=fields, zfields=zfields)
+    return State(nz=nz, ny=ny, nx=nx, cx=c, cy=c, grid=g, fields=fields, zfields=zfields,
+                 n_tracers=max(n_tracers, 0))
 
 
 def cfl_dt(state: State, cfl: float = 0.3) -> float:

@@ -23,7 +23,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libmonc_tvd.so"
 OK, ERR_ARG, ERR_HIP, ERR_STATE = 0, 1, 2, 3
 CENTRED_STEPPING, FORWARD_STEPPING = 0, 1  # state.F90:15
 FP64, FP32 = 0, 1
-ADVECT_FLOW, ADVECT_THETA, ADVECT_Q = 1, 2, 4
+ADVECT_FLOW, ADVECT_THETA, ADVECT_Q, ADVECT_TRACER = 1, 2, 4, 8
 U, V, W, TH = 0, 1, 2, 3
 PROG, ZFIELD, SOURCE, ADJOINT, TANGENT = 0, 1, 2, 3, 4
 
@@ -36,7 +36,7 @@ EXPORTS = (
     "monc_tvd_adjoint_step", "monc_tvd_adjoint_fold", "monc_tvd_tangent_steps", "monc_tvd_march_shapes",
     "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic", "monc_tvd_autotune",
     "monc_tvd_leapfrog_steps", "monc_tvd_forward_step_into", "monc_thadv_tendencies",
-    "monc_pw_tendencies",
+    "monc_pw_tendencies", "monc_tvd_set_field_active", "monc_tvd_test_fail_allocations",
     "monc_tvd_ensemble_create", "monc_tvd_ensemble_destroy", "monc_tvd_ensemble_layout",
     "monc_tvd_ensemble_set_field", "monc_tvd_ensemble_get_field", "monc_tvd_ensemble_copy_pool",
     "monc_tvd_ensemble_set_dt", "monc_tvd_ensemble_set_stream", "monc_tvd_ensemble_synchronize",
@@ -45,12 +45,16 @@ EXPORTS = (
 )
 DIAG_TENDENCY, DIAG_ZFACE = 1, 2
 DIAG_3D, DIAG_PROFILE, DIAG_TABS_3D, DIAG_TABS_PROFILE, DIAG_ZFACES = 0, 1, 2, 3, 4
-KERNEL_MARCH, KERNEL_CELL, KERNEL_SPLIT = 0, 1, 2
+KERNEL_MARCH, KERNEL_CELL = 0, 1
 KERNEL_KINDS = ("u", "v", "w", "scalars", "halo", "cell")
 
 
 def Q(n: int) -> int:
     return 4 + n
+
+
+def TRACER(n_q: int, n: int) -> int:
+    return 4 + n_q + n
 
 
 class TvdError(RuntimeError):
@@ -72,8 +76,10 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib = ctypes.CDLL(str(path))
     c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
     dp = ctypes.POINTER(ctypes.c_double)
-    lib.monc_tvd_create.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_double,
+    lib.monc_tvd_create.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_double,
                                     dp, dp, dp, dp, dp, dp, dp, dp, c_int, ctypes.POINTER(c_void_p)]
+    lib.monc_tvd_set_field_active.argtypes = [c_void_p, c_int, c_int]
+    lib.monc_tvd_test_fail_allocations.argtypes = [c_int]
     lib.monc_tvd_destroy.argtypes = [c_void_p]
     lib.monc_tvd_set_field.argtypes = [c_void_p, c_int, c_int, dp]
     lib.monc_tvd_get_field.argtypes = [c_void_p, c_int, c_int, dp]
@@ -141,9 +147,10 @@ class TvdAdvection:
     """One device-resident local domain of the tvd_advection component."""
 
     def __init__(self, nz: int, ny: int, nx: int, cx: float, cy: float, metrics: dict, n_q: int = 1,
-                 advect: int = ADVECT_FLOW | ADVECT_THETA | ADVECT_Q, precision: int = FP64, halo: int = 2):
+                 advect: int = ADVECT_FLOW | ADVECT_THETA | ADVECT_Q, precision: int = FP64, halo: int = 2,
+                 n_tracers: int = 0):
         self._lib = load_library()
-        self.nz, self.ny, self.nx, self.n_q = nz, ny, nx, n_q
+        self.nz, self.ny, self.nx, self.n_q, self.n_tracers = nz, ny, nx, n_q, n_tracers
         self.precision = precision
         self._keep = [np.ascontiguousarray(metrics[k], dtype=np.float64) for k in
                       ("rdz", "rdzn", "dz", "dzn", "tzc1", "tzc2", "tzd1", "tzd2")]
@@ -151,17 +158,23 @@ class TvdAdvection:
             if v.shape != (nz,):
                 raise ValueError("metric vectors must have length nz")
         h = ctypes.c_void_p()
-        _check(self._lib.monc_tvd_create(nz, ny, nx, halo, n_q, advect, cx, cy, *[_dptr(v) for v in self._keep],
-                                         precision, ctypes.byref(h)))
+        _check(self._lib.monc_tvd_create(nz, ny, nx, halo, n_q, n_tracers, advect, cx, cy,
+                                         *[_dptr(v) for v in self._keep], precision, ctypes.byref(h)))
         self._h = h
 
     # the component surface -------------------------------------------------
     @classmethod
     def initialise(cls, state: State, precision: int = FP64,
                    advect: int = ADVECT_FLOW | ADVECT_THETA | ADVECT_Q) -> "TvdAdvection":
-        """initialisation_callback: metrics to the device, fields staged."""
+        """initialisation_callback: metrics to the device, fields staged.  With
+        tracers in the state, ADVECT_THETA also sets ADVECT_TRACER, as the
+        reference derives advect_tracer from the theta option
+        (tvdadvection.F90:352)."""
+        n_tr = getattr(state, "n_tracers", 0)
+        if n_tr and advect & ADVECT_THETA:
+            advect |= ADVECT_TRACER
         t = cls(state.nz, state.ny, state.nx, state.cx, state.cy, state.grid.metric_vectors(), n_q=state.n_q,
-                advect=advect, precision=precision)
+                advect=advect, precision=precision, n_tracers=n_tr)
         for n, f in enumerate(state.fields):
             t.set_field(n, PROG, f)
         for n, f in enumerate(state.zfields):
@@ -205,6 +218,14 @@ class TvdAdvection:
             raise ValueError("bind_field needs a contiguous CUDA tensor of the context dtype and field shape")
         _check(self._lib.monc_tvd_bind_field(self._h, field, variant, ctypes.c_void_p(tensor.data_ptr())))
 
+    def set_field_active(self, field: int, active: bool) -> None:
+        """th%active / q(n)%active: an inactive th or q field is not advected."""
+        _check(self._lib.monc_tvd_set_field_active(self._h, field, 1 if active else 0))
+
+    @property
+    def nfields(self) -> int:
+        return 4 + self.n_q + self.n_tracers
+
     def device_ptr(self, field: int, variant: int) -> int:
         p = ctypes.c_void_p()
         _check(self._lib.monc_tvd_field_device_ptr(self._h, field, variant, ctypes.byref(p)))
@@ -212,7 +233,7 @@ class TvdAdvection:
 
     def set_tiling(self, kernel: int = KERNEL_MARCH, rows: int = 0, strip: int = 0) -> None:
         """kernel KERNEL_MARCH (default; rows = cell rows per block, 0 the
-        default shape), KERNEL_CELL or KERNEL_SPLIT; strip = x columns per
+        default shape) or KERNEL_CELL; strip = x columns per
         block (0: 32)."""
         _check(self._lib.monc_tvd_set_tiling(self._h, kernel, rows, strip))
 
@@ -289,12 +310,31 @@ class TvdAdvection:
     def forward_steps(self, dtm: float, nsteps: int) -> None:
         _check(self._lib.monc_tvd_forward_steps(self._h, dtm, nsteps))
 
+    def _check_tensor(self, t, what: str) -> None:
+        import torch
+        want = torch.float64 if self.precision == FP64 else torch.float32
+        if not t.is_cuda or t.dtype != want or tuple(t.shape) != self.field_shape() or not t.is_contiguous():
+            raise ValueError(f"{what} must be a contiguous CUDA tensor of the context dtype and field shape")
+
     def forward_step_into(self, dtm: float, src, dst) -> None:
         """One forward step from the torch tensors `src` (one per field) into
-        `dst`, without touching the context's own fields."""
+        `dst`, without touching the context's own fields.  Every tensor must be
+        a contiguous CUDA tensor of the context's dtype and field shape, and no
+        destination may share memory with a source or another destination."""
         n = len(src)
-        if n != len(dst) or n != 4 + self.n_q:
+        if n != len(dst) or n != self.nfields:
             raise ValueError("one source and one destination tensor per field")
+        for i, t in enumerate(src):
+            self._check_tensor(t, f"source {i}")
+        for i, t in enumerate(dst):
+            self._check_tensor(t, f"destination {i}")
+        nbytes = int(np.prod(self.field_shape())) * (8 if self.precision == FP64 else 4)
+        spans = [(t.data_ptr(), t.data_ptr() + nbytes) for t in src]
+        outs = [(t.data_ptr(), t.data_ptr() + nbytes) for t in dst]
+        for i, (a0, a1) in enumerate(outs):
+            for j, (b0, b1) in enumerate(spans + [o for m, o in enumerate(outs) if m != i]):
+                if a0 < b1 and b0 < a1:
+                    raise ValueError(f"destination {i} overlaps another source or destination")
         a = (ctypes.c_void_p * n)(*[t.data_ptr() for t in src])
         b = (ctypes.c_void_p * n)(*[t.data_ptr() for t in dst])
         _check(self._lib.monc_tvd_forward_step_into(self._h, dtm, a, b))

@@ -47,6 +47,7 @@ Parity status per oracle function (the checks live in tests/test_oracle.py):
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import subprocess
 from pathlib import Path
@@ -65,33 +66,61 @@ def build() -> Path:
     return LIB
 
 
+def build_variant(plant: int, outdir: Path) -> Path:
+    """Build the restatement with the deliberate mistake PLANT = plant into
+    outdir (for the pinning test only: it shows that each planted mistake
+    fails at least one check)."""
+    out = Path(outdir) / f"liboracle_plant{plant}.so"
+    subprocess.run(["gcc", "-O2", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math", f"-DPLANT={plant}",
+                    "-shared", "-o", str(out), str(HERE / "tvd_oracle.c"), "-lm"], check=True)
+    return out
+
+
+@contextlib.contextmanager
+def use_library(path):
+    """Temporarily route every call of this module through another build of
+    the restatement (a planted-mistake variant)."""
+    global _lib
+    saved = _lib
+    _lib = _load(Path(path))
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
         if not LIB.exists():
             build()
-        L = ctypes.CDLL(str(LIB))
-        dp = ctypes.POINTER(ctypes.c_double)
-        pp = ctypes.POINTER(dp)
-        ci, cd = ctypes.c_int, ctypes.c_double
-        L.oracle_tvd_timestep.argtypes = [ci, ci, ci, cd, cd, dp, dp, dp, dp, dp, dp, dp, dp, cd, ci, ci, ci, ci, ci,
-                                          pp, pp, pp, ci, dp, dp, dp]
-        L.oracle_tvd_timestep.restype = ci
-        L.oracle_halo_fill.argtypes = [dp, ci, ci, ci]
-        L.oracle_vertical_metrics.argtypes = [ci, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
-        L.oracle_horizontal_c.argtypes = [cd]
-        L.oracle_horizontal_c.restype = cd
-        L.oracle_forward_step.argtypes = [dp, dp, dp, ci, ci, ci, cd]
-        L.oracle_profile_total.argtypes = [dp, ci, ci, ci, dp]
-        L.oracle_step_field.argtypes = [dp, dp, dp, dp, ci, ci, ci, ci, ci, cd, cd, ci, cd, cd]
-        L.oracle_swap_classic.argtypes = [dp, dp, ci, ci, ci]
-        L.oracle_swap_robert.argtypes = [dp, dp, dp, ci, cd, cd, ci, ci, ci]
-        L.oracle_thadv.argtypes = [dp, dp, dp, dp, dp, dp, dp, dp, ci, ci, ci, ci, ci] + [cd] * 7
-        pp = ctypes.POINTER(dp)
-        L.oracle_pw.argtypes = [dp, dp, dp, dp, pp, dp, dp, dp, dp, pp, ci, dp, dp, dp, dp, cd, cd, cd, cd, ci, ci,
-                                ci, ci, ci, ci]
-        _lib = L
+        _lib = _load(LIB)
     return _lib
+
+
+def _load(path: Path) -> ctypes.CDLL:
+    L = ctypes.CDLL(str(path))
+    dp = ctypes.POINTER(ctypes.c_double)
+    pp = ctypes.POINTER(dp)
+    ci, cd = ctypes.c_int, ctypes.c_double
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.oracle_tvd_timestep.argtypes = [ci, ci, ci, cd, cd, dp, dp, dp, dp, dp, dp, dp, dp, cd, ci, ci, ci, ci, ci,
+                                      ci, ip, pp, pp, pp, ci, dp, dp, dp]
+    L.oracle_tvd_timestep.restype = ci
+    L.oracle_halo_fill.argtypes = [dp, ci, ci, ci]
+    L.oracle_vertical_metrics.argtypes = [ci, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+    L.oracle_horizontal_c.argtypes = [cd]
+    L.oracle_horizontal_c.restype = cd
+    L.oracle_forward_step.argtypes = [dp, dp, dp, ci, ci, ci, cd]
+    L.oracle_profile_total.argtypes = [dp, ci, ci, ci, dp]
+    L.oracle_step_field.argtypes = [dp, dp, dp, dp, ci, ci, ci, ci, ci, cd, cd, ci, cd, cd]
+    L.oracle_swap_classic.argtypes = [dp, dp, ci, ci, ci]
+    L.oracle_swap_robert.argtypes = [dp, dp, dp, ci, cd, cd, ci, ci, ci]
+    L.oracle_thadv.argtypes = [dp, dp, dp, dp, dp, dp, dp, dp, ci, ci, ci, ci, ci] + [cd] * 7
+    pp = ctypes.POINTER(dp)
+    L.oracle_pw.argtypes = [dp, dp, dp, dp, pp, dp, dp, dp, dp, pp, ci, dp, dp, dp, dp, cd, cd, cd, cd, ci, ci,
+                            ci, ci, ci, ci]
+    return L
 
 
 def _p(a: np.ndarray):
@@ -107,11 +136,21 @@ def _pp(arrs: Sequence[np.ndarray]):
 def tendencies(nz: int, ny: int, nx: int, cx: float, cy: float, metrics: dict, dtm: float, field_stepping: int,
                momentum_stepping: int, scalar_stepping: int, fields: Sequence[np.ndarray],
                zfields: Optional[Sequence[np.ndarray]] = None, sources: Optional[Sequence[np.ndarray]] = None,
-               advect: int = 7, rec_field: int = -1):
+               advect: int = 7, rec_field: int = -1, n_tracers: int = 0, active: Optional[Sequence[int]] = None):
     """One timestep_callback sweep.  Returns (sources, faces) where sources are
     the accumulated s-fields (copies of `sources` or zeros) and faces is
-    (fx, fy, fz) for rec_field (or None)."""
+    (fx, fy, fz) for rec_field (or None).  fields: u, v, w, th, the q fields,
+    then n_tracers tracers; advect bits 1 flow, 2 theta, 4 q, 8 tracers;
+    active: per-field flags (None: all active)."""
     n = len(fields)
+    n_q = n - 4 - n_tracers
+    if n_q < 0:
+        raise ValueError("fewer fields than u, v, w, th and the tracers")
+    act = None
+    if active is not None:
+        if len(active) != n:
+            raise ValueError("one active flag per field")
+        act = (ctypes.c_int * n)(*[1 if a else 0 for a in active])
     shape = (nx + 4, ny + 4, nz)
     fields = [np.ascontiguousarray(f, dtype=np.float64) for f in fields]
     zfields = [np.ascontiguousarray(f, dtype=np.float64) for f in (zfields if zfields else fields)]
@@ -125,18 +164,18 @@ def tendencies(nz: int, ny: int, nx: int, cx: float, cy: float, metrics: dict, d
         rec = [_p(a) for a in faces]
     rc = lib().oracle_tvd_timestep(nz, ny, nx, cx, cy, _p(m["rdz"]), _p(m["rdzn"]), _p(m["dz"]), _p(m["dzn"]),
                                    _p(m["tzc1"]), _p(m["tzc2"]), _p(m["tzd1"]), _p(m["tzd2"]), dtm,
-                                   field_stepping, momentum_stepping, scalar_stepping, advect, n - 4,
-                                   _pp(fields), _pp(zfields), _pp(srcs), rec_field, *rec)
+                                   field_stepping, momentum_stepping, scalar_stepping, advect, n_q, n_tracers,
+                                   act, _pp(fields), _pp(zfields), _pp(srcs), rec_field, *rec)
     if rc != 0:
         raise RuntimeError(f"oracle_tvd_timestep failed with {rc}")
     return srcs, faces
 
 
 def state_tendencies(state, dtm: float, field_stepping: int = 1, momentum_stepping: int = 1,
-                     scalar_stepping: int = 1, sources=None, advect: int = 7, rec_field: int = -1):
+                     scalar_stepping: int = 1, sources=None, advect: int = 7, rec_field: int = -1, active=None):
     return tendencies(state.nz, state.ny, state.nx, state.cx, state.cy, state.grid.metric_vectors(), dtm,
                       field_stepping, momentum_stepping, scalar_stepping, state.fields, state.zfields or None,
-                      sources, advect, rec_field)
+                      sources, advect, rec_field, getattr(state, "n_tracers", 0), active)
 
 
 def halo_fill(f: np.ndarray) -> np.ndarray:
@@ -169,26 +208,32 @@ def forward_step(f: np.ndarray, s: np.ndarray, dtm: float) -> np.ndarray:
     return out
 
 
-def forward_run(state, dtm: float, nsteps: int, advect: int = 7):
+def forward_run(state, dtm: float, nsteps: int, advect: int = 7, active=None):
     """nsteps of: halo fill, FORWARD tendencies from zero, forward step, for all
     advected fields (the resident stepping the library fuses)."""
     fields = [halo_fill(f.copy()) for f in state.fields]
+    n_tr = getattr(state, "n_tracers", 0)
     for _ in range(nsteps):
         srcs, _ = tendencies(state.nz, state.ny, state.nx, state.cx, state.cy, state.grid.metric_vectors(), dtm,
-                             1, 1, 1, fields, None, None, advect)
-        adv = _advected(len(fields), advect)
+                             1, 1, 1, fields, None, None, advect, -1, n_tr, active)
+        adv = advected(len(fields), advect, n_tr, active)
         fields = [halo_fill(forward_step(f, srcs[n], dtm)) if n in adv else f for n, f in enumerate(fields)]
     return fields
 
 
-def _advected(n: int, advect: int):
+def advected(n: int, advect: int, n_tracers: int = 0, active=None):
+    """Field ids a sweep advects (tvdadvection.F90:591-594, :661, :706)."""
+    act = (lambda f: True) if active is None else (lambda f: bool(active[f]))
+    n_q = n - 4 - n_tracers
     s = set()
     if advect & 1:
         s |= {0, 1, 2}
-    if advect & 2:
+    if advect & 2 and act(3):
         s.add(3)
     if advect & 4:
-        s |= set(range(4, n))
+        s |= {f for f in range(4, 4 + n_q) if act(f)}
+    if advect & 8:
+        s |= set(range(4 + n_q, n))
     return s
 
 
@@ -310,8 +355,9 @@ def leapfrog_run(state, dtm: float, tsmth: float, nsteps: int, ugal: float = 0.0
         direction = FORWARD if (first_forward and step == 0) else CENTRED
         fields = [halo_fill(f) for f in fields]
         zfs = [halo_fill(z) for z in zfs]
+        n_tr = getattr(state, "n_tracers", 0)
         srcs, _ = tendencies(state.nz, state.ny, state.nx, state.cx, state.cy, state.grid.metric_vectors(), dtm,
-                             direction, CENTRED, CENTRED, fields, zfs, None, 7)
+                             direction, CENTRED, CENTRED, fields, zfs, None, 15 if n_tr else 7, -1, n_tr)
         new_f, new_z = [], []
         for n, (f, z, s) in enumerate(zip(fields, zfs, srcs)):
             flow = n in (0, 1)

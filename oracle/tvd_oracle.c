@@ -27,6 +27,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* PLANT selects one deliberate restatement mistake (0: none).  Only the
+ * pinning test of tests/test_oracle.py builds variants with PLANT > 0, to show
+ * that each mistake fails at least one of its checks; the oracle proper is
+ * always built with PLANT = 0. */
+#ifndef PLANT
+#define PLANT 0
+#endif
+
 #define OR_CENTRED 0 /* state.F90:15 CENTRED_STEPPING */
 #define OR_FORWARD 1 /* state.F90:15 FORWARD_STEPPING */
 
@@ -70,7 +78,7 @@ static double flow_at(const or_flow *f, int k, int j, int i) {
 static double flux_in_w(int k, double wval, double dt, double sum_cfl_out, double rdc, double fdels, double fd,
                         double fc, double fu, double rdu, double fgt1, double fgt2, const double *dzn,
                         const double *rdzn) {
-  double dd = V1(dzn, k + 1), rdd = V1(rdzn, k + 1);
+  double dd = V1(dzn, PLANT == 5 ? k : k + 1), rdd = V1(rdzn, k + 1);
   double cflmod = fabs(wval * V1(rdzn, k + 1) * dt);
   double hrcfl = 1.0 / (sum_cfl_out + 1.e-30);
   double hcfl = 0.5 * cflmod;
@@ -174,15 +182,18 @@ static void vert_middle(const or_col *c, double *flux_z) {
       flux_z[k + 1] = fc;
     } else {
       double s = 0.0;
-      s = s + V1(c->rdz, k + ko) * (dmax(0.0, FL(W, k + ko, jf, ifl)) + fabs(dmin(0.0, FL(W, k - 1 + ko, jf, ifl))));
+      s = s + V1(c->rdz, PLANT == 7 ? k : k + ko) *
+                  (dmax(0.0, FL(W, k + ko, jf, ifl)) + fabs(dmin(0.0, FL(W, k - 1 + ko, jf, ifl))));
       s = s + c->cy * (dmax(0.0, FL(V, k + ko, jf, ifl)) + fabs(dmin(0.0, FL(V, k + ko, jf - 1, ifl))));
       s = s + c->cx * (dmax(0.0, FL(U, k + ko, jf, ifl)) + fabs(dmin(0.0, FL(U, k + ko, jf, ifl - 1))));
       s = s * c->dt;
       double fgt1 = grad_y_one(k, V, jf, ifl, c->cy, c->dt, fc, F, js, is, ko);
       double fgt2 = grad_x(k, U, jf, ifl - 1, jf, ifl, jf, ifl - 1, jf, ifl, c->cx, c->dt, fc, A(F, k + ko, js, is + 1),
                            A(F, k + ko, js, is - 1), 1);
-      double rdu = V1(c->rdzn, k) * advpos + V1(c->rdzn, k + 2) * advneg;
-      double rdc = V1(c->rdz, k + c->kdof) * advpos + V1(c->rdz, k + 1 + c->kdof) * advneg;
+      double rdu = PLANT == 1 ? V1(c->rdzn, k + 2) * advpos + V1(c->rdzn, k) * advneg
+                              : V1(c->rdzn, k) * advpos + V1(c->rdzn, k + 2) * advneg;
+      double rdc = PLANT == 4 ? V1(c->rdz, k + 1 + c->kdof) * advpos + V1(c->rdz, k + c->kdof) * advneg
+                              : V1(c->rdz, k + c->kdof) * advpos + V1(c->rdz, k + 1 + c->kdof) * advneg;
       flux_z[k + 1] = flux_in_w(k, FL(W, k, jf, ifl), c->dt, s, rdc, fdels, fd, fc, fu, rdu, fgt1, fgt2, c->dzn,
                                 c->rdzn);
     }
@@ -231,8 +242,8 @@ static void vert_bottom(const or_col *c, double *flux_z) {
     fgt1 = grad_y_one(k, V, jf, ifl, c->cy, c->dt, fc, F, js, is, 1);
     fgt2 = grad_x(k, U, jf, ifl - 1, jf, ifl, jf, ifl - 1, jf, ifl, c->cx, c->dt, fc, A(F, k + 1, js, is + 1),
                   A(F, k + 1, js, is - 1), 1);
-    rdu = V1(c->rdzn, k + 2);
-    rdc = V1(c->rdz, k + 1 + c->kdof);
+    rdu = V1(c->rdzn, PLANT == 6 ? k + 1 : k + 2);
+    rdc = V1(c->rdz, PLANT == 9 ? k + c->kdof : k + 1 + c->kdof);
   }
   flux_z[k + 1] = flux_in_w(k, wk, c->dt, s, rdc, fdels, fd, fc, fu, rdu, fgt1, fgt2, c->dzn, c->rdzn);
 }
@@ -274,14 +285,14 @@ static void vert_top(const or_col *c, double *flux_z) {
     rdc = V1(c->rdz, k + c->kdof);
   } else {
     int kt = (k + 1 + c->kdof < nz) ? k + 1 + c->kdof : nz;
-    s = s + (-(wk * V1(c->rdz, kt)));
+    s = s + (-(wk * V1(c->rdz, PLANT == 8 ? nz - 1 : kt)));
     s = s + c->cy * (dmax(0.0, FL(V, k + 1, jf, ifl)) + fabs(dmin(0.0, FL(V, k + 1, jf - 1, ifl))));
     s = s + c->cx * (dmax(0.0, FL(U, k + 1, jf, ifl)) + fabs(dmin(0.0, FL(U, k + 1, jf, ifl - 1))));
     s = s * c->dt;
     fgt1 = grad_y_one(k, V, jf, ifl, c->cy, c->dt, fc, F, js, is, 1);
     fgt2 = grad_x(k, U, jf, ifl - 1, jf, ifl, jf, ifl - 1, jf, ifl, c->cx, c->dt, fc, A(F, k + 1, js, is + 1),
                   A(F, k + 1, js, is - 1), 1);
-    rdu = V1(c->rdzn, nz);
+    rdu = V1(c->rdzn, PLANT == 2 ? nz - 1 : nz);
     rdc = V1(c->rdz, kt);
   }
   flux_z[k + 1] = flux_in_w(k, wk, c->dt, s, rdc, fdels, fd, fc, fu, rdu, fgt1, fgt2, c->dzn, c->rdzn);
@@ -439,9 +450,9 @@ static void advect_field(const or_grid *g, int x, int y, int halo_column, double
   const double *t1, *t2;
   if (is_w) { /* :881-904 */
     c.kdof = 1;
-    c.rdz = g->rdzn;
-    c.rdzn = g->rdz;
-    c.dzn = g->dz;
+    c.rdz = PLANT == 3 ? g->rdz : g->rdzn;
+    c.rdzn = PLANT == 3 ? g->rdzn : g->rdz;
+    c.dzn = PLANT == 3 ? g->dzn : g->dz;
     kmin = 1;
     kmax = g_nz - 1;
     t1 = g->tzd1;
@@ -498,18 +509,24 @@ static void scratch_free(or_fieldscratch *f) {
  * of timestepper.F90:65-85 (x outer, y inner, halos included).
  *
  * fields[] / zfields[] / sfields[]: index 0,1,2 = u,v,w; 3 = th; 4.. = the
- * n_scalar q fields followed by tracers (all advected by advect_scalar_field).
- * `advect` bit 0 = flow fields, bit 1 = theta, bit 2 = q/tracer fields.
- * The s-fields are accumulated in place (the reference's `s += ...`).
+ * n_q q fields, then the n_tracers tracers (all advected by
+ * advect_scalar_field).  `advect` bit 0 = flow fields (advect_flow), bit 1 =
+ * theta (advect_th), bit 2 = q fields (advect_q), bit 3 = tracers
+ * (advect_tracer, which the reference sets from the theta option,
+ * tvdadvection.F90:349-352).  active[] (NULL: all active) is the per-field
+ * `active` flag: theta is advected only when active (:706), each q field only
+ * when active (:661); tracers and the flow fields are not gated (:603-646,
+ * :684-694).  The s-fields are accumulated in place (the reference's
+ * `s += ...`).
  * rec_field: index of the field whose face values are written to rec (-1: none).
  */
 int oracle_tvd_timestep(int nz, int ny, int nx, double cx, double cy, const double *rdz, const double *rdzn,
                         const double *dz, const double *dzn, const double *tzc1, const double *tzc2,
                         const double *tzd1, const double *tzd2, double dtm, int field_stepping,
-                        int momentum_stepping, int scalar_stepping, int advect, int n_scalar,
-                        const double *const *fields, const double *const *zfields, double *const *sfields,
+                        int momentum_stepping, int scalar_stepping, int advect, int n_q, int n_tracers,
+                        const int *active, const double *const *fields, const double *const *zfields, double *const *sfields,
                         int rec_field, double *rec_fx, double *rec_fy, double *rec_fz) {
-  if (nz < 4 || ny < 1 || nx < 1) return 1;
+  if (nz < 4 || ny < 1 || nx < 1 || n_q < 0 || n_tracers < 0) return 1;
   g_nz = nz;
   g_ny = ny;
   g_nx = nx;
@@ -517,7 +534,7 @@ int oracle_tvd_timestep(int nz, int ny, int nx, double cx, double cy, const doub
   m_flux_x = (double *)calloc((size_t)nz + 2, sizeof(double));
   m_flux_y = (double *)calloc((size_t)nz + 2, sizeof(double));
   m_flux_z = (double *)calloc((size_t)nz + 2, sizeof(double));
-  int nf = 4 + n_scalar;
+  int nf = 4 + n_q + n_tracers;
   or_fieldscratch *fs = (or_fieldscratch *)malloc(sizeof(or_fieldscratch) * (size_t)nf);
   for (int n = 0; n < nf; n++) fs[n] = scratch_new();
   or_facerec rec = {rec_fx, rec_fy, rec_fz};
@@ -550,13 +567,19 @@ int oracle_tvd_timestep(int nz, int ny, int nx, double cx, double cy, const doub
         advect_field(&g, x, y, halo_column, dt_mom, &dw_u, &dw_v, &dw_w, adv[2], sfields[2], &fs[2], 1,
                      rec_field == 2 ? &rec : NULL);
       }
-      if (advect & 2)
+      /* :591-594: flow, theta, q fields, tracers */
+      if ((advect & 2) && (!active || active[3]))
         advect_field(&g, x, y, halo_column, dt_sca, &raw_u, &raw_v, &raw_w, adv[3], sfields[3], &fs[3], 0,
                      rec_field == 3 ? &rec : NULL);
       if (advect & 4)
-        for (int n = 0; n < n_scalar; n++)
-          advect_field(&g, x, y, halo_column, dt_sca, &raw_u, &raw_v, &raw_w, adv[4 + n], sfields[4 + n],
-                       &fs[4 + n], 0, rec_field == 4 + n ? &rec : NULL);
+        for (int n = 4; n < 4 + n_q; n++)
+          if (!active || active[n])
+            advect_field(&g, x, y, halo_column, dt_sca, &raw_u, &raw_v, &raw_w, adv[n], sfields[n], &fs[n], 0,
+                         rec_field == n ? &rec : NULL);
+      if (advect & 8)
+        for (int n = 4 + n_q; n < nf; n++)
+          advect_field(&g, x, y, halo_column, dt_sca, &raw_u, &raw_v, &raw_w, adv[n], sfields[n], &fs[n], 0,
+                       rec_field == n ? &rec : NULL);
     }
   }
   for (int n = 0; n < nf; n++) scratch_free(&fs[n]);

@@ -2,9 +2,16 @@
 import numpy as np
 
 
-def write_input(path, state, fs, ms, ss, dtm, precision=0):
+def write_input(path, state, fs, ms, ss, dtm, precision=0, advect=(1, 1, 1), th_active=1, q_active=None):
+    """Header: local_grid%size in MONC's (Z_INDEX, Y_INDEX, X_INDEX) order,
+    number_q_fields, n_tracers, the stepping flags, precision, the three
+    advection selectors, th%active, then q(n)%active per q field."""
+    nq = state.n_q
+    qa = [1] * nq if q_active is None else [int(bool(a)) for a in q_active]
     with open(path, "wb") as f:
-        np.array([state.nz, state.ny, state.nx, state.n_q, fs, ms, ss, precision], dtype=np.int32).tofile(f)
+        np.array([state.nz, state.ny, state.nx, nq, state.n_tracers, fs, ms, ss, precision, *advect,
+                  int(bool(th_active))], dtype=np.int32).tofile(f)
+        np.array(qa, dtype=np.int32).tofile(f)
         np.array([state.cx, state.cy, dtm], dtype=np.float64).tofile(f)
         for k in ("rdz", "rdzn", "dz", "dzn", "tzc1", "tzc2", "tzd1", "tzd2"):
             np.asarray(getattr(state.grid, k), dtype=np.float64).tofile(f)

@@ -18,13 +18,17 @@ def rel_err(a: np.ndarray, b: np.ndarray) -> float:
     return num / den if den > 0 else num
 
 
-def gpu_tendencies(state, dtm, fs, ms, ss, sources=None, precision=tvd.FP64, advect=7, tiling=None):
+def gpu_tendencies(state, dtm, fs, ms, ss, sources=None, precision=tvd.FP64, advect=7, tiling=None, active=None):
     """Library timestep over a State; returns the s-fields (FP64 numpy).
-    tiling = (kernel, rows, strip) for monc_tvd_set_tiling, or None."""
+    tiling = (kernel, rows, strip) for monc_tvd_set_tiling, or None; active:
+    the th and q fields' active flags by field id (None: all active)."""
     t = tvd.TvdAdvection.initialise(state, precision=precision, advect=advect)
     try:
         if tiling is not None:
             t.set_tiling(*tiling)
+        if active is not None:
+            for f in range(3, 4 + state.n_q):
+                t.set_field_active(f, bool(active[f]))
         n = len(state.fields)
         for f in range(n):
             t.set_field(f, tvd.SOURCE, np.zeros(t.field_shape()) if sources is None else sources[f])

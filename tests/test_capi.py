@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
-def _create(nz=8, ny=6, nx=6, halo=2, n_q=1, flags=7, cx=0.02, cy=0.02, prec=0, null_metric=False):
+def _create(nz=8, ny=6, nx=6, halo=2, n_q=1, flags=7, cx=0.02, cy=0.02, prec=0, null_metric=False, n_tracers=0):
     lib = tvd.load_library()
     v = np.ones(nz)
     dp = v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
@@ -36,7 +36,7 @@ def _create(nz=8, ny=6, nx=6, halo=2, n_q=1, flags=7, cx=0.02, cy=0.02, prec=0, 
     if null_metric:
         mets[3] = ctypes.POINTER(ctypes.c_double)()
     h = ctypes.c_void_p()
-    rc = lib.monc_tvd_create(nz, ny, nx, halo, n_q, flags, cx, cy, *mets, prec, ctypes.byref(h))
+    rc = lib.monc_tvd_create(nz, ny, nx, halo, n_q, n_tracers, flags, cx, cy, *mets, prec, ctypes.byref(h))
     return rc, lib.monc_tvd_last_error().decode(), h
 
 
@@ -46,7 +46,9 @@ def _create(nz=8, ny=6, nx=6, halo=2, n_q=1, flags=7, cx=0.02, cy=0.02, prec=0, 
     (dict(nz=3), "too small"),
     (dict(ny=3), "too small"),
     (dict(prec=7), "precision"),
-    (dict(flags=8), "advect flags"),
+    (dict(flags=16), "advect flags"),
+    (dict(n_tracers=-1), "n_tracers"),
+    (dict(n_tracers=65), "n_tracers"),
     (dict(cx=0.0), "positive"),
     (dict(n_q=-1), "n_q"),
     (dict(null_metric=True), "metric vector"),
@@ -113,3 +115,34 @@ def test_ensemble_workload_sizes():
     for st in sts:
         assert st.nx in (8, 12) and st.ny in (8, 12) and st.nz in (16, 20)
         assert len(st.fields) == 5
+
+
+@pytest.mark.parametrize("kw", [dict(cx=float("inf")), dict(cy=float("nan"))])
+def test_create_rejects_non_finite_spacings(kw):
+    rc, msg, h = _create(**kw)
+    assert rc == tvd.ERR_ARG and "finite" in msg and not h.value
+
+
+def test_failed_allocation_is_a_hip_error():
+    """The allocation hook makes create's first device allocation (the metric
+    upload) fail before reaching the runtime: the call returns
+    MONC_TVD_ERR_HIP with the runtime's message and no context, and the hook
+    is consumed."""
+    lib = tvd.load_library()
+    assert lib.monc_tvd_test_fail_allocations(1) == 0
+    rc, msg, h = _create()
+    assert rc == tvd.ERR_HIP and "out of memory" in msg.lower() and not h.value
+    assert lib.monc_tvd_test_fail_allocations(0) == 0
+
+
+def test_forward_step_into_checks_its_tensors():
+    """The Python layer refuses buffers the kernels would overrun or alias,
+    before any call reaches the library (CPU tensors are refused first)."""
+    import torch
+    t = tvd.TvdAdvection.__new__(tvd.TvdAdvection)
+    t.nz, t.ny, t.nx, t.n_q, t.n_tracers, t.precision = 8, 6, 6, 1, 0, tvd.FP64
+    good = [torch.zeros(t.field_shape(), dtype=torch.float64) for _ in range(5)]
+    with pytest.raises(ValueError, match="one source"):
+        t.forward_step_into(1.0, good[:4], good)
+    with pytest.raises(ValueError, match="CUDA tensor"):
+        t.forward_step_into(1.0, good, [g.clone() for g in good])

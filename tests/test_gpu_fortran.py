@@ -15,15 +15,27 @@ pytestmark = pytest.mark.gpu
 DRIVER = Path(__file__).resolve().parents[1] / "monc_amd" / "fortran" / "_build" / "tvd_dropin_driver"
 
 
-@pytest.mark.parametrize("fs,ms,sc", [(1, 1, 1), (0, 0, 0)])
-def test_fortran_driver_matches_oracle(gpu, tmp_path, fs, ms, sc):
+def _advect_bits(flow, th, q):
+    # the shim's flags: tracers follow the theta selector (tvdadvection.F90:352)
+    return (1 if flow else 0) | ((2 | 8) if th else 0) | (4 if q else 0)
+
+
+# nz != ny != nx so that a swap of local_grid%size's Z/X entries fails
+@pytest.mark.parametrize("fs,ms,sc,selectors,th_active,q_active", [
+    (1, 1, 1, (1, 1, 1), 1, (1,)),
+    (0, 0, 0, (1, 1, 1), 1, (1,)),
+    (1, 0, 0, (1, 1, 0), 1, (1,)),   # th = tvd, q = another scheme: tracers still advected
+    (1, 1, 1, (0, 1, 1), 0, (0,)),   # inactive th and q: only the tracers
+])
+def test_fortran_driver_matches_oracle(gpu, tmp_path, fs, ms, sc, selectors, th_active, q_active):
     assert DRIVER.exists(), "Fortran driver not built (run __graft_entry__.build())"
-    st = ss.make_state(24, 16, 20, 50.0, 800.0, seed=21, n_q=1)
+    st = ss.make_state(24, 16, 20, 50.0, 800.0, seed=21, n_q=1, n_tracers=2)
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
-    write_input(inp, st, fs, ms, sc, 1.5)
+    write_input(inp, st, fs, ms, sc, 1.5, advect=selectors, th_active=th_active, q_active=q_active)
     r = subprocess.run([str(DRIVER), str(inp), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     got = read_output(out, st)
-    ref, _ = oracle.state_tendencies(st, 1.5, fs, ms, sc)
+    active = [1, 1, 1, th_active, *q_active, 1, 1]
+    ref, _ = oracle.state_tendencies(st, 1.5, fs, ms, sc, advect=_advect_bits(*selectors), active=active)
     for n, (a, b) in enumerate(zip(got, ref)):
         assert rel_err(a, b) <= 1e-12, (n, rel_err(a, b))

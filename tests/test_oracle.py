@@ -590,3 +590,217 @@ def test_leapfrog_run_conserves_scalar_mass(anel):
         m0 = mass(st.fields[n])
         assert abs(mass(f[n]) - m0) <= 1e-12 * abs(m0), n
         assert abs(mass(z[n]) - m0) <= 1e-12 * abs(m0), n
+
+
+# --------------------------------------------------------------------------
+# The vertical arms on non-uniform grids.  Every z-face check above runs on a
+# uniform dz, where the metric roles of ultflx's vertical branches
+# (ultimateflux.F90:236-523: rdu, rdd, rdc, dd, the outflow sums) and w's
+# metric swap (tvdadvection.F90:881-894) cannot be told apart.  The checks
+# below fix each role by a property of the mathematics on stretched grids:
+#
+# * linear fields: the QUICKEST curvature term (fd-fc)*rdd - (fc-fu)*rdu is
+#   the difference of the two one-sided slopes, zero for a linear field only
+#   when each reciprocal spacing pairs with its own interval; the face value is
+#   then the node average minus the half-step shift, F(mid) - W t/2 dF/dz;
+# * quadratic fields on an arithmetic grid (dz(k+1) - dz(k) constant): the
+#   (W t)^2 coefficient of the face value is W^2 F''/6, that of the exact
+#   swept average, when the curvature is normalised by the centre cell
+#   (dd^2 * rdc * (rdd^2) = 1/dz_centre); w's dual cells make it exact on any
+#   grid;
+# * Leonard's universal limiter: where its upper bound binds, the normalised
+#   face value is phi~_C / c_out, c_out the outflow Courant number of the
+#   upwind cell |W| dt / dz_centre.
+VKINDS = {"th": 3, "u": 0, "v": 1, "w": 2}
+
+
+def _vstate(z, nxy, F_of_node, kind, U, V, W, n_q=1):
+    """State with uniform flows (U, V, W) and, as the advected z-field of
+    `kind`, F evaluated at that kind's nodes (zn for th, u, v; z for w)."""
+    g = ss.vertical_grid(np.asarray(z, dtype=np.float64))
+    nz = len(z)
+    shape = (nxy + 4, nxy + 4, nz)
+    fields = [np.full(shape, U), np.full(shape, V), np.full(shape, W)]
+    fields += [np.full(shape, 1.0) for _ in range(1 + n_q)]
+    zf = [f.copy() for f in fields]
+    node = g.z if kind == 2 else g.zn
+    zf[kind] = np.ascontiguousarray(np.broadcast_to(F_of_node(node)[None, None, :], shape))
+    c = ss.horizontal_c(50.0)
+    return ss.State(nz=nz, ny=nxy, nx=nxy, cx=c, cy=c, grid=g, fields=fields, zfields=zf)
+
+
+def _zfaces(st, kind, dtm, stepping):
+    """flux_z of `kind` (index m = flux_z(m+1)) with the z-fields advected."""
+    _, faces = tend(st, dtm=dtm, fs=CS, ms=stepping, sc=stepping, rec_field=kind)
+    return faces[2][2:-2, 2:-2, :]
+
+
+def check_linear_z(kind, sgn, stepping):
+    nz = 24
+    z = ss.tanh_stretched_z(nz, 575.0)
+    c0, e, W, dtm = 1.5, 0.02, 0.4 * sgn, 1.0
+    st = _vstate(z, 5, lambda h: c0 + e * h, kind, 0.8, -0.6, W)
+    fz = _zfaces(st, kind, dtm, stepping)
+    t = dtm if stepping == FS else 2.0 * dtm
+    g = st.grid
+    node = g.z if kind == 2 else g.zn
+    for m in range(1, nz):  # flux_z(m+1): the top face of level k = m
+        if kind != 2 and m == 1:
+            exp = 0.0  # flux_z(2) = 0 for scalars, u and v (:78)
+        elif kind == 2 and m == 1 and W > 0:
+            exp = c0 + e * node[0]  # the w bottom face upwinds: fu = fc = F(1)
+        else:
+            exp = c0 + e * (0.5 * (node[m - 1] + node[m]) - 0.5 * W * t)
+        got = fz[:, :, m]
+        assert np.max(np.abs(got - exp)) <= 1e-12 * 20.0, (kind, sgn, stepping, m, float(got.flat[0]), exp)
+
+
+def check_quadratic_t2(kind, sgn):
+    nz, d0, dd = 20, 10.0, 1.5
+    z = np.concatenate([[0.0], np.cumsum(d0 + dd * np.arange(nz - 1))])  # dz(k) = d0 + (k-2) dd
+    z0, W, h = 300.0, 0.4 * sgn, 0.5
+    vals = []
+    for t in (1.0, 1.0 + h, 1.0 + 2 * h):
+        st = _vstate(z, 4, lambda q: (q + z0) ** 2, kind, 0.0, 0.0, W)
+        vals.append(_zfaces(st, kind, t, FS))
+    c2 = (vals[0] - 2 * vals[1] + vals[2]) / (2 * h * h)  # coefficient of t^2
+    want = W * W / 3.0  # W^2 F''/6 with F'' = 2
+    for m in range(1, nz):
+        k = m
+        if kind != 2:
+            if m == 1:
+                continue  # flux_z(2) = 0
+            if W > 0 and k == 2:
+                continue  # reads F(1) at zn(1) = -zn(2), off the arithmetic progression
+            exp = 0.0 if (W < 0 and k == nz - 1) else want  # top W < 0 extrapolates linearly
+        else:
+            exp = 0.0 if ((W > 0 and k == 1) or (W < 0 and k == nz - 1)) else want
+        got = c2[:, :, m]
+        assert np.max(np.abs(got - exp)) <= 1e-6 * want, (kind, sgn, m, float(got.flat[0]), exp)
+
+
+def check_courant_bound(sgn):
+    """Scalar faces where Leonard's upper bound phi~_C / c_out binds: each
+    interior column holds a step profile around one target face, so that
+    phi~_C is small (W > 0, W < 0 middle and top W > 0) or 1/2 (top W < 0,
+    the linear extrapolation), with U = V = 0."""
+    dzs = [20.0, 35.0, 12.0, 28.0, 40.0, 15.0, 30.0, 22.0, 40.0, 10.0]  # dz(2..nz); thin top layer
+    z = np.concatenate([[0.0], np.cumsum(dzs)])
+    nz = len(z)
+    nxy = 4
+    W, dtm, eps = 1.0 * sgn, 9.0, 0.02
+    g = ss.vertical_grid(z)
+    shape = (nxy + 4, nxy + 4, nz)
+    F = np.zeros(shape)
+    targets = {}
+    cols = [(i, j) for i in range(2, nxy + 2) for j in range(2, nxy + 2)]
+    for n, k in enumerate(range(2, nz)):  # the face above level k, k = 2..nz-1
+        i, j = cols[n]
+        prof = np.zeros(nz)
+        kc = k if W > 0 else k + 1  # upwind (centre) level
+        if W < 0 and k == nz - 1:
+            prof[:] = np.linspace(3.0, 1.0, nz)  # any monotone profile: phi~_C = 1/2
+            a, b = prof[nz - 1], prof[nz - 2]
+            fu, fc = 2 * a - b, a
+        else:
+            up = k - 1 if W > 0 else k + 2
+            prof[:] = 1.0
+            if W > 0:
+                prof[:k - 1] = 0.0
+            else:
+                prof[k + 1:] = 0.0
+            prof[kc - 1] = eps
+            fu, fc = prof[up - 1], eps
+        cout = abs(W) * dtm / g.dz[kc - 1]
+        targets[k] = (i, j, fu + (fc - fu) / cout)
+        F[i, j, :] = prof
+    ss.halo_fill(F)
+    zero = np.zeros(shape)
+    st = ss.State(nz=nz, ny=nxy, nx=nxy, cx=0.02, cy=0.02, grid=g,
+                  fields=[zero, zero.copy(), np.full(shape, W), F, F.copy()], zfields=[])
+    _, faces = tend(st, dtm=dtm, rec_field=3)
+    for k, (i, j, exp) in targets.items():
+        got = faces[2][i, j, k]  # flux_z(k+1)
+        assert abs(got - exp) <= 1e-12 * max(1.0, abs(exp)), (sgn, k, got, exp)
+
+
+@pytest.mark.parametrize("stepping", [FS, CS])
+@pytest.mark.parametrize("sgn", [1.0, -1.0])
+@pytest.mark.parametrize("kind", list(VKINDS))
+def test_linear_field_vertical_faces_on_stretched_grid(kind, sgn, stepping):
+    check_linear_z(VKINDS[kind], sgn, stepping)
+
+
+@pytest.mark.parametrize("sgn", [1.0, -1.0])
+@pytest.mark.parametrize("kind", list(VKINDS))
+def test_quadratic_field_curvature_coefficient(kind, sgn):
+    check_quadratic_t2(VKINDS[kind], sgn)
+
+
+@pytest.mark.parametrize("sgn", [1.0, -1.0])
+def test_universal_limiter_courant_bound(sgn):
+    check_courant_bound(sgn)
+
+
+# Deliberate restatement mistakes (PLANT in tvd_oracle.c) and what each is:
+PLANTED = {
+    1: "middle face: rdu takes rdzn(k+2) for W >= 0 and rdzn(k) for W < 0",
+    2: "top face, W < 0: rdu = rdzn(nz-1)",
+    3: "w: the metric swap (rdz <-> rdzn, dzn -> dz) dropped",
+    4: "middle face: rdc branches exchanged",
+    5: "limiter: dd = dzn(k) instead of dzn(k+1)",
+    6: "w bottom face, W < 0: rdu = rdzn(k+1)",
+    7: "middle face: outflow sum always at level k",
+    8: "top face, W < 0: outflow sum at rdz(nz-1)",
+    9: "w bottom face, W < 0: rdc = rdz(k+kdof)",
+}
+
+
+def _all_vertical_checks():
+    for kind in VKINDS.values():
+        for sgn in (1.0, -1.0):
+            yield lambda kind=kind, sgn=sgn: check_linear_z(kind, sgn, FS)
+            yield lambda kind=kind, sgn=sgn: check_quadratic_t2(kind, sgn)
+    for sgn in (1.0, -1.0):
+        yield lambda sgn=sgn: check_courant_bound(sgn)
+
+
+def test_planted_vertical_mistakes_are_caught(tmp_path):
+    """Each mistake of PLANTED, compiled into its own build of the
+    restatement, fails at least one of the checks above (which all pass on the
+    oracle proper)."""
+    for plant, what in PLANTED.items():
+        path = oracle.build_variant(plant, tmp_path)
+        failed = 0
+        with oracle.use_library(path):
+            for chk in _all_vertical_checks():
+                try:
+                    chk()
+                except AssertionError:
+                    failed += 1
+        assert failed > 0, f"planted mistake {plant} ({what}) passes every vertical check"
+
+
+# --------------------------------------------------------------------------
+# Field selection (tvdadvection.F90:591-594, :661, :684-694, :706)
+def test_tracers_follow_the_scalar_arithmetic_and_the_selectors():
+    st = ss.make_state(12, 6, 7, 50.0, 275.0, seed=30, n_q=1, n_tracers=2)
+    st.fields[5] = st.fields[4].copy()  # tracer 1 holds q(1)'s data
+    st.zfields[5] = st.zfields[4].copy()
+    s, _ = tend(st, advect=15)
+    assert np.array_equal(s[5], s[4]) and np.abs(inner(s[6])).max() > 0
+    s7, _ = tend(st, advect=7)  # no tracer bit: tracers untouched
+    assert np.all(s7[5] == 0) and np.all(s7[6] == 0)
+    np.testing.assert_array_equal(s7[4], s[4])
+    s8, _ = tend(st, advect=8)  # tracers alone
+    np.testing.assert_array_equal(s8[5], s[5])
+    assert all(np.all(s8[n] == 0) for n in range(5))
+
+
+def test_inactive_theta_and_q_are_not_advected():
+    st = ss.make_state(12, 6, 7, 50.0, 275.0, seed=31, n_q=2)
+    full, _ = tend(st)
+    part, _ = tend(st, active=[1, 1, 1, 0, 1, 0])
+    assert np.all(part[3] == 0) and np.all(part[5] == 0)
+    for n in (0, 1, 2, 4):
+        np.testing.assert_array_equal(part[n], full[n])

@@ -142,7 +142,10 @@ def test_overwritten_and_accumulated_sources():
 # The HIP kernel through the C ABI
 
 # (nz, ny, nx, seed, anelastic scale or None, n_q)
-GRIDS = [(16, 12, 12, 0, None, 1), (33, 10, 6, 1, 2000.0, 2), (70, 8, 9, 2, None, 1), (130, 5, 7, 3, 1500.0, 3)]
+# nx = 37 and 32 make the kernel march three strips of 16 columns with a
+# partial last strip, and two exact strips (seeding of later strips)
+GRIDS = [(16, 12, 12, 0, None, 1), (33, 10, 6, 1, 2000.0, 2), (70, 8, 9, 2, None, 1), (130, 5, 7, 3, 1500.0, 3),
+         (24, 6, 37, 4, 3000.0, 1), (20, 7, 32, 5, None, 2)]
 
 
 def _gpu_case(case, flags, precision=None):
@@ -181,9 +184,37 @@ def test_pw_matches_oracle(gpu, case, flags):
 
 
 @pytest.mark.gpu
-def test_pw_fp32_vs_fp64(gpu):
+@pytest.mark.parametrize("case", [GRIDS[1], GRIDS[4]], ids=["6x10x33", "37x6x24"])
+def test_pw_fp32_vs_fp64(gpu, case):
     from monc_amd import tvd
-    got, ref, before = _gpu_case(GRIDS[1], (True, True, True), precision=tvd.FP32)
+    got, ref, before = _gpu_case(case, (True, True, True), precision=tvd.FP32)
     for n in range(len(got)):
         l2 = float(np.linalg.norm(got[n] - ref[n]) / np.linalg.norm(ref[n]))
         assert l2 <= 1e-5, (n, l2)
+
+
+@pytest.mark.gpu
+def test_pw_skips_tracers_and_inactive_theta(gpu):
+    """pw_advection's q loop covers the q fields only (pwadvection.F90:333)
+    and th only when active (:403): tracers and an inactive th keep their
+    sources."""
+    from monc_amd import tvd
+    st = ss.make_state(20, 8, 10, 50.0, 475.0, seed=9, n_q=1, n_tracers=1)
+    rng = np.random.default_rng(9)
+    src = [rng.standard_normal(f.shape) * 1e-2 for f in st.fields]
+    tcx, tcy = _tc(st)
+    ref = oracle.pw(st.fields[:5], src[:5], st.grid, st.cx, st.cy, tcx, tcy, True, False, True)
+    t = tvd.TvdAdvection.initialise(st)
+    try:
+        t.set_field_active(tvd.TH, False)
+        for n in range(len(st.fields)):
+            t.set_field(n, tvd.SOURCE, src[n])
+        t.pw_tendencies(True, True, True, tcx=tcx, tcy=tcy)
+        got = [t.get_field(n, tvd.SOURCE) for n in range(len(st.fields))]
+    finally:
+        t.close()
+    np.testing.assert_array_equal(got[3], src[3])
+    np.testing.assert_array_equal(got[5], src[5])
+    for n in (0, 1, 2, 4):
+        err = float(np.max(np.abs(got[n] - ref[n]))) / float(np.max(np.abs(ref[n])))
+        assert err <= 1e-12, (n, err)
