@@ -255,6 +255,16 @@ int monc_tvd_leapfrog_steps(monc_tvd_ctx *ctx, double dtm, double tsmth, double 
  * reference has no global diagnostic of this path.  Synchronous. */
 int monc_tvd_weighted_sum(monc_tvd_ctx *ctx, int field, int variant, const double *weights, double *out);
 
+/* monc_tvd_weighted_sum without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The same
+ * kernels and the same result, bit for bit; the weights, partial sums and
+ * result live in a fixed scratch array in device memory, one per GPU, held
+ * until the process ends, sized for nz <= 4096 (larger nz: MONC_TVD_ERR_ARG).
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (other host threads or streams) race.  Synchronous. */
+int monc_tvd_weighted_sum_noalloc(monc_tvd_ctx *ctx, int field, int variant, const double *weights, double *out);
+
 /*
  * Adjoint of one monc_tvd_forward_steps step (no counterpart in the
  * reference: the differentiable form of the path).  With the prognostics

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1042,6 +1043,34 @@ int weighted_sum_on(monc_tvd_ctx* c, const void* field, const double* weights, d
   return MONC_TVD_OK;
 }
 
+// Scratch of monc_tvd_weighted_sum_noalloc: a file-scope device array (one
+// copy per GPU, held until the process ends) of weights, partials and the
+// result for columns of up to WSUM_NOALLOC_MAX_NZ levels.  Host code never
+// passes the array's name to a kernel or copy: it is a host-side stand-in.
+// Each device's own address is taken once, with that device current, by
+// hipGetSymbolAddress and kept in wsum_noalloc_ptr.
+constexpr int WSUM_NOALLOC_MAX_NZ = 4096;
+constexpr int WSUM_NOALLOC_LEN = WSUM_NOALLOC_MAX_NZ + WSUM_BLOCKS + 1;
+static __device__ __align__(256) double scratch[WSUM_NOALLOC_LEN];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+int wsum_noalloc_ptr(double** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(MONC_TVD_ERR_ARG, "device %d beyond the %d supported", dev, MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MONC_TVD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1221,6 +1250,22 @@ int monc_tvd_weighted_sum(monc_tvd_ctx* c, int field, int variant, const double*
   // scratch of the context, allocated on the first call and kept
   if (!c->wsum_scratch) HIP_TRY(dev_alloc(&c->wsum_scratch, sizeof(double) * wsum_scratch_len(c->nz)));
   return weighted_sum_on(c, slot(c, field, variant), weights, c->wsum_scratch, out);
+}
+
+int monc_tvd_weighted_sum_noalloc(monc_tvd_ctx* c, int field, int variant, const double* weights, double* out) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_fv(c, field, variant)) return rc;
+  if (!weights || !out) return fail(MONC_TVD_ERR_ARG, "null weights or output");
+  if (c->nz > WSUM_NOALLOC_MAX_NZ)
+    return fail(MONC_TVD_ERR_ARG, "nz = %d exceeds the %d levels of the fixed scratch of the no-allocation sum", c->nz,
+                WSUM_NOALLOC_MAX_NZ);
+  if (int rc = need(c, field, variant)) return rc;
+  double* ptr = nullptr;
+  if (int rc = wsum_noalloc_ptr(&ptr)) return rc;
+  // every word of the scratch a call reads is written earlier in the same call
+  // (weights by the copy, partials by the first kernel, the result by the
+  // second): nothing needs clearing between calls
+  return weighted_sum_on(c, slot(c, field, variant), weights, ptr, out);
 }
 
 int monc_tvd_set_profiling(monc_tvd_ctx* c, int on) {

@@ -37,6 +37,7 @@ EXPORTS = (
     "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic", "monc_tvd_autotune",
     "monc_tvd_leapfrog_steps", "monc_tvd_forward_step_into", "monc_thadv_tendencies",
     "monc_pw_tendencies", "monc_tvd_set_field_active", "monc_tvd_test_fail_allocations",
+    "monc_tvd_weighted_sum_noalloc",
     "monc_tvd_ensemble_create", "monc_tvd_ensemble_destroy", "monc_tvd_ensemble_layout",
     "monc_tvd_ensemble_set_field", "monc_tvd_ensemble_get_field", "monc_tvd_ensemble_copy_pool",
     "monc_tvd_ensemble_set_dt", "monc_tvd_ensemble_set_stream", "monc_tvd_ensemble_synchronize",
@@ -93,6 +94,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_get_diagnostic.argtypes = [c_void_p, c_int, c_int, dp]
     lib.monc_tvd_set_profiling.argtypes = [c_void_p, c_int]
     lib.monc_tvd_weighted_sum.argtypes = [c_void_p, c_int, c_int, dp, dp]
+    lib.monc_tvd_weighted_sum_noalloc.argtypes = [c_void_p, c_int, c_int, dp, dp]
     lib.monc_tvd_kernel_times.argtypes = [c_void_p, ctypes.POINTER(c_double), ctypes.POINTER(ctypes.c_longlong),
                                           c_int, c_int]
     lib.monc_tvd_synchronize.argtypes = [c_void_p]
@@ -282,6 +284,16 @@ class TvdAdvection:
             raise ValueError("weights must have length nz")
         out = np.zeros(1)
         _check(self._lib.monc_tvd_weighted_sum(self._h, field, variant, _dptr(w), _dptr(out)))
+        return float(out[0])
+
+    def weighted_sum_noalloc(self, field: int, variant: int, weights: np.ndarray) -> float:
+        """weighted_sum through the entry point that allocates no device
+        memory (fixed per-GPU scratch, nz <= 4096)."""
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        if w.shape != (self.nz,):
+            raise ValueError("weights must have length nz")
+        out = np.zeros(1)
+        _check(self._lib.monc_tvd_weighted_sum_noalloc(self._h, field, variant, _dptr(w), _dptr(out)))
         return float(out[0])
 
     def set_profiling(self, on: bool) -> None:
