@@ -120,17 +120,27 @@ int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **
  * default for the column height; otherwise one of the shapes listed by
  * monc_tvd_march_shapes), strip = x columns per block (0: 32).
  * MONC_TVD_KERNEL_CELL: one thread per cell, every face recomputed (any
- * column height; the fallback beyond 256 levels or 4 GiB per field). */
+ * column height; the fallback beyond 256 levels or 4 GiB per field).
+ * MONC_TVD_KERNEL_STAGE: the forward step (monc_tvd_forward_steps,
+ * monc_tvd_forward_step_into) of each field kind with u, v, w (and the
+ * kind's scalars) staged in LDS; rows = cell rows per block (4, 6, 8; 11 in
+ * FP64, where the scalar pair takes 4, 6 or 8; 0: the default), strip as
+ * above.  Other calls use the march kernel.  Without a tiling set (or after
+ * kernel 0 with rows = strip = 0) the forward step stages FP64 u, v and w
+ * where the domain gives at least 512 staged blocks, and marches the rest. */
 #define MONC_TVD_KERNEL_MARCH 0
 #define MONC_TVD_KERNEL_CELL 1
+#define MONC_TVD_KERNEL_STAGE 2
 int monc_tvd_set_tiling(monc_tvd_ctx *ctx, int kernel, int rows, int strip);
 /* Tile-shape autotuner (no counterpart in the reference).  For each kernel
  * of the forward step (u, v, w, the scalar group) it times every compiled
- * march shape and strip length (8, 16, 32, 64) on the context's own fields,
- * reps launches each after one warm-up, writing into library scratch (the
- * state is not advanced), and keeps the fastest per kernel.  chosen (8 ints,
- * may be null) receives rows and strip per kernel in the order u, v, w,
- * scalars.  Needs the prognostics of u, v, w and the advected fields. */
+ * march shape and strip length (8, 16, 32, 64) and every stage shape (rows
+ * 4, 6, 8 x strips 16, 32, 64) on the context's own fields, reps launches
+ * each after one warm-up, writing into library scratch (the state is not
+ * advanced), and keeps the fastest kernel and shape per field kind.  chosen
+ * (12 ints, may be null) receives kernel, rows and strip per kind in the
+ * order u, v, w, scalars.  Needs the prognostics of u, v, w and the advected
+ * fields. */
 int monc_tvd_autotune(monc_tvd_ctx *ctx, double dtm, int reps, int *chosen);
 
 /* The march shapes (cell rows per block) compiled for this context's column
@@ -177,9 +187,10 @@ int monc_tvd_get_diagnostic(monc_tvd_ctx *ctx, int field, int which, double *hos
  * stream; monc_tvd_kernel_times synchronises, folds the events into per-kind
  * totals and returns the first n of them: milliseconds and launch counts for
  * kinds 0 = u, 1 = v, 2 = w, 3 = scalars, 4 = whole-field kernels (halo
- * fill, th_advection), 5 = one-cell-per-thread kernel (all fields).
+ * fill, th_advection), 5 = one-cell-per-thread kernel (all fields), 6 = the
+ * fused stage kernel (all fields).
  * reset != 0 clears the totals afterwards. */
-#define MONC_TVD_NKERNEL_KINDS 6
+#define MONC_TVD_NKERNEL_KINDS 7
 int monc_tvd_set_profiling(monc_tvd_ctx *ctx, int on);
 int monc_tvd_kernel_times(monc_tvd_ctx *ctx, double *ms, long long *launches, int n, int reset);
 

@@ -20,6 +20,7 @@
 #include "tvd_host.hpp"
 #include "tvd_kernels.hpp"
 #include "tvd_march.hpp"
+#include "tvd_stage.hpp"
 
 using namespace tvd;
 
@@ -79,9 +80,10 @@ struct monc_tvd_ctx {
   std::vector<void*> adj_tmp;  // copy of the incoming adjoint of each field (adjoint step)
   std::vector<void*> dual[2];  // packed (value, tangent) state, double-buffered (tangent steps)
   hipStream_t stream;
-  int kernel;                  // MONC_TVD_KERNEL_MARCH or _CELL
-  int rows[4], strip[4];       // march tile shape per kind (u, v, w, scalars): cell rows and x columns
+  int kernel;                  // MONC_TVD_KERNEL_MARCH, _CELL or _STAGE
+  int rows[4], strip[4];       // tile shape per kind (u, v, w, scalars): cell rows and x columns
                                // per block (0: defaults)
+  int kind_kernel[4];          // per kind: MONC_TVD_KERNEL_MARCH or _STAGE (forward step)
   int buffer_ok;               // fields small enough for 32-bit buffer offsets
   // optional per-kernel timing: HIP events recorded around every launch on
   // the context stream, folded into per-kind totals on query
@@ -260,6 +262,93 @@ void launch_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
 // Launches over a field set: per field kind (u, v, w, scalars in pairs) the
 // slab kernel for columns of up to 256 levels; the one-cell-per-thread kernel
 // (all fields in one launch) beyond that or when forced.
+// Stage kernel (MONC_TVD_KERNEL_STAGE): the step of one field kind with u, v,
+// w (and the kind's scalars) staged in LDS, over 64-level bands x R rows x x
+// strips.  The register budget (waves per SIMD the compiler plans for) is set
+// by the kind's demand, not by the LDS: FP32 fits 128 registers (4 waves),
+// FP64 168 (3 waves) and the FP64 scalar pair 256 (2 waves); a smaller budget
+// spills.
+template <typename T, int KIND, int NS, int R>
+constexpr int stage_wpe() {
+  return sizeof(T) == 4 ? 4 : (KIND == KIND_SCALAR && NS == 2 ? 2 : 3);
+}
+
+template <typename T, int KIND, int NS, int R>
+void launch_stage(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G, int strip) {
+  const int nbands = (c->nz + 63) / 64, tiles_y = (c->ny + R - 1) / R, tiles_x = (c->nx + strip - 1) / strip;
+  hipLaunchKernelGGL((stage_kernel<T, KIND, NS, R, stage_wpe<T, KIND, NS, R>()>), dim3(nbands * tiles_y * tiles_x),
+                     dim3(64, R + 1, 1), 0, c->stream, L, G, strip, nbands, tiles_y);
+}
+
+// Rows per block of the stage kernel (11 only for the FP64 flow kinds).
+constexpr int STAGE_ROWS[4] = {4, 6, 8, 11};
+
+bool stage_shape_compiled(int precision, int kind, int rows) {
+  return rows != 11 || (precision == MONC_TVD_FP64 && kind != KIND_SCALAR);
+}
+
+// Default kernel per field kind for the forward step, from timings at
+// 512x512x256 on MI355X: FP64 u, v and w staged in 11-row tiles of 128
+// columns (1.67-1.71 ms per launch against 2.03-2.14 ms marching); the FP64
+// scalar pair and every FP32 kind march (their staged forms measured slower).
+// Small domains keep the march kernel: the staged form runs one block per CU,
+// so it needs at least two blocks per CU (512) to fill the chip; the strip is
+// the longest of 128, 64 and 32 columns that still gives them.  Returns the
+// strip, or 0 for the march kernel.
+int stage_default_strip(const monc_tvd_ctx* c, int kind) {
+  if (c->precision != MONC_TVD_FP64 || kind == KIND_SCALAR) return 0;
+  const long tiles = (long)((c->nz + 63) / 64) * ((c->ny + 10) / 11);
+  for (int strip : {128, 64, 32})
+    if (tiles * ((c->nx + strip - 1) / strip) >= 512) return strip;
+  return 0;
+}
+
+bool stage_rows_ok(int rows) { return rows == 4 || rows == 6 || rows == 8 || rows == 11; }
+
+// A field group the stage kernel takes: MODE_STEP, a flow field advected by
+// itself or one or two scalars with the same timestep, fields under 4 GiB.
+template <typename T>
+bool stage_fits(const monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G) {
+  if (!c->buffer_ok || G.nf < 1 || G.nf > 2) return false;
+  for (int n = 0; n < G.nf; n++)
+    if (G.f[n] != G.F[n] || G.dt[n] != G.dt[0] || G.kind[n] != G.kind[0]) return false;
+  switch (G.kind[0]) {
+    case KIND_U: return G.nf == 1 && G.F[0] == L.u;
+    case KIND_V: return G.nf == 1 && G.F[0] == L.v;
+    case KIND_W: return G.nf == 1 && G.F[0] == L.w;
+    default: return true;
+  }
+}
+
+template <typename T, int KIND, int NS>
+void launch_stage_rows(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G, int rows, int strip) {
+  switch (rows) {
+    case 6: launch_stage<T, KIND, NS, 6>(c, L, G, strip); break;
+    case 8: launch_stage<T, KIND, NS, 8>(c, L, G, strip); break;
+    case 11:
+      if constexpr (sizeof(T) == 8 && KIND != KIND_SCALAR) {
+        launch_stage<T, KIND, NS, 11>(c, L, G, strip);
+        break;
+      }
+      [[fallthrough]];
+    default: launch_stage<T, KIND, NS, 4>(c, L, G, strip); break;
+  }
+}
+
+template <typename T>
+void launch_stage_group(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G, int kk, int rows,
+                        int strip) {
+  switch (G.kind[0]) {
+    case KIND_U: launch_stage_rows<T, KIND_U, 0>(c, L, G, rows, strip); break;
+    case KIND_V: launch_stage_rows<T, KIND_V, 0>(c, L, G, rows, strip); break;
+    case KIND_W: launch_stage_rows<T, KIND_W, 0>(c, L, G, rows, strip); break;
+    default:
+      if (G.nf == 2) launch_stage_rows<T, KIND_SCALAR, 2>(c, L, G, rows, strip);
+      else launch_stage_rows<T, KIND_SCALAR, 1>(c, L, G, rows, strip);
+  }
+  (void)kk;
+}
+
 template <typename T, int MODE>
 int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
   const int nch = (c->nz + 63) / 64;
@@ -301,6 +390,19 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
     const bool wide = c->precision == MONC_TVD_FP64 && kk == 3 && nch == 4 && rows <= 2;
     const int strip = c->strip[kk] > 0 && !leap ? c->strip[kk] : default_strip(c->ny, c->nx, rows, wide);
     LaunchTimer timer(c, kk);
+    if constexpr (MODE == MODE_STEP) {
+      const int dstrip = stage_default_strip(c, S.kind[n]);
+      const bool dflt = c->kind_kernel[kk] == MONC_TVD_KERNEL_MARCH && !c->rows[kk] && !c->strip[kk] &&
+                        c->kernel == MONC_TVD_KERNEL_MARCH && dstrip > 0;
+      if ((c->kind_kernel[kk] == MONC_TVD_KERNEL_STAGE || dflt) && stage_fits<T>(c, L, G)) {
+        const int srows = dflt ? 11 : (c->rows[kk] > 0 ? c->rows[kk] : 6);
+        const int sstrip = dflt ? dstrip : (c->strip[kk] > 0 ? c->strip[kk] : 64);
+        launch_stage_group<T>(c, L, G, kk, srows, sstrip);
+        HIP_TRY(hipGetLastError());
+        n += G.nf;
+        continue;
+      }
+    }
     switch (S.kind[n]) {
       case KIND_U: launch_shape<T, KIND_U, 1, MODE>(c, L, G, nch, kernel, rows, strip); break;
       case KIND_V: launch_shape<T, KIND_V, 1, MODE>(c, L, G, nch, kernel, rows, strip); break;
@@ -810,7 +912,30 @@ int autotune_impl(monc_tvd_ctx* c, double dtm, int reps, int* chosen) {
     }
     const int kk = kind == KIND_SCALAR ? 3 : kind;
     float best = 1e30f;
-    int best_rows = 0, best_strip = 0;
+    int best_rows = 0, best_strip = 0, best_kernel = MONC_TVD_KERNEL_MARCH;
+    // the stage kernel's shapes where it takes this field group
+    if (stage_fits<T>(c, L, G)) {
+      for (int rows : STAGE_ROWS) {
+        if (rows == 4 || !stage_shape_compiled(c->precision, kind, rows)) continue;
+        for (int strip : {32, 64, 128}) {
+          float ms = 0.f;
+          for (int it = 0; it <= reps; it++) {
+            if (it == 1) HIP_TRY(hipEventRecord(a, c->stream));
+            launch_stage_group<T>(c, L, G, kk, rows, strip);
+            HIP_TRY(hipGetLastError());
+          }
+          HIP_TRY(hipEventRecord(b, c->stream));
+          HIP_TRY(hipEventSynchronize(b));
+          HIP_TRY(hipEventElapsedTime(&ms, a, b));
+          if (ms < best) {
+            best = ms;
+            best_rows = rows;
+            best_strip = strip;
+            best_kernel = MONC_TVD_KERNEL_STAGE;
+          }
+        }
+      }
+    }
     for (int rows : MARCH_ROWS[nch - 1]) {
       if (!rows) continue;
       for (int strip : strips) {
@@ -834,11 +959,13 @@ int autotune_impl(monc_tvd_ctx* c, double dtm, int reps, int* chosen) {
           best = ms;
           best_rows = rows;
           best_strip = strip;
+          best_kernel = MONC_TVD_KERNEL_MARCH;
         }
       }
     }
     c->rows[kk] = best_rows;
     c->strip[kk] = best_strip;
+    c->kind_kernel[kk] = best_kernel;
     n += G.nf;
   }
   (void)hipEventDestroy(a);
@@ -846,8 +973,9 @@ int autotune_impl(monc_tvd_ctx* c, double dtm, int reps, int* chosen) {
   c->kernel = MONC_TVD_KERNEL_MARCH;
   if (chosen)
     for (int q = 0; q < 4; q++) {
-      chosen[2 * q] = c->rows[q];
-      chosen[2 * q + 1] = c->strip[q];
+      chosen[3 * q] = c->kind_kernel[q];
+      chosen[3 * q + 1] = c->rows[q];
+      chosen[3 * q + 2] = c->strip[q];
     }
   return MONC_TVD_OK;
 }
@@ -1130,7 +1258,10 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int n_tracers, in
   c->stream = nullptr;
   c->dmet = nullptr;
   c->kernel = MONC_TVD_KERNEL_MARCH;
-  for (int q = 0; q < 4; q++) c->rows[q] = c->strip[q] = 0;
+  for (int q = 0; q < 4; q++) {
+    c->rows[q] = c->strip[q] = 0;
+    c->kind_kernel[q] = MONC_TVD_KERNEL_MARCH;
+  }
   c->buffer_ok = n_elem * c->elem_size < ((size_t)1 << 32) - ((size_t)1 << 20);
   c->profiling = 0;
   c->diag_flags = 0;
@@ -1213,17 +1344,22 @@ int monc_tvd_destroy(monc_tvd_ctx* c) {
 
 int monc_tvd_set_tiling(monc_tvd_ctx* c, int kernel, int rows, int strip) {
   if (int rc = check_ctx(c)) return rc;
-  if (kernel != MONC_TVD_KERNEL_MARCH && kernel != MONC_TVD_KERNEL_CELL)
-    return fail(MONC_TVD_ERR_ARG, "kernel %d is not 0 (march) or 1 (cell)", kernel);
+  if (kernel != MONC_TVD_KERNEL_MARCH && kernel != MONC_TVD_KERNEL_CELL && kernel != MONC_TVD_KERNEL_STAGE)
+    return fail(MONC_TVD_ERR_ARG, "kernel %d is not 0 (march), 1 (cell) or 2 (stage)", kernel);
+  if (kernel == MONC_TVD_KERNEL_STAGE && rows != 0 && !stage_rows_ok(rows))
+    return fail(MONC_TVD_ERR_ARG, "no stage shape with %d rows (4, 6, 8 or 11)", rows);
+  if (kernel == MONC_TVD_KERNEL_STAGE && rows == 11 && c->precision != MONC_TVD_FP64)
+    return fail(MONC_TVD_ERR_ARG, "the 11-row stage shape exists in FP64 only");
   if (strip < 0 || strip > 4096) return fail(MONC_TVD_ERR_ARG, "strip %d out of range [0,4096]", strip);
   const int nch = (c->nz + 63) / 64;
   if (kernel == MONC_TVD_KERNEL_MARCH && rows != 0 && !march_rows_ok(nch, rows))
     return fail(MONC_TVD_ERR_ARG, "no march shape with %d rows for %d-level columns", rows, c->nz);
   if (rows < 0) return fail(MONC_TVD_ERR_ARG, "rows %d < 0", rows);
-  c->kernel = kernel;
+  c->kernel = kernel == MONC_TVD_KERNEL_CELL ? MONC_TVD_KERNEL_CELL : MONC_TVD_KERNEL_MARCH;
   for (int q = 0; q < 4; q++) {
-    c->rows[q] = kernel == MONC_TVD_KERNEL_MARCH ? rows : 0;
+    c->rows[q] = kernel == MONC_TVD_KERNEL_CELL ? 0 : rows;
     c->strip[q] = strip;
+    c->kind_kernel[q] = kernel == MONC_TVD_KERNEL_STAGE ? MONC_TVD_KERNEL_STAGE : MONC_TVD_KERNEL_MARCH;
   }
   return MONC_TVD_OK;
 }

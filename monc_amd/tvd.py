@@ -46,8 +46,8 @@ EXPORTS = (
 )
 DIAG_TENDENCY, DIAG_ZFACE = 1, 2
 DIAG_3D, DIAG_PROFILE, DIAG_TABS_3D, DIAG_TABS_PROFILE, DIAG_ZFACES = 0, 1, 2, 3, 4
-KERNEL_MARCH, KERNEL_CELL = 0, 1
-KERNEL_KINDS = ("u", "v", "w", "scalars", "halo", "cell")
+KERNEL_MARCH, KERNEL_CELL, KERNEL_STAGE = 0, 1, 2
+KERNEL_KINDS = ("u", "v", "w", "scalars", "halo", "cell", "stage")
 
 
 def Q(n: int) -> int:
@@ -240,11 +240,12 @@ class TvdAdvection:
         _check(self._lib.monc_tvd_set_tiling(self._h, kernel, rows, strip))
 
     def autotune(self, dtm: float, reps: int = 3) -> dict:
-        """Time every compiled march shape and strip per kernel kind on this
-        context's fields and keep the fastest; returns {kind: (rows, strip)}."""
-        out = (ctypes.c_int * 8)()
+        """Time every compiled march and stage shape per field kind on this
+        context's fields and keep the fastest; returns {kind: (kernel, rows,
+        strip)} with kernel KERNEL_MARCH or KERNEL_STAGE."""
+        out = (ctypes.c_int * 12)()
         _check(self._lib.monc_tvd_autotune(self._h, dtm, reps, out))
-        return {k: (out[2 * i], out[2 * i + 1]) for i, k in enumerate(("u", "v", "w", "scalars"))}
+        return {k: (out[3 * i], out[3 * i + 1], out[3 * i + 2]) for i, k in enumerate(("u", "v", "w", "scalars"))}
 
     def march_shapes(self) -> list:
         """Cell rows per block of the march shapes compiled for this column height, default first."""

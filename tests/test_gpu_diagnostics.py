@@ -94,7 +94,9 @@ def test_diagnostics_need_recording(gpu):
             t.get_diagnostic(3, tvd.DIAG_3D)
         with pytest.raises(tvd.TvdError):
             t.set_diagnostics(8)
-        with pytest.raises(tvd.TvdError):  # the split kernel no longer exists
-            t.set_tiling(2, 0, 0)
+        with pytest.raises(tvd.TvdError):  # no kernel 3
+            t.set_tiling(3, 0, 0)
+        with pytest.raises(tvd.TvdError):  # no 5-row stage shape
+            t.set_tiling(tvd.KERNEL_STAGE, 5, 0)
     finally:
         t.close()

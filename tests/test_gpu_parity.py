@@ -181,8 +181,11 @@ def test_autotune_keeps_state_and_parity(gpu, precision):
         chosen = t.autotune(dt, reps=1)
         shapes = t.march_shapes()
         assert set(chosen) == {"u", "v", "w", "scalars"}
-        for rows, strip in chosen.values():
-            assert rows in shapes and strip in (8, 16, 32, 64)
+        for kernel, rows, strip in chosen.values():
+            if kernel == tvd.KERNEL_STAGE:
+                assert rows in (4, 6, 8) and strip in (16, 32, 64)
+            else:
+                assert kernel == tvd.KERNEL_MARCH and rows in shapes and strip in (8, 16, 32, 64)
         for n in range(5):
             np.testing.assert_array_equal(t.get_field(n, tvd.PROG), before[n])
         t.forward_steps(dt, 3)
@@ -258,3 +261,35 @@ def test_forward_steps_with_tracers_and_inactive_q(gpu):
         e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
         assert e <= TOL, (n, e)
     np.testing.assert_array_equal(got[5], oracle.halo_fill(st.fields[5].copy()))  # inactive q(2) unchanged
+
+
+# ---------------------------------------------------------------------------
+# The stage kernel (per field kind, fields staged in LDS): column heights that
+# fill one to four 64-level bands partially and exactly, row counts that do
+# and do not divide ny, strips that do and do not divide nx, one and two
+# scalars, against the oracle's forward run.
+STAGE_CASES = [(nz, rows) for nz in (4, 5, 63, 64, 65, 70, 128, 150, 193, 256) for rows in (4, 6, 8, 11)]
+
+
+@pytest.mark.parametrize("nz,rows", STAGE_CASES)
+@pytest.mark.parametrize("strip,n_q", [(5, 1), (16, 2)])
+def test_stage_kernel_matches_oracle(gpu, nz, rows, strip, n_q):
+    """rows = 11 is compiled for u, v and w only: the scalars then take 4 rows."""
+    st = ss.make_state(nz, 11, 13, 50.0, 25.0 * (nz - 1), seed=nz + rows + strip, n_q=n_q, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    ref = oracle.forward_run(st, dt, 3)
+    got = gpu_forward(st, dt, 3, tiling=(tvd.KERNEL_STAGE, rows, strip))
+    for n, (a, b) in enumerate(zip(got, ref)):
+        e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))  # halos included
+        assert e <= TOL, (n, e)
+
+
+@pytest.mark.parametrize("rows", [4, 6, 8])
+def test_stage_kernel_fp32_matches_march(gpu, rows):
+    st = ss.make_state(130, 20, 18, 50.0, 2500.0, seed=77, n_q=1, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    a = gpu_forward(st, dt, 4, precision=tvd.FP32, tiling=(tvd.KERNEL_STAGE, rows, 16))
+    b = gpu_forward(st, dt, 4, precision=tvd.FP32)
+    for n, (x, y) in enumerate(zip(a, b)):
+        e = float(np.max(np.abs(x - y)) / np.max(np.abs(y)))
+        assert e <= 1e-6, (n, e)
