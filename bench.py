@@ -5,19 +5,21 @@ Workload (the plain FP64 form at real shape): a 512x512x256 local domain
 seeded distributions of the synthetic set-up, advection-only forward-Euler
 steps at CFL 0.3.  One step = the TVD tendencies of all five fields from the
 pre-step state plus the update field <- field + dt*s and the periodic halo of
-the new state (monc_tvd_forward_steps): four launches of the march
-kernel, one each for u, v, w and the (th, q) pair, with tile shapes picked by
-the library's autotuner before the untimed warm-up.  The same step in FP32 is
-reported beside it.
+the new state (monc_tvd_forward_steps): one launch per field kind (u, v, w
+and the (th, q) pair), each in the library's default kernel and tile shape
+(FP64 u, v, w: the LDS-staged kernel; the rest: the march kernel; --tune runs
+the library's autotuner first).  The same step in FP32 is reported beside it.
 
 Output: one JSON line.  value = interior cell-updates per second (all five
 fields) of the FP64 form, wall clock over the timed steps; roofline = the
 dominant kernel's algorithmic bytes per launch over its average duration
-(HIP events recorded by the library around every launch on its stream), with
-`traffic` the HBM bytes rocprofv3 measured for that kernel at that tile shape
-(profiles/march_kernel_traffic_512x512x256.json; null for another shape);
-cpu_baseline = the C restatement (oracle) timed on a bounded sample on this
-host's CPU.  Beside it: the FP32 form, the FP32 adjoint step on the same
+(HIP events recorded by the library around every launch on its stream),
+with the HBM traffic, VALU instruction count and vector-memory busy that
+rocprofv3 measured for that kernel at that shape
+(profiles/step_counters_512x512x256.json; marked stale for another kernel or
+shape); cpu_baseline = the C restatement (oracle) of the path on the host's
+cores, one process per core on x slabs of the same workload, run before the
+GPU is touched.  Beside it: the FP32 form, the FP32 adjoint step on the same
 domain, the MONC leapfrog timestep (FP64), the sibling components
 th_advection and pw_advection on the same FP64 state (kernel time and HBM
 fraction), and the batched ensemble (48 FP32 members of seeded sizes
@@ -35,6 +37,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (FP64 and unpacked
+# FP32 alike), lane-instructions per second
+VALU_PEAK = 256 * 4 * 16 * 2.4e9
+PROFILE = os.path.join(ROOT, "profiles", "step_counters_512x512x256.json")
 METRIC = "Gcell-updates/s & %HBM peak: FP32/FP64 fwd + adjoint at stated error vs FP64"
 
 
@@ -46,7 +52,7 @@ def log(msg: str) -> None:
 # (element size es): u, v and w kernels read u, v, w and write their field;
 # the scalar kernel reads u, v, w and its two scalars and writes both.
 KIND_FIELDS = {"u": 4, "v": 4, "w": 4, "scalars": 7, "cell": 10}
-KERNEL_NAMES = {0: "march_kernel", 1: "cell_kernel"}
+KERNEL_NAMES = {0: "march_kernel", 1: "cell_kernel", 2: "stage_kernel"}
 
 
 def run_form(state, precision: int, steps: int, warmup: int, dt: float, kernel: int = 0, rows: int = 0,
@@ -59,9 +65,12 @@ def run_form(state, precision: int, steps: int, warmup: int, dt: float, kernel: 
     try:
         stream = torch.cuda.current_stream()
         t.set_stream(stream.cuda_stream)
-        t.set_tiling(kernel, rows, 0)
+        if kernel or rows:
+            t.set_tiling(kernel, rows, 0)
         t.halo_fill()
-        tuned = t.autotune(dt, reps=2) if tune else None
+        if tune:
+            t.autotune(dt, reps=2)
+        tuned = {k: list(v) for k, v in t.tiling().items()}
         for _ in range(warmup):
             t.forward_steps(dt, 1)
         t.synchronize()
@@ -89,24 +98,140 @@ def roofline(kt: dict, cells: int, es: int):
     return kind, avg_ms, bytes_launch, achieved
 
 
-def default_shape(precision: str, kind: str):
-    """The library's default march tile (rows, x columns) for this workload
-    (256 levels over 512 x 512 columns)."""
-    return [2, 64] if precision == "fp64" and kind == "scalars" else [3, 32]
+def profiled(precision: str, kind: str, nz: int, tiling):
+    """The rocprofv3 counters of this kernel kind at the kernel and tile shape
+    it ran in (kernel, rows, strip), or None when the profile holds another
+    kernel or shape (stale) or another column height."""
+    if nz != 256 or not os.path.exists(PROFILE) or tiling is None:
+        return None
+    with open(PROFILE) as fh:
+        prof = json.load(fh)
+    entry = prof.get(precision, {}).get(kind)
+    if entry is None or list(entry["tiling"]) != list(tiling):
+        return None
+    return entry
 
 
-def measured_traffic(precision: str, kind: str, nz: int, shape):
-    """HBM bytes per launch that rocprofv3 measured for this kernel at the
-    tile shape it was profiled in, or None for another shape."""
-    p = os.path.join(ROOT, "profiles", "march_kernel_traffic_512x512x256.json")
-    if nz != 256 or not os.path.exists(p):
+def roofline_entry(precision: str, kind: str, kavg_ms: float, alg_bytes: float, nz: int, tiling):
+    """HBM fraction at algorithmic bytes, and, from the profile at the same
+    kernel and shape, the measured HBM traffic, the VALU fraction
+    (SQ_INSTS_VALU x 64 / time / VALU_PEAK) and the vector-memory busy (TA);
+    `limiter` is the largest of the three fractions."""
+    t = kavg_ms * 1e-3
+    achieved = alg_bytes / t / 1e9
+    out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel_ms": round(kavg_ms, 4),
+           "algorithmic_bytes_per_launch": alg_bytes, "tiling": tiling}
+    prof = profiled(precision, kind, nz, tiling)
+    if prof is None:
+        out["counters"] = "stale: no profile of this kernel at this shape"
+        return out
+    traffic = prof["fetch_bytes"] + prof["write_bytes"]
+    valu = prof["valu_insts"] * 64 / t / VALU_PEAK
+    fracs = {"hbm": traffic / t / (HBM_PEAK_GBS * 1e9), "valu": valu, "vmem": prof["ta_busy"]}
+    out.update({"traffic": traffic, "traffic_over_algorithmic": round(traffic / alg_bytes, 3),
+                "traffic_frac": round(fracs["hbm"], 4),
+                "valu": {"insts_per_launch": prof["valu_insts"], "achieved": round(prof["valu_insts"] * 64 / t, 0),
+                         "peak": VALU_PEAK, "unit": "lane-instructions/s", "frac": round(valu, 4)},
+                "vmem_busy": prof["ta_busy"], "limiter": max(fracs, key=fracs.get),
+                "profiled_kernel_ms": prof["ms"],
+                "counters": f"profile {os.path.relpath(PROFILE, ROOT)} ({prof['kernel']})"})
+    return out
+
+
+_KIND_OF = {0: "u", 1: "v", 2: "w", 3: "scalars"}
+
+
+def _step_kernel(name: str):
+    """(precision, kind) of a forward-step kernel named in a rocprofv3 record,
+    or None: march_kernel<T, KIND, NS, MODE=1, ...> and stage_kernel<T, KIND,
+    ...>."""
+    import re
+
+    m = re.search(r"(march|stage)_kernel<(float|double), (\d+), (\d+)(?:, (\d+))?", name)
+    if m is None or (m.group(1) == "march" and m.group(5) != "1"):
         return None
-    with open(p) as fh:
-        t = json.load(fh)
-    prof = t.get("shapes", {}).get(precision, {}).get(kind)
-    if shape is None or prof is None or list(shape) != list(prof):
-        return None
-    return t["bytes_per_launch"].get(precision, {}).get(kind)
+    return ("fp64" if m.group(2) == "double" else "fp32"), _KIND_OF[int(m.group(3))]
+
+
+def collate(d: str) -> None:
+    """Reduce the rocprofv3 runs of the step under directory d to the counters
+    bench.py prices the dominant kernel with (written to the profile file).
+    Expected: d/kt/kt_kernel_stats.csv (--kernel-trace --stats), d/bench.json
+    (the JSON line of that run, for the tile shapes), and the counter passes
+    d/f (FETCH_SIZE), d/w (WRITE_SIZE), d/sq (SQ_INSTS_VALU, SQ_INSTS_VMEM_RD,
+    GRBM_GUI_ACTIVE), d/ta (TA_TA_BUSY_sum, TD_TD_BUSY_sum, GRBM_GUI_ACTIVE),
+    each d/<p>/<p>_counter_collection.csv.  The f and w runs end with three
+    512 MiB device-to-device copies (--calibrate-copy): the last three copy
+    dispatches give the factors from FETCH_SIZE and WRITE_SIZE (KiB) to bytes.
+    Busy fractions: VALU = SQ_INSTS_VALU x 4 cycles / (GRBM_GUI_ACTIVE / 8 XCDs
+    x 1024 SIMDs); TA and TD = *_BUSY_sum / (GRBM_GUI_ACTIVE / 8 x 256 CUs)."""
+    import csv
+    from collections import defaultdict
+
+    def counters(p):
+        per = defaultdict(lambda: defaultdict(list))   # kernel -> counter -> values
+        copies = defaultdict(dict)                     # dispatch -> counter -> value
+        with open(os.path.join(d, p, f"{p}_counter_collection.csv")) as fh:
+            for r in csv.DictReader(fh):
+                v = float(r["Counter_Value"])
+                if "copyBuffer" in r["Kernel_Name"]:
+                    copies[int(r["Dispatch_Id"])][r["Counter_Name"]] = v
+                key = _step_kernel(r["Kernel_Name"])
+                if key is not None:
+                    per[key][r["Counter_Name"]].append(v)
+        return ({k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in per.items()},
+                [copies[i] for i in sorted(copies)[-3:]])
+
+    def factor(copy_list, counter):
+        vals = sorted(c[counter] for c in copy_list)
+        return 512 * 1024 / vals[1]
+
+    stats, names = {}, {}
+    with open(os.path.join(d, "kt", "kt_kernel_stats.csv")) as fh:
+        for r in csv.DictReader(fh):
+            key = _step_kernel(r["Name"])
+            if key is not None:
+                stats[key] = float(r["AverageNs"]) * 1e-6
+                names[key] = r["Name"].split("(")[0].replace("void ", "")
+    with open(os.path.join(d, "bench.json")) as fh:
+        shapes = json.loads(fh.read().strip().splitlines()[-1])["tile_shapes"]
+    f, fcp = counters("f")
+    w, wcp = counters("w")
+    sq, _ = counters("sq")
+    ta, _ = counters("ta")
+    ff, wf = factor(fcp, "FETCH_SIZE"), factor(wcp, "WRITE_SIZE")
+    prof = {"workload": "bench.py default FP64 and FP32 step, 512x512x256, library default shapes",
+            "calibration": {"fetch_bytes_per_KiB_count": round(ff * 1024, 1),
+                            "write_bytes_per_KiB_count": round(wf * 1024, 1)}}
+    for key in sorted(stats):
+        prec, kind = key
+        cyc_xcd = sq[key]["GRBM_GUI_ACTIVE"] / 8
+        prof.setdefault(prec, {})[kind] = {
+            "kernel": names[key], "tiling": shapes[prec][kind], "ms": round(stats[key], 4),
+            "fetch_bytes": round(f[key]["FETCH_SIZE"] * 1024 * ff),
+            "write_bytes": round(w[key]["WRITE_SIZE"] * 1024 * wf),
+            "valu_insts": round(sq[key]["SQ_INSTS_VALU"]), "vmem_rd": round(sq[key]["SQ_INSTS_VMEM_RD"]),
+            "valu_busy": round(sq[key]["SQ_INSTS_VALU"] * 4 / (cyc_xcd * 1024), 3),
+            "ta_busy": round(ta[key]["TA_TA_BUSY_sum"] / (ta[key]["GRBM_GUI_ACTIVE"] / 8 * 256), 3),
+            "td_busy": round(ta[key]["TD_TD_BUSY_sum"] / (ta[key]["GRBM_GUI_ACTIVE"] / 8 * 256), 3)}
+    with open(PROFILE, "w") as fh:
+        json.dump(prof, fh, indent=1)
+        fh.write("\n")
+    cells = 512 * 512 * 256
+    print("| precision | kind | kernel | tiling | avg ms | fetch GB | write GB | traffic / alg. | "
+          "alg. GB/s | VALU instr. (M) | VALU busy | VMEM reads (M) | TA busy | TD busy |")
+    print("|---|---|---|---|---|---|---|---|---|---|---|---|---|---|")
+    for prec in ("fp64", "fp32"):
+        es = 8 if prec == "fp64" else 4
+        for kind, e in prof.get(prec, {}).items():
+            alg = KIND_FIELDS[kind] * es * cells
+            print(f"| {prec} | {kind} | `{e['kernel']}` | {e['tiling']} | {e['ms']:.3f} | "
+                  f"{e['fetch_bytes'] / 1e9:.2f} | {e['write_bytes'] / 1e9:.2f} | "
+                  f"{(e['fetch_bytes'] + e['write_bytes']) / alg:.2f} | {alg / e['ms'] / 1e6:.0f} | "
+                  f"{e['valu_insts'] / 1e6:.0f} | {e['valu_busy']:.2f} | {e['vmem_rd'] / 1e6:.1f} | "
+                  f"{e['ta_busy']:.2f} | {e['td_busy']:.2f} |")
+    print(f"calibration: FETCH_SIZE x {ff:.3f}, WRITE_SIZE x {wf:.3f}")
 
 
 def adjoint_form(state, dt: float, steps: int):
@@ -275,25 +400,62 @@ def ensemble_form(steps: int):
         e.close()
 
 
-def cpu_baseline(nz: int, seconds_hint: float = 12.0):
-    """The oracle (single-threaded C restatement) on a bounded sample of the
-    same workload: one forward step of a 288x288 column block at full depth."""
+_CPU_STATE = None
+
+
+def _cpu_slab(args):
+    """One process's share of the CPU baseline: the TVD tendencies and the
+    forward step of an x slab (its x halo taken from the neighbouring
+    columns, y periodic), timed around the oracle calls only."""
     import numpy as np
 
     import oracle
-    from monc_amd import setup_state as ss
-
-    n = 288
-    st = ss.make_state(nz, n, n, 25.0, 5000.0, seed=0, n_q=1, with_zfields=False)
-    dt = ss.cfl_dt(st, 0.3)
+    i0, i1 = args
+    st, dt = _CPU_STATE
+    fields = [np.ascontiguousarray(f[i0 - 2:i1 + 2]) for f in st.fields]
     t0 = time.perf_counter()
-    srcs, _ = oracle.tendencies(nz, n, n, st.cx, st.cy, st.grid.metric_vectors(), dt, 1, 1, 1, st.fields)
-    _ = [oracle.forward_step(f, s, dt) for f, s in zip(st.fields, srcs)]
-    el = time.perf_counter() - t0
-    cells = n * n * nz
-    return {"value": cells / el / 1e9, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"one forward step (5 fields) of a {n}x{n}x{nz} block of the workload, "
-                      f"{el:.1f} s on 1 core"}
+    srcs, _ = oracle.tendencies(st.nz, st.ny, i1 - i0, st.cx, st.cy, st.grid.metric_vectors(), dt, 1, 1, 1, fields)
+    _ = [oracle.forward_step(f, s, dt) for f, s in zip(fields, srcs)]
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(nz: int, ny: int, nx: int):
+    """The oracle (the C restatement of the path, -O2, no OpenMP of its own)
+    on the host's cores: one forward step of the whole workload, split into
+    x slabs, one process per core (as MONC's MPI ranks split the domain; the
+    halo exchange is the slabs' copy of neighbouring columns, outside the
+    timed region).  Runs before the GPU is initialised (the workers are
+    forked)."""
+    import multiprocessing as mp
+
+    from monc_amd import setup_state as ss
+    global _CPU_STATE
+    cores = max(1, min(os.cpu_count() or 1, 16))
+    st = ss.make_state(nz, ny, nx, 25.0, 5000.0, seed=0, n_q=1, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    _CPU_STATE = (st, dt)
+    edges = [2 + (nx * p) // cores for p in range(cores + 1)]
+    slabs = [(edges[p], edges[p + 1]) for p in range(cores)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        times = pool.map(_cpu_slab, slabs)
+    wall = time.perf_counter() - t0
+    _CPU_STATE = None
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cells = nx * ny * nz
+    return {"value": round(cells / max(times) / 1e9, 5), "unit": "Gcell-updates/s", "cores": cores,
+            "kind": "port", "nproc": os.cpu_count(), "cpu_model": model,
+            "sample": f"one forward step (5 fields) of the whole {nx}x{ny}x{nz} workload by the C restatement "
+                      f"(not the reference Fortran) in {cores} processes on x slabs; slowest slab "
+                      f"{max(times):.2f} s, pool wall {wall:.2f} s"}
 
 
 def main() -> None:
@@ -304,9 +466,11 @@ def main() -> None:
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--nz", type=int, default=256)
-    ap.add_argument("--kernel", type=int, default=0, help="0 march (default), 1 cell")
-    ap.add_argument("--rows", type=int, default=0, help="march kernel cell rows per block (0: default)")
-    ap.add_argument("--no-tune", action="store_true", help="skip the tile-shape autotuner (untimed)")
+    ap.add_argument("--kernel", type=int, default=0,
+                    help="0 the library's default per kind, 1 cell, 2 stage for every kind")
+    ap.add_argument("--rows", type=int, default=0, help="cell rows per block (0: default)")
+    ap.add_argument("--tune", action="store_true", help="run the library's autotuner first (untimed)")
+    ap.add_argument("--no-tune", action="store_true", help="(the default) keep the library's default shapes")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
@@ -316,11 +480,22 @@ def main() -> None:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed work, run three 512 MiB device-to-device copies: the known-size "
                          "reference that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
+    ap.add_argument("--collate", metavar="DIR", default=None,
+                    help="no GPU work: reduce the rocprofv3 runs of the step under DIR to the profile file")
     args = ap.parse_args()
+    if args.collate:
+        collate(args.collate)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+
+    cpu = None
+    if not args.no_cpu and not args.only:
+        # before anything initialises the GPU: the baseline forks its workers
+        log("bench: CPU baseline (the C restatement on the host's cores)")
+        cpu = cpu_baseline(args.nz, args.ny, args.nx)
 
     import torch
 
@@ -347,14 +522,22 @@ def main() -> None:
         return
 
     log("bench: FP64 form")
-    tune = not args.no_tune and args.kernel == 0 and args.rows == 0
+    tune = args.tune and args.kernel == 0 and args.rows == 0
     wall64, kt64, tuned64 = run_form(st, tvd.FP64, args.steps, args.warmup, dt, args.kernel, args.rows, tune)
     ms_step = wall64 / args.steps * 1e3
     value = cells * args.steps / wall64 / 1e9
     kind, kavg, kbytes, achieved = roofline(kt64, cells, 8)
-    traffic = measured_traffic("fp64", kind, nz, (tuned64 or {}).get(kind) if tuned64 else
-                               (default_shape("fp64", kind) if args.kernel == 0 and args.rows == 0 else None))
     step_kernel_ms = sum(v[0] for k, v in kt64.items() if k in KIND_FIELDS) / args.steps
+    roof = roofline_entry("fp64", kind, kavg, kbytes, nz, tuned64.get(kind))
+    roof["kernel"] = f"{KERNEL_NAMES[tuned64[kind][0]]} ({kind})"
+    step_traffic = [profiled("fp64", k, nz, tuned64.get(k)) for k in ("u", "v", "w", "scalars")]
+    roof["step"] = {"kernel_ms": round(step_kernel_ms, 4), "bytes_per_cell_update": 80,
+                    "achieved_GBs": round(80 * cells / (step_kernel_ms * 1e-3) / 1e9, 2),
+                    "frac": round(80 * cells / (step_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "traffic_over_algorithmic": (round(sum(p["fetch_bytes"] + p["write_bytes"] for p in step_traffic)
+                                                       / (80.0 * cells), 3)
+                                                 if all(step_traffic) else None)}
+    roof["kernels_ms_per_launch"] = {k: round(v[0] / v[1], 4) for k, v in kt64.items()}
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -372,13 +555,7 @@ def main() -> None:
         "config": {"workload": "plain FP64 TVD advection forward-Euler step, 512x512x256, dx=dy=25 m, "
                                "z tanh-stretched to 5 km, fields u,v,w,th,q, CFL 0.3",
                    "nx": nx, "ny": ny, "nz": nz, "fields": 5, "dt_s": dt},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"{KERNEL_NAMES[args.kernel]} ({kind})", "kernel_ms": round(kavg, 4),
-                     "algorithmic_bytes_per_launch": kbytes,
-                     "step": {"kernel_ms": round(step_kernel_ms, 4), "bytes_per_cell_update": 80,
-                              "achieved_GBs": round(80 * cells / (step_kernel_ms * 1e-3) / 1e9, 2)},
-                     "kernels_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kt64.items()}},
+        "roofline": roof,
         "tile_shapes": {"fp64": tuned64},
     }
     if not args.no_fp32:
@@ -387,11 +564,17 @@ def main() -> None:
         v32 = cells * args.steps / wall32 / 1e9
         kind32, kavg32, kbytes32, a32 = roofline(kt32, cells, 4)
         step32 = sum(v[0] for k, v in kt32.items() if k in KIND_FIELDS) / args.steps
+        r32 = roofline_entry("fp32", kind32, kavg32, kbytes32, nz, tuned32.get(kind32))
+        st32 = [profiled("fp32", k, nz, tuned32.get(k)) for k in ("u", "v", "w", "scalars")]
         out["fp32"] = {"value": round(v32, 4), "unit": "Gcell-updates/s",
                        "ms_per_step": round(wall32 / args.steps * 1e3, 4),
-                       "dominant_kernel": kind32, "kernel_ms": round(kavg32, 4), "achieved_GBs": round(a32, 2),
-                       "frac": round(a32 / HBM_PEAK_GBS, 4),
+                       "dominant_kernel": f"{KERNEL_NAMES[tuned32[kind32][0]]} ({kind32})",
+                       "roofline": r32,
                        "step_achieved_GBs": round(40 * cells / (step32 * 1e-3) / 1e9, 2),
+                       "step_frac": round(40 * cells / (step32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "step_traffic_over_algorithmic": (round(sum(p["fetch_bytes"] + p["write_bytes"] for p in st32)
+                                                               / (40.0 * cells), 3) if all(st32) else None),
+                       "kernels_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kt32.items()},
                        "speedup_vs_fp64": round(v32 / value, 3)}
         out["tile_shapes"]["fp32"] = tuned32
     if not args.no_extra:
@@ -412,9 +595,8 @@ def main() -> None:
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    if not args.no_cpu:
-        log("bench: CPU baseline (oracle) on a bounded sample")
-        out["cpu_baseline"] = cpu_baseline(nz)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

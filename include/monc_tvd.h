@@ -143,6 +143,12 @@ int monc_tvd_set_tiling(monc_tvd_ctx *ctx, int kernel, int rows, int strip);
  * fields. */
 int monc_tvd_autotune(monc_tvd_ctx *ctx, double dtm, int reps, int *chosen);
 
+/* The kernel and tile shape the forward step uses per field kind, defaults
+ * resolved (no counterpart in the reference): chosen (12 ints) receives
+ * kernel, rows and strip for u, v, w and the scalars, as monc_tvd_autotune
+ * reports them. */
+int monc_tvd_get_tiling(monc_tvd_ctx *ctx, int *chosen);
+
 /* The march shapes (cell rows per block) compiled for this context's column
  * height, default first: writes up to n of them to rows and returns how many
  * exist (0 when the march kernel cannot run this context), or a negative

@@ -1364,6 +1364,37 @@ int monc_tvd_set_tiling(monc_tvd_ctx* c, int kernel, int rows, int strip) {
   return MONC_TVD_OK;
 }
 
+int monc_tvd_get_tiling(monc_tvd_ctx* c, int* chosen) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!chosen) return fail(MONC_TVD_ERR_ARG, "null output array");
+  const int nch = (c->nz + 63) / 64;
+  const bool march_fits = nch <= 4 && c->buffer_ok;
+  for (int q = 0; q < 4; q++) {
+    const int kind = q == 3 ? KIND_SCALAR : q;
+    int kernel = c->kernel == MONC_TVD_KERNEL_CELL || !march_fits ? MONC_TVD_KERNEL_CELL : c->kind_kernel[q];
+    int rows = c->rows[q], strip = c->strip[q];
+    const int dstrip = stage_default_strip(c, kind);
+    if (kernel == MONC_TVD_KERNEL_MARCH && !rows && !strip && c->buffer_ok && dstrip > 0) {
+      kernel = MONC_TVD_KERNEL_STAGE;
+      rows = 11;
+      strip = dstrip;
+    } else if (kernel == MONC_TVD_KERNEL_STAGE) {
+      rows = rows > 0 ? rows : 6;
+      strip = strip > 0 ? strip : 64;
+    } else if (kernel == MONC_TVD_KERNEL_MARCH) {
+      rows = rows > 0 ? rows : default_rows(c->precision, nch, q);
+      const bool wide = c->precision == MONC_TVD_FP64 && q == 3 && nch == 4 && rows <= 2;
+      strip = strip > 0 ? strip : default_strip(c->ny, c->nx, rows, wide);
+    } else {
+      rows = strip = 0;
+    }
+    chosen[3 * q] = kernel;
+    chosen[3 * q + 1] = rows;
+    chosen[3 * q + 2] = strip;
+  }
+  return MONC_TVD_OK;
+}
+
 int monc_tvd_march_shapes(monc_tvd_ctx* c, int* rows, int n) {
   if (int rc = check_ctx(c)) return rc;
   if (n < 0 || (n > 0 && !rows)) return fail(MONC_TVD_ERR_ARG, "null output array");

@@ -21,8 +21,17 @@ module tvd_shim_core_mod
   !> Dimension indices of local_grid%size (model_core/src/grid/grids.F90:14).
   integer, parameter, public :: Z_INDEX = 1, Y_INDEX = 2, X_INDEX = 3
 
-  public :: tvd_shim_flags, tvd_shim_create, tvd_shim_q_id, tvd_shim_tracer_id
+  public :: tvd_shim_selected, tvd_shim_flags, tvd_shim_create, tvd_shim_q_id, tvd_shim_tracer_id
 contains
+
+  !> Whether an advection option names this scheme: an empty value, "tvd" or
+  !! "any" selects it (the rule of tvdadvection.F90:1218-1230).
+  logical function tvd_shim_selected(option_value)
+    character(len=*), intent(in) :: option_value
+
+    tvd_shim_selected = len_trim(option_value) == 0 .or. trim(option_value) == "tvd" .or. &
+         trim(option_value) == "any"
+  end function tvd_shim_selected
 
   !> The library's advect flags for the three selectors.  The reference
   !! advects tracers when the *theta* option says so (advect_tracer is set

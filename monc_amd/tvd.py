@@ -37,7 +37,7 @@ EXPORTS = (
     "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic", "monc_tvd_autotune",
     "monc_tvd_leapfrog_steps", "monc_tvd_forward_step_into", "monc_thadv_tendencies",
     "monc_pw_tendencies", "monc_tvd_set_field_active", "monc_tvd_test_fail_allocations",
-    "monc_tvd_weighted_sum_noalloc",
+    "monc_tvd_weighted_sum_noalloc", "monc_tvd_get_tiling",
     "monc_tvd_ensemble_create", "monc_tvd_ensemble_destroy", "monc_tvd_ensemble_layout",
     "monc_tvd_ensemble_set_field", "monc_tvd_ensemble_get_field", "monc_tvd_ensemble_copy_pool",
     "monc_tvd_ensemble_set_dt", "monc_tvd_ensemble_set_stream", "monc_tvd_ensemble_synchronize",
@@ -91,6 +91,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_march_shapes.argtypes = [c_void_p, ctypes.POINTER(c_int), c_int]
     lib.monc_tvd_set_diagnostics.argtypes = [c_void_p, c_int, dp]
     lib.monc_tvd_autotune.argtypes = [c_void_p, c_double, c_int, ctypes.POINTER(c_int)]
+    lib.monc_tvd_get_tiling.argtypes = [c_void_p, ctypes.POINTER(c_int)]
     lib.monc_tvd_get_diagnostic.argtypes = [c_void_p, c_int, c_int, dp]
     lib.monc_tvd_set_profiling.argtypes = [c_void_p, c_int]
     lib.monc_tvd_weighted_sum.argtypes = [c_void_p, c_int, c_int, dp, dp]
@@ -245,6 +246,12 @@ class TvdAdvection:
         strip)} with kernel KERNEL_MARCH or KERNEL_STAGE."""
         out = (ctypes.c_int * 12)()
         _check(self._lib.monc_tvd_autotune(self._h, dtm, reps, out))
+        return {k: (out[3 * i], out[3 * i + 1], out[3 * i + 2]) for i, k in enumerate(("u", "v", "w", "scalars"))}
+
+    def tiling(self) -> dict:
+        """{kind: (kernel, rows, strip)} the forward step uses, defaults resolved."""
+        out = (ctypes.c_int * 12)()
+        _check(self._lib.monc_tvd_get_tiling(self._h, out))
         return {k: (out[3 * i], out[3 * i + 1], out[3 * i + 2]) for i, k in enumerate(("u", "v", "w", "scalars"))}
 
     def march_shapes(self) -> list:
