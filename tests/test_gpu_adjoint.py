@@ -288,7 +288,8 @@ def _c4_gradient(st, dt, precision, k=8):
         dtype = eng.dtype
         x0 = [torch.tensor(f, device="cuda", dtype=dtype, requires_grad=(n in (0, 3)))
               for n, f in enumerate(st.fields)]
-        for _ in range(2):  # warm-up pass, then the timed one
+        fwd, bwd = [], []
+        for rep in range(4):  # a warm-up pass, then three timed ones
             for x in x0:
                 x.grad = None
             torch.cuda.synchronize()
@@ -300,8 +301,10 @@ def _c4_gradient(st, dt, precision, k=8):
             loss.backward()
             torch.cuda.synchronize()
             t2 = time.perf_counter()
-        return (float(loss), x0[3].grad.double().cpu().numpy(), x0[0].grad.double().cpu().numpy(),
-                t1 - t0, t2 - t1)
+            if rep:
+                fwd.append(t1 - t0)
+                bwd.append(t2 - t1)
+        return (float(loss), x0[3].grad.double().cpu().numpy(), x0[0].grad.double().cpu().numpy(), fwd, bwd)
     finally:
         eng.close()
 
@@ -337,8 +340,10 @@ def test_c4_fp32_adjoint_vs_fp64_tangent_linear(c4):
     (isolated points, which dominate the L2 distance and the noise direction)."""
     st, dt = c4["st"], c4["dt"]
     rng = np.random.default_rng(14)
-    report = {"timing_s": c4["t"], "bwd_over_fwd_fp32": c4["t"]["bwd32"] / c4["t"]["fwd32"],
-              "bwd_over_fwd_fp64": c4["t"]["bwd64"] / c4["t"]["fwd64"]}
+    t = c4["t"]
+    report = {"timing_s": t,
+              "bwd_over_fwd_fp32": float(np.median(np.array(t["bwd32"]) / np.array(t["fwd32"]))),
+              "bwd_over_fwd_fp64": float(np.median(np.array(t["bwd64"]) / np.array(t["fwd64"])))}
     for name, fid, gi, amp in (("th", 3, 0, 0.05), ("u", 0, 1, 0.3)):
         g32, g64 = interior(c4["g32"][gi]), interior(c4["g64"][gi])
         report[name] = {"grad_rel_l2_32_vs_64": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))}
@@ -358,7 +363,13 @@ def test_c4_fp32_adjoint_vs_fp64_tangent_linear(c4):
         for dname in ("noise", "smooth"):
             assert r[dname]["rel_adj64_vs_tl"] <= 1e-10, (name, dname, r)
         assert r["smooth"]["rel_adj32_vs_tl"] <= 1e-4, (name, r)
+        # measured on MI355X: noise 5.4e-3 (th), gradients 1.0e-3 (th) and
+        # 3.4e-4 (u) apart -- above the 1e-4 aim: the FP32 trajectory takes
+        # other limiter branches than the FP64 one at isolated points, and
+        # the FP32 gradient is exact for its own trajectory
+        assert r["noise"]["rel_adj32_vs_tl"] <= 1e-2, (name, r)
         assert r["grad_rel_l2_32_vs_64"] <= 3e-3, (name, r)
+    # median over three timed rollouts (recompute + adjoint over forward)
     assert report["bwd_over_fwd_fp32"] <= 3.0, report
 
 
