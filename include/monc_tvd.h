@@ -188,6 +188,25 @@ int monc_tvd_set_diagnostics(monc_tvd_ctx *ctx, int flags, const double *rprefrc
 #define MONC_TVD_DIAG_ZFACES 4
 int monc_tvd_get_diagnostic(monc_tvd_ctx *ctx, int field, int which, double *host);
 
+/* The same tendency diagnostics of the two sibling components on this
+ * context: with MONC_TVD_DIAG_TENDENCY set, every monc_pw_tendencies call
+ * records tend_3d = s_after - s_before and its per-level total for each field
+ * it changes (u, v, w with advect_flow; th with advect_th and th active; each
+ * q with advect_q), and the tabs tendency with rprefrcp given: the published
+ * fields tend_*_pwadvection_3d_local / _profile_total_local
+ * (pwadvection.F90:56-120, save_precomponent_tendencies and
+ * compute_component_tendencies :619-743); every monc_thadv_tendencies call
+ * records th's (and tabs') while th is active, also when no term is switched
+ * on (a zero tendency): tend_th/tabs_thadvection_3d_local /
+ * _profile_total_local (thadvection.F90:47-51, 84-88, 152-159, 228-270).
+ * component selects whose record is read; monc_tvd_get_diagnostic reads the
+ * TVD component's.  `which` as above; z-faces exist for TVD only. */
+#define MONC_TVD_COMPONENT_TVD 0
+#define MONC_TVD_COMPONENT_PW 1
+#define MONC_TVD_COMPONENT_THADV 2
+#define MONC_TVD_NCOMPONENTS 3
+int monc_tvd_get_component_diagnostic(monc_tvd_ctx *ctx, int component, int field, int which, double *host);
+
 /* Per-kernel timing (no counterpart in the reference: measurement support).
  * With profiling on, every launch is bracketed by HIP events on the context
  * stream; monc_tvd_kernel_times synchronises, folds the events into per-kind

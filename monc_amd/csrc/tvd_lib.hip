@@ -92,16 +92,18 @@ struct monc_tvd_ctx {
   std::vector<Timed> pending;
   double prof_ms[MONC_TVD_NKERNEL_KINDS];
   long long prof_n[MONC_TVD_NKERNEL_KINDS];
-  // tendency diagnostics (per field): s before the call, tend_3d (nz,ny,nx),
-  // per-level totals (FP64), flux_z copies; tabs from th and rprefrcp
+  // tendency diagnostics per component (TVD, pw_advection, th_advection)
+  // and field: s before the call (shared), tend_3d (nz,ny,nx), per-level
+  // totals (FP64), flux_z copies (TVD); tabs from th and rprefrcp
   int diag_flags;
   double* rprefrcp;            // device, nz (or null)
-  std::vector<void*> diag_before, diag_3d, diag_zf;
-  std::vector<double*> diag_prof;
-  std::vector<char> diag_have;  // tend_3d/profile recorded
-  void* tabs_3d;
-  double* tabs_prof;
-  int tabs_have;
+  std::vector<void*> diag_before, diag_zf;
+  std::vector<void*> diag_3d[MONC_TVD_NCOMPONENTS];
+  std::vector<double*> diag_prof[MONC_TVD_NCOMPONENTS];
+  std::vector<char> diag_have[MONC_TVD_NCOMPONENTS];  // tend_3d/profile recorded
+  void* tabs_3d[MONC_TVD_NCOMPONENTS];
+  double* tabs_prof[MONC_TVD_NCOMPONENTS];
+  int tabs_have[MONC_TVD_NCOMPONENTS];
   double* diag_partial;        // ny*nz FP64 scratch of the level totals
   double* thadv_tab;           // device, 2*nz: dthref, thref of th_advection (allocated on first use)
   double* wsum_scratch;        // device, weights + partials + result of monc_tvd_weighted_sum (first use)
@@ -489,6 +491,71 @@ __global__ void __launch_bounds__(256) thadv_kernel(T* sth, const T* u, const T*
   sth[o] = s;
 }
 
+// Diagnostics around one call of a component (TVD, pw_advection or
+// th_advection): snapshot the s-fields it changes before it
+// (save_precomponent_tendencies), allocate the z-face buffers (TVD only).
+int diag_before_call(monc_tvd_ctx* c, const std::vector<int>& fields, int comp) {
+  if (!c->diag_flags) return MONC_TVD_OK;
+  const size_t bytes = c->n_elem * c->elem_size;
+  for (int fl : fields) {
+    if (is_tracer(c, fl)) continue;
+    if (c->diag_flags & MONC_TVD_DIAG_TENDENCY) {
+      if (!c->diag_before[fl]) HIP_TRY(dev_alloc(&c->diag_before[fl], bytes));
+      HIP_TRY(hipMemcpyAsync(c->diag_before[fl], slot(c, fl, MONC_TVD_SOURCE), bytes, hipMemcpyDeviceToDevice,
+                             c->stream));
+    }
+    if (comp == MONC_TVD_COMPONENT_TVD && (c->diag_flags & MONC_TVD_DIAG_ZFACE) && !c->diag_zf[fl]) {
+      HIP_TRY(dev_alloc(&c->diag_zf[fl], bytes));
+      HIP_TRY(hipMemsetAsync(c->diag_zf[fl], 0, bytes, c->stream));
+    }
+  }
+  return MONC_TVD_OK;
+}
+
+// tend_3d = after - before and its per-level totals (compute_component_tendencies).
+template <typename T>
+int tend_and_profile(monc_tvd_ctx* c, const T* after, const T* before, const double* r, T* t3d, double* prof) {
+  const Dims d = dims_of(c);
+  hipLaunchKernelGGL((tend3d_kernel<T>), dim3((c->nz + 63) / 64, (c->ny + 3) / 4, c->nx), dim3(64, 4, 1), 0,
+                     c->stream, after, before, r, t3d, d);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL((profile_rows_kernel<T>), dim3((c->nz + 63) / 64, c->ny, 1), dim3(64, 1, 1), 0, c->stream,
+                     (const T*)t3d, c->nz, c->ny, c->nx, c->diag_partial);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(profile_sum_kernel, dim3((c->nz + 63) / 64, 1, 1), dim3(64, 1, 1), 0, c->stream,
+                     (const double*)c->diag_partial, c->nz, c->ny, prof);
+  HIP_TRY(hipGetLastError());
+  return MONC_TVD_OK;
+}
+
+template <typename T>
+int diag_after_call(monc_tvd_ctx* c, const std::vector<int>& fields, int comp) {
+  if (!(c->diag_flags & MONC_TVD_DIAG_TENDENCY)) return MONC_TVD_OK;
+  const size_t n3d = (size_t)c->nz * c->ny * c->nx;
+  if (!c->diag_partial) HIP_TRY(dev_alloc(&c->diag_partial, sizeof(double) * (size_t)c->nz * c->ny));
+  int rc;
+  for (int fl : fields) {
+    if (is_tracer(c, fl)) continue;
+    void*& t3d = c->diag_3d[comp][fl];
+    double*& prof = c->diag_prof[comp][fl];
+    if (!t3d) HIP_TRY(dev_alloc(&t3d, n3d * sizeof(T)));
+    if (!prof) HIP_TRY(dev_alloc(&prof, sizeof(double) * c->nz));
+    if ((rc = tend_and_profile<T>(c, (const T*)slot(c, fl, MONC_TVD_SOURCE), (const T*)c->diag_before[fl], nullptr,
+                                  (T*)t3d, prof)))
+      return rc;
+    c->diag_have[comp][fl] = 1;
+    if (fl == MONC_TVD_TH && c->rprefrcp) {
+      if (!c->tabs_3d[comp]) HIP_TRY(dev_alloc(&c->tabs_3d[comp], n3d * sizeof(T)));
+      if (!c->tabs_prof[comp]) HIP_TRY(dev_alloc(&c->tabs_prof[comp], sizeof(double) * c->nz));
+      if ((rc = tend_and_profile<T>(c, (const T*)slot(c, fl, MONC_TVD_SOURCE), (const T*)c->diag_before[fl],
+                                    c->rprefrcp, (T*)c->tabs_3d[comp], c->tabs_prof[comp])))
+        return rc;
+      c->tabs_have[comp] = 1;
+    }
+  }
+  return MONC_TVD_OK;
+}
+
 template <typename T>
 int thadv_impl(monc_tvd_ctx* c, int anelastic, const double* dthref, int baro, const double* thref, double thref0,
                double fcoriol, double g, double rate_x, double rate_y, double ugal, double vgal) {
@@ -503,15 +570,22 @@ int thadv_impl(monc_tvd_ctx* c, int anelastic, const double* dthref, int baro, c
   const Dims d = dims_of(c);
   const long long ncol = (long long)(c->ny + 4) * (c->nx + 4);
   if (ncol > 0x7ffffff0LL) return fail(MONC_TVD_ERR_ARG, "too many columns for th_advection's grid");
-  dim3 block(64, 4, 1), grid((nz + 63) / 64, (unsigned)((ncol + 3) / 4), 1);
-  LaunchTimer timer(c, 4);
-  hipLaunchKernelGGL((thadv_kernel<T>), grid, block, 0, c->stream, (T*)slot(c, MONC_TVD_TH, MONC_TVD_SOURCE),
-                     (const T*)slot(c, MONC_TVD_U, MONC_TVD_PROG), (const T*)slot(c, MONC_TVD_V, MONC_TVD_PROG),
-                     (const T*)slot(c, MONC_TVD_W, MONC_TVD_PROG), c->dmet + MET_TZC1 * nz, c->dmet + MET_TZC2 * nz,
-                     c->thadv_tab, c->thadv_tab + nz, d, anelastic, baro, fcog, rate_x, rate_y, mfx, mfy, (T)ugal,
-                     (T)vgal);
-  HIP_TRY(hipGetLastError());
-  return MONC_TVD_OK;
+  // th's tendency diagnostics when th is active (thadvection.F90:84-88, 152-159)
+  std::vector<int> diag_fields;
+  if (c->active[MONC_TVD_TH]) diag_fields.push_back(MONC_TVD_TH);
+  int rc;
+  if ((rc = diag_before_call(c, diag_fields, MONC_TVD_COMPONENT_THADV))) return rc;
+  if (anelastic || baro) {
+    dim3 block(64, 4, 1), grid((nz + 63) / 64, (unsigned)((ncol + 3) / 4), 1);
+    LaunchTimer timer(c, 4);
+    hipLaunchKernelGGL((thadv_kernel<T>), grid, block, 0, c->stream, (T*)slot(c, MONC_TVD_TH, MONC_TVD_SOURCE),
+                       (const T*)slot(c, MONC_TVD_U, MONC_TVD_PROG), (const T*)slot(c, MONC_TVD_V, MONC_TVD_PROG),
+                       (const T*)slot(c, MONC_TVD_W, MONC_TVD_PROG), c->dmet + MET_TZC1 * nz,
+                       c->dmet + MET_TZC2 * nz, c->thadv_tab, c->thadv_tab + nz, d, anelastic, baro, fcog, rate_x,
+                       rate_y, mfx, mfy, (T)ugal, (T)vgal);
+    HIP_TRY(hipGetLastError());
+  }
+  return diag_after_call<T>(c, diag_fields, MONC_TVD_COMPONENT_THADV);
 }
 
 // pw_advection (pwadvection.F90:258-610): the centred second-order scheme,
@@ -627,78 +701,28 @@ int pw_impl(monc_tvd_ctx* c, int flow, int th, int q, double tcx, double tcy) {
   P.th = th && c->active[MONC_TVD_TH];  // advect_th_field: only when th%active (pwadvection.F90:403)
   P.q = q;
   const Dims d = dims_of(c);
-  dim3 block(64, 4, 1), grid((c->nz + 63) / 64, (c->ny + 3) / 4, (c->nx + PW_STRIP - 1) / PW_STRIP);
-  LaunchTimer timer(c, 4);
-  hipLaunchKernelGGL((pw_kernel<T>), grid, block, 0, c->stream, P, d);
-  HIP_TRY(hipGetLastError());
-  return MONC_TVD_OK;
+  // tendency diagnostics of the fields pw_advection changes
+  // (pwadvection.F90:94-120, 310-318)
+  std::vector<int> diag_fields;
+  if (P.flow)
+    for (int f = MONC_TVD_U; f <= MONC_TVD_W; f++) diag_fields.push_back(f);
+  if (P.th) diag_fields.push_back(MONC_TVD_TH);
+  if (P.q)
+    for (int n = 0; n < c->n_q; n++) diag_fields.push_back(MONC_TVD_Q(n));
+  int rc;
+  if ((rc = diag_before_call(c, diag_fields, MONC_TVD_COMPONENT_PW))) return rc;
+  {
+    dim3 block(64, 4, 1), grid((c->nz + 63) / 64, (c->ny + 3) / 4, (c->nx + PW_STRIP - 1) / PW_STRIP);
+    LaunchTimer timer(c, 4);
+    hipLaunchKernelGGL((pw_kernel<T>), grid, block, 0, c->stream, P, d);
+    HIP_TRY(hipGetLastError());
+  }
+  return diag_after_call<T>(c, diag_fields, MONC_TVD_COMPONENT_PW);
 }
 
 int need(monc_tvd_ctx* c, int field, int variant) {
   if (!slot(c, field, variant))
     return fail(MONC_TVD_ERR_STATE, "field %d variant %d is needed but was never set or bound", field, variant);
-  return MONC_TVD_OK;
-}
-
-// Diagnostics around one tendency call: snapshot the s-fields before it
-// (save_precomponent_tendencies), allocate the z-face buffers.
-int diag_before_call(monc_tvd_ctx* c, const std::vector<int>& fields) {
-  if (!c->diag_flags) return MONC_TVD_OK;
-  const size_t bytes = c->n_elem * c->elem_size;
-  for (int fl : fields) {
-    if (is_tracer(c, fl)) continue;
-    if (c->diag_flags & MONC_TVD_DIAG_TENDENCY) {
-      if (!c->diag_before[fl]) HIP_TRY(dev_alloc(&c->diag_before[fl], bytes));
-      HIP_TRY(hipMemcpyAsync(c->diag_before[fl], slot(c, fl, MONC_TVD_SOURCE), bytes, hipMemcpyDeviceToDevice,
-                             c->stream));
-    }
-    if ((c->diag_flags & MONC_TVD_DIAG_ZFACE) && !c->diag_zf[fl]) {
-      HIP_TRY(dev_alloc(&c->diag_zf[fl], bytes));
-      HIP_TRY(hipMemsetAsync(c->diag_zf[fl], 0, bytes, c->stream));
-    }
-  }
-  return MONC_TVD_OK;
-}
-
-// tend_3d = after - before and its per-level totals (compute_component_tendencies).
-template <typename T>
-int tend_and_profile(monc_tvd_ctx* c, const T* after, const T* before, const double* r, T* t3d, double* prof) {
-  const Dims d = dims_of(c);
-  hipLaunchKernelGGL((tend3d_kernel<T>), dim3((c->nz + 63) / 64, (c->ny + 3) / 4, c->nx), dim3(64, 4, 1), 0,
-                     c->stream, after, before, r, t3d, d);
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL((profile_rows_kernel<T>), dim3((c->nz + 63) / 64, c->ny, 1), dim3(64, 1, 1), 0, c->stream,
-                     (const T*)t3d, c->nz, c->ny, c->nx, c->diag_partial);
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(profile_sum_kernel, dim3((c->nz + 63) / 64, 1, 1), dim3(64, 1, 1), 0, c->stream,
-                     (const double*)c->diag_partial, c->nz, c->ny, prof);
-  HIP_TRY(hipGetLastError());
-  return MONC_TVD_OK;
-}
-
-template <typename T>
-int diag_after_call(monc_tvd_ctx* c, const std::vector<int>& fields) {
-  if (!(c->diag_flags & MONC_TVD_DIAG_TENDENCY)) return MONC_TVD_OK;
-  const size_t n3d = (size_t)c->nz * c->ny * c->nx;
-  if (!c->diag_partial) HIP_TRY(dev_alloc(&c->diag_partial, sizeof(double) * (size_t)c->nz * c->ny));
-  int rc;
-  for (int fl : fields) {
-    if (is_tracer(c, fl)) continue;
-    if (!c->diag_3d[fl]) HIP_TRY(dev_alloc(&c->diag_3d[fl], n3d * sizeof(T)));
-    if (!c->diag_prof[fl]) HIP_TRY(dev_alloc(&c->diag_prof[fl], sizeof(double) * c->nz));
-    if ((rc = tend_and_profile<T>(c, (const T*)slot(c, fl, MONC_TVD_SOURCE), (const T*)c->diag_before[fl], nullptr,
-                                  (T*)c->diag_3d[fl], c->diag_prof[fl])))
-      return rc;
-    c->diag_have[fl] = 1;
-    if (fl == MONC_TVD_TH && c->rprefrcp) {
-      if (!c->tabs_3d) HIP_TRY(dev_alloc(&c->tabs_3d, n3d * sizeof(T)));
-      if (!c->tabs_prof) HIP_TRY(dev_alloc(&c->tabs_prof, sizeof(double) * c->nz));
-      if ((rc = tend_and_profile<T>(c, (const T*)slot(c, fl, MONC_TVD_SOURCE), (const T*)c->diag_before[fl],
-                                    c->rprefrcp, (T*)c->tabs_3d, c->tabs_prof)))
-        return rc;
-      c->tabs_have = 1;
-    }
-  }
   return MONC_TVD_OK;
 }
 
@@ -714,7 +738,7 @@ int tendencies_impl(monc_tvd_ctx* c, double dtm, int field_stepping, int momentu
   for (int fl : fields) {
     if ((rc = need(c, fl, adv_var)) || (rc = need(c, fl, MONC_TVD_SOURCE))) return rc;
   }
-  if ((rc = diag_before_call(c, fields))) return rc;
+  if ((rc = diag_before_call(c, fields, MONC_TVD_COMPONENT_TVD))) return rc;
   const LaunchArgs<T> L = launch_args<T>(c, dtm);
   for (size_t first = 0; first < fields.size(); first += MAX_FIELDS) {
     FieldSet<T> S{};
@@ -730,7 +754,7 @@ int tendencies_impl(monc_tvd_ctx* c, double dtm, int field_stepping, int momentu
     }
     if ((rc = launch_set<T, MODE_ACCUM>(c, L, S))) return rc;
   }
-  return diag_after_call<T>(c, fields);
+  return diag_after_call<T>(c, fields, MONC_TVD_COMPONENT_TVD);
 }
 
 template <typename T>
@@ -1267,13 +1291,15 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int n_tracers, in
   c->diag_flags = 0;
   c->rprefrcp = nullptr;
   c->diag_before.assign((size_t)c->nfields, nullptr);
-  c->diag_3d.assign((size_t)c->nfields, nullptr);
   c->diag_zf.assign((size_t)c->nfields, nullptr);
-  c->diag_prof.assign((size_t)c->nfields, nullptr);
-  c->diag_have.assign((size_t)c->nfields, 0);
-  c->tabs_3d = nullptr;
-  c->tabs_prof = nullptr;
-  c->tabs_have = 0;
+  for (int q = 0; q < MONC_TVD_NCOMPONENTS; q++) {
+    c->diag_3d[q].assign((size_t)c->nfields, nullptr);
+    c->diag_prof[q].assign((size_t)c->nfields, nullptr);
+    c->diag_have[q].assign((size_t)c->nfields, 0);
+    c->tabs_3d[q] = nullptr;
+    c->tabs_prof[q] = nullptr;
+    c->tabs_have[q] = 0;
+  }
   c->diag_partial = nullptr;
   for (int k = 0; k < MONC_TVD_NKERNEL_KINDS; k++) {
     c->prof_ms[k] = 0.0;
@@ -1331,10 +1357,16 @@ int monc_tvd_destroy(monc_tvd_ctx* c) {
       if (c->dual[b][f]) (void)hipFree(c->dual[b][f]);
   }
   for (int f = 0; f < c->nfields; f++) {
-    for (void* p : {c->diag_before[f], c->diag_3d[f], c->diag_zf[f], (void*)c->diag_prof[f]})
+    for (void* p : {c->diag_before[f], c->diag_zf[f]})
       if (p) (void)hipFree(p);
+    for (int q = 0; q < MONC_TVD_NCOMPONENTS; q++)
+      for (void* p : {c->diag_3d[q][f], (void*)c->diag_prof[q][f]})
+        if (p) (void)hipFree(p);
   }
-  for (void* p : {(void*)c->rprefrcp, c->tabs_3d, (void*)c->tabs_prof, (void*)c->diag_partial, (void*)c->thadv_tab,
+  for (int q = 0; q < MONC_TVD_NCOMPONENTS; q++)
+    for (void* p : {c->tabs_3d[q], (void*)c->tabs_prof[q]})
+      if (p) (void)hipFree(p);
+  for (void* p : {(void*)c->rprefrcp, (void*)c->diag_partial, (void*)c->thadv_tab,
                   (void*)c->wsum_scratch})
     if (p) (void)hipFree(p);
   if (c->dmet) (void)hipFree(c->dmet);
@@ -1575,7 +1607,7 @@ int monc_thadv_tendencies(monc_tvd_ctx* c, int anelastic, const double* dthref, 
                           const double* thref, double thref0, double fcoriol, double g, double rate_x, double rate_y,
                           double ugal, double vgal) {
   if (int rc = check_ctx(c)) return rc;
-  if (!anelastic && !baroclinicity) return MONC_TVD_OK;
+  if (!anelastic && !baroclinicity && !(c->diag_flags & MONC_TVD_DIAG_TENDENCY)) return MONC_TVD_OK;
   if (anelastic && !dthref) return fail(MONC_TVD_ERR_ARG, "anelastic th_advection needs dthref");
   if (baroclinicity && anelastic && !thref)
     return fail(MONC_TVD_ERR_ARG, "anelastic advection of the mean baroclinicity needs thref");
@@ -1663,8 +1695,15 @@ int monc_tvd_set_diagnostics(monc_tvd_ctx* c, int flags, const double* rprefrcp)
 }
 
 int monc_tvd_get_diagnostic(monc_tvd_ctx* c, int field, int which, double* host) {
+  return monc_tvd_get_component_diagnostic(c, MONC_TVD_COMPONENT_TVD, field, which, host);
+}
+
+int monc_tvd_get_component_diagnostic(monc_tvd_ctx* c, int comp, int field, int which, double* host) {
   if (int rc = check_ctx(c)) return rc;
   if (!host) return fail(MONC_TVD_ERR_ARG, "null host pointer");
+  if (comp < 0 || comp >= MONC_TVD_NCOMPONENTS) return fail(MONC_TVD_ERR_ARG, "unknown component %d", comp);
+  if (which == MONC_TVD_DIAG_ZFACES && comp != MONC_TVD_COMPONENT_TVD)
+    return fail(MONC_TVD_ERR_ARG, "z-face diagnostics belong to the TVD component only");
   const bool tabs = which == MONC_TVD_DIAG_TABS_3D || which == MONC_TVD_DIAG_TABS_PROFILE;
   if (!tabs && (field < 0 || field >= c->nfields))
     return fail(MONC_TVD_ERR_ARG, "field id %d out of range [0,%d)", field, c->nfields);
@@ -1673,24 +1712,24 @@ int monc_tvd_get_diagnostic(monc_tvd_ctx* c, int field, int which, double* host)
   bool fp64 = false;
   switch (which) {
     case MONC_TVD_DIAG_3D:
-      if (!c->diag_have[field]) break;
-      src = c->diag_3d[field];
+      if (!c->diag_have[comp][field]) break;
+      src = c->diag_3d[comp][field];
       n = (size_t)c->nz * c->ny * c->nx;
       break;
     case MONC_TVD_DIAG_PROFILE:
-      if (!c->diag_have[field]) break;
-      src = c->diag_prof[field];
+      if (!c->diag_have[comp][field]) break;
+      src = c->diag_prof[comp][field];
       n = c->nz;
       fp64 = true;
       break;
     case MONC_TVD_DIAG_TABS_3D:
-      if (!c->tabs_have) break;
-      src = c->tabs_3d;
+      if (!c->tabs_have[comp]) break;
+      src = c->tabs_3d[comp];
       n = (size_t)c->nz * c->ny * c->nx;
       break;
     case MONC_TVD_DIAG_TABS_PROFILE:
-      if (!c->tabs_have) break;
-      src = c->tabs_prof;
+      if (!c->tabs_have[comp]) break;
+      src = c->tabs_prof[comp];
       n = c->nz;
       fp64 = true;
       break;
@@ -1701,7 +1740,8 @@ int monc_tvd_get_diagnostic(monc_tvd_ctx* c, int field, int which, double* host)
     default:
       return fail(MONC_TVD_ERR_ARG, "unknown diagnostic %d", which);
   }
-  if (!src) return fail(MONC_TVD_ERR_STATE, "diagnostic %d of field %d was never recorded", which, field);
+  if (!src)
+    return fail(MONC_TVD_ERR_STATE, "diagnostic %d of field %d (component %d) was never recorded", which, field, comp);
   if (fp64 || c->precision == MONC_TVD_FP64) {
     HIP_TRY(hipMemcpyAsync(host, src, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -37,7 +37,7 @@ EXPORTS = (
     "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic", "monc_tvd_autotune",
     "monc_tvd_leapfrog_steps", "monc_tvd_forward_step_into", "monc_thadv_tendencies",
     "monc_pw_tendencies", "monc_tvd_set_field_active", "monc_tvd_test_fail_allocations",
-    "monc_tvd_weighted_sum_noalloc", "monc_tvd_get_tiling",
+    "monc_tvd_weighted_sum_noalloc", "monc_tvd_get_tiling", "monc_tvd_get_component_diagnostic",
     "monc_tvd_ensemble_create", "monc_tvd_ensemble_destroy", "monc_tvd_ensemble_layout",
     "monc_tvd_ensemble_set_field", "monc_tvd_ensemble_get_field", "monc_tvd_ensemble_copy_pool",
     "monc_tvd_ensemble_set_dt", "monc_tvd_ensemble_set_stream", "monc_tvd_ensemble_synchronize",
@@ -46,6 +46,7 @@ EXPORTS = (
 )
 DIAG_TENDENCY, DIAG_ZFACE = 1, 2
 DIAG_3D, DIAG_PROFILE, DIAG_TABS_3D, DIAG_TABS_PROFILE, DIAG_ZFACES = 0, 1, 2, 3, 4
+COMPONENT_TVD, COMPONENT_PW, COMPONENT_THADV = 0, 1, 2
 KERNEL_MARCH, KERNEL_CELL, KERNEL_STAGE = 0, 1, 2
 KERNEL_KINDS = ("u", "v", "w", "scalars", "halo", "cell", "stage")
 
@@ -93,6 +94,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_autotune.argtypes = [c_void_p, c_double, c_int, ctypes.POINTER(c_int)]
     lib.monc_tvd_get_tiling.argtypes = [c_void_p, ctypes.POINTER(c_int)]
     lib.monc_tvd_get_diagnostic.argtypes = [c_void_p, c_int, c_int, dp]
+    lib.monc_tvd_get_component_diagnostic.argtypes = [c_void_p, c_int, c_int, c_int, dp]
     lib.monc_tvd_set_profiling.argtypes = [c_void_p, c_int]
     lib.monc_tvd_weighted_sum.argtypes = [c_void_p, c_int, c_int, dp, dp]
     lib.monc_tvd_weighted_sum_noalloc.argtypes = [c_void_p, c_int, c_int, dp, dp]
@@ -263,8 +265,10 @@ class TvdAdvection:
         return [buf[i] for i in range(min(n, 16))]
 
     def set_diagnostics(self, flags: int, rprefrcp: Optional[np.ndarray] = None) -> None:
-        """DIAG_TENDENCY and/or DIAG_ZFACE recording on every tendencies() call;
-        rprefrcp (length nz) enables the tabs tendency."""
+        """DIAG_TENDENCY and/or DIAG_ZFACE recording on every tendencies() call
+        (DIAG_TENDENCY also on every pw_tendencies() and thadv_tendencies()
+        call, kept per component); rprefrcp (length nz) enables the tabs
+        tendency."""
         r = None
         if rprefrcp is not None:
             r = np.ascontiguousarray(rprefrcp, dtype=np.float64)
@@ -272,17 +276,21 @@ class TvdAdvection:
                 raise ValueError("rprefrcp must have length nz")
         _check(self._lib.monc_tvd_set_diagnostics(self._h, flags, _dptr(r) if r is not None else None))
 
-    def get_diagnostic(self, field: int, which: int) -> np.ndarray:
-        """A recorded diagnostic: DIAG_3D / DIAG_TABS_3D as (nx, ny, nz) arrays
+    def get_diagnostic(self, field: int, which: int, component: int = COMPONENT_TVD) -> np.ndarray:
+        """A recorded diagnostic of a component (COMPONENT_TVD, COMPONENT_PW,
+        COMPONENT_THADV): DIAG_3D / DIAG_TABS_3D as (nx, ny, nz) arrays
         (x slowest, the (z,y,x) column-major layout), DIAG_PROFILE /
-        DIAG_TABS_PROFILE as (nz,), DIAG_ZFACES in the field shape."""
+        DIAG_TABS_PROFILE as (nz,), DIAG_ZFACES (TVD only) in the field shape."""
         if which in (DIAG_3D, DIAG_TABS_3D):
             out = np.zeros((self.nx, self.ny, self.nz))
         elif which in (DIAG_PROFILE, DIAG_TABS_PROFILE):
             out = np.zeros(self.nz)
         else:
             out = np.zeros(self.field_shape())
-        _check(self._lib.monc_tvd_get_diagnostic(self._h, field, which, _dptr(out)))
+        if component == COMPONENT_TVD:
+            _check(self._lib.monc_tvd_get_diagnostic(self._h, field, which, _dptr(out)))
+        else:
+            _check(self._lib.monc_tvd_get_component_diagnostic(self._h, component, field, which, _dptr(out)))
         return out
 
     def weighted_sum(self, field: int, variant: int, weights: np.ndarray) -> float:

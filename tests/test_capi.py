@@ -69,6 +69,10 @@ def test_null_context_is_an_argument_error():
     assert "null context" in lib.monc_tvd_last_error().decode()
     assert lib.monc_pw_tendencies(None, 1, 1, 1, 0.005, 0.005) == tvd.ERR_ARG
     assert "null context" in lib.monc_tvd_last_error().decode()
+    out = np.zeros(4)
+    assert lib.monc_tvd_get_component_diagnostic(None, tvd.COMPONENT_PW, 0, tvd.DIAG_PROFILE,
+                                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == tvd.ERR_ARG
+    assert "null context" in lib.monc_tvd_last_error().decode()
     assert lib.monc_tvd_destroy(None) == tvd.OK
 
 

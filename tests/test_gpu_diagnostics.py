@@ -100,3 +100,142 @@ def test_diagnostics_need_recording(gpu):
             t.set_tiling(tvd.KERNEL_STAGE, 5, 0)
     finally:
         t.close()
+
+
+# ---------------------------------------------------------------------------
+# The sibling components' tendency diagnostics on the same context
+# (pwadvection.F90:619-743, thadvection.F90:228-270): tend_3d = s_after -
+# s_before of each field the component changes, over the interior columns,
+# its per-level total and the tabs tendency, recorded per component.
+
+def _diag_ref(after, before, fields, r):
+    t3d = {f: after[f][2:-2, 2:-2, :] - before[f][2:-2, 2:-2, :] for f in fields}
+    prof = {f: oracle.profile_total(t3d[f]) for f in fields}
+    tabs = None
+    if tvd.TH in fields:
+        tabs = after[tvd.TH][2:-2, 2:-2, :] * r - before[tvd.TH][2:-2, 2:-2, :] * r
+    return t3d, prof, tabs
+
+
+def _state_error(fn):
+    with pytest.raises(tvd.TvdError) as e:
+        fn()
+    assert e.value.code == tvd.ERR_STATE
+
+
+@pytest.mark.parametrize("flags", [(True, True, True), (True, False, False), (False, True, True)],
+                         ids=["all", "flow", "scalars"])
+def test_pw_tendency_diagnostics(gpu, flags):
+    st = ss.make_state(24, 12, 37, 50.0, 575.0, seed=31, n_q=2, anelastic_scale=5000.0)
+    rng = np.random.default_rng(31)
+    src = [ss.halo_fill(0.01 * rng.standard_normal(f.shape)) for f in st.fields]
+    r = 1.0 + 0.05 * rng.random(st.nz)
+    tcx, tcy = 0.25 * st.cx, 0.25 * st.cy
+    ref = oracle.pw(st.fields, src, st.grid, st.cx, st.cy, tcx, tcy, *flags)
+    fields = ([0, 1, 2] if flags[0] else []) + ([3] if flags[1] else []) + ([4, 5] if flags[2] else [])
+    t3d, prof, tabs = _diag_ref(ref, src, fields, r)
+    t = tvd.TvdAdvection.initialise(st, precision=tvd.FP64)
+    try:
+        for f, s in enumerate(src):
+            t.set_field(f, tvd.SOURCE, s)
+        t.set_diagnostics(tvd.DIAG_TENDENCY, r)
+        t.pw_tendencies(*flags, tcx=tcx, tcy=tcy)
+        for f in range(len(st.fields)):
+            if f in fields:
+                got = t.get_diagnostic(f, tvd.DIAG_3D, tvd.COMPONENT_PW)
+                assert _rel(got, t3d[f]) <= TOL, f
+                _prof_ok(t.get_diagnostic(f, tvd.DIAG_PROFILE, tvd.COMPONENT_PW), prof[f], t3d[f])
+            else:
+                _state_error(lambda: t.get_diagnostic(f, tvd.DIAG_3D, tvd.COMPONENT_PW))
+        if tabs is not None:
+            got = t.get_diagnostic(0, tvd.DIAG_TABS_3D, tvd.COMPONENT_PW)
+            assert _rel(got, tabs) <= TOL
+            _prof_ok(t.get_diagnostic(0, tvd.DIAG_TABS_PROFILE, tvd.COMPONENT_PW), oracle.profile_total(tabs), tabs)
+        else:
+            _state_error(lambda: t.get_diagnostic(0, tvd.DIAG_TABS_3D, tvd.COMPONENT_PW))
+        # each component keeps its own record: no TVD call was made
+        _state_error(lambda: t.get_diagnostic(0, tvd.DIAG_3D))
+        with pytest.raises(tvd.TvdError) as e:
+            t.get_diagnostic(0, tvd.DIAG_ZFACES, tvd.COMPONENT_PW)
+        assert e.value.code == tvd.ERR_ARG
+    finally:
+        t.close()
+
+
+def test_component_records_are_separate(gpu):
+    """pw_advection, then TVD, then th_advection on one context: each
+    component's tend_3d is its own increment of the sources."""
+    st = ss.make_state(20, 9, 11, 50.0, 475.0, seed=33, n_q=1, anelastic_scale=4000.0)
+    rng = np.random.default_rng(33)
+    s0 = [ss.halo_fill(0.01 * rng.standard_normal(f.shape)) for f in st.fields]
+    tcx, tcy = 0.25 * st.cx, 0.25 * st.cy
+    s1 = oracle.pw(st.fields, s0, st.grid, st.cx, st.cy, tcx, tcy, True, True, True)
+    s2, _ = oracle.state_tendencies(st, 0.7, 1, 1, 1, s1)
+    thref = 290.0 + 0.004 * st.grid.zn
+    dthref = np.zeros(st.nz)
+    dthref[:-1] = thref[1:] - thref[:-1]
+    kw = dict(dthref=dthref, thref=thref, anelastic=True, baroclinicity=True, thref0=300.0, fcoriol=1.0e-4,
+              g=9.81, rate_x=2.0e-3, rate_y=-1.0e-3, ugal=0.4, vgal=-0.2)
+    s3 = oracle.thadv(s2[3], st.fields[0], st.fields[1], st.fields[2], st.grid.tzc1, st.grid.tzc2, **kw)
+    t = tvd.TvdAdvection.initialise(st, precision=tvd.FP64)
+    try:
+        for f, s in enumerate(s0):
+            t.set_field(f, tvd.SOURCE, s)
+        t.set_diagnostics(tvd.DIAG_TENDENCY)
+        t.halo_fill()
+        t.pw_tendencies(True, True, True, tcx=tcx, tcy=tcy)
+        t.tendencies(0.7, 1, 1, 1)
+        t.thadv_tendencies(**kw)
+        for f in range(len(st.fields)):
+            want = {tvd.COMPONENT_PW: s1[f] - s0[f], tvd.COMPONENT_TVD: s2[f] - s1[f]}
+            for comp, d in want.items():
+                assert _rel(t.get_diagnostic(f, tvd.DIAG_3D, comp), d[2:-2, 2:-2, :]) <= TOL, (f, comp)
+        got = t.get_diagnostic(tvd.TH, tvd.DIAG_3D, tvd.COMPONENT_THADV)
+        assert _rel(got, (s3 - s2[3])[2:-2, 2:-2, :]) <= TOL
+        _state_error(lambda: t.get_diagnostic(tvd.U, tvd.DIAG_3D, tvd.COMPONENT_THADV))
+    finally:
+        t.close()
+
+
+@pytest.mark.parametrize("terms", [(True, False), (False, True), (False, False)],
+                         ids=["refstate", "baroclinicity", "none"])
+def test_thadv_tendency_diagnostics(gpu, terms):
+    """th's increment and tabs over the interior columns (the component
+    updates halo columns too; its diagnostics skip them), zero when no term
+    is switched on, and nothing while th is inactive."""
+    st = ss.make_state(30, 8, 10, 50.0, 725.0, seed=35, n_q=1, anelastic_scale=3000.0)
+    rng = np.random.default_rng(35)
+    sth = ss.halo_fill(1e-3 * rng.standard_normal(st.fields[3].shape))
+    r = 1.0 + 0.05 * rng.random(st.nz)
+    thref = 290.0 + 0.004 * st.grid.zn
+    dthref = np.zeros(st.nz)
+    dthref[:-1] = thref[1:] - thref[:-1]
+    kw = dict(dthref=dthref, thref=thref, anelastic=terms[0], baroclinicity=terms[1], thref0=300.0,
+              fcoriol=1.1e-4, g=9.81, rate_x=2.5e-3, rate_y=-1.2e-3, ugal=1.3, vgal=-0.6)
+    ref = oracle.thadv(sth, st.fields[0], st.fields[1], st.fields[2], st.grid.tzc1, st.grid.tzc2, **kw)
+    t3d = (ref - sth)[2:-2, 2:-2, :]
+    tabs = ref[2:-2, 2:-2, :] * r - sth[2:-2, 2:-2, :] * r
+    for active in (True, False):
+        t = tvd.TvdAdvection(st.nz, st.ny, st.nx, st.cx, st.cy, st.grid.metric_vectors(), n_q=1)
+        try:
+            for n in range(3):
+                t.set_field(n, tvd.PROG, st.fields[n])
+            t.set_field(tvd.TH, tvd.SOURCE, sth)
+            t.set_field_active(tvd.TH, active)
+            t.set_diagnostics(tvd.DIAG_TENDENCY, r)
+            t.thadv_tendencies(**kw)
+            if not active:
+                _state_error(lambda: t.get_diagnostic(tvd.TH, tvd.DIAG_3D, tvd.COMPONENT_THADV))
+                continue
+            got = t.get_diagnostic(tvd.TH, tvd.DIAG_3D, tvd.COMPONENT_THADV)
+            gtabs = t.get_diagnostic(0, tvd.DIAG_TABS_3D, tvd.COMPONENT_THADV)
+            if terms == (False, False):
+                assert not np.any(got) and not np.any(gtabs)
+            else:
+                assert _rel(got, t3d) <= TOL
+                assert _rel(gtabs, tabs) <= TOL
+            _prof_ok(t.get_diagnostic(tvd.TH, tvd.DIAG_PROFILE, tvd.COMPONENT_THADV), oracle.profile_total(t3d), t3d)
+            _prof_ok(t.get_diagnostic(0, tvd.DIAG_TABS_PROFILE, tvd.COMPONENT_THADV), oracle.profile_total(tabs),
+                     tabs)
+        finally:
+            t.close()
