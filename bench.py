@@ -20,7 +20,9 @@ rocprofv3 measured for that kernel at that shape
 shape); cpu_baseline = the C restatement (oracle) of the path on the host's
 cores, one process per core on x slabs of the same workload, run before the
 GPU is touched.  Beside it: the FP32 form, the FP32 adjoint step on the same
-domain, the MONC leapfrog timestep (FP64), the sibling components
+domain, the MONC leapfrog timestep (FP64), the drop-in tendency call itself
+(64^3 forward and centred, and the bench domain; 120 / 160 B per
+cell-update), the sibling components
 th_advection and pw_advection on the same FP64 state (kernel time and HBM
 fraction), and the batched ensemble (48 FP32 members of seeded sizes
 64..256 x 64..256 x 64..128, one persistent launch per step, graph-replayed).
@@ -292,6 +294,45 @@ def leapfrog_form(state, dt: float, steps: int):
         t.close()
 
 
+def tendency_form(state, dtm: float, stepping: int, calls: int):
+    """The drop-in call itself (monc_tvd_tendencies, the timestep callback over
+    every column) on resident FP64 fields: forward stepping reads the five
+    prognostics and read-modify-writes the five source fields (120 B per
+    cell-update), centred stepping also reads the five z-fields (160 B).
+    Kernel time from the library's HIP events, summed over the launches of
+    one call."""
+    import numpy as np
+
+    from monc_amd import tvd
+
+    t = tvd.TvdAdvection.initialise(state, precision=tvd.FP64)
+    try:
+        t.zero_sources(range(len(state.fields)))
+        t.halo_fill()
+        t.tendencies(dtm, stepping, stepping, stepping)
+        t.synchronize()
+        t.set_profiling(True)
+        t.kernel_times(reset=True)
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            t.tendencies(dtm, stepping, stepping, stepping)
+        t.synchronize()
+        wall = (time.perf_counter() - t0) / calls
+        kt = t.kernel_times(reset=True)
+        t.set_profiling(False)
+        ms = sum(v[0] for v in kt.values()) / calls
+        cells = state.nx * state.ny * state.nz
+        nbytes = (120.0 if stepping == tvd.FORWARD_STEPPING else 160.0) * cells
+        return {"cells": cells, "stepping": "forward" if stepping == tvd.FORWARD_STEPPING else "centred",
+                "kernel_ms": round(ms, 4), "wall_ms": round(wall * 1e3, 4),
+                "value": round(cells / (ms * 1e-3) / 1e9, 3), "unit": "Gcell-updates/s",
+                "bytes_per_cell_update": nbytes / cells, "achieved_GBs": round(nbytes / ms / 1e6, 1),
+                "frac_hbm": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4), "dtype": "f64",
+                "kernels_ms_per_call": {k: round(v[0] / calls, 4) for k, v in kt.items()}}
+    finally:
+        t.close()
+
+
 def thadv_form(state, steps: int):
     """th_advection's source terms (monc_thadv_tendencies: reference-state
     vertical advection and mean baroclinicity, every column) on the FP64
@@ -475,7 +516,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the adjoint, leapfrog and ensemble measurements")
-    ap.add_argument("--only", choices=["adjoint", "leapfrog", "ensemble", "thadv", "pw"], default=None,
+    ap.add_argument("--only", choices=["adjoint", "leapfrog", "ensemble", "thadv", "pw", "tendency"], default=None,
                     help="run only this side measurement (for profiling) and print its JSON object")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed work, run three 512 MiB device-to-device copies: the known-size "
@@ -517,6 +558,7 @@ def main() -> None:
                 "leapfrog": lambda: leapfrog_form(st, dt, max(2, args.steps // 2)),
                 "ensemble": lambda: ensemble_form(max(4, args.steps // 2)),
                 "thadv": lambda: thadv_form(st, max(4, args.steps)),
+                "tendency": lambda: tendency_form(st, dt, tvd.FORWARD_STEPPING, max(4, args.steps // 2)),
                 "pw": lambda: pw_form(st, max(4, args.steps))}[args.only]()
         print(json.dumps({args.only: side}), flush=True)
         return
@@ -582,6 +624,13 @@ def main() -> None:
         out["adjoint_fp32"] = adjoint_form(st, dt, max(2, args.steps // 2))
         log("bench: FP64 leapfrog timestep")
         out["leapfrog_fp64"] = leapfrog_form(st, dt, max(2, args.steps // 2))
+        log("bench: drop-in tendency calls (64^3 and the bench domain, FP64)")
+        c1 = ss.make_state(64, 64, 64, 50.0, 2000.0, seed=0, n_q=1)
+        out["tendency_fp64"] = {
+            "c1_64cubed_forward": tendency_form(c1, 1.0, tvd.FORWARD_STEPPING, 50),
+            "c1_64cubed_centred": tendency_form(c1, 1.0, tvd.CENTRED_STEPPING, 50),
+            "512x512x256_forward": tendency_form(st, dt, tvd.FORWARD_STEPPING, max(4, args.steps // 2))}
+        del c1
         log("bench: th_advection")
         out["thadv_fp64"] = thadv_form(st, max(4, args.steps))
         log("bench: pw_advection")

@@ -122,12 +122,16 @@ int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **
  * MONC_TVD_KERNEL_CELL: one thread per cell, every face recomputed (any
  * column height; the fallback beyond 256 levels or 4 GiB per field).
  * MONC_TVD_KERNEL_STAGE: the forward step (monc_tvd_forward_steps,
- * monc_tvd_forward_step_into) of each field kind with u, v, w (and the
- * kind's scalars) staged in LDS; rows = cell rows per block (4, 6, 8; 11 in
- * FP64, where the scalar pair takes 4, 6 or 8; 0: the default), strip as
- * above.  Other calls use the march kernel.  Without a tiling set (or after
- * kernel 0 with rows = strip = 0) the forward step stages FP64 u, v and w
- * where the domain gives at least 512 staged blocks, and marches the rest. */
+ * monc_tvd_forward_step_into) and the drop-in call (monc_tvd_tendencies) of
+ * each field kind with u, v, w (and the kind's scalars) staged in LDS;
+ * rows = cell rows per block (4, 6, 8; 11 in FP64, where the scalar pair
+ * takes 4, 6 or 8; 0: the default), strip as above.  A drop-in call whose
+ * momentum fields advect their z-fields (centred field stepping), or that
+ * records z-face diagnostics, marches those kinds; the leapfrog step and the
+ * other calls use the march kernel.  Without a tiling set (or after kernel 0
+ * with rows = strip = 0) the forward step and the drop-in call stage FP64 u,
+ * v and w where the domain gives at least 512 staged blocks, and march the
+ * rest. */
 #define MONC_TVD_KERNEL_MARCH 0
 #define MONC_TVD_KERNEL_CELL 1
 #define MONC_TVD_KERNEL_STAGE 2

@@ -275,11 +275,12 @@ constexpr int stage_wpe() {
   return sizeof(T) == 4 ? 4 : (KIND == KIND_SCALAR && NS == 2 ? 2 : 3);
 }
 
-template <typename T, int KIND, int NS, int R>
+template <typename T, int KIND, int NS, int R, int MODE>
 void launch_stage(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G, int strip) {
   const int nbands = (c->nz + 63) / 64, tiles_y = (c->ny + R - 1) / R, tiles_x = (c->nx + strip - 1) / strip;
-  hipLaunchKernelGGL((stage_kernel<T, KIND, NS, R, stage_wpe<T, KIND, NS, R>()>), dim3(nbands * tiles_y * tiles_x),
-                     dim3(64, R + 1, 1), 0, c->stream, L, G, strip, nbands, tiles_y);
+  hipLaunchKernelGGL((stage_kernel<T, KIND, NS, R, stage_wpe<T, KIND, NS, R>(), MODE>),
+                     dim3(nbands * tiles_y * tiles_x), dim3(64, R + 1, 1), 0, c->stream, L, G, strip, nbands,
+                     tiles_y);
 }
 
 // Rows per block of the stage kernel (11 only for the FP64 flow kinds).
@@ -307,13 +308,18 @@ int stage_default_strip(const monc_tvd_ctx* c, int kind) {
 
 bool stage_rows_ok(int rows) { return rows == 4 || rows == 6 || rows == 8 || rows == 11; }
 
-// A field group the stage kernel takes: MODE_STEP, a flow field advected by
-// itself or one or two scalars with the same timestep, fields under 4 GiB.
-template <typename T>
+// A field group the stage kernel takes: a flow field advected by itself or
+// one or two scalars with the same timestep, fields under 4 GiB; MODE_STEP
+// with each field stepped from the field it advects, MODE_ACCUM without
+// z-face diagnostics.
+template <typename T, int MODE>
 bool stage_fits(const monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G) {
   if (!c->buffer_ok || G.nf < 1 || G.nf > 2) return false;
-  for (int n = 0; n < G.nf; n++)
-    if (G.f[n] != G.F[n] || G.dt[n] != G.dt[0] || G.kind[n] != G.kind[0]) return false;
+  for (int n = 0; n < G.nf; n++) {
+    if (G.dt[n] != G.dt[0] || G.kind[n] != G.kind[0]) return false;
+    if (MODE == MODE_STEP && G.f[n] != G.F[n]) return false;
+    if (MODE == MODE_ACCUM && G.zf[n]) return false;
+  }
   switch (G.kind[0]) {
     case KIND_U: return G.nf == 1 && G.F[0] == L.u;
     case KIND_V: return G.nf == 1 && G.F[0] == L.v;
@@ -322,31 +328,31 @@ bool stage_fits(const monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>
   }
 }
 
-template <typename T, int KIND, int NS>
+template <typename T, int KIND, int NS, int MODE>
 void launch_stage_rows(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G, int rows, int strip) {
   switch (rows) {
-    case 6: launch_stage<T, KIND, NS, 6>(c, L, G, strip); break;
-    case 8: launch_stage<T, KIND, NS, 8>(c, L, G, strip); break;
+    case 6: launch_stage<T, KIND, NS, 6, MODE>(c, L, G, strip); break;
+    case 8: launch_stage<T, KIND, NS, 8, MODE>(c, L, G, strip); break;
     case 11:
       if constexpr (sizeof(T) == 8 && KIND != KIND_SCALAR) {
-        launch_stage<T, KIND, NS, 11>(c, L, G, strip);
+        launch_stage<T, KIND, NS, 11, MODE>(c, L, G, strip);
         break;
       }
       [[fallthrough]];
-    default: launch_stage<T, KIND, NS, 4>(c, L, G, strip); break;
+    default: launch_stage<T, KIND, NS, 4, MODE>(c, L, G, strip); break;
   }
 }
 
-template <typename T>
+template <typename T, int MODE = MODE_STEP>
 void launch_stage_group(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G, int kk, int rows,
                         int strip) {
   switch (G.kind[0]) {
-    case KIND_U: launch_stage_rows<T, KIND_U, 0>(c, L, G, rows, strip); break;
-    case KIND_V: launch_stage_rows<T, KIND_V, 0>(c, L, G, rows, strip); break;
-    case KIND_W: launch_stage_rows<T, KIND_W, 0>(c, L, G, rows, strip); break;
+    case KIND_U: launch_stage_rows<T, KIND_U, 0, MODE>(c, L, G, rows, strip); break;
+    case KIND_V: launch_stage_rows<T, KIND_V, 0, MODE>(c, L, G, rows, strip); break;
+    case KIND_W: launch_stage_rows<T, KIND_W, 0, MODE>(c, L, G, rows, strip); break;
     default:
-      if (G.nf == 2) launch_stage_rows<T, KIND_SCALAR, 2>(c, L, G, rows, strip);
-      else launch_stage_rows<T, KIND_SCALAR, 1>(c, L, G, rows, strip);
+      if (G.nf == 2) launch_stage_rows<T, KIND_SCALAR, 2, MODE>(c, L, G, rows, strip);
+      else launch_stage_rows<T, KIND_SCALAR, 1, MODE>(c, L, G, rows, strip);
   }
   (void)kk;
 }
@@ -388,18 +394,24 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
     const int kernel = leap ? MONC_TVD_KERNEL_MARCH : c->kernel;
     // the leapfrog step keeps the general shape: its FP64 pair measured
     // slower in the wide one (6.26 against 6.42 Gcell/s)
-    const int rows = c->rows[kk] > 0 && !leap ? c->rows[kk] : default_rows(c->precision, nch, leap ? -1 : kk);
+    // a kind set to the stage kernel that does not fit it (a drop-in call
+    // whose momentum fields advect their z-fields, z-face diagnostics)
+    // marches in the default shape: its rows and strip are the stage's
+    const bool own = !leap && c->kind_kernel[kk] != MONC_TVD_KERNEL_STAGE;
+    const int rows = own && c->rows[kk] > 0 && march_rows_ok(nch, c->rows[kk])
+                         ? c->rows[kk]
+                         : default_rows(c->precision, nch, leap ? -1 : kk);
     const bool wide = c->precision == MONC_TVD_FP64 && kk == 3 && nch == 4 && rows <= 2;
-    const int strip = c->strip[kk] > 0 && !leap ? c->strip[kk] : default_strip(c->ny, c->nx, rows, wide);
+    const int strip = own && c->strip[kk] > 0 ? c->strip[kk] : default_strip(c->ny, c->nx, rows, wide);
     LaunchTimer timer(c, kk);
-    if constexpr (MODE == MODE_STEP) {
+    if constexpr (MODE == MODE_STEP || MODE == MODE_ACCUM) {
       const int dstrip = stage_default_strip(c, S.kind[n]);
       const bool dflt = c->kind_kernel[kk] == MONC_TVD_KERNEL_MARCH && !c->rows[kk] && !c->strip[kk] &&
                         c->kernel == MONC_TVD_KERNEL_MARCH && dstrip > 0;
-      if ((c->kind_kernel[kk] == MONC_TVD_KERNEL_STAGE || dflt) && stage_fits<T>(c, L, G)) {
+      if ((c->kind_kernel[kk] == MONC_TVD_KERNEL_STAGE || dflt) && stage_fits<T, MODE>(c, L, G)) {
         const int srows = dflt ? 11 : (c->rows[kk] > 0 ? c->rows[kk] : 6);
         const int sstrip = dflt ? dstrip : (c->strip[kk] > 0 ? c->strip[kk] : 64);
-        launch_stage_group<T>(c, L, G, kk, srows, sstrip);
+        launch_stage_group<T, MODE>(c, L, G, kk, srows, sstrip);
         HIP_TRY(hipGetLastError());
         n += G.nf;
         continue;
@@ -938,7 +950,7 @@ int autotune_impl(monc_tvd_ctx* c, double dtm, int reps, int* chosen) {
     float best = 1e30f;
     int best_rows = 0, best_strip = 0, best_kernel = MONC_TVD_KERNEL_MARCH;
     // the stage kernel's shapes where it takes this field group
-    if (stage_fits<T>(c, L, G)) {
+    if (stage_fits<T, MODE_STEP>(c, L, G)) {
       for (int rows : STAGE_ROWS) {
         if (rows == 4 || !stage_shape_compiled(c->precision, kind, rows)) continue;
         for (int strip : {32, 64, 128}) {

@@ -261,11 +261,13 @@ __device__ __forceinline__ void skind_faces(const SView<T, R, NA>& s, const Stag
   for (int n = 0; n < NSF; n++) fzt[n] = has_z ? zf[n] : T(0);
 }
 
-// Divergence and forward step of the fields of one kind at (k, j, i)
-// (tvdadvection.F90:925-959; stepfields.F90:597-627): writes f + dtm*s at the
-// cell and its periodic halo images; levels the reference leaves untouched
-// (k = 1; k = nz for w) keep f.
-template <typename T, int KIND, int NSF, int R, int NA>
+// Divergence of the fields of one kind at (k, j, i) (tvdadvection.F90:
+// 925-959).  MODE_STEP, the forward step (stepfields.F90:597-627): writes
+// f + dtm*s at the cell and its periodic halo images; levels the reference
+// leaves untouched (k = 1; k = nz for w) keep f.  MODE_ACCUM, the drop-in
+// call: adds s to the source field `out` at levels 2..nz (w: 2..nz-1),
+// overwriting it at k = nz for u, v and the scalars (:942-959).
+template <typename T, int KIND, int NSF, int R, int NA, int MODE>
 __device__ __forceinline__ void skind_step(const SView<T, R, NA>& s, const StageLds<T, R, NA, NSF>& lds,
                                            const __amdgpu_buffer_rsrc_t (&out)[NSF], const Geo& g, const Dims& d,
                                            int k, int j, int i, int li, int lane, int r, T dtm, T cx, T cy, T ul,
@@ -284,6 +286,29 @@ __device__ __forceinline__ void skind_step(const SView<T, R, NA>& s, const Stage
   const bool top = !IS_W && k == nz;
   const unsigned vo = (unsigned)(k - 1) * (unsigned)sizeof(T);
   const unsigned so = g.so(j, i);
+  if constexpr (MODE == MODE_ACCUM) {
+    // the source read is clamped to the column (lanes past nz store nothing)
+    const unsigned vc = (unsigned)((k < nz ? k : nz) - 1) * (unsigned)sizeof(T);
+#pragma unroll
+    for (int n = 0; n < NSF; n++) {
+      const T fys = lds.yn[n][r][lane];
+      T fzb = __shfl_up(fzt[n], 1);
+      if (lane == 0) fzb = lds.zb[n][r];
+      const T s0 = ldb<T>(out[n], vc, so);
+      T sv;
+      if (top) {
+        sv = (vs * fys - vn * fyn[n]) * cy;
+        sv = sv + T(4.0) * t1 * wb * fzb;
+        sv = sv + (ul * fxl[n] - ur * fxr[n]) * cx;
+      } else {
+        sv = s0 + (vs * fys - vn * fyn[n]) * cy;
+        sv = sv + T(4.0) * (wb * fzb * t1 - wt * fzt[n] * t2);
+        sv = sv + (ul * fxl[n] - ur * fxr[n]) * cx;
+      }
+      if (cell) stb<T>(out[n], sv, vo, so);
+    }
+    return;
+  }
   const int jh = (j <= 4) ? j + ny : (j >= ny + 1 ? j - ny : 0);
   const int ih = (i <= 4) ? i + nx : (i >= nx + 1 ? i - nx : 0);
   const unsigned soj = g.so(jh ? jh : j, i), soi = g.so(j, ih ? ih : i), soji = g.so(jh ? jh : j, ih ? ih : i);
@@ -406,8 +431,10 @@ __device__ __forceinline__ void skind_edge(const SView<T, R, NA>& s, StageLds<T,
 // by the same flows): a 1-D grid of (row tile, band, strip) tiles, row tiles
 // fastest so that consecutive tiles of one XCD share halo rows in its L2
 // (blocks are dealt to the 8 XCDs round-robin).  MODE_STEP with every advected
-// field its own prognostic; G holds the kind's fields (F, out) in order.
-template <typename T, int KIND, int NS, int R, int WPE>
+// field its own prognostic, or MODE_ACCUM (the drop-in tendency call) with
+// the flow kinds advecting their own prognostic; G holds the kind's fields
+// (F, out) in order.
+template <typename T, int KIND, int NS, int R, int WPE, int MODE>
 __global__ void __launch_bounds__(64 * (R + 1)) __attribute__((amdgpu_waves_per_eu(WPE)))
 stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y) {
   constexpr bool SC = KIND == KIND_SCALAR;
@@ -483,7 +510,8 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
     }
     __syncthreads();
     if (active)
-      skind_step<T, KIND, NSF>(sv, lds, out, g, L.d, k, j, i, li, lane, r, dtm, cx, cy, ul, fxl, fxr, fyn, fzt);
+      skind_step<T, KIND, NSF, R, NA, MODE>(sv, lds, out, g, L.d, k, j, i, li, lane, r, dtm, cx, cy, ul, fxl, fxr,
+                                            fyn, fzt);
     if (more) stage_store<T, R, NA>(fetch, lds, wv, lane, i + 3);
 #pragma unroll
     for (int n = 0; n < NSF; n++) fxl[n] = fxr[n];

@@ -284,6 +284,52 @@ def test_stage_kernel_matches_oracle(gpu, nz, rows, strip, n_q):
         assert e <= TOL, (n, e)
 
 
+STAGE_ACCUM_CASES = [(nz, rows) for nz in (5, 64, 70, 150, 256) for rows in (4, 6, 11)]
+
+
+@pytest.mark.parametrize("nz,rows", STAGE_ACCUM_CASES)
+@pytest.mark.parametrize("fs,ms,sc", [(1, 1, 1), (1, 0, 0), (0, 0, 0)])
+def test_stage_kernel_drop_in_call(gpu, nz, rows, fs, ms, sc):
+    """The drop-in call (s += tendency, overwritten at k = nz) through the
+    stage kernel, into non-zero sources; with centred field stepping the
+    momentum fields advect their z-fields and march, the scalars stage."""
+    st = ss.make_state(nz, 11, 13, 50.0, 25.0 * (nz - 1), seed=nz + 7 * rows + fs, n_q=2,
+                       anelastic_scale=6000.0 if rows == 6 else None)
+    rng = np.random.default_rng(nz + rows)
+    src = [ss.halo_fill(0.01 * rng.standard_normal(f.shape)) for f in st.fields]
+    ref, _ = oracle.state_tendencies(st, 1.0, fs, ms, sc, src)
+    got = gpu_tendencies(st, 1.0, fs, ms, sc, sources=src, tiling=(tvd.KERNEL_STAGE, rows, 5))
+    _compare(got, ref)
+
+
+def test_default_drop_in_call_stages_at_scale(gpu):
+    """512x512x256 FP64: the default drop-in call (u, v, w staged) against the
+    march kernel forced for every kind, both into the same sources."""
+    st = ss.make_state(256, 512, 512, 25.0, 5000.0, seed=0, n_q=1, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    rng = np.random.default_rng(5)
+    src = [ss.halo_fill(0.01 * rng.standard_normal(st.fields[0].shape)) for _ in range(2)]
+    src += [np.zeros(st.fields[0].shape)] * 3
+    t = tvd.TvdAdvection.initialise(st, precision=tvd.FP64)
+    try:
+        assert [t.tiling()[k][0] for k in ("u", "v", "w")] == [tvd.KERNEL_STAGE] * 3  # the default
+        out = []
+        for tiling in (None, (tvd.KERNEL_MARCH, 3, 32)):
+            if tiling is not None:
+                t.set_tiling(*tiling)
+            for f in range(5):
+                t.set_field(f, tvd.SOURCE, src[f])
+            t.halo_fill()
+            t.tendencies(dt, 1, 1, 1)
+            out.append([t.get_field(f, tvd.SOURCE) for f in range(5)])
+        assert [k[0] for k in list(t.tiling().values())[:3]] == [tvd.KERNEL_MARCH] * 3  # forced
+    finally:
+        t.close()
+    for n in range(5):
+        e = rel_err(out[0][n], out[1][n])
+        assert e <= TOL, (n, e)
+
+
 @pytest.mark.parametrize("rows", [4, 6, 8])
 def test_stage_kernel_fp32_matches_march(gpu, rows):
     st = ss.make_state(130, 20, 18, 50.0, 2500.0, seed=77, n_q=1, with_zfields=False)
