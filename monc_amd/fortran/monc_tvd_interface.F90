@@ -15,6 +15,11 @@ module monc_tvd_interface_mod
        MONC_TVD_ADVECT_TRACER = 8
   integer(c_int), parameter :: MONC_TVD_U = 0, MONC_TVD_V = 1, MONC_TVD_W = 2, MONC_TVD_TH = 3
   integer(c_int), parameter :: MONC_TVD_PROG = 0, MONC_TVD_ZFIELD = 1, MONC_TVD_SOURCE = 2
+  integer(c_int), parameter :: MONC_TVD_ERR_ARG = 1, MONC_TVD_ERR_HIP = 2, MONC_TVD_ERR_STATE = 3
+  integer(c_int), parameter :: MONC_TVD_DIAG_TENDENCY = 1, MONC_TVD_DIAG_ZFACE = 2
+  integer(c_int), parameter :: MONC_TVD_DIAG_3D = 0, MONC_TVD_DIAG_PROFILE = 1, MONC_TVD_DIAG_TABS_3D = 2, &
+       MONC_TVD_DIAG_TABS_PROFILE = 3, MONC_TVD_DIAG_ZFACES = 4
+  integer(c_int), parameter :: MONC_TVD_COMPONENT_TVD = 0, MONC_TVD_COMPONENT_PW = 1, MONC_TVD_COMPONENT_THADV = 2
 
   interface
     integer(c_int) function monc_tvd_create(nz, ny, nx, halo, n_q, n_tracers, advect_flags, cx, cy, rdz, rdzn, &
@@ -104,6 +109,29 @@ module monc_tvd_interface_mod
       integer(c_int), value :: field, variant
       type(c_ptr), value :: device_ptr
     end function monc_tvd_bind_field
+
+    !> Tendency diagnostics (the published tend_* fields): rprefrcp may be
+    !! c_null_ptr, else the address of a length-nz vector.
+    integer(c_int) function monc_tvd_set_diagnostics(ctx, flags, rprefrcp) bind(C, name="monc_tvd_set_diagnostics")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, rprefrcp
+      integer(c_int), value :: flags
+    end function monc_tvd_set_diagnostics
+
+    integer(c_int) function monc_tvd_get_diagnostic(ctx, field, which, host) bind(C, name="monc_tvd_get_diagnostic")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field, which
+      real(c_double), dimension(*), intent(out) :: host
+    end function monc_tvd_get_diagnostic
+
+    integer(c_int) function monc_tvd_get_component_diagnostic(ctx, component, field, which, host) &
+        bind(C, name="monc_tvd_get_component_diagnostic")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: component, field, which
+      real(c_double), dimension(*), intent(out) :: host
+    end function monc_tvd_get_component_diagnostic
 
     integer(c_int) function monc_tvd_synchronize(ctx) bind(C, name="monc_tvd_synchronize")
       import :: c_int, c_ptr

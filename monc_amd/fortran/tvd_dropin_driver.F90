@@ -12,9 +12,13 @@
 !! cy, dtm), the 8 metric vectors (rdz, rdzn, dz, dzn, tzc1, tzc2, tzd1, tzd2;
 !! nz each), then the prognostics and the z-fields of u, v, w, th, q(:),
 !! tracer(:), each (nz, ny+4, nx+4) float64.
-!! Output file: the source fields in the same layout and order.
+!! Output file: the source fields in the same layout and order, then for u,
+!! v, w, th and each q the TVD tendency diagnostics as the component's
+!! field_value_retrieval would hand them out (tend_*_tvdadvection_3d_local,
+!! (nz, ny, nx), and _profile_total_local, nz), zeros for a field that was
+!! not advected (no diagnostic recorded).
 program tvd_dropin_driver
-  use iso_c_binding, only : c_int, c_double, c_ptr
+  use iso_c_binding, only : c_int, c_double, c_ptr, c_null_ptr
   use monc_tvd_interface_mod
   use tvd_shim_core_mod
   implicit none
@@ -23,8 +27,9 @@ program tvd_dropin_driver
   integer(c_int) :: hdr(13), fs, ms, ss, prec, n
   integer :: local_size(3), nq, ntr, nf
   integer(c_int), allocatable :: qact(:)
+  integer(c_int) :: rc
   real(c_double) :: rhdr(3)
-  real(c_double), allocatable :: met(:,:), fld(:,:,:), zero(:,:,:)
+  real(c_double), allocatable :: met(:,:), fld(:,:,:), zero(:,:,:), t3d(:,:,:), prof(:)
   type(c_ptr) :: ctx
   integer :: u_in, u_out
 
@@ -60,12 +65,26 @@ program tvd_dropin_driver
   close(u_in)
 
   call check(monc_tvd_halo_fill(ctx), "halo fill")
+  call check(monc_tvd_set_diagnostics(ctx, MONC_TVD_DIAG_TENDENCY, c_null_ptr), "diagnostics")
   call check(monc_tvd_tendencies(ctx, rhdr(3), fs, ms, ss), "tendencies")
 
   open(newunit=u_out, file=trim(out_path), access="stream", form="unformatted", status="replace", action="write")
   do n = 0, nf - 1
     call check(monc_tvd_get_field(ctx, n, MONC_TVD_SOURCE, fld), "get source")
     write(u_out) fld
+  end do
+  allocate(t3d(local_size(Z_INDEX), local_size(Y_INDEX), local_size(X_INDEX)), prof(local_size(Z_INDEX)))
+  do n = 0, 3 + nq
+    rc = monc_tvd_get_diagnostic(ctx, n, MONC_TVD_DIAG_3D, t3d)
+    if (rc == MONC_TVD_ERR_STATE) then
+      t3d = 0.0_c_double
+      prof = 0.0_c_double
+    else
+      call check(rc, "tend_3d")
+      call check(monc_tvd_get_diagnostic(ctx, n, MONC_TVD_DIAG_PROFILE, prof), "profile total")
+    end if
+    write(u_out) t3d
+    write(u_out) prof
   end do
   close(u_out)
   call check(monc_tvd_destroy(ctx), "destroy")

@@ -22,7 +22,20 @@ def write_input(path, state, fs, ms, ss, dtm, precision=0, advect=(1, 1, 1), th_
 
 
 def read_output(path, state):
+    """The source fields, then per non-tracer field (u, v, w, th, q(:)) the
+    tend_3d diagnostic as an (nx, ny, nz) array and its profile total."""
     data = np.fromfile(path, dtype=np.float64)
     shape = state.shape()
     n = int(np.prod(shape))
-    return [data[i * n:(i + 1) * n].reshape(shape) for i in range(len(state.fields))]
+    nf = len(state.fields)
+    sources = [data[i * n:(i + 1) * n].reshape(shape) for i in range(nf)]
+    m = state.nx * state.ny * state.nz
+    off = nf * n
+    diags = []
+    for _ in range(4 + state.n_q):
+        t3d = data[off:off + m].reshape(state.nx, state.ny, state.nz)
+        prof = data[off + m:off + m + state.nz]
+        diags.append((t3d, prof))
+        off += m + state.nz
+    assert off == data.size
+    return sources, diags
