@@ -216,8 +216,8 @@ int monc_tvd_get_component_diagnostic(monc_tvd_ctx *ctx, int component, int fiel
  * stream; monc_tvd_kernel_times synchronises, folds the events into per-kind
  * totals and returns the first n of them: milliseconds and launch counts for
  * kinds 0 = u, 1 = v, 2 = w, 3 = scalars, 4 = whole-field kernels (halo
- * fill, th_advection), 5 = one-cell-per-thread kernel (all fields), 6 = the
- * fused stage kernel (all fields).
+ * fill, th_advection), 5 = one-cell-per-thread kernel (all fields); kind 6
+ * is reserved (always empty).
  * reset != 0 clears the totals afterwards. */
 #define MONC_TVD_NKERNEL_KINDS 7
 int monc_tvd_set_profiling(monc_tvd_ctx *ctx, int on);

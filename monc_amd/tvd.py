@@ -48,7 +48,7 @@ DIAG_TENDENCY, DIAG_ZFACE = 1, 2
 DIAG_3D, DIAG_PROFILE, DIAG_TABS_3D, DIAG_TABS_PROFILE, DIAG_ZFACES = 0, 1, 2, 3, 4
 COMPONENT_TVD, COMPONENT_PW, COMPONENT_THADV = 0, 1, 2
 KERNEL_MARCH, KERNEL_CELL, KERNEL_STAGE = 0, 1, 2
-KERNEL_KINDS = ("u", "v", "w", "scalars", "halo", "cell", "stage")
+KERNEL_KINDS = ("u", "v", "w", "scalars", "halo", "cell", "unused")
 
 
 def Q(n: int) -> int:
@@ -238,8 +238,8 @@ class TvdAdvection:
 
     def set_tiling(self, kernel: int = KERNEL_MARCH, rows: int = 0, strip: int = 0) -> None:
         """kernel KERNEL_MARCH (default; rows = cell rows per block, 0 the
-        default shape) or KERNEL_CELL; strip = x columns per
-        block (0: 32)."""
+        default shape), KERNEL_CELL or KERNEL_STAGE (each kind staged in
+        LDS); strip = x columns per block (0: the default)."""
         _check(self._lib.monc_tvd_set_tiling(self._h, kernel, rows, strip))
 
     def autotune(self, dtm: float, reps: int = 3) -> dict:
