@@ -118,7 +118,8 @@ int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **
  * library).  kernel MONC_TVD_KERNEL_MARCH (default, columns of up to 256
  * levels): each face computed once, rows = cell rows per block (0: the
  * default for the column height; otherwise one of the shapes listed by
- * monc_tvd_march_shapes), strip = x columns per block (0: 32).
+ * monc_tvd_march_shapes), strip = x columns per block (0: 32 on large
+ * domains, narrower down to 1 until a launch has 256 blocks on small ones).
  * MONC_TVD_KERNEL_CELL: one thread per cell, every face recomputed (any
  * column height; the fallback beyond 256 levels or 4 GiB per field).
  * MONC_TVD_KERNEL_STAGE: the forward step (monc_tvd_forward_steps,

@@ -218,10 +218,16 @@ int default_rows(int precision, int nch, int kind) {
   return (precision == MONC_TVD_FP64 ? fp64 : fp32)[nch - 1];
 }
 
+// Small domains take narrower strips until the launch has 256 blocks (one
+// per CU): at 64^3 the step went from 0.28 to 0.055 ms (FP64) and 0.17 to
+// 0.036 ms (FP32) in strips of 2 columns instead of 16.
 int default_strip(int ny, int nx, int rows, bool wide) {
   const long tiles_y = (ny + rows - 1) / rows;
   if (wide && tiles_y * ((nx + 63) / 64) >= 1024) return 64;
-  return tiles_y * ((nx + 31) / 32) >= 1024 ? 32 : 16;
+  if (tiles_y * ((nx + 31) / 32) >= 1024) return 32;
+  int strip = 16;
+  while (strip > 1 && tiles_y * ((nx + strip - 1) / strip) < 256) strip /= 2;
+  return strip;
 }
 
 bool march_rows_ok(int nch, int rows) {
