@@ -114,6 +114,11 @@ def profiled(precision: str, kind: str, nz: int, tiling):
     return entry
 
 
+def cells_of(nz: int) -> int:
+    """Interior cells of the profiled workload (512 x 512 columns)."""
+    return 512 * 512 * nz
+
+
 def roofline_entry(precision: str, kind: str, kavg_ms: float, alg_bytes: float, nz: int, tiling):
     """HBM fraction at algorithmic bytes, and, from the profile at the same
     kernel and shape, the measured HBM traffic, the VALU fraction
@@ -136,6 +141,8 @@ def roofline_entry(precision: str, kind: str, kavg_ms: float, alg_bytes: float, 
                 "valu": {"insts_per_launch": prof["valu_insts"], "achieved": round(prof["valu_insts"] * 64 / t, 0),
                          "peak": VALU_PEAK, "unit": "lane-instructions/s", "frac": round(valu, 4)},
                 "vmem_busy": prof["ta_busy"], "limiter": max(fracs, key=fracs.get),
+                # vector-memory read instructions (each 64 lanes) per 64-cell column of the launch
+                "vmem_wave_loads_per_column": round(prof["vmem_rd"] / (cells_of(nz) / 64.0), 2),
                 "profiled_kernel_ms": prof["ms"],
                 "counters": f"profile {os.path.relpath(PROFILE, ROOT)} ({prof['kernel']})"})
     return out
