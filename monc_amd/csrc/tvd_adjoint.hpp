@@ -58,6 +58,14 @@ struct LdsAcc {
     if (v != T(0)) unsafeAtomicAdd(&s[((i & 3) * rows + (j - j0 + 1)) * ADJ_KB + (k - k0 + 1)], v);
   }
   __device__ __forceinline__ void dyn(int k, int j, int i, T v) const { add(k, j, i, v); }
+  // Plain read-add-write, for a location no other lane or wave touches until
+  // the next barrier.
+  __device__ __forceinline__ void put(int k, int j, int i, T v) const {
+    if (v != T(0)) {
+      T* p = &s[((i & 3) * rows + (j - j0 + 1)) * ADJ_KB + (k - k0 + 1)];
+      *p = *p + v;
+    }
+  }
 };
 
 // One face owner's scatter gathered in registers before it reaches the LDS
@@ -104,14 +112,21 @@ template <typename T>
 __device__ __forceinline__ bool any_nz(T x) { return __builtin_amdgcn_ballot_w64(x != T(0)) != 0; }
 
 // Adds the slots of every lane of the wave (lane L = level k0+L of row j,
-// column i) to the rings.  Called by all 64 lanes together.
+// column i) to the rings, without atomics.  Called by every wave of the block
+// together (it holds barriers), after each face.  The slots of row offset dj
+// go in phase dj+1: in one phase the wave of owner row r writes ring row
+// r+dj only, so no two waves write one row, and within a wave each lane
+// writes its own level (lane 0 and 63 alone the band's edges); a barrier
+// closes each phase.  The first barrier waits for every wave's face, whose
+// run-time-offset adds (dyn) go to the rings with LDS atomics.
 template <typename T>
 __device__ __forceinline__ void flush_slots(const Slots<T>& s, const LdsAcc<T> (&ring)[4], int k, int j, int i,
                                             int lane) {
+  __syncthreads();
 #pragma unroll
-  for (int t = 0; t < 4; t++)
+  for (int b = 0; b < 4; b++) {
 #pragma unroll
-    for (int b = 0; b < 4; b++)
+    for (int t = 0; t < 4; t++)
 #pragma unroll
       for (int c = 0; c < 4; c++) {
         const T vm = s.v[t][0][b][c], v0 = s.v[t][1][b][c], v1 = s.v[t][2][b][c], v2 = s.v[t][3][b][c];
@@ -125,12 +140,14 @@ __device__ __forceinline__ void flush_slots(const Slots<T>& s, const LdsAcc<T> (
         T tot = v0;
         if (h1 || h2) tot += from_below(up);
         if (hm) tot += from_above(vm);
-        ring[t].add(k, jj, ii, tot);
-        // the band's edges: level k0-1 from lane 0, k0+64 and k0+65 from lanes 62 and 63
-        if (hm) ring[t].add(k - 1, jj, ii, lane == 0 ? vm : T(0));
-        if (h1 || h2) ring[t].add(k + 1, jj, ii, lane == 63 ? up : T(0));
-        if (h2) ring[t].add(k + 2, jj, ii, lane == 63 ? v2 : T(0));
+        ring[t].put(k, jj, ii, tot);
+        // the band's edges: level k0-1 from lane 0, k0+64 and k0+65 from lane 63
+        if (hm && lane == 0) ring[t].put(k - 1, jj, ii, vm);
+        if ((h1 || h2) && lane == 63) ring[t].put(k + 1, jj, ii, up);
+        if (h2 && lane == 63) ring[t].put(k + 2, jj, ii, v2);
       }
+    __syncthreads();
+  }
 }
 
 // Adjoint of Flow<T,DK,DJ,DI>: the dual-grid interpolation scatters to both points.
@@ -626,7 +643,9 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
                                               const AF& aF, const AU& aU, const AV& aV, const AW& aW,
                                               const FL& after_face) {
   const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;
-  if (j > ny + 2) return;  // uniform over the wave
+  // rows past ny+2 own no face, but their waves still reach every
+  // after_face() (it may hold barriers); uniform over the wave
+  const bool own = j <= ny + 2;
   const Flows<T, KIND> fl(L);
   Adv<T, 1> F{};
   F.a[0] = Arr<T>(A.F, L.d);
@@ -643,7 +662,7 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
     const bool cell = jj >= 3 && jj <= ny + 2 && ii >= 3 && ii <= nx + 2 && kk >= 2 && kk <= kmax;
     return cell ? A.lin[at(L.d, kk, jj, ii)] : T(0.0);
   };
-  const bool row_in = j >= 3, col_in = i >= 3;
+  const bool row_in = own && j >= 3, col_in = own && i >= 3;
   const T lself = lraw(k, j, i);
   // x face (k, j, i+1/2): s(i) -= U fx cx, s(i+1) += U fx cx
   if (row_in && k >= 2 && k <= kmax) {
