@@ -58,13 +58,8 @@ struct LdsAcc {
     if (v != T(0)) unsafeAtomicAdd(&s[((i & 3) * rows + (j - j0 + 1)) * ADJ_KB + (k - k0 + 1)], v);
   }
   __device__ __forceinline__ void dyn(int k, int j, int i, T v) const { add(k, j, i, v); }
-  // Plain read-add-write, for a location no other lane or wave touches until
-  // the next barrier.
-  __device__ __forceinline__ void put(int k, int j, int i, T v) const {
-    if (v != T(0)) {
-      T* p = &s[((i & 3) * rows + (j - j0 + 1)) * ADJ_KB + (k - k0 + 1)];
-      *p = *p + v;
-    }
+  __device__ __forceinline__ T* ptr(int k, int j, int i) const {
+    return &s[((i & 3) * rows + (j - j0 + 1)) * ADJ_KB + (k - k0 + 1)];
   }
 };
 
@@ -107,9 +102,10 @@ struct SlotAcc {
   __device__ __forceinline__ void dyn(int kk, int jj, int ii, T v) const { ring[TGT].add(kk, jj, ii, v); }
 };
 
-// any lane of the wave non-zero (uniform; folds to false for a constant zero)
+// A slot that some add() reaches (not the constant zero of one no call site
+// reaches); decided at compile time.
 template <typename T>
-__device__ __forceinline__ bool any_nz(T x) { return __builtin_amdgcn_ballot_w64(x != T(0)) != 0; }
+__device__ __forceinline__ bool live(T x) { return !(__builtin_constant_p(x) && x == T(0)); }
 
 // Adds the slots of every lane of the wave (lane L = level k0+L of row j,
 // column i) to the rings, without atomics.  Called by every wave of the block
@@ -118,34 +114,87 @@ __device__ __forceinline__ bool any_nz(T x) { return __builtin_amdgcn_ballot_w64
 // r+dj only, so no two waves write one row, and within a wave each lane
 // writes its own level (lane 0 and 63 alone the band's edges); a barrier
 // closes each phase.  The first barrier waits for every wave's face, whose
-// run-time-offset adds (dyn) go to the rings with LDS atomics.
+// run-time-offset adds (dyn) go to the rings with LDS atomics.  Within a
+// phase every ring value is read before any is written (the locations are
+// distinct), so the reads go out together instead of one LDS round trip per
+// read-add-write.
 template <typename T>
 __device__ __forceinline__ void flush_slots(const Slots<T>& s, const LdsAcc<T> (&ring)[4], int k, int j, int i,
                                             int lane) {
   __syncthreads();
 #pragma unroll
   for (int b = 0; b < 4; b++) {
+    const int jj = j + b - 1;
+    // level k0+L collects dk = 0 of lane L, +1 of lane L-1, +2 of lane L-2 and
+    // -1 of lane L+1; `up` at lane L is what lanes L and L-1 send to level
+    // k0+L+1
+    T tot[4][4], up[4][4];
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
       for (int c = 0; c < 4; c++) {
         const T vm = s.v[t][0][b][c], v0 = s.v[t][1][b][c], v1 = s.v[t][2][b][c], v2 = s.v[t][3][b][c];
-        const bool hm = any_nz(vm), h1 = any_nz(v1), h2 = any_nz(v2);
-        const int jj = j + b - 1, ii = i + c - 1;
-        // level k0+L collects dk = 0 of lane L, +1 of lane L-1, +2 of lane L-2
-        // and -1 of lane L+1; `up` at lane L is what lanes L and L-1 send to
-        // level k0+L+1
-        T up = v1;
-        if (h2) up += from_below(v2);
-        T tot = v0;
-        if (h1 || h2) tot += from_below(up);
-        if (hm) tot += from_above(vm);
-        ring[t].put(k, jj, ii, tot);
-        // the band's edges: level k0-1 from lane 0, k0+64 and k0+65 from lane 63
-        if (hm && lane == 0) ring[t].put(k - 1, jj, ii, vm);
-        if ((h1 || h2) && lane == 63) ring[t].put(k + 1, jj, ii, up);
-        if (h2 && lane == 63) ring[t].put(k + 2, jj, ii, v2);
+        up[t][c] = v1;
+        if (live(v2)) up[t][c] += from_below(v2);
+        tot[t][c] = v0;
+        if (live(v1) || live(v2)) tot[t][c] += from_below(up[t][c]);
+        if (live(vm)) tot[t][c] += from_above(vm);
       }
+    T* p[4][4];
+    T old[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (live(tot[t][c])) {
+          p[t][c] = ring[t].ptr(k, jj, i + c - 1);
+          old[t][c] = *p[t][c];
+        }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (live(tot[t][c])) *p[t][c] = old[t][c] + tot[t][c];
+    // the band's edges: level k0-1 from lane 0, k0+64 and k0+65 from lane 63
+    if (lane == 0) {
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          if (live(s.v[t][0][b][c])) {
+            p[t][c] = ring[t].ptr(k - 1, jj, i + c - 1);
+            old[t][c] = *p[t][c];
+          }
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          if (live(s.v[t][0][b][c])) *p[t][c] = old[t][c] + s.v[t][0][b][c];
+    }
+    if (lane == 63) {
+      T* q[4][4];
+      T old2[4][4];
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          if (live(up[t][c])) {
+            p[t][c] = ring[t].ptr(k + 1, jj, i + c - 1);
+            old[t][c] = *p[t][c];
+          }
+          if (live(s.v[t][3][b][c])) {
+            q[t][c] = ring[t].ptr(k + 2, jj, i + c - 1);
+            old2[t][c] = *q[t][c];
+          }
+        }
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          if (live(up[t][c])) *p[t][c] = old[t][c] + up[t][c];
+          if (live(s.v[t][3][b][c])) *q[t][c] = old2[t][c] + s.v[t][3][b][c];
+        }
+    }
     __syncthreads();
   }
 }
@@ -716,7 +765,7 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjAr
   const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;
   const int lane = threadIdx.x, r = (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
   const int tid = lane + 64 * r;
-  const int k0 = 1 + 64 * band, j0 = 2 + ty * R;
+  const int k0 = __builtin_amdgcn_readfirstlane(1 + 64 * band), j0 = __builtin_amdgcn_readfirstlane(2 + ty * R);
   const int i_begin = 2 + tx * strip, i_end = min(i_begin + strip, nx + 3);
   // An owner's face adjoints are proportional to the incoming adjoint at
   // (k,j,i), (k,j,i+1), (k,j+1,i) and (k+1,j,i): a tile that reads only zeros
@@ -759,7 +808,11 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjAr
       *p = T(0);
     }
   };
-  for (int i = i_begin; i < i_end; i++) {
+  for (int ic = i_begin; ic < i_end; ic++) {
+    // the column is uniform over the block; saying so keeps it in a scalar
+    // register (a column offset in a vector register makes every buffer
+    // load a loop over the distinct offsets of the wave)
+    const int i = __builtin_amdgcn_readfirstlane(ic);
     {
       const int k = k0 + lane, j = j0 + r;
       Slots<T> sl{};
