@@ -74,32 +74,50 @@ struct LdsAcc {
 // with wave shifts (lane L of a wave holds level k0+L), leaving one LDS atomic
 // per (t, dj, di) and the band's edge levels.  dyn() takes an offset chosen at
 // run time (an upwind pick) straight to the ring.
-template <typename T>
+template <typename T, int NT>
 struct Slots {
-  T v[4][4][4][4];  // [target][dk+1][dj+1][di+1]
+  T v[NT][4][4][4];  // [target][dk+1][dj+1][di+1]
 };
 
-template <typename T, int TGT>
+// Target t's slot for (kk, jj, ii), a fixed offset from the owner (k, j, i).
+template <typename T, int NT>
+__device__ __forceinline__ void slot_add(Slots<T, NT>* s, const LdsAcc<T>* ring, int t, int k, int j, int i, int kk,
+                                         int jj, int ii, T v) {
+  const int dk = kk - k, dj = jj - j, di = ii - i;
+  if (dk < -1 || dk > 2 || dj < -1 || dj > 2 || di < -1 || di > 2) {  // none by construction
+    ring[t].add(kk, jj, ii, v);
+    return;
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (dk == a - 1 && dj == b - 1 && di == c - 1) s->v[t][a][b][c] += v;
+}
+
+// A flow's accumulator: target TGT.
+template <typename T, int TGT, int NT>
 struct SlotAcc {
-  Slots<T>* s;
+  Slots<T, NT>* s;
   const LdsAcc<T>* ring;
   int k, j, i;
-  // (kk, jj, ii) a fixed offset from the owner (k, j, i)
-  __device__ __forceinline__ void add(int kk, int jj, int ii, T v) const {
-    const int dk = kk - k, dj = jj - j, di = ii - i;
-    if (dk < -1 || dk > 2 || dj < -1 || dj > 2 || di < -1 || di > 2) {  // none by construction
-      ring[TGT].add(kk, jj, ii, v);
-      return;
-    }
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-      for (int b = 0; b < 4; b++)
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-          if (dk == a - 1 && dj == b - 1 && di == c - 1) s->v[TGT][a][b][c] += v;
-  }
+  __device__ __forceinline__ void add(int kk, int jj, int ii, T v) const { slot_add(s, ring, TGT, k, j, i, kk, jj, ii, v); }
   __device__ __forceinline__ void dyn(int kk, int jj, int ii, T v) const { ring[TGT].add(kk, jj, ii, v); }
+};
+
+// The advected fields' accumulators: field n is target n (n a constant once
+// the loops over the fields are unrolled).
+template <typename T, int NT>
+struct SlotAccF {
+  Slots<T, NT>* s;
+  const LdsAcc<T>* ring;
+  int k, j, i;
+  __device__ __forceinline__ void add(int n, int kk, int jj, int ii, T v) const {
+    slot_add(s, ring, n, k, j, i, kk, jj, ii, v);
+  }
+  __device__ __forceinline__ void dyn(int n, int kk, int jj, int ii, T v) const { ring[n].add(kk, jj, ii, v); }
 };
 
 // A slot that some add() reaches (not the constant zero of one no call site
@@ -118,8 +136,8 @@ __device__ __forceinline__ bool live(T x) { return !(__builtin_constant_p(x) && 
 // phase every ring value is read before any is written (the locations are
 // distinct), so the reads go out together instead of one LDS round trip per
 // read-add-write.
-template <typename T>
-__device__ __forceinline__ void flush_slots(const Slots<T>& s, const LdsAcc<T> (&ring)[4], int k, int j, int i,
+template <typename T, int NT>
+__device__ __forceinline__ void flush_slots(const Slots<T, NT>& s, const LdsAcc<T> (&ring)[NT], int k, int j, int i,
                                             int lane) {
   __syncthreads();
 #pragma unroll
@@ -128,9 +146,9 @@ __device__ __forceinline__ void flush_slots(const Slots<T>& s, const LdsAcc<T> (
     // level k0+L collects dk = 0 of lane L, +1 of lane L-1, +2 of lane L-2 and
     // -1 of lane L+1; `up` at lane L is what lanes L and L-1 send to level
     // k0+L+1
-    T tot[4][4], up[4][4];
+    T tot[NT][4], up[NT][4];
 #pragma unroll
-    for (int t = 0; t < 4; t++)
+    for (int t = 0; t < NT; t++)
 #pragma unroll
       for (int c = 0; c < 4; c++) {
         const T vm = s.v[t][0][b][c], v0 = s.v[t][1][b][c], v1 = s.v[t][2][b][c], v2 = s.v[t][3][b][c];
@@ -140,10 +158,10 @@ __device__ __forceinline__ void flush_slots(const Slots<T>& s, const LdsAcc<T> (
         if (live(v1) || live(v2)) tot[t][c] += from_below(up[t][c]);
         if (live(vm)) tot[t][c] += from_above(vm);
       }
-    T* p[4][4];
-    T old[4][4];
+    T* p[NT][4];
+    T old[NT][4];
 #pragma unroll
-    for (int t = 0; t < 4; t++)
+    for (int t = 0; t < NT; t++)
 #pragma unroll
       for (int c = 0; c < 4; c++)
         if (live(tot[t][c])) {
@@ -151,14 +169,14 @@ __device__ __forceinline__ void flush_slots(const Slots<T>& s, const LdsAcc<T> (
           old[t][c] = *p[t][c];
         }
 #pragma unroll
-    for (int t = 0; t < 4; t++)
+    for (int t = 0; t < NT; t++)
 #pragma unroll
       for (int c = 0; c < 4; c++)
         if (live(tot[t][c])) *p[t][c] = old[t][c] + tot[t][c];
     // the band's edges: level k0-1 from lane 0, k0+64 and k0+65 from lane 63
     if (lane == 0) {
 #pragma unroll
-      for (int t = 0; t < 4; t++)
+      for (int t = 0; t < NT; t++)
 #pragma unroll
         for (int c = 0; c < 4; c++)
           if (live(s.v[t][0][b][c])) {
@@ -166,16 +184,16 @@ __device__ __forceinline__ void flush_slots(const Slots<T>& s, const LdsAcc<T> (
             old[t][c] = *p[t][c];
           }
 #pragma unroll
-      for (int t = 0; t < 4; t++)
+      for (int t = 0; t < NT; t++)
 #pragma unroll
         for (int c = 0; c < 4; c++)
           if (live(s.v[t][0][b][c])) *p[t][c] = old[t][c] + s.v[t][0][b][c];
     }
     if (lane == 63) {
-      T* q[4][4];
-      T old2[4][4];
+      T* q[NT][4];
+      T old2[NT][4];
 #pragma unroll
-      for (int t = 0; t < 4; t++)
+      for (int t = 0; t < NT; t++)
 #pragma unroll
         for (int c = 0; c < 4; c++) {
           if (live(up[t][c])) {
@@ -188,7 +206,7 @@ __device__ __forceinline__ void flush_slots(const Slots<T>& s, const LdsAcc<T> (
           }
         }
 #pragma unroll
-      for (int t = 0; t < 4; t++)
+      for (int t = 0; t < NT; t++)
 #pragma unroll
         for (int c = 0; c < 4; c++) {
           if (live(up[t][c])) *p[t][c] = old[t][c] + up[t][c];
@@ -311,12 +329,16 @@ __device__ __forceinline__ T courant_adj(T vel, T ch, T dt, T geom, T lhcfl, T l
   return lcfl * sgn_abs(x) * ch * dt;
 }
 
-// Right x face of column (j,i) at level k: value and reverse.  Returns the
-// face value (the divergence adjoint of the flow coefficient needs it).
-template <typename T, class AF, class FU, class FV, class FW, class AU, class AV, class AW>
-__device__ __forceinline__ T face_x_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, const FU& U, const FV& V,
-                        const FW& W, const AU& aU, const AV& aV, const AW& aW, const VMet& m, int kdof, T dt, T cx,
-                        T cy, int nz, T lam) {
+// Right x face of column (j,i) at level k for the NS advected fields: values
+// (face[n], for the divergence adjoint of the flow coefficient) and reverse.
+// lam[n] is the adjoint of field n's face value.  The velocity-only
+// quantities are recomputed once and their adjoints summed over the fields
+// before they reach the flows.
+template <typename T, int NS, class AF, class FU, class FV, class FW, class AU, class AV, class AW>
+__device__ __forceinline__ void face_x_adj(int k, int j, int i, const Adv<T, NS>& F, const AF& aF, const FU& U,
+                                           const FV& V, const FW& W, const AU& aU, const AV& aV, const AW& aW,
+                                           const VMet& m, int kdof, T dt, T cx, T cy, int nz, const T (&lam)[NS],
+                                           T (&face)[NS]) {
   const int kp1 = k + 1 < nz ? k + 1 : nz;
   const int kp0 = kp1 - 1;
   const T uk = U(k, j, i);
@@ -343,38 +365,48 @@ __device__ __forceinline__ T face_x_adj(int k, int j, int i, const Adv<T, 1>& F,
   const int js = sign_pos(sy) ? j - 1 : j + 1;
   const T cyt = fabs(sy) * cy * dt;
   const HLim<T> lim(uk, cx, dt, s);
-  const T fa = F(0, k, j, i), fb = F(0, k, j, i + 1);
-  const T fm = F(0, k, j, i - 1), fp = F(0, k, j, i + 2);
-  const T fu = fm * ap + fp * an, fc = fa * ap + fb * an, fd = fb * ap + fa * an;
-  const T l0 = F(0, ksl, j, i), u0 = F(0, ksu, j, i), l1 = F(0, ksl, j, i + 1), u1 = F(0, ksu, j, i + 1);
-  const T tmp = (l0 - u0) * ap + (l1 - u1) * an;
-  const T g1 = cz * tmp;
-  const T s0 = F.lane(0, k, js, i), s1 = F.lane(0, k, js, i + 1);
-  const T t = s0 * ap + s1 * an;
-  const T g2 = cyt * (fc - t);
-  const T face = lim.face(fu, fc, fd, g1, g2);
-  if (lam == T(0.0)) return face;
-  const LimAdj<T> r = lim_adjoint<T, false>(lam, fu, fc, fd, g1, g2, lim.hrcfl, lim.hcfl, lim.cflcoef, T(0), T(0));
-  // transverse terms
-  const T lcyt = r.g2 * (fc - t);
-  const T lfc = r.fc + r.g2 * cyt;
-  const T lt = -(r.g2 * cyt);
-  const T lcz = r.g1 * tmp;
-  const T ltmp = r.g1 * cz;
-  // advected field
-  aF.add(k, j, i, lfc * ap + r.fd * an);
-  aF.add(k, j, i + 1, lfc * an + r.fd * ap);
-  aF.add(k, j, i - 1, r.fu * ap);
-  aF.add(k, j, i + 2, r.fu * an);
-  aF.dyn(ksl, j, i, ltmp * ap);
-  aF.dyn(ksu, j, i, -(ltmp * ap));
-  aF.dyn(ksl, j, i + 1, ltmp * an);
-  aF.dyn(ksu, j, i + 1, -(ltmp * an));
-  aF.dyn(k, js, i, lt * ap);
-  aF.dyn(k, js, i + 1, lt * an);
-  // flows
-  const T lS = -(lim.hrcfl * lim.hrcfl) * r.hrcfl;
-  T luk = courant_adj(uk, cx, dt, T(1.0), r.hcfl, r.cflcoef);
+  // adjoints of cz, cyt, hrcfl, hcfl and cflcoef, summed over the fields
+  T lcz = T(0.0), lcyt = T(0.0), lhr = T(0.0), lhc = T(0.0), lcc = T(0.0);
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    const T fa = F(n, k, j, i), fb = F(n, k, j, i + 1);
+    const T fm = F(n, k, j, i - 1), fp = F(n, k, j, i + 2);
+    const T fu = fm * ap + fp * an, fc = fa * ap + fb * an, fd = fb * ap + fa * an;
+    const T l0 = F(n, ksl, j, i), u0 = F(n, ksu, j, i), l1 = F(n, ksl, j, i + 1), u1 = F(n, ksu, j, i + 1);
+    const T tmp = (l0 - u0) * ap + (l1 - u1) * an;
+    const T g1 = cz * tmp;
+    const T s0 = F.lane(n, k, js, i), s1 = F.lane(n, k, js, i + 1);
+    const T t = s0 * ap + s1 * an;
+    const T g2 = cyt * (fc - t);
+    face[n] = lim.face(fu, fc, fd, g1, g2);
+    LimAdj<T> r{};
+    if (lam[n] != T(0.0))
+      r = lim_adjoint<T, false>(lam[n], fu, fc, fd, g1, g2, lim.hrcfl, lim.hcfl, lim.cflcoef, T(0), T(0));
+    // transverse terms
+    lcyt += r.g2 * (fc - t);
+    const T lfc = r.fc + r.g2 * cyt;
+    const T lt = -(r.g2 * cyt);
+    lcz += r.g1 * tmp;
+    const T ltmp = r.g1 * cz;
+    lhr += r.hrcfl;
+    lhc += r.hcfl;
+    lcc += r.cflcoef;
+    // advected field
+    aF.add(n, k, j, i, lfc * ap + r.fd * an);
+    aF.add(n, k, j, i + 1, lfc * an + r.fd * ap);
+    aF.add(n, k, j, i - 1, r.fu * ap);
+    aF.add(n, k, j, i + 2, r.fu * an);
+    aF.dyn(n, ksl, j, i, ltmp * ap);
+    aF.dyn(n, ksu, j, i, -(ltmp * ap));
+    aF.dyn(n, ksl, j, i + 1, ltmp * an);
+    aF.dyn(n, ksu, j, i + 1, -(ltmp * an));
+    aF.dyn(n, k, js, i, lt * ap);
+    aF.dyn(n, k, js, i + 1, lt * an);
+  }
+  // flows (hrcfl multiplies its adjoint first: hrcfl^2 may overflow where
+  // the outflow sum is zero, and no face there has a non-zero adjoint)
+  const T lS = -(lim.hrcfl * (lim.hrcfl * lhr));
+  T luk = courant_adj(uk, cx, dt, T(1.0), lhc, lcc);
   const T lsz = T(0.125) * (lcz * sgn_abs(sz) * rzk * dt);
   const T lsy = T(0.125) * (lcyt * sgn_abs(sy) * cy * dt);
   const T lS3 = lS * dt;
@@ -400,14 +432,14 @@ __device__ __forceinline__ T face_x_adj(int k, int j, int i, const Adv<T, 1>& F,
   aU.add(k, j, i, luk);
   aU.add(k, j, i - 1, lum);
   aU.add(k, j, i + 1, lup1);
-  return face;
 }
 
-// North y face of row j at column i, level k: value and reverse.
-template <typename T, class AF, class FU, class FV, class FW, class AU, class AV, class AW>
-__device__ __forceinline__ T face_y_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, const FU& U, const FV& V,
-                        const FW& W, const AU& aU, const AV& aV, const AW& aW, const VMet& m, int kdof, T dt, T cx,
-                        T cy, int nz, T lam) {
+// North y face of row j at column i, level k: values and reverse.
+template <typename T, int NS, class AF, class FU, class FV, class FW, class AU, class AV, class AW>
+__device__ __forceinline__ void face_y_adj(int k, int j, int i, const Adv<T, NS>& F, const AF& aF, const FU& U,
+                                           const FV& V, const FW& W, const AU& aU, const AV& aV, const AW& aW,
+                                           const VMet& m, int kdof, T dt, T cx, T cy, int nz, const T (&lam)[NS],
+                                           T (&face)[NS]) {
   const int kp1 = k + 1 < nz ? k + 1 : nz;
   const int kp0 = kp1 - 1;
   const T vk = V(k, j, i);
@@ -436,32 +468,40 @@ __device__ __forceinline__ T face_y_adj(int k, int j, int i, const Adv<T, 1>& F,
   const T posuf = T(0.5) + half_sign(sx);
   const T cxt = fabs(sx) * cx * dt;
   const HLim<T> lim(vk, cy, dt, s);
-  const T fa = F(0, k, j, i), fb = F(0, k, j + 1, i);
-  const T fm = F(0, k, j - 1, i), fp = F(0, k, j + 2, i);
-  const T fu = fm * ap + fp * an, fc = fa * ap + fb * an, fd = fb * ap + fa * an;
-  const T fl = F.lane(0, ksl, jj, i), fs = F.lane(0, ksu, jj, i);
-  const T g1 = cz * (fl - fs);
-  const T fe = F.lane(0, k, jj, i + 1), fw = F.lane(0, k, jj, i - 1);
-  const T tx = fe * (T(1.0) - posuf) + fw * posuf;
-  const T g2 = cxt * (fc - tx);
-  const T face = lim.face(fu, fc, fd, g1, g2);
-  if (lam == T(0.0)) return face;
-  const LimAdj<T> r = lim_adjoint<T, false>(lam, fu, fc, fd, g1, g2, lim.hrcfl, lim.hcfl, lim.cflcoef, T(0), T(0));
-  const T lcxt = r.g2 * (fc - tx);
-  const T lfc = r.fc + r.g2 * cxt;
-  const T ltx = -(r.g2 * cxt);
-  const T lcz = r.g1 * (fl - fs);
-  const T ldiff = r.g1 * cz;
-  aF.add(k, j, i, lfc * ap + r.fd * an);
-  aF.add(k, j + 1, i, lfc * an + r.fd * ap);
-  aF.add(k, j - 1, i, r.fu * ap);
-  aF.add(k, j + 2, i, r.fu * an);
-  aF.dyn(ksl, jj, i, ldiff);
-  aF.dyn(ksu, jj, i, -ldiff);
-  aF.dyn(k, jj, i + 1, ltx * (T(1.0) - posuf));
-  aF.dyn(k, jj, i - 1, ltx * posuf);
-  const T lS = -(lim.hrcfl * lim.hrcfl) * r.hrcfl;
-  T lvk = courant_adj(vk, cy, dt, T(1.0), r.hcfl, r.cflcoef);
+  T lcz = T(0.0), lcxt = T(0.0), lhr = T(0.0), lhc = T(0.0), lcc = T(0.0);
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    const T fa = F(n, k, j, i), fb = F(n, k, j + 1, i);
+    const T fm = F(n, k, j - 1, i), fp = F(n, k, j + 2, i);
+    const T fu = fm * ap + fp * an, fc = fa * ap + fb * an, fd = fb * ap + fa * an;
+    const T fl = F.lane(n, ksl, jj, i), fs = F.lane(n, ksu, jj, i);
+    const T g1 = cz * (fl - fs);
+    const T fe = F.lane(n, k, jj, i + 1), fw = F.lane(n, k, jj, i - 1);
+    const T tx = fe * (T(1.0) - posuf) + fw * posuf;
+    const T g2 = cxt * (fc - tx);
+    face[n] = lim.face(fu, fc, fd, g1, g2);
+    LimAdj<T> r{};
+    if (lam[n] != T(0.0))
+      r = lim_adjoint<T, false>(lam[n], fu, fc, fd, g1, g2, lim.hrcfl, lim.hcfl, lim.cflcoef, T(0), T(0));
+    lcxt += r.g2 * (fc - tx);
+    const T lfc = r.fc + r.g2 * cxt;
+    const T ltx = -(r.g2 * cxt);
+    lcz += r.g1 * (fl - fs);
+    const T ldiff = r.g1 * cz;
+    lhr += r.hrcfl;
+    lhc += r.hcfl;
+    lcc += r.cflcoef;
+    aF.add(n, k, j, i, lfc * ap + r.fd * an);
+    aF.add(n, k, j + 1, i, lfc * an + r.fd * ap);
+    aF.add(n, k, j - 1, i, r.fu * ap);
+    aF.add(n, k, j + 2, i, r.fu * an);
+    aF.dyn(n, ksl, jj, i, ldiff);
+    aF.dyn(n, ksu, jj, i, -ldiff);
+    aF.dyn(n, k, jj, i + 1, ltx * (T(1.0) - posuf));
+    aF.dyn(n, k, jj, i - 1, ltx * posuf);
+  }
+  const T lS = -(lim.hrcfl * (lim.hrcfl * lhr));
+  T lvk = courant_adj(vk, cy, dt, T(1.0), lhc, lcc);
   const T lsz = T(0.125) * (lcz * sgn_abs(sz) * rzk * dt);
   const T lsx = T(0.125) * (lcxt * sgn_abs(sx) * cx * dt);
   const T lS3 = lS * dt;
@@ -486,50 +526,23 @@ __device__ __forceinline__ T face_y_adj(int k, int j, int i, const Adv<T, 1>& F,
   aU.add(k, j, i - 1, lsx + (vpos ? lub : T(0.0)));
   aU.add(k, j + 1, i, lsx + (vpos ? T(0.0) : lua));
   aU.add(k, j + 1, i - 1, lsx + (vpos ? T(0.0) : lub));
-  return face;
 }
 
-// Reverse of the z faces' transverse terms (ZGrad) for one field.
-template <typename T, class AF, class FU, class FV, class AU, class AV>
-__device__ __forceinline__ void zgrad_adj(int k, int j, int i, int kk, T fc, const Adv<T, 1>& F, const AF& aF,
-                                          const FU& U, const FV& V, const AU& aU, const AV& aV, T dt, T cx, T cy,
-                                          T lg1, T lg2, T& lfc) {
-  const T v00 = V(k, j - 1, i), v01 = V(k, j, i), v10 = V(k + 1, j - 1, i), v11 = V(k + 1, j, i);
-  const T sy = T(0.125) * (v00 + v01 + v10 + v11);
-  const int js = sign_pos(sy) ? j - 1 : j + 1;
-  const T cyt = fabs(sy) * cy * dt;
-  const T u00 = U(k, j, i - 1), u01 = U(k, j, i), u10 = U(k + 1, j, i - 1), u11 = U(k + 1, j, i);
-  const T sx = T(0.125) * (u00 + u01 + u10 + u11);
-  const T posuf = T(0.5) + half_sign(sx);
-  const T cxt = fabs(sx) * cx * dt;
-  const T fsj = F.lane(0, kk, js, i);
-  const T fe = F(0, kk, j, i + 1), fw = F(0, kk, j, i - 1);
-  const T tx = fe * (T(1.0) - posuf) + fw * posuf;
-  // g1 = cyt*(fc - fsj), g2 = cxt*(fc - tx)
-  lfc = lfc + lg1 * cyt + lg2 * cxt;
-  aF.dyn(kk, js, i, -(lg1 * cyt));
-  aF.dyn(kk, j, i + 1, -(lg2 * cxt) * (T(1.0) - posuf));
-  aF.dyn(kk, j, i - 1, -(lg2 * cxt) * posuf);
-  const T lsy = T(0.125) * (lg1 * (fc - fsj) * sgn_abs(sy) * cy * dt);
-  const T lsx = T(0.125) * (lg2 * (fc - tx) * sgn_abs(sx) * cx * dt);
-  aV.add(k, j - 1, i, lsy);
-  aV.add(k, j, i, lsy);
-  aV.add(k + 1, j - 1, i, lsy);
-  aV.add(k + 1, j, i, lsy);
-  aU.add(k, j, i - 1, lsx);
-  aU.add(k, j, i, lsx);
-  aU.add(k + 1, j, i - 1, lsx);
-  aU.add(k + 1, j, i, lsx);
-}
-
-// Top face of level k (flux_z(k+1)), 1 <= k <= nz-1: value and reverse.
-template <typename T, class AF, class FU, class FV, class FW, class AU, class AV, class AW>
-__device__ __forceinline__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F, const AF& aF, const FU& U, const FV& V,
-                        const FW& W, const AU& aU, const AV& aV, const AW& aW, const VMet& m, int kdof, int kmin, T dt,
-                        T cx, T cy, int nz, T lam) {
-  if (k == 1 && kmin > 1) return T(0.0);
+// Top face of level k (flux_z(k+1)), 1 <= k <= nz-1: values and reverse.
+// The transverse terms (ZGrad) are reversed per field, their weights' adjoints
+// summed over the fields.
+template <typename T, int NS, class AF, class FU, class FV, class FW, class AU, class AV, class AW>
+__device__ __forceinline__ void face_z_adj(int k, int j, int i, const Adv<T, NS>& F, const AF& aF, const FU& U,
+                                           const FV& V, const FW& W, const AU& aU, const AV& aV, const AW& aW,
+                                           const VMet& m, int kdof, int kmin, T dt, T cx, T cy, int nz,
+                                           const T (&lam)[NS], T (&face)[NS]) {
+  if (k == 1 && kmin > 1) {
+#pragma unroll
+    for (int n = 0; n < NS; n++) face[n] = T(0.0);
+    return;
+  }
   const T wk = W(k, j, i);
-  T s, rdu, rdc, fu, fc, fd;
+  T s, rdu, rdc;
   int kk;
   // S = ((a0 + cy*B) + cx*C) * dt with a0 the W part; record its inputs
   int kw0 = k, kw1 = k - 1, kv = k;  // W(kw0), W(kw1), V(kv, j/j-1), U(kv, i/i-1)
@@ -538,16 +551,13 @@ __device__ __forceinline__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F,
   const bool bottom = k == 1, top = k == nz - 1;
   T ap = T(0.0), an = T(0.0);
   if (bottom) {
-    const bool up = wk >= T(0.0);
-    const T f1 = F(0, 1, j, i), f2 = F(0, 2, j, i);
-    if (up) {
+    if (wk >= T(0.0)) {
       mode = 1;
       kv = 1;
       rzw = vm<T>(m.rdz, 1 + kdof);
       s = wk * rzw;
       rdu = vm<T>(m.rdzn, 2);
       rdc = vm<T>(m.rdz, 1 + kdof);
-      fu = f1, fc = f1, fd = f2;
       kk = 1;
     } else {
       mode = 3;
@@ -557,20 +567,16 @@ __device__ __forceinline__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F,
       s = rzw * (pos(W(2, j, i)) - wk);
       rdu = vm<T>(m.rdzn, 3);
       rdc = vm<T>(m.rdz, 2 + kdof);
-      fu = F(0, 3, j, i), fc = f2, fd = f1;
       kk = 2;
     }
   } else if (top) {
-    const bool up = wk >= T(0.0);
-    const T fk = F(0, k, j, i), fk1 = F(0, k + 1, j, i);
-    if (up) {
+    if (wk >= T(0.0)) {
       mode = 1;
       kv = k;
       rzw = vm<T>(m.rdz, k + kdof);
       s = rzw * (wk + negabs(W(k - 1, j, i)));
       rdu = vm<T>(m.rdzn, k);
       rdc = vm<T>(m.rdz, k + kdof);
-      fu = F(0, k - 1, j, i), fc = fk, fd = fk1;
       kk = k;
     } else {
       mode = 2;
@@ -580,7 +586,6 @@ __device__ __forceinline__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F,
       s = -(wk * rzw);
       rdu = vm<T>(m.rdzn, nz);
       rdc = rzw;
-      fu = T(2.0) * fk1 - fk, fc = fk1, fd = fk;
       kk = k + 1;
     }
   } else {
@@ -595,55 +600,103 @@ __device__ __forceinline__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F,
     s = rzw * (pos(W(kk, j, i)) + negabs(W(kk - 1, j, i)));
     rdu = vm<T>(m.rdzn, k) * ap + vm<T>(m.rdzn, k + 2) * an;
     rdc = vm<T>(m.rdz, k + kdof) * ap + vm<T>(m.rdz, k + 1 + kdof) * an;
-    fu = F(0, k - 1, j, i) * ap + F(0, k + 2, j, i) * an;
-    fc = F(0, kk, j, i);
-    fd = F(0, k + 1, j, i) * ap + F(0, k, j, i) * an;
   }
   const T vb0 = V(kv, j, i), vb1 = V(kv, j - 1, i), ub0 = U(kv, j, i), ub1 = U(kv, j, i - 1);
   s = s + cy * (pos(vb0) + negabs(vb1));
   s = s + cx * (pos(ub0) + negabs(ub1));
   s = s * dt;
-  const ZGrad<T, FU, FV> zg(k, j, i, U, V, dt, cx, cy);
-  T g1, g2;
-  zg.apply(F, 0, kk, j, i, fc, g1, g2);
+  // transverse weights (ZGrad), kept with their sums for the reverse
+  const T v00 = V(k, j - 1, i), v01 = V(k, j, i), v10 = V(k + 1, j - 1, i), v11 = V(k + 1, j, i);
+  const T sy = T(0.125) * (v00 + v01 + v10 + v11);
+  const int js = sign_pos(sy) ? j - 1 : j + 1;
+  const T cyt = fabs(sy) * cy * dt;
+  const T u00 = U(k, j, i - 1), u01 = U(k, j, i), u10 = U(k + 1, j, i - 1), u11 = U(k + 1, j, i);
+  const T sx = T(0.125) * (u00 + u01 + u10 + u11);
+  const T posuf = T(0.5) + half_sign(sx);
+  const T cxt = fabs(sx) * cx * dt;
   const WLim<T> lim(k, wk, dt, s, rdc, rdu, m);
-  const T face = lim.face(fu, fc, fd, g1, g2);
-  if (lam == T(0.0)) return face;
-  const LimAdj<T> r = lim_adjoint<T, true>(lam, fu, fc, fd, g1, g2, lim.hrcfl, lim.hcfl, lim.cflcoef, lim.rdd,
-                                           lim.rdu);
-  T lfc = r.fc;
-  zgrad_adj(k, j, i, kk, fc, F, aF, U, V, aU, aV, dt, cx, cy, r.g1, r.g2, lfc);
-  // stencil values
-  if (mode == 0) {
-    aF.add(k - 1, j, i, r.fu * ap);
-    aF.add(k + 2, j, i, r.fu * an);
-    aF.dyn(kk, j, i, lfc);
-    aF.add(k + 1, j, i, r.fd * ap);
-    aF.add(k, j, i, r.fd * an);
-  } else if (bottom) {
-    if (mode == 1) {  // fu = fc = F(1), fd = F(2)
-      aF.dyn(1, j, i, r.fu + lfc);
-      aF.dyn(2, j, i, r.fd);
-    } else {  // fu = F(3), fc = F(2), fd = F(1)
-      aF.dyn(3, j, i, r.fu);
-      aF.dyn(2, j, i, lfc);
-      aF.dyn(1, j, i, r.fd);
+  T ly = T(0.0), lx = T(0.0), lhr = T(0.0), lhc = T(0.0), lcc = T(0.0);
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    T fu, fc, fd;
+    if (bottom) {
+      const T f1 = F(n, 1, j, i), f2 = F(n, 2, j, i);
+      const bool up = mode == 1;
+      fu = up ? f1 : F(n, 3, j, i);
+      fc = up ? f1 : f2;
+      fd = up ? f2 : f1;
+    } else if (top) {
+      const T fk = F(n, k, j, i), fk1 = F(n, k + 1, j, i);
+      const bool up = mode == 1;
+      // W < 0: linear extrapolation to a level equidistant above the top
+      fu = up ? F(n, k - 1, j, i) : T(2.0) * fk1 - fk;
+      fc = up ? fk : fk1;
+      fd = up ? fk1 : fk;
+    } else {
+      fu = F(n, k - 1, j, i) * ap + F(n, k + 2, j, i) * an;
+      fc = F(n, kk, j, i);
+      fd = F(n, k + 1, j, i) * ap + F(n, k, j, i) * an;
     }
-  } else {
-    if (mode == 1) {  // fu = F(k-1), fc = F(k), fd = F(k+1)
-      aF.add(k - 1, j, i, r.fu);
-      aF.add(k, j, i, lfc);
-      aF.add(k + 1, j, i, r.fd);
-    } else {  // fu = 2F(k+1) - F(k), fc = F(k+1), fd = F(k)
-      aF.add(k + 1, j, i, T(2.0) * r.fu + lfc);
-      aF.add(k, j, i, r.fd - r.fu);
+    const T fsj = F.lane(n, kk, js, i);
+    const T fe = F(n, kk, j, i + 1), fw = F(n, kk, j, i - 1);
+    const T tx = fe * (T(1.0) - posuf) + fw * posuf;
+    const T g1 = cyt * (fc - fsj), g2 = cxt * (fc - tx);
+    face[n] = lim.face(fu, fc, fd, g1, g2);
+    LimAdj<T> r{};
+    if (lam[n] != T(0.0))
+      r = lim_adjoint<T, true>(lam[n], fu, fc, fd, g1, g2, lim.hrcfl, lim.hcfl, lim.cflcoef, lim.rdd, lim.rdu);
+    lhr += r.hrcfl;
+    lhc += r.hcfl;
+    lcc += r.cflcoef;
+    // g1 = cyt*(fc - fsj), g2 = cxt*(fc - tx)
+    const T lfc = r.fc + r.g1 * cyt + r.g2 * cxt;
+    aF.dyn(n, kk, js, i, -(r.g1 * cyt));
+    aF.dyn(n, kk, j, i + 1, -(r.g2 * cxt) * (T(1.0) - posuf));
+    aF.dyn(n, kk, j, i - 1, -(r.g2 * cxt) * posuf);
+    ly += r.g1 * (fc - fsj);
+    lx += r.g2 * (fc - tx);
+    // stencil values
+    if (mode == 0) {
+      aF.add(n, k - 1, j, i, r.fu * ap);
+      aF.add(n, k + 2, j, i, r.fu * an);
+      aF.dyn(n, kk, j, i, lfc);
+      aF.add(n, k + 1, j, i, r.fd * ap);
+      aF.add(n, k, j, i, r.fd * an);
+    } else if (bottom) {
+      if (mode == 1) {  // fu = fc = F(1), fd = F(2)
+        aF.dyn(n, 1, j, i, r.fu + lfc);
+        aF.dyn(n, 2, j, i, r.fd);
+      } else {  // fu = F(3), fc = F(2), fd = F(1)
+        aF.dyn(n, 3, j, i, r.fu);
+        aF.dyn(n, 2, j, i, lfc);
+        aF.dyn(n, 1, j, i, r.fd);
+      }
+    } else {
+      if (mode == 1) {  // fu = F(k-1), fc = F(k), fd = F(k+1)
+        aF.add(n, k - 1, j, i, r.fu);
+        aF.add(n, k, j, i, lfc);
+        aF.add(n, k + 1, j, i, r.fd);
+      } else {  // fu = 2F(k+1) - F(k), fc = F(k+1), fd = F(k)
+        aF.add(n, k + 1, j, i, T(2.0) * r.fu + lfc);
+        aF.add(n, k, j, i, r.fd - r.fu);
+      }
     }
   }
+  const T lsy = T(0.125) * (ly * sgn_abs(sy) * cy * dt);
+  const T lsx = T(0.125) * (lx * sgn_abs(sx) * cx * dt);
+  aV.add(k, j - 1, i, lsy);
+  aV.add(k, j, i, lsy);
+  aV.add(k + 1, j - 1, i, lsy);
+  aV.add(k + 1, j, i, lsy);
+  aU.add(k, j, i - 1, lsx);
+  aU.add(k, j, i, lsx);
+  aU.add(k + 1, j, i - 1, lsx);
+  aU.add(k + 1, j, i, lsx);
   // flows: courant number, outflow sum
   const T rdd = lim.rdd;
   const T dd = vm<T>(m.dzn, k + 1);
-  T lwk = courant_adj(wk, rdd, dt, dd * dd * rdc, r.hcfl, r.cflcoef);
-  const T lS = -(lim.hrcfl * lim.hrcfl) * r.hrcfl;
+  T lwk = courant_adj(wk, rdd, dt, dd * dd * rdc, lhc, lcc);
+  const T lS = -(lim.hrcfl * (lim.hrcfl * lhr));
   const T lS3 = lS * dt;
   const T lB = lS3 * cy, lC = lS3 * cx;
   aV.dyn(kv, j, i, lB * dpos(vb0));
@@ -667,14 +720,15 @@ __device__ __forceinline__ T face_z_adj(int k, int j, int i, const Adv<T, 1>& F,
     lwk = lwk - lW * rzw;
   }
   aW.add(k, j, i, lwk);
-  return face;
 }
 
-template <typename T>
+// One kernel kind's adjoint: NS advected fields (a momentum field alone, or
+// up to two scalars that share their flows).
+template <typename T, int NS>
 struct AdjArgs {
-  const T* F;        // advected field (forward stepping: the prognostic)
-  const T* lin;      // adjoint of the new state for this field (interior)
-  T* lf;             // adjoint accumulator of this field
+  const T* F[NS];    // advected fields (forward stepping: the prognostics)
+  const T* lin[NS];  // adjoint of the new state of each field (interior)
+  T* lf[NS];         // adjoint accumulators of the fields
   T* lu;             // adjoint accumulators of the flows u, v, w
   T* lv;
   T* lw;
@@ -682,13 +736,14 @@ struct AdjArgs {
 };
 
 // The face owner (k, j, i), 1 <= k <= nz, 2 <= j <= ny+2, 2 <= i <= nx+2: the
-// right x face, north y face and top z face of that point, their adjoints
-// scattered into the accumulators aF (advected field) and aU, aV, aW (flows,
-// through the dual-grid interpolation of the field kind).  `after_face()` runs
-// after each face in control flow that every lane of the wave reaches (lanes
-// past nz take no face); register accumulators are emptied there.
-template <typename T, int KIND, class AF, class AU, class AV, class AW, class FL>
-__device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjArgs<T>& A, int k, int j, int i,
+// right x face, north y face and top z face of that point for the NS fields,
+// their adjoints scattered into the accumulators aF (advected fields, target
+// n for field n) and aU, aV, aW (flows, through the dual-grid interpolation of
+// the field kind).  `after_face()` runs after each face in control flow that
+// every lane of the wave reaches (lanes past nz take no face); register
+// accumulators are emptied there.
+template <typename T, int KIND, int NS, class AF, class AU, class AV, class AW, class FL>
+__device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjArgs<T, NS>& A, int k, int j, int i,
                                               const AF& aF, const AU& aU, const AV& aV, const AW& aW,
                                               const FL& after_face) {
   const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;
@@ -696,8 +751,9 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
   // after_face() (it may hold barriers); uniform over the wave
   const bool own = j <= ny + 2;
   const Flows<T, KIND> fl(L);
-  Adv<T, 1> F{};
-  F.a[0] = Arr<T>(A.F, L.d);
+  Adv<T, NS> F{};
+#pragma unroll
+  for (int n = 0; n < NS; n++) F.a[n] = Arr<T>(A.F[n], L.d);
   const VMet m = KIND == KIND_W ? L.mw : L.mc;
   const int kdof = KIND == KIND_W ? 1 : 0;
   const int kmin = KIND == KIND_W ? 1 : 2;
@@ -707,26 +763,44 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
   // Differences of lambda' are taken before scaling by dtm: lambda' of a
   // smooth loss (e.g. 2 th) is large against its differences, and scaling
   // first would round each term before the cancellation.
-  auto lraw = [&](int kk, int jj, int ii) -> T {
+  auto lraw = [&](int n, int kk, int jj, int ii) -> T {
     const bool cell = jj >= 3 && jj <= ny + 2 && ii >= 3 && ii <= nx + 2 && kk >= 2 && kk <= kmax;
-    return cell ? A.lin[at(L.d, kk, jj, ii)] : T(0.0);
+    return cell ? A.lin[n][at(L.d, kk, jj, ii)] : T(0.0);
   };
   const bool row_in = own && j >= 3, col_in = own && i >= 3;
-  const T lself = lraw(k, j, i);
+  T lself[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) lself[n] = lraw(n, k, j, i);
   // x face (k, j, i+1/2): s(i) -= U fx cx, s(i+1) += U fx cx
   if (row_in && k >= 2 && k <= kmax) {
-    const T d = A.dtm * (lraw(k, j, i + 1) - lself);
     const T uf = fl.U(k, j, i);
-    const T fx = face_x_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, dt, L.cx, L.cy, nz, uf * L.cx * d);
-    aU.add(k, j, i, fx * L.cx * d);
+    T d[NS], lam[NS], fx[NS];
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      d[n] = A.dtm * (lraw(n, k, j, i + 1) - lself[n]);
+      lam[n] = uf * L.cx * d[n];
+    }
+    face_x_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, dt, L.cx, L.cy, nz, lam, fx);
+    T ld = T(0.0);
+#pragma unroll
+    for (int n = 0; n < NS; n++) ld += fx[n] * L.cx * d[n];
+    aU.add(k, j, i, ld);
   }
   after_face();
   // y face (k, j+1/2, i)
   if (col_in && k >= 2 && k <= kmax) {
-    const T d = A.dtm * (lraw(k, j + 1, i) - lself);
     const T vf = fl.V(k, j, i);
-    const T fy = face_y_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, dt, L.cx, L.cy, nz, vf * L.cy * d);
-    aV.add(k, j, i, fy * L.cy * d);
+    T d[NS], lam[NS], fy[NS];
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      d[n] = A.dtm * (lraw(n, k, j + 1, i) - lself[n]);
+      lam[n] = vf * L.cy * d[n];
+    }
+    face_y_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, dt, L.cx, L.cy, nz, lam, fy);
+    T ld = T(0.0);
+#pragma unroll
+    for (int n = 0; n < NS; n++) ld += fy[n] * L.cy * d[n];
+    aV.add(k, j, i, ld);
   }
   after_face();
   // z face at level k (flux_z(k+1)): s(k) -= 4 W Fz t2(k), s(k+1) += 4 W Fz t1(k+1);
@@ -734,34 +808,43 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
   // the metric difference formed in FP64
   if (row_in && col_in && k <= nz - 1 && !(k == 1 && kmin > 1)) {
     const T dmet = (T)(m.t1[k] - m.t2[k - 1]);
-    const T e = T(4.0) * A.dtm * (vm<T>(m.t1, k + 1) * (lraw(k + 1, j, i) - lself) + dmet * lself);
     const T wf = fl.W(k, j, i);
-    const T fz = face_z_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, kmin, dt, L.cx, L.cy, nz, wf * e);
-    aW.add(k, j, i, fz * e);
+    T e[NS], lam[NS], fz[NS];
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      e[n] = T(4.0) * A.dtm * (vm<T>(m.t1, k + 1) * (lraw(n, k + 1, j, i) - lself[n]) + dmet * lself[n]);
+      lam[n] = wf * e[n];
+    }
+    face_z_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, kmin, dt, L.cx, L.cy, nz, lam, fz);
+    T ld = T(0.0);
+#pragma unroll
+    for (int n = 0; n < NS; n++) ld += fz[n] * e[n];
+    aW.add(k, j, i, ld);
   }
   after_face();
 }
 
-// LDS of the march adjoint: the rings of the four target fields (advected
-// field, u, v, w flows) for R owner rows.
-template <typename T, int R>
+// LDS of the march adjoint: the rings of the NT target fields (the advected
+// fields, then the u, v, w flows) for R owner rows.
+template <typename T, int R, int NT = 4>
 struct AdjLds {
-  T ring[4][4][R + 3][ADJ_KB];
+  T ring[NT][4][R + 3][ADJ_KB];
 };
 
 // March adjoint tile: 64 levels (band `band`) x R owner rows (from row 2 +
-// ty*R) x the x strip of owner columns from 2 + tx*strip.  Each column of
-// owners scatters into the LDS rings with LDS atomics; a column leaves the
-// ring (x-1 after owners x) once no later owner of the tile can reach it, and
-// is added to the global adjoints with one atomic per non-zero value.  Device-
-// scope atomics are carried out beyond the L2, so the ring cuts the atomics
-// that reach memory from ~70 per face owner to ~(strip+3)(R+3)/(strip R) per
-// target.  Every thread of the block calls this with the same tile.
-template <typename T, int KIND, int R>
-__device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjArgs<T>& A, int band, int ty, int tx,
-                                             int strip, AdjLds<T, R>& lds) {
+// ty*R) x the x strip of owner columns from 2 + tx*strip, for NS fields of
+// kind KIND.  Each column of owners adds into the LDS rings (register slots
+// folded by wave shifts, in row phases); a column leaves the ring (x-1 after
+// owners x) once no later owner of the tile can reach it, and is added to the
+// global adjoints with one atomic per non-zero value.  Device-scope atomics
+// are carried out beyond the L2, so the ring cuts the atomics that reach
+// memory from ~70 per face owner to ~(strip+3)(R+3)/(strip R) per target.
+// Every thread of the block calls this with the same tile.
+template <typename T, int KIND, int R, int NS = 1>
+__device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjArgs<T, NS>& A, int band, int ty,
+                                             int tx, int strip, AdjLds<T, R, NS + 3>& lds) {
   constexpr int DK = Flows<T, KIND>::DK, DJ = Flows<T, KIND>::DJ, DI = Flows<T, KIND>::DI;
-  constexpr int ROWS = R + 3, PER = ROWS * ADJ_KB, NT = 64 * R;
+  constexpr int NT = NS + 3, ROWS = R + 3, PER = ROWS * ADJ_KB, NTH = 64 * R;
   const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;
   const int lane = threadIdx.x, r = (int)__builtin_amdgcn_readfirstlane(threadIdx.y);
   const int tid = lane + 64 * r;
@@ -779,26 +862,32 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjAr
     int any = 0;
     if (j <= ny + 2 && k <= nz) {
       const bool row_up = r == R - 1 || j == ny + 2, lev_up = lane == 63 && k < nz;
-      for (int i = i_begin; i <= i_end; i++) {
-        any |= A.lin[at(L.d, k, j, i)] != T(0);
-        if (row_up) any |= A.lin[at(L.d, k, j + 1, i)] != T(0);
-        if (lev_up) any |= A.lin[at(L.d, k + 1, j, i)] != T(0);
-      }
+      for (int i = i_begin; i <= i_end; i++)
+#pragma unroll
+        for (int n = 0; n < NS; n++) {
+          any |= A.lin[n][at(L.d, k, j, i)] != T(0);
+          if (row_up) any |= A.lin[n][at(L.d, k, j + 1, i)] != T(0);
+          if (lev_up) any |= A.lin[n][at(L.d, k + 1, j, i)] != T(0);
+        }
     }
     if (!__syncthreads_or(any)) return;
   }
   T* flat = &lds.ring[0][0][0][0];
-  for (int e = tid; e < 4 * 4 * PER; e += NT) flat[e] = T(0);
+  for (int e = tid; e < NT * 4 * PER; e += NTH) flat[e] = T(0);
   __syncthreads();
-  const LdsAcc<T> ring[4] = {{&lds.ring[0][0][0][0], k0, j0, ROWS},
-                             {&lds.ring[1][0][0][0], k0, j0, ROWS},
-                             {&lds.ring[2][0][0][0], k0, j0, ROWS},
-                             {&lds.ring[3][0][0][0], k0, j0, ROWS}};
-  T* const dst[4] = {A.lf, A.lu, A.lv, A.lw};
-  // add ring column c (all four targets) to the global adjoints and clear it
+  LdsAcc<T> ring[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) ring[t] = LdsAcc<T>{&lds.ring[t][0][0][0], k0, j0, ROWS};
+  T* dst[NT];
+#pragma unroll
+  for (int n = 0; n < NS; n++) dst[n] = A.lf[n];
+  dst[NS] = A.lu;
+  dst[NS + 1] = A.lv;
+  dst[NS + 2] = A.lw;
+  // add ring column c (all targets) to the global adjoints and clear it
   auto flush = [&](int c) {
     if (c < 1 || c > nx + 4) return;
-    for (int e = tid; e < 4 * PER; e += NT) {
+    for (int e = tid; e < NT * PER; e += NTH) {
       const int t = e / PER, rem = e - t * PER;
       const int rr = rem / ADJ_KB, lev = rem - rr * ADJ_KB;
       T* p = &lds.ring[t][c & 3][rr][lev];
@@ -815,14 +904,14 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjAr
     const int i = __builtin_amdgcn_readfirstlane(ic);
     {
       const int k = k0 + lane, j = j0 + r;
-      Slots<T> sl{};
-      const SlotAcc<T, 0> aF{&sl, ring, k, j, i};
-      const AccFlow<T, DK, DJ, DI, SlotAcc<T, 1>> aU{{&sl, ring, k, j, i}, nz};
-      const AccFlow<T, DK, DJ, DI, SlotAcc<T, 2>> aV{{&sl, ring, k, j, i}, nz};
-      const AccFlow<T, DK, DJ, DI, SlotAcc<T, 3>> aW{{&sl, ring, k, j, i}, nz};
-      adjoint_point<T, KIND>(L, A, k, j, i, aF, aU, aV, aW, [&] {
+      Slots<T, NT> sl{};
+      const SlotAccF<T, NT> aF{&sl, ring, k, j, i};
+      const AccFlow<T, DK, DJ, DI, SlotAcc<T, NS, NT>> aU{{&sl, ring, k, j, i}, nz};
+      const AccFlow<T, DK, DJ, DI, SlotAcc<T, NS + 1, NT>> aV{{&sl, ring, k, j, i}, nz};
+      const AccFlow<T, DK, DJ, DI, SlotAcc<T, NS + 2, NT>> aW{{&sl, ring, k, j, i}, nz};
+      adjoint_point<T, KIND, NS>(L, A, k, j, i, aF, aU, aV, aW, [&] {
         flush_slots(sl, ring, k, j, i, lane);
-        sl = Slots<T>{};
+        sl = Slots<T, NT>{};
       });
     }
     __syncthreads();
@@ -838,10 +927,10 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjAr
 
 // March adjoint: grid (64-level bands, owner-row tiles, x strips).
 constexpr int ADJ_ROWS = 4;
-template <typename T, int KIND>
-__global__ void __launch_bounds__(64 * ADJ_ROWS) adjoint_march_kernel(LaunchArgs<T> L, AdjArgs<T> A, int strip) {
-  __shared__ AdjLds<T, ADJ_ROWS> lds;
-  adjoint_tile<T, KIND, ADJ_ROWS>(L, A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, strip, lds);
+template <typename T, int KIND, int NS = 1>
+__global__ void __launch_bounds__(64 * ADJ_ROWS) adjoint_march_kernel(LaunchArgs<T> L, AdjArgs<T, NS> A, int strip) {
+  __shared__ AdjLds<T, ADJ_ROWS, NS + 3> lds;
+  adjoint_tile<T, KIND, ADJ_ROWS, NS>(L, A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, strip, lds);
 }
 
 // Adjoint of the periodic halo fill, in two passes so every target is

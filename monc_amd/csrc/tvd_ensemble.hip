@@ -209,11 +209,11 @@ __global__ void __launch_bounds__(64 * ADJ_ROWS) ens_adjoint_kernel(const EnsMem
     r /= nbx;
     const int ty = r % nty, tx = r / nty;
     const LaunchArgs<T> L = member_args<T>(M, state, met);
-    AdjArgs<T> A;
+    AdjArgs<T, 1> A;
     const long long fo = M.off + (long long)f * M.fstride;
-    A.F = state + fo;
-    A.lin = lin + fo;
-    A.lf = lam + fo;
+    A.F[0] = state + fo;
+    A.lin[0] = lin + fo;
+    A.lf[0] = lam + fo;
     A.lu = lam + M.off;
     A.lv = lam + M.off + M.fstride;
     A.lw = lam + M.off + 2 * M.fstride;

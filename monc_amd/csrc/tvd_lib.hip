@@ -1094,34 +1094,55 @@ int adjoint_step_impl(monc_tvd_ctx* c, double dtm) {
   // march adjoint: 64-level bands x 4 owner rows x strips of 4 owner columns.
   // Short strips keep the backward sweep of a localised loss short: its few
   // non-zero tiles each march their strip serially (a 64-step rollout at
-  // 256x256x128 runs its backward 1.6x faster than with 16 columns), and a
-  // dense adjoint step costs the same at 512x512x256.
+  // 256x256x128 runs its backward 1.6x faster than with 16 columns).
   constexpr int ADJ_STRIP = 4;
   dim3 block(64, ADJ_ROWS, 1);
   dim3 grid((c->nz + 63) / 64, (c->ny + 1 + ADJ_ROWS - 1) / ADJ_ROWS, (c->nx + 1 + ADJ_STRIP - 1) / ADJ_STRIP);
-  for (int fl : fields) {
-    AdjArgs<T> A;
-    A.F = (const T*)slot(c, fl, MONC_TVD_PROG);
-    A.lin = (const T*)c->adj_tmp[fl];
-    A.lf = (T*)slot(c, fl, MONC_TVD_ADJOINT);
-    A.lu = (T*)slot(c, MONC_TVD_U, MONC_TVD_ADJOINT);
-    A.lv = (T*)slot(c, MONC_TVD_V, MONC_TVD_ADJOINT);
-    A.lw = (T*)slot(c, MONC_TVD_W, MONC_TVD_ADJOINT);
+  T* const lu = (T*)slot(c, MONC_TVD_U, MONC_TVD_ADJOINT);
+  T* const lv = (T*)slot(c, MONC_TVD_V, MONC_TVD_ADJOINT);
+  T* const lw = (T*)slot(c, MONC_TVD_W, MONC_TVD_ADJOINT);
+  auto args1 = [&](int fl) {
+    AdjArgs<T, 1> A;
+    A.F[0] = (const T*)slot(c, fl, MONC_TVD_PROG);
+    A.lin[0] = (const T*)c->adj_tmp[fl];
+    A.lf[0] = (T*)slot(c, fl, MONC_TVD_ADJOINT);
+    A.lu = lu, A.lv = lv, A.lw = lw;
     A.dtm = (T)dtm;
-    LaunchTimer timer(c, kind_of(fl));
-    switch (kind_of(fl)) {
-      case KIND_U:
-        hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_U>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
-        break;
-      case KIND_V:
-        hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_V>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
-        break;
-      case KIND_W:
-        hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_W>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
-        break;
-      default:
-        hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_SCALAR>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
-        break;
+    return A;
+  };
+  std::vector<int> scalars;
+  for (int fl : fields) {
+    const int kind = kind_of(fl);
+    if (kind == KIND_SCALAR) {
+      scalars.push_back(fl);
+      continue;
+    }
+    const AdjArgs<T, 1> A = args1(fl);
+    LaunchTimer timer(c, kind);
+    if (kind == KIND_U)
+      hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_U, 1>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
+    else if (kind == KIND_V)
+      hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_V, 1>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
+    else
+      hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_W, 1>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
+    HIP_TRY(hipGetLastError());
+  }
+  // scalars in pairs sharing their flows' recompute and reverse (th with the
+  // first q, ...), an odd one alone
+  for (size_t n = 0; n < scalars.size(); n += 2) {
+    LaunchTimer timer(c, KIND_SCALAR);
+    if (n + 1 < scalars.size()) {
+      const AdjArgs<T, 1> A0 = args1(scalars[n]), A1 = args1(scalars[n + 1]);
+      AdjArgs<T, 2> A;
+      A.F[0] = A0.F[0], A.F[1] = A1.F[0];
+      A.lin[0] = A0.lin[0], A.lin[1] = A1.lin[0];
+      A.lf[0] = A0.lf[0], A.lf[1] = A1.lf[0];
+      A.lu = lu, A.lv = lv, A.lw = lw;
+      A.dtm = (T)dtm;
+      hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_SCALAR, 2>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
+    } else {
+      hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_SCALAR, 1>), grid, block, 0, c->stream, L, args1(scalars[n]),
+                         ADJ_STRIP);
     }
     HIP_TRY(hipGetLastError());
   }
