@@ -269,6 +269,29 @@ int monc_tvd_forward_steps(monc_tvd_ctx *ctx, double dtm, int nsteps);
 int monc_tvd_forward_step_into(monc_tvd_ctx *ctx, double dtm, void *const *in, void *const *out);
 
 /*
+ * monc_tvd_forward_step_into that also records, for the adjoint of the step,
+ * the limiter arm every face took (no counterpart in the reference: the
+ * branch masks of the differentiable form).  rec holds one device pointer per
+ * field (null for a field the step does not advect); rec[f] is a buffer of
+ * one byte per point of field f's layout (nz*(ny+4)*(nx+4)), receiving at
+ * face owner (k, j, i) the arms of its right x, north y and top z faces in
+ * bits 0-1, 2-3 and 4-5 (0: the face is fc, 1: fd, 2: the QUICKEST value,
+ * 3: fu + (fc - fu)/S; ultimateflux.F90:556-614).  Runs the march kernel in
+ * its default shape; columns of at most 256 levels.  Asynchronous.
+ */
+int monc_tvd_forward_step_record(monc_tvd_ctx *ctx, double dtm, void *const *in, void *const *out,
+                                 unsigned char *const *rec);
+
+/*
+ * Binds the record buffer monc_tvd_adjoint_step reads the limiter arms of
+ * `field` from (a buffer monc_tvd_forward_step_record wrote for the state the
+ * adjoint step starts from); null unbinds.  When every advected field has one
+ * the adjoint takes the recorded arms, otherwise it re-derives them from the
+ * state.  No counterpart in the reference.
+ */
+int monc_tvd_bind_record(monc_tvd_ctx *ctx, int field, const unsigned char *rec);
+
+/*
  * Device-resident MONC timesteps of an advection-only run, nsteps times: per
  * step the sources are cleared (clearsourceterms), the TVD tendencies added
  * with momentum and scalar stepping CENTRED (steppingdirection.F90:27-32:
@@ -314,8 +337,11 @@ int monc_tvd_weighted_sum_noalloc(monc_tvd_ctx *ctx, int field, int variant, con
  * (interior values, zero halo), replaces them by lambda = lambda' +
  * dtm * J(X)^T lambda', J the Jacobian of the tendencies; halo contributions
  * are folded back onto their periodic sources.  The adjoint fields of u, v, w
- * are allocated (zero) if missing.  Branches of the limiter are re-derived
- * from X, so they are those the forward step took.  Asynchronous.
+ * are allocated (zero) if missing.  The limiter arms are those bound with
+ * monc_tvd_bind_record (recorded by the forward step from X), or re-derived
+ * from X with the forward step's arithmetic; flow signs and upwind picks are
+ * re-derived from X.  Either way they are those the forward step took.
+ * Asynchronous.
  */
 int monc_tvd_adjoint_step(monc_tvd_ctx *ctx, double dtm);
 

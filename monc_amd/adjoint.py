@@ -10,16 +10,20 @@ with their periodic halo) as a `torch.autograd.Function`:
 * backward: segment by segment from the last, the states inside the segment
   are recomputed from its checkpoint, then the hand-written adjoint step
   (monc_tvd_adjoint_step) carries lambda back one step at a time.  The
-  limiter's branches are re-derived from each recomputed state, which is
-  bitwise the state the forward pass stepped from, so the adjoint follows the
-  branches the forward pass took.
+  recompute records the limiter arm of every face
+  (monc_tvd_forward_step_record) and each adjoint step reads the arms of its
+  state (monc_tvd_bind_record); every recomputed state is bitwise the state
+  the forward pass stepped from, so the arms are those the forward pass took.
+  With `record_arms=False`, or columns of more than 256 levels, the adjoint
+  re-derives them from the state instead.
 
 The states of the last segment are kept as the forward pass computes them
 (in the segment buffers the backward pass uses), so the backward pass
 recomputes every segment but the last.
 
-Memory: ceil(n/k) checkpoints plus k segment states, each one state (all
-fields with halo) in the context precision.
+Memory: ceil(n/k) checkpoints plus k + 1 segment states, each one state (all
+fields with halo) in the context precision, and k records of one byte per
+point and advected field.
 
 The incoming gradient may be non-zero on halo entries (a loss over whole
 arrays): it is folded onto the periodic sources first (monc_tvd_adjoint_fold).
@@ -40,7 +44,7 @@ class TvdRollout:
     """Device domain + checkpoint buffers for differentiable rollouts."""
 
     def __init__(self, state: State, precision: int = tvd.FP32, checkpoint_every: int = 8,
-                 advect: int = tvd.ADVECT_FLOW | tvd.ADVECT_THETA | tvd.ADVECT_Q):
+                 advect: int = tvd.ADVECT_FLOW | tvd.ADVECT_THETA | tvd.ADVECT_Q, record_arms: bool = True):
         if checkpoint_every < 1:
             raise ValueError("checkpoint_every must be >= 1")
         if not torch.cuda.is_available():
@@ -71,7 +75,10 @@ class TvdRollout:
         if advect & tvd.ADVECT_TRACER:
             adv |= set(range(4 + state.n_q, self.nfields))
         self._fixed = [f for f in range(self.nfields) if f not in adv]  # fields the step leaves as they are
+        # the recording step runs the march kernel: columns of at most 256 levels
+        self.record = bool(record_arms) and state.nz <= 256
         self._seg: List[List[torch.Tensor]] = []  # recompute buffers, reused across segments
+        self._rec: List[List] = []  # limiter arms of the step from each segment state
         self._seg_tag = None  # the forward pass whose last segment the buffers hold
 
     # helpers -------------------------------------------------------------------
@@ -86,11 +93,24 @@ class TvdRollout:
         for p, f in zip(self.prog, fields):
             p.copy_(f)
 
-    def _step_into(self, dt: float, src: Sequence[torch.Tensor], dst: Sequence[torch.Tensor]) -> None:
-        """dst = one forward step of src (monc_tvd_forward_step_into)."""
-        self.t.forward_step_into(dt, src, dst)
+    def _step_into(self, dt: float, src: Sequence[torch.Tensor], dst: Sequence[torch.Tensor], rec=None) -> None:
+        """dst = one forward step of src (monc_tvd_forward_step_into), recording
+        the limiter arms into `rec` when given."""
+        self.t.forward_step_into(dt, src, dst, rec)
         for f in self._fixed:
             dst[f].copy_(src[f])
+
+    def _rec_buffers(self, n: int) -> List[List]:
+        while len(self._rec) < n:
+            self._rec.append([None if f in self._fixed else
+                              torch.empty(self.shape, dtype=torch.uint8, device=self.prog[0].device)
+                              for f in range(self.nfields)])
+        return self._rec
+
+    def _bind_rec(self, rec) -> None:
+        for f in range(self.nfields):
+            if f not in self._fixed:
+                self.t.bind_record(f, None if rec is None else rec[f])
 
     def _snapshot(self) -> List[torch.Tensor]:
         return [p.clone() for p in self.prog]
@@ -116,6 +136,7 @@ class TvdRollout:
     def close(self) -> None:
         self.t.close()
         self._seg = []
+        self._rec = []
 
 
 class RolloutFn(torch.autograd.Function):
@@ -132,13 +153,14 @@ class RolloutFn(torch.autograd.Function):
             if done + n < nsteps:
                 ckpts.append((done, eng._snapshot()))
                 eng.t.forward_steps(dt, n)
-            else:  # last segment: keep every state the backward pass needs
+            else:  # last segment: keep every state (and arms) the backward pass needs
                 seg = eng._seg_buffers(n)
+                rec = eng._rec_buffers(n) if eng.record else [None] * n
                 for b, p in zip(seg[0], eng.prog):
                     b.copy_(p)
                 for m in range(1, n):
-                    eng._step_into(dt, seg[m - 1], seg[m])
-                eng._step_into(dt, seg[n - 1], eng.prog)
+                    eng._step_into(dt, seg[m - 1], seg[m], rec[m - 1])
+                eng._step_into(dt, seg[n - 1], eng.prog, rec[n - 1])
                 ckpts.append((done, None))
                 eng._seg_tag = tag
             done += n
@@ -155,22 +177,31 @@ class RolloutFn(torch.autograd.Function):
             else:
                 a.copy_(g)
         eng.t.adjoint_fold()
-        seg = eng._seg_buffers(eng.k)
+        seg = eng._seg_buffers(eng.k + 1)
+        rec = eng._rec_buffers(eng.k) if eng.record else [None] * eng.k
         for start, ck in reversed(ctx.ckpts):
             n = min(eng.k, ctx.nsteps - start)
             if ck is None and eng._seg_tag is not ctx.tag:
                 raise RuntimeError("another rollout of this engine reused its segment buffers before this backward")
             if ck is not None:
-                # recompute X_start .. X_{start+n-1} in the working buffers
+                # recompute X_start .. X_{start+n-1} in the working buffers, and
+                # the arms of the step from each (the last step's state is the
+                # next checkpoint, written to a spare buffer)
                 for s, c in zip(seg[0], ck):
                     s.copy_(c)
                 for m in range(1, n):
-                    eng._step_into(dt, seg[m - 1], seg[m])
+                    eng._step_into(dt, seg[m - 1], seg[m], rec[m - 1])
+                if eng.record:
+                    eng._step_into(dt, seg[n - 1], seg[n], rec[n - 1])
             # adjoint sweep over the segment, newest state first
             for m in reversed(range(n)):
                 eng._bind_prog(seg[m])
+                if eng.record:
+                    eng._bind_rec(rec[m])
                 eng.t.adjoint_step(dt)
         eng._bind_prog(eng.prog)
+        if eng.record:
+            eng._bind_rec(None)
         eng._seg_tag = None
         ctx.ckpts = None
         return (None, None, None) + tuple(a.clone() for a in eng.adj)

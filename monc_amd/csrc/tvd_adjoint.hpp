@@ -5,13 +5,15 @@
 // (ultflx dt = dtm, advected field = the prognostic).  Given the adjoint of the
 // new state, lambda' (interior), the adjoint of the old state is
 //   lambda = lambda' + dtm * J(X)^T lambda'
-// assembled face by face: each face owner recomputes its face value from the
-// state (so every branch -- upwind or limited, which of the limiter's min/max
-// arms, flow signs, upwind picks of the transverse terms -- is re-derived from
-// the same inputs and is the one the forward pass took), receives the face's
-// adjoint from the two cells it bounds, and scatters the reverse-mode partials
-// to the field and flow adjoints with hardware floating-point atomics.  A fold
-// then returns the adjoint that landed on halo points to their periodic
+// assembled face by face: each face owner recomputes its face's operands from
+// the state (flow signs and the upwind picks of the transverse terms are
+// re-derived from the same inputs, so they are the ones the forward pass
+// took), takes the limiter arm the forward step recorded (MODE_REC) or
+// re-derives it from the same arithmetic, receives the face's adjoint from
+// the two cells it bounds, and scatters the reverse-mode partials to the field
+// and flow adjoints: in registers, then in per-block LDS rings, then to the
+// global adjoints with one device-scope atomic per value and ring column.  A
+// fold then returns the adjoint that landed on halo points to their periodic
 // sources (the adjoint of the periodic fill).
 //
 // Derivative conventions at kinks follow the tangent-linear code obtained by
@@ -30,23 +32,9 @@ __device__ __forceinline__ T dpos(T x) { return T(0.0) > x ? T(0.0) : T(1.0); }
 template <typename T>
 __device__ __forceinline__ T dnegabs(T x) { return T(0.0) < x ? T(0.0) : sgn_abs(x); }
 
-template <typename T>
-__device__ __forceinline__ void atomic_add(T* p, T v) {
-  if (v != T(0)) unsafeAtomicAdd(p, v);
-}
-
-// Adjoint of a field: atomic scatter with the field's indexing.
-template <typename T>
-struct Acc {
-  T* a;
-  Dims d;
-  __device__ __forceinline__ void add(int k, int j, int i, T v) const { atomic_add(a + at(d, k, j, i), v); }
-  __device__ __forceinline__ void dyn(int k, int j, int i, T v) const { add(k, j, i, v); }
-};
-
 // Adjoint of one target field accumulated in LDS: a ring of 4 x columns
-// (slot = i mod 4) x `rows` rows (from j0 - 1) x ADJ_KB levels (from k0 - 1),
-// with LDS atomics.  Every contribution of the face owners at (k, j, i) lands
+// (slot = i mod 4) x `rows` rows (from j0 - 1) x ADJ_KB levels (from k0 - 1).
+// Every contribution of the face owners at (k, j, i) lands
 // within levels k-1..k+2, rows j-1..j+2 and columns i-1..i+2 (the stencils of
 // the three faces and the dual-grid interpolation of the flows).
 constexpr int ADJ_KB = 67;
@@ -54,6 +42,7 @@ template <typename T>
 struct LdsAcc {
   T* s;
   int k0, j0, rows;
+  // LDS atomic, for the adds whose offset is chosen at run time (dyn)
   __device__ __forceinline__ void add(int k, int j, int i, T v) const {
     if (v != T(0)) unsafeAtomicAdd(&s[((i & 3) * rows + (j - j0 + 1)) * ADJ_KB + (k - k0 + 1)], v);
   }
@@ -71,9 +60,10 @@ struct LdsAcc {
 // dk, dj, di in -1..2.  The offsets are compile-time constants at the call
 // sites of add(), so the slots no call reaches stay constant zero and vanish.
 // After each face the level offsets are folded onto each lane's own level
-// with wave shifts (lane L of a wave holds level k0+L), leaving one LDS atomic
-// per (t, dj, di) and the band's edge levels.  dyn() takes an offset chosen at
-// run time (an upwind pick) straight to the ring.
+// with wave shifts (lane L of a wave holds level k0+L), leaving one LDS
+// read-add-write per (t, dj, di) and the band's edge levels (flush_slots).
+// dyn() takes an offset chosen at run time (an upwind pick) straight to the
+// ring with an LDS atomic.
 template <typename T, int NT>
 struct Slots {
   T v[NT][4][4][4];  // [target][dk+1][dj+1][di+1]
@@ -218,7 +208,7 @@ __device__ __forceinline__ void flush_slots(const Slots<T, NT>& s, const LdsAcc<
 }
 
 // Adjoint of Flow<T,DK,DJ,DI>: the dual-grid interpolation scatters to both points.
-template <typename T, int DK, int DJ, int DI, class ACC = Acc<T>>
+template <typename T, int DK, int DJ, int DI, class ACC>
 struct AccFlow {
   ACC a;
   int nz;
@@ -252,72 +242,6 @@ struct LimAdj {
   T fu, fc, fd, g1, g2, hrcfl, hcfl, cflcoef;
 };
 
-// Reverse of HLim::face.  `q_rdd`, `q_rdu` non-zero select the z form's
-// curvature term (fd-fc)*rdd - (fc-fu)*rdu instead of fd - 2fc + fu.
-template <typename T, bool ZFORM>
-__device__ __forceinline__ LimAdj<T> lim_adjoint(T lam, T fu, T fc, T fd, T g1, T g2, T hrcfl, T hcfl, T cflcoef,
-                                                 T rdd, T rdu) {
-  LimAdj<T> r{};
-  const T fdels = fd - fu;
-  const T fcurvs = fd - T(2.0) * fc + fu;
-  if (fabs(fcurvs) >= fabs(fdels)) {  // upwind branch: the face is fc
-    r.fc = lam;
-    return r;
-  }
-  const T rf = recip(guard(fdels));
-  const T curv = ZFORM ? (fd - fc) * rdd - (fc - fu) * rdu : fcurvs;
-  const T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * curv - g1 - g2;
-  const T fnfs = (ffs - fu) * rf;
-  const T fncs = (fc - fu) * rf;
-  const T fnrefs = fncs * hrcfl;
-  const T m1 = tmin(fnfs, fnrefs);
-  const T m2 = tmin(m1, T(1.0));
-  const T ftemp = tmax(fncs, m2);
-  // lim = ftemp * fdels + fu
-  const T lftemp = lam * fdels;
-  T lfdels = lam * ftemp;
-  T lfu = lam;
-  const bool pick_ncs = fncs > m2;
-  T lfncs = pick_ncs ? lftemp : T(0.0);
-  const T lm2 = pick_ncs ? T(0.0) : lftemp;
-  const T lm1 = m1 < T(1.0) ? lm2 : T(0.0);
-  const bool pick_nfs = fnfs < fnrefs;
-  const T lfnfs = pick_nfs ? lm1 : T(0.0);
-  const T lfnrefs = pick_nfs ? T(0.0) : lm1;
-  lfncs = lfncs + lfnrefs * hrcfl;
-  r.hrcfl = lfnrefs * fncs;
-  const T lffs = lfnfs * rf;
-  lfu = lfu - lfnfs * rf;
-  T lrf = lfnfs * (ffs - fu);
-  T lfc = lfncs * rf;
-  lfu = lfu - lfncs * rf;
-  lrf = lrf + lfncs * (fc - fu);
-  lfdels = lfdels - (rf * rf) * lrf;
-  // ffs = 0.5(fd+fc) - hcfl(fd-fc) - cflcoef*curv - g1 - g2
-  T lfd = T(0.5) * lffs - hcfl * lffs;
-  lfc = lfc + T(0.5) * lffs + hcfl * lffs;
-  r.hcfl = -lffs * (fd - fc);
-  r.cflcoef = -lffs * curv;
-  const T lcurv = -lffs * cflcoef;
-  r.g1 = -lffs;
-  r.g2 = -lffs;
-  if (ZFORM) {
-    lfd = lfd + lcurv * rdd;
-    lfc = lfc - lcurv * (rdd + rdu);
-    lfu = lfu + lcurv * rdu;
-  } else {
-    lfd = lfd + lcurv;
-    lfc = lfc - T(2.0) * lcurv;
-    lfu = lfu + lcurv;
-  }
-  lfd = lfd + lfdels;
-  lfu = lfu - lfdels;
-  r.fu = lfu;
-  r.fc = lfc;
-  r.fd = lfd;
-  return r;
-}
-
 // Adjoint of cflmod = |vel*ch*dt| through hcfl = cflmod/2 and
 // cflcoef = r6*(1 - cflmod^2)*geom: returns d/dvel.
 template <typename T>
@@ -329,6 +253,35 @@ __device__ __forceinline__ T courant_adj(T vel, T ch, T dt, T geom, T lhcfl, T l
   return lcfl * sgn_abs(x) * ch * dt;
 }
 
+// Reverse of a face whose limiter arm is `arm` (LIM_*): with the arm fixed
+// the face is linear in the advected field (fc, fd, the QUICKEST value ffs,
+// or fu + (fc - fu)*hrcfl), so the adjoints of its operands follow from the
+// arm alone.  Returns the face value (from the same arm).  The arm's value
+// equals the forward's ftemp*fdels + fu up to the rounding of rf*fdels = 1
+// (rf = 1/(fdels + 1e-30)), so the derivative is that of the forward arm up
+// to a relative 1e-30/|fdels|.  ZFORM: `rdd`, `rdu` give the z form's curvature
+// (fd-fc)*rdd - (fc-fu)*rdu instead of fd - 2fc + fu.
+template <typename T, bool ZFORM>
+__device__ __forceinline__ T arm_adjoint(int arm, T lam, T fu, T fc, T fd, T g1, T g2, T hrcfl, T hcfl, T cflcoef,
+                                         T rdd, T rdu, LimAdj<T>& r) {
+  const T curv = ZFORM ? (fd - fc) * rdd - (fc - fu) * rdu : fd - T(2.0) * fc + fu;
+  const T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * curv - g1 - g2;
+  const T fr = fu + (fc - fu) * hrcfl;
+  const bool ac = arm == LIM_C, ad = arm == LIM_D, aq = arm == LIM_Q, ar = arm == LIM_R;
+  const T lq = aq ? lam : T(0.0), lr = ar ? lam : T(0.0);
+  // ffs = 0.5(fd+fc) - hcfl(fd-fc) - cflcoef*curv - g1 - g2
+  const T lcurv = -(lq * cflcoef);
+  r.fd = (ad ? lam : T(0.0)) + lq * (T(0.5) - hcfl) + (ZFORM ? lcurv * rdd : lcurv);
+  r.fc = (ac ? lam : T(0.0)) + lq * (T(0.5) + hcfl) - (ZFORM ? lcurv * (rdd + rdu) : T(2.0) * lcurv) + lr * hrcfl;
+  r.fu = (ZFORM ? lcurv * rdu : lcurv) + lr * (T(1.0) - hrcfl);
+  r.hcfl = -(lq * (fd - fc));
+  r.cflcoef = -(lq * curv);
+  r.g1 = -lq;
+  r.g2 = -lq;
+  r.hrcfl = lr * (fc - fu);
+  return ac ? fc : (ad ? fd : (aq ? ffs : fr));
+}
+
 // Right x face of column (j,i) at level k for the NS advected fields: values
 // (face[n], for the divergence adjoint of the flow coefficient) and reverse.
 // lam[n] is the adjoint of field n's face value.  The velocity-only
@@ -338,7 +291,7 @@ template <typename T, int NS, class AF, class FU, class FV, class FW, class AU, 
 __device__ __forceinline__ void face_x_adj(int k, int j, int i, const Adv<T, NS>& F, const AF& aF, const FU& U,
                                            const FV& V, const FW& W, const AU& aU, const AV& aV, const AW& aW,
                                            const VMet& m, int kdof, T dt, T cx, T cy, int nz, const T (&lam)[NS],
-                                           T (&face)[NS]) {
+                                           T (&face)[NS], bool rec, const int (&arm)[NS]) {
   const int kp1 = k + 1 < nz ? k + 1 : nz;
   const int kp0 = kp1 - 1;
   const T uk = U(k, j, i);
@@ -378,10 +331,9 @@ __device__ __forceinline__ void face_x_adj(int k, int j, int i, const Adv<T, NS>
     const T s0 = F.lane(n, k, js, i), s1 = F.lane(n, k, js, i + 1);
     const T t = s0 * ap + s1 * an;
     const T g2 = cyt * (fc - t);
-    face[n] = lim.face(fu, fc, fd, g1, g2);
-    LimAdj<T> r{};
-    if (lam[n] != T(0.0))
-      r = lim_adjoint<T, false>(lam[n], fu, fc, fd, g1, g2, lim.hrcfl, lim.hcfl, lim.cflcoef, T(0), T(0));
+    LimAdj<T> r;
+    face[n] = arm_adjoint<T, false>(rec ? arm[n] : lim.cls_of(fu, fc, fd, g1, g2), lam[n], fu, fc, fd, g1, g2,
+                                    lim.hrcfl, lim.hcfl, lim.cflcoef, T(0), T(0), r);
     // transverse terms
     lcyt += r.g2 * (fc - t);
     const T lfc = r.fc + r.g2 * cyt;
@@ -439,7 +391,7 @@ template <typename T, int NS, class AF, class FU, class FV, class FW, class AU, 
 __device__ __forceinline__ void face_y_adj(int k, int j, int i, const Adv<T, NS>& F, const AF& aF, const FU& U,
                                            const FV& V, const FW& W, const AU& aU, const AV& aV, const AW& aW,
                                            const VMet& m, int kdof, T dt, T cx, T cy, int nz, const T (&lam)[NS],
-                                           T (&face)[NS]) {
+                                           T (&face)[NS], bool rec, const int (&arm)[NS]) {
   const int kp1 = k + 1 < nz ? k + 1 : nz;
   const int kp0 = kp1 - 1;
   const T vk = V(k, j, i);
@@ -479,10 +431,9 @@ __device__ __forceinline__ void face_y_adj(int k, int j, int i, const Adv<T, NS>
     const T fe = F.lane(n, k, jj, i + 1), fw = F.lane(n, k, jj, i - 1);
     const T tx = fe * (T(1.0) - posuf) + fw * posuf;
     const T g2 = cxt * (fc - tx);
-    face[n] = lim.face(fu, fc, fd, g1, g2);
-    LimAdj<T> r{};
-    if (lam[n] != T(0.0))
-      r = lim_adjoint<T, false>(lam[n], fu, fc, fd, g1, g2, lim.hrcfl, lim.hcfl, lim.cflcoef, T(0), T(0));
+    LimAdj<T> r;
+    face[n] = arm_adjoint<T, false>(rec ? arm[n] : lim.cls_of(fu, fc, fd, g1, g2), lam[n], fu, fc, fd, g1, g2,
+                                    lim.hrcfl, lim.hcfl, lim.cflcoef, T(0), T(0), r);
     lcxt += r.g2 * (fc - tx);
     const T lfc = r.fc + r.g2 * cxt;
     const T ltx = -(r.g2 * cxt);
@@ -535,7 +486,7 @@ template <typename T, int NS, class AF, class FU, class FV, class FW, class AU, 
 __device__ __forceinline__ void face_z_adj(int k, int j, int i, const Adv<T, NS>& F, const AF& aF, const FU& U,
                                            const FV& V, const FW& W, const AU& aU, const AV& aV, const AW& aW,
                                            const VMet& m, int kdof, int kmin, T dt, T cx, T cy, int nz,
-                                           const T (&lam)[NS], T (&face)[NS]) {
+                                           const T (&lam)[NS], T (&face)[NS], bool rec, const int (&arm)[NS]) {
   if (k == 1 && kmin > 1) {
 #pragma unroll
     for (int n = 0; n < NS; n++) face[n] = T(0.0);
@@ -641,10 +592,9 @@ __device__ __forceinline__ void face_z_adj(int k, int j, int i, const Adv<T, NS>
     const T fe = F(n, kk, j, i + 1), fw = F(n, kk, j, i - 1);
     const T tx = fe * (T(1.0) - posuf) + fw * posuf;
     const T g1 = cyt * (fc - fsj), g2 = cxt * (fc - tx);
-    face[n] = lim.face(fu, fc, fd, g1, g2);
-    LimAdj<T> r{};
-    if (lam[n] != T(0.0))
-      r = lim_adjoint<T, true>(lam[n], fu, fc, fd, g1, g2, lim.hrcfl, lim.hcfl, lim.cflcoef, lim.rdd, lim.rdu);
+    LimAdj<T> r;
+    face[n] = arm_adjoint<T, true>(rec ? arm[n] : lim.cls_of(fu, fc, fd, g1, g2), lam[n], fu, fc, fd, g1, g2,
+                                   lim.hrcfl, lim.hcfl, lim.cflcoef, lim.rdd, lim.rdu, r);
     lhr += r.hrcfl;
     lhc += r.hcfl;
     lcc += r.cflcoef;
@@ -729,6 +679,7 @@ struct AdjArgs {
   const T* F[NS];    // advected fields (forward stepping: the prognostics)
   const T* lin[NS];  // adjoint of the new state of each field (interior)
   T* lf[NS];         // adjoint accumulators of the fields
+  const unsigned char* rec[NS];  // limiter arms the forward step recorded (MODE_REC), or null
   T* lu;             // adjoint accumulators of the flows u, v, w
   T* lv;
   T* lw;
@@ -742,7 +693,7 @@ struct AdjArgs {
 // the field kind).  `after_face()` runs after each face in control flow that
 // every lane of the wave reaches (lanes past nz take no face); register
 // accumulators are emptied there.
-template <typename T, int KIND, int NS, class AF, class AU, class AV, class AW, class FL>
+template <typename T, int KIND, int NS, bool REC, class AF, class AU, class AV, class AW, class FL>
 __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjArgs<T, NS>& A, int k, int j, int i,
                                               const AF& aF, const AU& aU, const AV& aV, const AW& aW,
                                               const FL& after_face) {
@@ -771,6 +722,22 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
   T lself[NS];
 #pragma unroll
   for (int n = 0; n < NS; n++) lself[n] = lraw(n, k, j, i);
+  // the limiter arms of the owner's faces, as the forward step recorded them
+  // (REC), else re-derived from the state inside the face reverses
+  int code[NS] = {};
+  if constexpr (REC) {
+    if (own && k <= nz) {
+#pragma unroll
+      for (int n = 0; n < NS; n++) code[n] = A.rec[n][at(L.d, k, j, i)];
+    }
+  }
+  int ax[NS], ay[NS], az[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    ax[n] = code[n] & 3;
+    ay[n] = (code[n] >> 2) & 3;
+    az[n] = (code[n] >> 4) & 3;
+  }
   // x face (k, j, i+1/2): s(i) -= U fx cx, s(i+1) += U fx cx
   if (row_in && k >= 2 && k <= kmax) {
     const T uf = fl.U(k, j, i);
@@ -780,7 +747,7 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
       d[n] = A.dtm * (lraw(n, k, j, i + 1) - lself[n]);
       lam[n] = uf * L.cx * d[n];
     }
-    face_x_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, dt, L.cx, L.cy, nz, lam, fx);
+    face_x_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, dt, L.cx, L.cy, nz, lam, fx, REC, ax);
     T ld = T(0.0);
 #pragma unroll
     for (int n = 0; n < NS; n++) ld += fx[n] * L.cx * d[n];
@@ -796,7 +763,7 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
       d[n] = A.dtm * (lraw(n, k, j + 1, i) - lself[n]);
       lam[n] = vf * L.cy * d[n];
     }
-    face_y_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, dt, L.cx, L.cy, nz, lam, fy);
+    face_y_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, dt, L.cx, L.cy, nz, lam, fy, REC, ay);
     T ld = T(0.0);
 #pragma unroll
     for (int n = 0; n < NS; n++) ld += fy[n] * L.cy * d[n];
@@ -815,7 +782,7 @@ __device__ __forceinline__ void adjoint_point(const LaunchArgs<T>& L, const AdjA
       e[n] = T(4.0) * A.dtm * (vm<T>(m.t1, k + 1) * (lraw(n, k + 1, j, i) - lself[n]) + dmet * lself[n]);
       lam[n] = wf * e[n];
     }
-    face_z_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, kmin, dt, L.cx, L.cy, nz, lam, fz);
+    face_z_adj(k, j, i, F, aF, fl.U, fl.V, fl.W, aU, aV, aW, m, kdof, kmin, dt, L.cx, L.cy, nz, lam, fz, REC, az);
     T ld = T(0.0);
 #pragma unroll
     for (int n = 0; n < NS; n++) ld += fz[n] * e[n];
@@ -840,7 +807,7 @@ struct AdjLds {
 // are carried out beyond the L2, so the ring cuts the atomics that reach
 // memory from ~70 per face owner to ~(strip+3)(R+3)/(strip R) per target.
 // Every thread of the block calls this with the same tile.
-template <typename T, int KIND, int R, int NS = 1>
+template <typename T, int KIND, int R, int NS = 1, bool REC = false>
 __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjArgs<T, NS>& A, int band, int ty,
                                              int tx, int strip, AdjLds<T, R, NS + 3>& lds) {
   constexpr int DK = Flows<T, KIND>::DK, DJ = Flows<T, KIND>::DJ, DI = Flows<T, KIND>::DI;
@@ -909,7 +876,7 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjAr
       const AccFlow<T, DK, DJ, DI, SlotAcc<T, NS, NT>> aU{{&sl, ring, k, j, i}, nz};
       const AccFlow<T, DK, DJ, DI, SlotAcc<T, NS + 1, NT>> aV{{&sl, ring, k, j, i}, nz};
       const AccFlow<T, DK, DJ, DI, SlotAcc<T, NS + 2, NT>> aW{{&sl, ring, k, j, i}, nz};
-      adjoint_point<T, KIND, NS>(L, A, k, j, i, aF, aU, aV, aW, [&] {
+      adjoint_point<T, KIND, NS, REC>(L, A, k, j, i, aF, aU, aV, aW, [&] {
         flush_slots(sl, ring, k, j, i, lane);
         sl = Slots<T, NT>{};
       });
@@ -927,10 +894,10 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjAr
 
 // March adjoint: grid (64-level bands, owner-row tiles, x strips).
 constexpr int ADJ_ROWS = 4;
-template <typename T, int KIND, int NS = 1>
+template <typename T, int KIND, int NS = 1, bool REC = false>
 __global__ void __launch_bounds__(64 * ADJ_ROWS) adjoint_march_kernel(LaunchArgs<T> L, AdjArgs<T, NS> A, int strip) {
   __shared__ AdjLds<T, ADJ_ROWS, NS + 3> lds;
-  adjoint_tile<T, KIND, ADJ_ROWS, NS>(L, A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, strip, lds);
+  adjoint_tile<T, KIND, ADJ_ROWS, NS, REC>(L, A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, strip, lds);
 }
 
 // Adjoint of the periodic halo fill, in two passes so every target is

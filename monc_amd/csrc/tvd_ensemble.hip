@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(64 * ADJ_ROWS) ens_adjoint_kernel(const EnsMem
     r /= nbx;
     const int ty = r % nty, tx = r / nty;
     const LaunchArgs<T> L = member_args<T>(M, state, met);
-    AdjArgs<T, 1> A;
+    AdjArgs<T, 1> A{};
     const long long fo = M.off + (long long)f * M.fstride;
     A.F[0] = state + fo;
     A.lin[0] = lin + fo;

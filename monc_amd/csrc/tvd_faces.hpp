@@ -184,6 +184,21 @@ struct Adv {
   __device__ __forceinline__ T lane(int n, int k, int j, int i) const { return a[n].lane(k, j, i); }
 };
 
+// The arm of the limiter a face takes, in the terms of
+// ultimateflux.F90:556-606: the face is fc (upwind branch, or ftemp = fncs),
+// fd (ftemp = 1), the QUICKEST value ffs (ftemp = fnfs) or
+// fu + (fc - fu)*hrcfl (ftemp = fnrefs).  With the arm fixed the face is linear
+// in the advected field; the forward pass records it for the adjoint.
+enum { LIM_C = 0, LIM_D = 1, LIM_Q = 2, LIM_R = 3 };
+// The arm the selections of face() take, from the same operands.
+template <typename T>
+__device__ __forceinline__ int lim_class(T fcurvs, T fdels, T fnfs, T fncs, T fnrefs) {
+  const T m1 = tmin(fnfs, fnrefs);
+  const T m2 = tmin(m1, T(1.0));
+  const int lim = fncs > m2 ? LIM_C : (m1 < T(1.0) ? (fnfs < fnrefs ? LIM_Q : LIM_R) : LIM_D);
+  return fabs(fcurvs) >= fabs(fdels) ? LIM_C : lim;
+}
+
 // Limiter of ultimateflux.F90:556-606 (calculate_flux_in_x/_y): the
 // velocity-only part, shared by every field advected by the same flows...
 template <typename T>
@@ -209,6 +224,26 @@ struct HLim {
     T ftemp = tmax(fncs, tmin(tmin(fnfs, fnrefs), T(1.0)));
     T lim = ftemp * fdels + fu;
     return fabs(fcurvs) >= fabs(fdels) ? fc : lim;
+  }
+  // face() and the arm it takes
+  __device__ __forceinline__ T face_cls(T fu, T fc, T fd, T fgt1, T fgt2, int& cls) const {
+    T fdels = fd - fu;
+    T fcurvs = fd - T(2.0) * fc + fu;
+    T rfdels = recip(guard(fdels));
+    T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * fcurvs - fgt1 - fgt2;
+    T fnfs = (ffs - fu) * rfdels;
+    T fncs = (fc - fu) * rfdels;
+    T fnrefs = fncs * hrcfl;
+    T ftemp = tmax(fncs, tmin(tmin(fnfs, fnrefs), T(1.0)));
+    T lim = ftemp * fdels + fu;
+    cls = lim_class(fcurvs, fdels, fnfs, fncs, fnrefs);
+    return fabs(fcurvs) >= fabs(fdels) ? fc : lim;
+  }
+  // the arm alone
+  __device__ __forceinline__ int cls_of(T fu, T fc, T fd, T fgt1, T fgt2) const {
+    int cls;
+    (void)face_cls(fu, fc, fd, fgt1, fgt2, cls);
+    return cls;
   }
 };
 
@@ -238,6 +273,25 @@ struct WLim {
     T ftemp = tmax(fncs, tmin(tmin(fnfs, fnrefs), T(1.0)));
     T lim = ftemp * fdels + fu;
     return fabs(fcurvs) >= fabs(fdels) ? fc : lim;
+  }
+  __device__ __forceinline__ T face_cls(T fu, T fc, T fd, T fgt1, T fgt2, int& cls) const {
+    T fcurvs = fd - T(2.0) * fc + fu;
+    T fdels = fd - fu;
+    T rfdels = recip(guard(fdels));
+    T ffs = T(0.5) * (fd + fc) - hcfl * (fd - fc) - cflcoef * ((fd - fc) * rdd - (fc - fu) * rdu) - fgt1 - fgt2;
+    T fnfs = (ffs - fu) * rfdels;
+    T fncs = (fc - fu) * rfdels;
+    T fnrefs = fncs * hrcfl;
+    T ftemp = tmax(fncs, tmin(tmin(fnfs, fnrefs), T(1.0)));
+    T lim = ftemp * fdels + fu;
+    cls = lim_class(fcurvs, fdels, fnfs, fncs, fnrefs);
+    return fabs(fcurvs) >= fabs(fdels) ? fc : lim;
+  }
+  // the arm alone
+  __device__ __forceinline__ int cls_of(T fu, T fc, T fd, T fgt1, T fgt2) const {
+    int cls;
+    (void)face_cls(fu, fc, fd, fgt1, fgt2, cls);
+    return cls;
   }
 };
 

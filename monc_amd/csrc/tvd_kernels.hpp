@@ -10,7 +10,10 @@ enum FieldKind { KIND_U = 0, KIND_V = 1, KIND_W = 2, KIND_SCALAR = 3 };
 // fused forward-Euler step; MODE_LEAP: the fused MONC step of an
 // advection-only run (stepfields + swap_smooth: leapfrog with the Robert
 // filter, or the forward first step with the classic swap).
-enum Mode { MODE_ACCUM = 0, MODE_STEP = 1, MODE_LEAP = 2 };
+enum Mode { MODE_ACCUM = 0, MODE_STEP = 1, MODE_LEAP = 2, MODE_REC = 3 };
+// MODE_REC: MODE_STEP that also records each face's limiter arm (LIM_*) for
+// the adjoint, one byte per face owner (k, j, i) and field in the field's
+// layout: bits 0-1 the right x face, 2-3 the north y face, 4-5 the top z face.
 
 // Largest number of fields one launch advects (u, v, w, th and up to 12
 // scalars); the host splits longer scalar lists over several launches.
@@ -28,6 +31,7 @@ struct FieldSet {
   const T* z[MAX_FIELDS];  // MODE_LEAP: current z-field (time level t-1)
   T* out2[MAX_FIELDS];     // MODE_LEAP: next z-field
   T gal[MAX_FIELDS];       // MODE_LEAP: galilean shift of the field (ugal, vgal; 0 otherwise)
+  unsigned char* rec[MAX_FIELDS];  // MODE_REC: limiter arms of the field's faces
 };
 
 template <typename T>

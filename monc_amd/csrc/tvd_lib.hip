@@ -78,6 +78,7 @@ struct monc_tvd_ctx {
   std::vector<void*> spare;    // library-owned second buffer of each prognostic (stepping)
   std::vector<void*> spare_z;  // library-owned second buffer of each z-field (leapfrog stepping)
   std::vector<void*> adj_tmp;  // copy of the incoming adjoint of each field (adjoint step)
+  std::vector<const unsigned char*> rec;  // caller buffer of the limiter arms the adjoint step reads, per field
   std::vector<void*> dual[2];  // packed (value, tangent) state, double-buffered (tangent steps)
   hipStream_t stream;
   int kernel;                  // MONC_TVD_KERNEL_MARCH, _CELL or _STAGE
@@ -240,7 +241,7 @@ bool march_rows_ok(int nch, int rows) {
 template <typename T, int KIND, int NS, int MODE>
 void launch_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S, int nch, int kernel, int rows,
                   int strip) {
-  if constexpr (MODE == MODE_LEAP) {  // compiled for the default shapes only
+  if constexpr (MODE == MODE_LEAP || MODE == MODE_REC) {  // compiled for the default shapes only
     switch (nch * 100 + rows) {
       case 108: launch_march<T, KIND, NS, MODE, 1, 8>(c, L, S, strip); break;
       case 204: launch_march<T, KIND, NS, MODE, 2, 4>(c, L, S, strip); break;
@@ -370,6 +371,10 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
   if constexpr (MODE == MODE_LEAP) {
     if (!march_fits)
       return fail(MONC_TVD_ERR_ARG, "leapfrog stepping needs columns of at most 256 levels and fields under 4 GiB");
+  } else if constexpr (MODE == MODE_REC) {
+    if (!march_fits)
+      return fail(MONC_TVD_ERR_ARG,
+                  "recording limiter arms needs columns of at most 256 levels and fields under 4 GiB");
   } else if (c->kernel == MONC_TVD_KERNEL_CELL || !march_fits) {
     const int nk = MODE == MODE_STEP ? c->nz : c->nz - 1;
     dim3 block(64, 4, 1);
@@ -393,10 +398,12 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
       G.z[m] = S.z[n + m];
       G.out2[m] = S.out2[n + m];
       G.gal[m] = S.gal[n + m];
+      G.rec[m] = S.rec[n + m];
     }
     const int kk = pair || S.kind[n] == KIND_SCALAR ? 3 : S.kind[n];
-    // the leapfrog step runs the march kernel in its default shape
-    const bool leap = MODE == MODE_LEAP;
+    // the leapfrog step and the recording step run the march kernel in its
+    // default shape
+    const bool leap = MODE == MODE_LEAP || MODE == MODE_REC;
     const int kernel = leap ? MONC_TVD_KERNEL_MARCH : c->kernel;
     // the leapfrog step keeps the general shape: its FP64 pair measured
     // slower in the wide one (6.26 against 6.42 Gcell/s)
@@ -852,6 +859,43 @@ int forward_into_impl(monc_tvd_ctx* c, double dtm, void* const* in, void* const*
   return MONC_TVD_OK;
 }
 
+// forward_into_impl that also records every face's limiter arm into rec[f]
+// (one byte per point of field f's layout) for the adjoint of the step.
+template <typename T>
+int forward_record_impl(monc_tvd_ctx* c, double dtm, void* const* in, void* const* out, unsigned char* const* rec) {
+  std::vector<int> fields = advected_fields(c);
+  for (int f = 0; f < c->nfields; f++)
+    if (!in[f] || !out[f]) return fail(MONC_TVD_ERR_ARG, "null buffer for field %d", f);
+  for (int fl : fields) {
+    if (!rec[fl]) return fail(MONC_TVD_ERR_ARG, "null record buffer for advected field %d", fl);
+    for (int g = 0; g < c->nfields; g++)
+      if (out[fl] == in[g]) return fail(MONC_TVD_ERR_ARG, "output of field %d aliases the input of field %d", fl, g);
+    for (int g : fields)
+      if (g != fl && (out[fl] == out[g] || rec[fl] == rec[g]))
+        return fail(MONC_TVD_ERR_ARG, "fields %d and %d share an output", fl, g);
+  }
+  LaunchArgs<T> L = launch_args<T>(c, dtm);
+  L.u = (const T*)in[MONC_TVD_U];
+  L.v = (const T*)in[MONC_TVD_V];
+  L.w = (const T*)in[MONC_TVD_W];
+  for (size_t first = 0; first < fields.size(); first += MAX_FIELDS) {
+    FieldSet<T> S{};
+    S.nf = (int)std::min(fields.size() - first, (size_t)MAX_FIELDS);
+    for (int n = 0; n < S.nf; n++) {
+      const int fl = fields[first + n];
+      if (in[fl] == out[fl]) return fail(MONC_TVD_ERR_ARG, "field %d: the step cannot run in place", fl);
+      S.kind[n] = kind_of(fl);
+      S.F[n] = (const T*)in[fl];
+      S.f[n] = S.F[n];
+      S.out[n] = (T*)out[fl];
+      S.dt[n] = (T)dtm;
+      S.rec[n] = rec[fl];
+    }
+    if (int rc = launch_set<T, MODE_REC>(c, L, S)) return rc;
+  }
+  return MONC_TVD_OK;
+}
+
 // The MONC timestep of an advection-only run, resident: per step the TVD
 // tendencies from cleared sources (momentum and scalar stepping CENTRED, so
 // ultflx takes dt = 2 dtm, steppingdirection.F90:27-32; the advected field is
@@ -1101,14 +1145,35 @@ int adjoint_step_impl(monc_tvd_ctx* c, double dtm) {
   T* const lu = (T*)slot(c, MONC_TVD_U, MONC_TVD_ADJOINT);
   T* const lv = (T*)slot(c, MONC_TVD_V, MONC_TVD_ADJOINT);
   T* const lw = (T*)slot(c, MONC_TVD_W, MONC_TVD_ADJOINT);
+  // the limiter arms the forward step recorded, when every advected field
+  // has them; otherwise the adjoint re-derives them from the state
+  bool rec = true;
+  for (int fl : fields) rec = rec && c->rec[fl];
   auto args1 = [&](int fl) {
-    AdjArgs<T, 1> A;
+    AdjArgs<T, 1> A{};
     A.F[0] = (const T*)slot(c, fl, MONC_TVD_PROG);
     A.lin[0] = (const T*)c->adj_tmp[fl];
     A.lf[0] = (T*)slot(c, fl, MONC_TVD_ADJOINT);
+    A.rec[0] = c->rec[fl];
     A.lu = lu, A.lv = lv, A.lw = lw;
     A.dtm = (T)dtm;
     return A;
+  };
+  auto launch1 = [&](int kind, const AdjArgs<T, 1>& A) {
+#define MONC_ADJ1(K, R)                                                                                      \
+  hipLaunchKernelGGL((adjoint_march_kernel<T, K, 1, R>), grid, block, 0, c->stream, L, A, ADJ_STRIP)
+    if (rec) {
+      if (kind == KIND_U) MONC_ADJ1(KIND_U, true);
+      else if (kind == KIND_V) MONC_ADJ1(KIND_V, true);
+      else if (kind == KIND_W) MONC_ADJ1(KIND_W, true);
+      else MONC_ADJ1(KIND_SCALAR, true);
+    } else {
+      if (kind == KIND_U) MONC_ADJ1(KIND_U, false);
+      else if (kind == KIND_V) MONC_ADJ1(KIND_V, false);
+      else if (kind == KIND_W) MONC_ADJ1(KIND_W, false);
+      else MONC_ADJ1(KIND_SCALAR, false);
+    }
+#undef MONC_ADJ1
   };
   std::vector<int> scalars;
   for (int fl : fields) {
@@ -1117,14 +1182,8 @@ int adjoint_step_impl(monc_tvd_ctx* c, double dtm) {
       scalars.push_back(fl);
       continue;
     }
-    const AdjArgs<T, 1> A = args1(fl);
     LaunchTimer timer(c, kind);
-    if (kind == KIND_U)
-      hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_U, 1>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
-    else if (kind == KIND_V)
-      hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_V, 1>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
-    else
-      hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_W, 1>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
+    launch1(kind, args1(fl));
     HIP_TRY(hipGetLastError());
   }
   // scalars in pairs sharing their flows' recompute and reverse (th with the
@@ -1137,12 +1196,17 @@ int adjoint_step_impl(monc_tvd_ctx* c, double dtm) {
       A.F[0] = A0.F[0], A.F[1] = A1.F[0];
       A.lin[0] = A0.lin[0], A.lin[1] = A1.lin[0];
       A.lf[0] = A0.lf[0], A.lf[1] = A1.lf[0];
+      A.rec[0] = A0.rec[0], A.rec[1] = A1.rec[0];
       A.lu = lu, A.lv = lv, A.lw = lw;
       A.dtm = (T)dtm;
-      hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_SCALAR, 2>), grid, block, 0, c->stream, L, A, ADJ_STRIP);
+      if (rec)
+        hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_SCALAR, 2, true>), grid, block, 0, c->stream, L, A,
+                           ADJ_STRIP);
+      else
+        hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_SCALAR, 2, false>), grid, block, 0, c->stream, L, A,
+                           ADJ_STRIP);
     } else {
-      hipLaunchKernelGGL((adjoint_march_kernel<T, KIND_SCALAR, 1>), grid, block, 0, c->stream, L, args1(scalars[n]),
-                         ADJ_STRIP);
+      launch1(KIND_SCALAR, args1(scalars[n]));
     }
     HIP_TRY(hipGetLastError());
   }
@@ -1316,6 +1380,7 @@ int monc_tvd_create(int nz, int ny, int nx, int halo, int n_q, int n_tracers, in
   c->spare.assign((size_t)c->nfields, nullptr);
   c->spare_z.assign((size_t)c->nfields, nullptr);
   c->adj_tmp.assign((size_t)c->nfields, nullptr);
+  c->rec.assign((size_t)c->nfields, nullptr);
   c->dual[0].assign((size_t)c->nfields, nullptr);
   c->dual[1].assign((size_t)c->nfields, nullptr);
   c->stream = nullptr;
@@ -1700,6 +1765,22 @@ int monc_tvd_forward_step_into(monc_tvd_ctx* c, double dtm, void* const* in, voi
   if (!std::isfinite(dtm)) return fail(MONC_TVD_ERR_ARG, "dtm must be finite");
   return c->precision == MONC_TVD_FP64 ? forward_into_impl<double>(c, dtm, in, out)
                                        : forward_into_impl<float>(c, dtm, in, out);
+}
+
+int monc_tvd_forward_step_record(monc_tvd_ctx* c, double dtm, void* const* in, void* const* out,
+                                 unsigned char* const* rec) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!in || !out || !rec) return fail(MONC_TVD_ERR_ARG, "null buffer array");
+  if (!std::isfinite(dtm)) return fail(MONC_TVD_ERR_ARG, "dtm must be finite");
+  return c->precision == MONC_TVD_FP64 ? forward_record_impl<double>(c, dtm, in, out, rec)
+                                       : forward_record_impl<float>(c, dtm, in, out, rec);
+}
+
+int monc_tvd_bind_record(monc_tvd_ctx* c, int field, const unsigned char* rec) {
+  if (int rc = check_ctx(c)) return rc;
+  if (field < 0 || field >= c->nfields) return fail(MONC_TVD_ERR_ARG, "field %d out of range", field);
+  c->rec[field] = rec;
+  return MONC_TVD_OK;
 }
 
 int monc_tvd_leapfrog_steps(monc_tvd_ctx* c, double dtm, double tsmth, double ugal, double vgal, int nsteps,

@@ -170,7 +170,7 @@ template <typename T, int NS, bool VD>
 __device__ __forceinline__ void mx_face(const __amdgpu_buffer_rsrc_t (&F)[NS], const Geo& g, const Levels& lv, int j,
                                         int i, const T (&xw)[NS][4], const VNb<T> (&c0)[NS], const VNb<T> (&c1)[NS],
                                         T u_m, T u_0, T u_p, T v_0, T v_s, T v_0p, T v_sp, T w_0, T w_p, T wm_0,
-                                        T wm_p, T rdz_k, T rdz_kd, T dt, T cx, T cy, T out[NS]) {
+                                        T wm_p, T rdz_k, T rdz_kd, T dt, T cx, T cy, T out[NS], int cls[NS]) {
   const bool upos = u_0 > T(0.0);
   T s = rdz_k * (pos(upos ? w_0 : w_p) + negabs(upos ? wm_0 : wm_p));
   s = s + cy * (pos(upos ? v_0 : v_0p) + negabs(upos ? v_s : v_sp));
@@ -207,7 +207,7 @@ __device__ __forceinline__ void mx_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
     const T t0 = ldb<T>(F[n], ojs, sj0), t1 = ldb<T>(F[n], ojs, sj1);
     const T t = ap ? t0 : t1;
     const T g2 = cyt * (fc - t);
-    out[n] = lim.face(fu, fc, fd, g1, g2);
+    out[n] = lim.face_cls(fu, fc, fd, g1, g2, cls[n]);
   }
 }
 
@@ -217,7 +217,8 @@ __device__ __forceinline__ void mx_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
 template <typename T, int NS>
 __device__ __forceinline__ void my_face(const __amdgpu_buffer_rsrc_t (&F)[NS], const Geo& g, const Levels& lv, int j,
                                         int i, const T (&fc0)[NS], T vk, T vn, T vs, T w0, T w0m, T w1, T w1m, T u0,
-                                        T u0m, T u1, T u1m, T rdz_k, T rdz_kd, T dt, T cx, T cy, T out[NS]) {
+                                        T u0m, T u1, T u1m, T rdz_k, T rdz_kd, T dt, T cx, T cy, T out[NS],
+                                        int cls[NS]) {
   const bool vpos = sign_pos(vk);
   T s = rdz_k * (pos(vpos ? w0 : w1) + negabs(vpos ? w0m : w1m));
   s = s + cy * (pos(vpos ? vk : vn) + negabs(vpos ? vs : vk));
@@ -245,7 +246,7 @@ __device__ __forceinline__ void my_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
     const T fd = vpos ? fb : fa;
     const T g1 = cz * (ldb<T>(F[n], osl, s_0) - ldb<T>(F[n], osu, s_0));
     const T g2 = cxt * (fc - ldb<T>(F[n], ot, s_im));
-    out[n] = lim.face(fu, fc, fd, g1, g2);
+    out[n] = lim.face_cls(fu, fc, fd, g1, g2, cls[n]);
   }
 }
 
@@ -261,7 +262,7 @@ __device__ __forceinline__ void mz_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
                                         bool edge, bool top, bool bot,
                                         T zs_up, T zs_dn,
                                         T rdc_up, T rdc_dn, T rdu_up, T rdu_dn, T rdd, T dd2, T dt, T cx, T cy,
-                                        T out[NS]) {
+                                        T out[NS], int cls[NS]) {
   const T wk = q.w_0;
   const bool up = edge ? wk >= T(0.0) : sign_pos(wk);
   // outflow sum of the upwind cell kk = k (up) or k+1
@@ -319,6 +320,7 @@ __device__ __forceinline__ void mz_face(const __amdgpu_buffer_rsrc_t (&F)[NS], c
     const T ftemp = tmax(fncs, tmin(tmin(fnfs, fnrefs), T(1.0)));
     const T lim = ftemp * fdels + fu;
     out[n] = fabs(fcurvs) >= fabs(fdels) ? fc : lim;
+    cls[n] = lim_class(fcurvs, fdels, fnfs, fncs, fnrefs);
   }
 }
 
@@ -437,8 +439,15 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
         e0[n] = e1[n];
       }
     }
+    int cpro[NS];
     mx_face<T, NS, VD>(F, g, lv, j, i, xw, c0, c1, um, c_um, c_u0, v0, vs, c_v0, c_vs, w0, c_w0, wm0, c_wm0, rdz_k,
-                   rdz_kd, dt, cx, cy, fxl);
+                   rdz_kd, dt, cx, cy, fxl, cpro);
+    if constexpr (MODE == MODE_REC) {  // the right x face of column 2 (owner (k, j, 2))
+      if (i == 2 && k <= nz) {
+#pragma unroll
+        for (int n = 0; n < NS; n++) S.rec[n][at(L.d, k, j, 2)] = (unsigned char)cpro[n];
+      }
+    }
 #pragma unroll
     for (int n = 0; n < NS; n++) {  // window of column i_begin: F(i_begin-1 .. i_begin+1), F(i_begin+2) below
       xw[n][0] = xw[n][1];
@@ -501,24 +510,35 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
       q.vk_s = V(lv.kp1, lv.kp2, j - 1, i);
       q.vk_0 = V(lv.kp1, lv.kp2, j, i);
       q.wk_0 = W(lv.kp1, lv.kp2, j, i);
+      int cyn[NS], cys[NS] = {}, cxr[NS], czf[NS];
       my_face<T, NS>(F, g, lv, j, i, fc0, q.v_0, q.v_n, q.v_s, q.w_0, q.wm_0, q.w_n, q.wm_n, q.u_0, q.u_m, q.u1_0,
-                     q.u1_m, rdz_k, rdz_kd, dt, cx, cy, fyn);
+                     q.u1_m, rdz_k, rdz_kd, dt, cx, cy, fyn, cyn);
       if (r == 0) {  // south face of the block's first row
         T fs0[NS];
 #pragma unroll
         for (int n = 0; n < NS; n++) fs0[n] = ldb<T>(F[n], lv.k, g.so(j - 1, i));
         my_face<T, NS>(F, g, lv, j - 1, i, fs0, q.v_s, q.v_0, V(lv.k, lv.kp1, j - 2, i), W(lv.k, lv.kp1, j - 1, i),
                        W(lv.km1, lv.k, j - 1, i), q.w_0, q.wm_0, U(lv.k, lv.kp1, j - 1, i),
-                       U(lv.k, lv.kp1, j - 1, i - 1), q.u_0, q.u_m, rdz_k, rdz_kd, dt, cx, cy, fys);
+                       U(lv.k, lv.kp1, j - 1, i - 1), q.u_0, q.u_m, rdz_k, rdz_kd, dt, cx, cy, fys, cys);
       }
       mx_face<T, NS, VD>(F, g, lv, j, i, xw, c0, c1, q.u_m, q.u_0, q.u_p, q.v_0, q.v_s, q.v_0p, q.v_sp, q.w_0, q.w_p,
-                     q.wm_0, q.wm_p, rdz_k, rdz_kd, dt, cx, cy, fxr);
+                     q.wm_0, q.wm_p, rdz_k, rdz_kd, dt, cx, cy, fxr, cxr);
       T zf[NS];
       mz_face<T, NS, VD>(F, g, lv, j, i, fc0, c0, q, edge, ztop, zbot, lds.met[M_ZS_UP][kl], lds.met[M_ZS_DN][kl],
                      lds.met[M_RDC_UP][kl], lds.met[M_RDC_DN][kl], lds.met[M_RDU_UP][kl], lds.met[M_RDU_DN][kl],
-                     lds.met[M_RDD][kl], lds.met[M_DD2][kl], dt, cx, cy, zf);
+                     lds.met[M_RDD][kl], lds.met[M_DD2][kl], dt, cx, cy, zf, czf);
 #pragma unroll
       for (int n = 0; n < NS; n++) fzt[n] = has_z ? zf[n] : T(0);
+      if constexpr (MODE == MODE_REC) {  // the arms of the faces cell (k, j, i) owns
+        if (k <= nz) {
+#pragma unroll
+          for (int n = 0; n < NS; n++) {
+            S.rec[n][at(L.d, k, j, i)] = (unsigned char)(cxr[n] | (cyn[n] << 2) | (czf[n] << 4));
+            // the south face of row 3 is the north face of owner (k, 2, i)
+            if (j == 3) S.rec[n][at(L.d, k, 2, i)] = (unsigned char)(cys[n] << 2);
+          }
+        }
+      }
     }
 #pragma unroll
     for (int n = 0; n < NS; n++) {
@@ -553,7 +573,7 @@ __device__ __forceinline__ void march_tile(const LaunchArgs<T>& L, const FieldSe
         T sv;
         const T s0 = MODE == MODE_ACCUM ? ldb<T>(Fo[n], lv.k, so) : T(0);
         T fcur = T(0);
-        if constexpr (MODE == MODE_STEP) fcur = S.f[n] == S.F[n] ? fc0[n] : ldb<T>(Ff[n], lv.k, so);
+        if constexpr (MODE == MODE_STEP || MODE == MODE_REC) fcur = S.f[n] == S.F[n] ? fc0[n] : ldb<T>(Ff[n], lv.k, so);
         else if constexpr (MODE == MODE_LEAP) fcur = ldb<T>(Ff[n], lv.k, so);
         if (top) {
           sv = (q.v_s * fys[n] - q.v_0 * fyn[n]) * cy;

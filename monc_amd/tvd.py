@@ -35,7 +35,8 @@ EXPORTS = (
     "monc_tvd_set_profiling", "monc_tvd_kernel_times", "monc_tvd_weighted_sum",
     "monc_tvd_adjoint_step", "monc_tvd_adjoint_fold", "monc_tvd_tangent_steps", "monc_tvd_march_shapes",
     "monc_tvd_set_diagnostics", "monc_tvd_get_diagnostic", "monc_tvd_autotune",
-    "monc_tvd_leapfrog_steps", "monc_tvd_forward_step_into", "monc_thadv_tendencies",
+    "monc_tvd_leapfrog_steps", "monc_tvd_forward_step_into", "monc_tvd_forward_step_record",
+    "monc_tvd_bind_record", "monc_thadv_tendencies",
     "monc_pw_tendencies", "monc_tvd_set_field_active", "monc_tvd_test_fail_allocations",
     "monc_tvd_weighted_sum_noalloc", "monc_tvd_get_tiling", "monc_tvd_get_component_diagnostic",
     "monc_tvd_ensemble_create", "monc_tvd_ensemble_destroy", "monc_tvd_ensemble_layout",
@@ -106,6 +107,9 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.monc_tvd_forward_steps.argtypes = [c_void_p, c_double, c_int]
     lib.monc_tvd_forward_step_into.argtypes = [c_void_p, c_double, ctypes.POINTER(c_void_p),
                                                ctypes.POINTER(c_void_p)]
+    lib.monc_tvd_forward_step_record.argtypes = [c_void_p, c_double, ctypes.POINTER(c_void_p),
+                                                 ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p)]
+    lib.monc_tvd_bind_record.argtypes = [c_void_p, c_int, c_void_p]
     lib.monc_tvd_leapfrog_steps.argtypes = [c_void_p, c_double, c_double, c_double, c_double, c_int, c_int]
     lib.monc_tvd_adjoint_step.argtypes = [c_void_p, c_double]
     lib.monc_tvd_adjoint_fold.argtypes = [c_void_p]
@@ -344,11 +348,14 @@ class TvdAdvection:
         if not t.is_cuda or t.dtype != want or tuple(t.shape) != self.field_shape() or not t.is_contiguous():
             raise ValueError(f"{what} must be a contiguous CUDA tensor of the context dtype and field shape")
 
-    def forward_step_into(self, dtm: float, src, dst) -> None:
+    def forward_step_into(self, dtm: float, src, dst, rec=None) -> None:
         """One forward step from the torch tensors `src` (one per field) into
         `dst`, without touching the context's own fields.  Every tensor must be
         a contiguous CUDA tensor of the context's dtype and field shape, and no
-        destination may share memory with a source or another destination."""
+        destination may share memory with a source or another destination.
+        With `rec` (one uint8 CUDA tensor of the field shape per field, or None
+        for a field the step leaves alone) the step also records every face's
+        limiter arm there (monc_tvd_forward_step_record)."""
         n = len(src)
         if n != len(dst) or n != self.nfields:
             raise ValueError("one source and one destination tensor per field")
@@ -365,7 +372,28 @@ class TvdAdvection:
                     raise ValueError(f"destination {i} overlaps another source or destination")
         a = (ctypes.c_void_p * n)(*[t.data_ptr() for t in src])
         b = (ctypes.c_void_p * n)(*[t.data_ptr() for t in dst])
-        _check(self._lib.monc_tvd_forward_step_into(self._h, dtm, a, b))
+        if rec is None:
+            _check(self._lib.monc_tvd_forward_step_into(self._h, dtm, a, b))
+            return
+        if len(rec) != n:
+            raise ValueError("one record tensor (or None) per field")
+        for i, t in enumerate(rec):
+            if t is not None:
+                self._check_record(t, f"record {i}")
+        r = (ctypes.c_void_p * n)(*[None if t is None else t.data_ptr() for t in rec])
+        _check(self._lib.monc_tvd_forward_step_record(self._h, dtm, a, b, r))
+
+    def _check_record(self, t, what: str) -> None:
+        import torch
+        if not t.is_cuda or t.dtype != torch.uint8 or tuple(t.shape) != self.field_shape() or not t.is_contiguous():
+            raise ValueError(f"{what} must be a contiguous CUDA uint8 tensor of the field shape")
+
+    def bind_record(self, field: int, tensor) -> None:
+        """The limiter arms adjoint_step reads for `field` (a uint8 tensor a
+        recording step wrote), or None to re-derive them from the state."""
+        if tensor is not None:
+            self._check_record(tensor, f"record of field {field}")
+        _check(self._lib.monc_tvd_bind_record(self._h, field, None if tensor is None else tensor.data_ptr()))
 
     def leapfrog_steps(self, dtm: float, tsmth: float, nsteps: int, ugal: float = 0.0, vgal: float = 0.0,
                        first_forward: bool = True) -> None:

@@ -105,7 +105,7 @@ def _load(path: Path) -> ctypes.CDLL:
     ci, cd = ctypes.c_int, ctypes.c_double
     ip = ctypes.POINTER(ctypes.c_int)
     L.oracle_tvd_timestep.argtypes = [ci, ci, ci, cd, cd, dp, dp, dp, dp, dp, dp, dp, dp, cd, ci, ci, ci, ci, ci,
-                                      ci, ip, pp, pp, pp, ci, dp, dp, dp]
+                                      ci, ip, pp, pp, pp, ci, dp, dp, dp, dp, dp, dp]
     L.oracle_tvd_timestep.restype = ci
     L.oracle_halo_fill.argtypes = [dp, ci, ci, ci]
     L.oracle_vertical_metrics.argtypes = [ci, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
@@ -136,10 +136,13 @@ def _pp(arrs: Sequence[np.ndarray]):
 def tendencies(nz: int, ny: int, nx: int, cx: float, cy: float, metrics: dict, dtm: float, field_stepping: int,
                momentum_stepping: int, scalar_stepping: int, fields: Sequence[np.ndarray],
                zfields: Optional[Sequence[np.ndarray]] = None, sources: Optional[Sequence[np.ndarray]] = None,
-               advect: int = 7, rec_field: int = -1, n_tracers: int = 0, active: Optional[Sequence[int]] = None):
+               advect: int = 7, rec_field: int = -1, n_tracers: int = 0, active: Optional[Sequence[int]] = None,
+               arms: bool = False):
     """One timestep_callback sweep.  Returns (sources, faces) where sources are
     the accumulated s-fields (copies of `sources` or zeros) and faces is
-    (fx, fy, fz) for rec_field (or None).  fields: u, v, w, th, the q fields,
+    (fx, fy, fz) for rec_field (or None); with `arms`, (fx, fy, fz, ax, ay, az),
+    a* the limiter arm of each face (0 fc, 1 fd, 2 QUICKEST, 3 fu + (fc-fu)/S;
+    NaN where no face was computed).  fields: u, v, w, th, the q fields,
     then n_tracers tracers; advect bits 1 flow, 2 theta, 4 q, 8 tracers;
     active: per-field flags (None: all active)."""
     n = len(fields)
@@ -158,10 +161,10 @@ def tendencies(nz: int, ny: int, nx: int, cx: float, cy: float, metrics: dict, d
             for i in range(n)]
     m = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in metrics.items()}
     faces = None
-    rec = [None, None, None]
+    rec = [None] * 6
     if rec_field >= 0:
-        faces = tuple(np.full(shape, np.nan) for _ in range(3))
-        rec = [_p(a) for a in faces]
+        faces = tuple(np.full(shape, np.nan) for _ in range(6 if arms else 3))
+        rec = [_p(a) for a in faces] + [None] * (6 - len(faces))
     rc = lib().oracle_tvd_timestep(nz, ny, nx, cx, cy, _p(m["rdz"]), _p(m["rdzn"]), _p(m["dz"]), _p(m["dzn"]),
                                    _p(m["tzc1"]), _p(m["tzc2"]), _p(m["tzd1"]), _p(m["tzd2"]), dtm,
                                    field_stepping, momentum_stepping, scalar_stepping, advect, n_q, n_tracers,
@@ -172,10 +175,11 @@ def tendencies(nz: int, ny: int, nx: int, cx: float, cy: float, metrics: dict, d
 
 
 def state_tendencies(state, dtm: float, field_stepping: int = 1, momentum_stepping: int = 1,
-                     scalar_stepping: int = 1, sources=None, advect: int = 7, rec_field: int = -1, active=None):
+                     scalar_stepping: int = 1, sources=None, advect: int = 7, rec_field: int = -1, active=None,
+                     arms: bool = False):
     return tendencies(state.nz, state.ny, state.nx, state.cx, state.cy, state.grid.metric_vectors(), dtm,
                       field_stepping, momentum_stepping, scalar_stepping, state.fields, state.zfields or None,
-                      sources, advect, rec_field, getattr(state, "n_tracers", 0), active)
+                      sources, advect, rec_field, getattr(state, "n_tracers", 0), active, arms)
 
 
 def halo_fill(f: np.ndarray) -> np.ndarray:

@@ -64,6 +64,24 @@ static int fnint(double x) { return (int)lround(x); }
 static double dmax(double a, double b) { return a > b ? a : b; }
 static double dmin(double a, double b) { return a < b ? a : b; }
 
+/* The arm of the limiter the last flux_in_w / flux_in_h call took, from its
+ * dmax(fncs, dmin(dmin(fnfs, fnrefs), 1)): 0 the face is fc (ftemp = fncs),
+ * 1 fd (ftemp = 1), 2 ffs (ftemp = fnfs), 3 fu + fncs*hrcfl*fdels (ftemp =
+ * fnrefs).  The upwind branch (|fcurvs| >= |fdels|) is arm 0.  Recorded per
+ * face for the adjoint's recorded branch masks. */
+static int g_arm;
+static double *m_arm_x, *m_arm_y, *m_arm_z; /* arm of each face of the column (NAN: none computed) */
+static void set_arm(double fnfs, double fncs, double fnrefs) {
+  double m1 = dmin(fnfs, fnrefs), m2 = dmin(m1, 1.0);
+#if PLANT == 10
+  g_arm = fncs > m2 ? 0 : (m1 < 1.0 ? (fnfs < fnrefs ? 3 : 2) : 1);
+#elif PLANT == 11
+  g_arm = fncs > m2 ? 1 : (m1 < 1.0 ? (fnfs < fnrefs ? 2 : 3) : 0);
+#else
+  g_arm = fncs > m2 ? 0 : (m1 < 1.0 ? (fnfs < fnrefs ? 2 : 3) : 1);
+#endif
+}
+
 /* stencil.F90:147-169 calculate_interpolated_cell_value: raw copy where the
  * shifted neighbour leaves [1, end+halo] or for the z shift at k=nz. */
 static double flow_at(const or_flow *f, int k, int j, int i) {
@@ -89,6 +107,7 @@ static double flux_in_w(int k, double wval, double dt, double sum_cfl_out, doubl
   double fncs = (fc - fu) * rfdels;
   double fnrefs = fncs * hrcfl;
   double ftemp = dmax(fncs, dmin(dmin(fnfs, fnrefs), 1.0));
+  set_arm(fnfs, fncs, fnrefs);
   return ftemp * fdels + fu;
 }
 
@@ -106,6 +125,7 @@ static double flux_in_h(double vel, double dt, double sum_cfl_out, double fdels,
   double fncs = (fc - fu) * rfdels;
   double fnrefs = fncs * hrcfl;
   double ftemp = dmax(fncs, dmin(dmin(fnfs, fnrefs), 1.0));
+  set_arm(fnfs, fncs, fnrefs);
   return ftemp * fdels + fu;
 }
 
@@ -180,6 +200,7 @@ static void vert_middle(const or_col *c, double *flux_z) {
     double fdels = fd - fu;
     if (fabs(fcurvs) >= fabs(fdels)) {
       flux_z[k + 1] = fc;
+      m_arm_z[k + 1] = 0;
     } else {
       double s = 0.0;
       s = s + V1(c->rdz, PLANT == 7 ? k : k + ko) *
@@ -196,6 +217,7 @@ static void vert_middle(const or_col *c, double *flux_z) {
                               : V1(c->rdz, k + c->kdof) * advpos + V1(c->rdz, k + 1 + c->kdof) * advneg;
       flux_z[k + 1] = flux_in_w(k, FL(W, k, jf, ifl), c->dt, s, rdc, fdels, fd, fc, fu, rdu, fgt1, fgt2, c->dzn,
                                 c->rdzn);
+      m_arm_z[k + 1] = g_arm;
     }
   }
 }
@@ -221,6 +243,7 @@ static void vert_bottom(const or_col *c, double *flux_z) {
   double fdels = fd - fu;
   if (fabs(fcurvs) >= fabs(fdels)) {
     flux_z[k + 1] = fc;
+    m_arm_z[k + 1] = 0;
     return;
   }
   double s = 0.0, fgt1, fgt2, rdu, rdc;
@@ -246,6 +269,7 @@ static void vert_bottom(const or_col *c, double *flux_z) {
     rdc = V1(c->rdz, PLANT == 9 ? k + c->kdof : k + 1 + c->kdof);
   }
   flux_z[k + 1] = flux_in_w(k, wk, c->dt, s, rdc, fdels, fd, fc, fu, rdu, fgt1, fgt2, c->dzn, c->rdzn);
+  m_arm_z[k + 1] = g_arm;
 }
 
 /* ultimateflux.F90:413-523 handle_vertical_fluxes_topofcolumn */
@@ -270,6 +294,7 @@ static void vert_top(const or_col *c, double *flux_z) {
   double fdels = fd - fu;
   if (fabs(fcurvs) >= fabs(fdels)) {
     flux_z[k + 1] = fc;
+    m_arm_z[k + 1] = 0;
     return;
   }
   double s = 0.0, fgt1, fgt2, rdu, rdc;
@@ -296,6 +321,7 @@ static void vert_top(const or_col *c, double *flux_z) {
     rdc = V1(c->rdz, kt);
   }
   flux_z[k + 1] = flux_in_w(k, wk, c->dt, s, rdc, fdels, fd, fc, fu, rdu, fgt1, fgt2, c->dzn, c->rdzn);
+  m_arm_z[k + 1] = g_arm;
 }
 
 /* ultimateflux.F90:81-166 handle_x_direction_fluxes: right face of column is */
@@ -317,6 +343,7 @@ static void x_faces(const or_col *c, int kmax, double *flux_x) {
     double fcurvs = fd - 2.0 * fc + fu;
     if (fabs(fcurvs) >= fabs(fdels)) {
       flux_x[k] = fc;
+      m_arm_x[k] = 0;
       continue;
     }
     double s = 0.0;
@@ -336,6 +363,7 @@ static void x_faces(const or_col *c, int kmax, double *flux_x) {
     double fgt2 =
         grad_y_two(k, V, jf, ifl + 1, jf, ifl, c->cy, c->dt, fc, F, js, is, advpos, js, is + 1, advneg);
     flux_x[k] = flux_in_h(uk, c->dt, s, fdels, fd, fc, fu, fgt1, fgt2, c->cx, fcurvs);
+    m_arm_x[k] = g_arm;
   }
 }
 
@@ -360,6 +388,7 @@ static void y_faces(const or_col *c, int kmax, double *flux_y) {
     double fcurvs = fd - 2.0 * fc + fu;
     if (fabs(fcurvs) >= fabs(fdels)) {
       flux_y[k] = fc;
+      m_arm_y[k] = 0;
       continue;
     }
     double s = 0.0;
@@ -372,6 +401,7 @@ static void y_faces(const or_col *c, int kmax, double *flux_y) {
     double fgt2 = grad_x(k, U, jf, ifl - 1, jf, ifl, jf + 1, ifl - 1, jf + 1, ifl, c->cx, c->dt, fc,
                          A(F, k, js + jo, is + 1), A(F, k, js + jo, is - 1), 0);
     flux_y[k] = flux_in_h(vk, c->dt, s, fdels, fd, fc, fu, fgt1, fgt2, c->cy, fcurvs);
+    m_arm_y[k] = g_arm;
   }
 }
 
@@ -388,10 +418,12 @@ static double *m_flux_x, *m_flux_y, *m_flux_z;
 /* Optional record of face values per column, for the face-level checks */
 typedef struct {
   double *fx, *fy, *fz; /* each (nz, ny+4, nx+4) or NULL */
+  double *ax, *ay, *az; /* the faces' limiter arms (g_arm), same layout, or NULL */
 } or_facerec;
 
 /* ultimateflux.F90:23-60 ultflx on a single rank (my_coords = dim_sizes-1 = 0) */
 static void ultflx(const or_col *c, int kmin, int kmax, double *flux_previous_x_col) {
+  for (int k = 0; k <= g_nz + 1; k++) m_arm_x[k] = m_arm_y[k] = m_arm_z[k] = NAN;
   /* :62-79 handle_vertical_fluxes */
   if (kmin == 1) vert_bottom(c, m_flux_z);
   vert_top(c, m_flux_z);
@@ -476,6 +508,9 @@ static void advect_field(const or_grid *g, int x, int y, int halo_column, double
       if (rec->fx) A(rec->fx, k, y, x) = fpx_col[k - 1];
       if (rec->fy) A(rec->fy, k, y, x) = m_flux_y[k];
       if (rec->fz) A(rec->fz, k, y, x) = m_flux_z[k];
+      if (rec->ax) A(rec->ax, k, y, x) = m_arm_x[k];
+      if (rec->ay) A(rec->ay, k, y, x) = m_arm_y[k];
+      if (rec->az) A(rec->az, k, y, x) = m_arm_z[k];
     }
   }
   if (!halo_column) {
@@ -518,14 +553,17 @@ static void scratch_free(or_fieldscratch *f) {
  * when active (:661); tracers and the flow fields are not gated (:603-646,
  * :684-694).  The s-fields are accumulated in place (the reference's
  * `s += ...`).
- * rec_field: index of the field whose face values are written to rec (-1: none).
+ * rec_field: index of the field whose face values are written to rec (-1: none),
+ * and the limiter arms of its faces to rec_a* (each may be NULL): at column
+ * (k, y, x) the right x face, the north y face and flux_z(k).
  */
 int oracle_tvd_timestep(int nz, int ny, int nx, double cx, double cy, const double *rdz, const double *rdzn,
                         const double *dz, const double *dzn, const double *tzc1, const double *tzc2,
                         const double *tzd1, const double *tzd2, double dtm, int field_stepping,
                         int momentum_stepping, int scalar_stepping, int advect, int n_q, int n_tracers,
                         const int *active, const double *const *fields, const double *const *zfields, double *const *sfields,
-                        int rec_field, double *rec_fx, double *rec_fy, double *rec_fz) {
+                        int rec_field, double *rec_fx, double *rec_fy, double *rec_fz, double *rec_ax,
+                        double *rec_ay, double *rec_az) {
   if (nz < 4 || ny < 1 || nx < 1 || n_q < 0 || n_tracers < 0) return 1;
   g_nz = nz;
   g_ny = ny;
@@ -534,10 +572,13 @@ int oracle_tvd_timestep(int nz, int ny, int nx, double cx, double cy, const doub
   m_flux_x = (double *)calloc((size_t)nz + 2, sizeof(double));
   m_flux_y = (double *)calloc((size_t)nz + 2, sizeof(double));
   m_flux_z = (double *)calloc((size_t)nz + 2, sizeof(double));
+  m_arm_x = (double *)calloc((size_t)nz + 2, sizeof(double));
+  m_arm_y = (double *)calloc((size_t)nz + 2, sizeof(double));
+  m_arm_z = (double *)calloc((size_t)nz + 2, sizeof(double));
   int nf = 4 + n_q + n_tracers;
   or_fieldscratch *fs = (or_fieldscratch *)malloc(sizeof(or_fieldscratch) * (size_t)nf);
   for (int n = 0; n < nf; n++) fs[n] = scratch_new();
-  or_facerec rec = {rec_fx, rec_fy, rec_fz};
+  or_facerec rec = {rec_fx, rec_fy, rec_fz, rec_ax, rec_ay, rec_az};
 
   /* tvdadvection.F90:608-609, 656-657, 704-705 */
   double dt_mom = (momentum_stepping == OR_FORWARD) ? dtm : dtm * 2.0;
@@ -587,6 +628,9 @@ int oracle_tvd_timestep(int nz, int ny, int nx, double cx, double cy, const doub
   free(m_flux_x);
   free(m_flux_y);
   free(m_flux_z);
+  free(m_arm_x);
+  free(m_arm_y);
+  free(m_arm_z);
   return 0;
 }
 

@@ -225,6 +225,24 @@ def ultimate_quickest_1d(phi_u, phi_c, phi_d, c):
     return out
 
 
+def ultimate_arm_1d(phi_u, phi_c, phi_d, c):
+    """Which part of Leonard's universal limiter sets the face: 0 the face is
+    phi_C (upwinding outside 0 < phi~_C < 1, or the lower bound phi~_C), 1
+    phi_D (the bound 1), 2 the QUICKEST value (no bound binds), 3
+    phi_U + (phi_C - phi_U)/c (the upper bound phi~_C/c)."""
+    delta = phi_d - phi_u
+    with np.errstate(divide="ignore", invalid="ignore"):
+        nc = (phi_c - phi_u) / delta
+    q = 0.5 * (phi_d + phi_c) - 0.5 * c * (phi_d - phi_c) - (1.0 - c * c) / 6.0 * (phi_d - 2.0 * phi_c + phi_u)
+    mono = (nc > 0.0) & (nc < 1.0) & (delta != 0.0)
+    nq = (q - phi_u) / np.where(delta == 0.0, 1.0, delta)
+    upper = nc / c
+    inner = np.minimum(nq, upper)
+    arm = np.where(inner < 1.0, np.where(nq < upper, 2, 3), 1)
+    arm = np.where(nc > np.minimum(inner, 1.0), 0, arm)
+    return np.where(mono, arm, 0)
+
+
 def _profile_1d(n, rng):
     x = np.linspace(0, 2 * np.pi, n, endpoint=False)
     p = np.sin(x) + 0.3 * np.sin(3 * x) + (x > 2.0) * 0.8 - (x > 4.5) * 1.1
@@ -257,7 +275,7 @@ def test_one_dimensional_ultimate_quickest(axis, sgn):
     V = np.full(shape, vel) if axis == "y" else zero.copy()
     st = ss.State(nz=nz, ny=ny, nx=nx, cx=ss.horizontal_c(dxy), cy=ss.horizontal_c(dxy), grid=g,
                   fields=[U, V, zero.copy(), F, F.copy()], zfields=[])
-    _, faces = tend(st, dtm=dtm, rec_field=3)
+    _, faces = tend(st, dtm=dtm, rec_field=3, arms=True)
     c = abs(vel) * dtm / dxy
     full = np.concatenate([prof[-2:], prof, prof[:2]])  # periodic, index p+2 = cell p
     ic = np.arange(n)  # face between cell p and p+1 (0-based interior p)
@@ -268,10 +286,16 @@ def test_one_dimensional_ultimate_quickest(axis, sgn):
     expect = ultimate_quickest_1d(pu, pc, pd, c)
     if axis == "x":
         got = faces[0][2:nx + 2, 3, 2]
+        arm = faces[3][2:nx + 2, 3, 2]
         # the last interior row's faces are recorded at the periodic wrap
     else:
         got = faces[1][3, 2:ny + 2, 2]
+        arm = faces[4][3, 2:ny + 1, 2]  # the wrapped last row records no arm
     np.testing.assert_allclose(got, expect, rtol=0, atol=1e-13)
+    # the limiter arm recorded for the adjoint is the bound Leonard's limiter takes
+    want = ultimate_arm_1d(pu, pc, pd, c)[:len(arm)]
+    np.testing.assert_array_equal(arm, want)
+    assert len(set(want.tolist())) >= 3
 
 
 @pytest.mark.parametrize("sgn", [1.0, -1.0])
@@ -287,7 +311,7 @@ def test_one_dimensional_ultimate_quickest_vertical(sgn):
     zero = np.zeros(shape)
     st = ss.State(nz=nz, ny=ny, nx=nx, cx=0.02, cy=0.02, grid=g, fields=[zero, zero.copy(), W, F, F.copy()],
                   zfields=[])
-    _, faces = tend(st, dtm=dtm, rec_field=3)
+    _, faces = tend(st, dtm=dtm, rec_field=3, arms=True)
     c = 0.8 * dtm / dz
     k = np.arange(2, nz - 1)  # 1-based level whose top face is flux_z(k+1)
     p = prof  # p[k-1] = F(k)
@@ -298,6 +322,9 @@ def test_one_dimensional_ultimate_quickest_vertical(sgn):
     expect = ultimate_quickest_1d(pu, pc, pd, c)
     got = faces[2][4, 4, k]  # 0-based index k = flux_z(k+1)
     np.testing.assert_allclose(got, expect, rtol=0, atol=1e-13)
+    want = ultimate_arm_1d(pu, pc, pd, c)
+    np.testing.assert_array_equal(faces[5][4, 4, k], want)
+    assert len(set(want.tolist())) >= 3
 
 
 # --------------------------------------------------------------------------
@@ -779,6 +806,33 @@ def test_planted_vertical_mistakes_are_caught(tmp_path):
                 except AssertionError:
                     failed += 1
         assert failed > 0, f"planted mistake {plant} ({what}) passes every vertical check"
+
+
+# Limiter arms recorded for the adjoint: a planted mislabelling of the arms
+# (the face values unchanged) fails the 1-D checks of Leonard's bounds.
+PLANTED_ARMS = {
+    10: "arms: QUICKEST and the upper bound exchanged",
+    11: "arms: fc and fd exchanged",
+}
+
+
+def test_planted_arm_mistakes_are_caught(tmp_path):
+    for plant, what in PLANTED_ARMS.items():
+        path = oracle.build_variant(plant, tmp_path)
+        failed = 0
+        with oracle.use_library(path):
+            for axis in ("x", "y"):
+                for sgn in (1.0, -1.0):
+                    try:
+                        test_one_dimensional_ultimate_quickest(axis, sgn)
+                    except AssertionError:
+                        failed += 1
+            for sgn in (1.0, -1.0):
+                try:
+                    test_one_dimensional_ultimate_quickest_vertical(sgn)
+                except AssertionError:
+                    failed += 1
+        assert failed > 0, f"planted mistake {plant} ({what}) passes every arm check"
 
 
 # --------------------------------------------------------------------------
