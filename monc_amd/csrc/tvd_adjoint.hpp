@@ -808,8 +808,13 @@ struct AdjLds {
 // memory from ~70 per face owner to ~(strip+3)(R+3)/(strip R) per target.
 // Every thread of the block calls this with the same tile.
 template <typename T, int KIND, int R, int NS = 1, bool REC = false>
-__device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L, const AdjArgs<T, NS>& A, int band, int ty,
-                                             int tx, int strip, AdjLds<T, R, NS + 3>& lds) {
+__device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L0, const AdjArgs<T, NS>& A0, int band0, int ty0,
+                                             int tx0, int strip0, AdjLds<T, R, NS + 3>& lds) {
+  // the tile's arguments are uniform over the block (the ensemble reads them
+  // from its member table)
+  const LaunchArgs<T> L = uniform(L0);
+  const AdjArgs<T, NS> A = uniform(A0);
+  const int band = uniform(band0), ty = uniform(ty0), tx = uniform(tx0), strip = uniform(strip0);
   constexpr int DK = Flows<T, KIND>::DK, DJ = Flows<T, KIND>::DJ, DI = Flows<T, KIND>::DI;
   constexpr int NT = NS + 3, ROWS = R + 3, PER = ROWS * ADJ_KB, NTH = 64 * R;
   const int nz = L.d.nz, ny = L.d.ny, nx = L.d.nx;

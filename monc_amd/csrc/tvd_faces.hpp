@@ -76,6 +76,23 @@ __device__ __forceinline__ T lane_value(T x, int l) {
   }
 }
 
+// A value every lane of the wave holds alike, kept in scalar registers: each
+// 32-bit word through readfirstlane.  A value the compiler cannot prove
+// uniform (one loaded from a table of members, say) otherwise stays in vector
+// registers, and a buffer load whose scalar offset derives from it becomes a
+// loop over the wave's distinct values.
+template <typename S>
+__device__ __forceinline__ S uniform(const S& x) {
+  static_assert(sizeof(S) % 4 == 0, "whole 32-bit words");
+  struct Words {
+    int w[sizeof(S) / 4];
+  };
+  Words a = __builtin_bit_cast(Words, x);
+#pragma unroll
+  for (int n = 0; n < (int)(sizeof(S) / 4); n++) a.w[n] = __builtin_amdgcn_readfirstlane(a.w[n]);
+  return __builtin_bit_cast(S, a);
+}
+
 // gfx9 raw-buffer descriptor word 3 (32-bit data, no swizzle)
 constexpr int BUFFER_RSRC_WORD3 = 0x00020000;
 
