@@ -812,8 +812,24 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L0, const AdjA
                                              int tx0, int strip0, AdjLds<T, R, NS + 3>& lds) {
   // the tile's arguments are uniform over the block (the ensemble reads them
   // from its member table)
-  const LaunchArgs<T> L = uniform(L0);
-  const AdjArgs<T, NS> A = uniform(A0);
+  LaunchArgs<T> L = L0;
+  L.d = uniform(L0.d);
+  L.u = uniform_ptr(L0.u), L.v = uniform_ptr(L0.v), L.w = uniform_ptr(L0.w);
+  for (VMet* m : {&L.mc, &L.mw}) {
+    m->rdz = uniform_ptr(m->rdz), m->rdzn = uniform_ptr(m->rdzn), m->dzn = uniform_ptr(m->dzn);
+    m->t1 = uniform_ptr(m->t1), m->t2 = uniform_ptr(m->t2);
+  }
+  L.dtm = uniform(L0.dtm), L.cx = uniform(L0.cx), L.cy = uniform(L0.cy);
+  AdjArgs<T, NS> A;
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    A.F[n] = uniform_ptr(A0.F[n]);
+    A.lin[n] = uniform_ptr(A0.lin[n]);
+    A.lf[n] = uniform_ptr(A0.lf[n]);
+    A.rec[n] = REC ? uniform_ptr(A0.rec[n]) : nullptr;
+  }
+  A.lu = uniform_ptr(A0.lu), A.lv = uniform_ptr(A0.lv), A.lw = uniform_ptr(A0.lw);
+  A.dtm = uniform(A0.dtm);
   const int band = uniform(band0), ty = uniform(ty0), tx = uniform(tx0), strip = uniform(strip0);
   constexpr int DK = Flows<T, KIND>::DK, DJ = Flows<T, KIND>::DJ, DI = Flows<T, KIND>::DI;
   constexpr int NT = NS + 3, ROWS = R + 3, PER = ROWS * ADJ_KB, NTH = 64 * R;
@@ -857,16 +873,20 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L0, const AdjA
   dst[NS + 1] = A.lv;
   dst[NS + 2] = A.lw;
   // add ring column c (all targets) to the global adjoints and clear it
+  // (target by target, so that each target's pointer is known to be global)
   auto flush = [&](int c) {
     if (c < 1 || c > nx + 4) return;
-    for (int e = tid; e < NT * PER; e += NTH) {
-      const int t = e / PER, rem = e - t * PER;
-      const int rr = rem / ADJ_KB, lev = rem - rr * ADJ_KB;
-      T* p = &lds.ring[t][c & 3][rr][lev];
-      const T v = *p;
-      const int kk = k0 - 1 + lev, jj = j0 - 1 + rr;
-      if (v != T(0) && kk >= 1 && kk <= nz && jj >= 1 && jj <= ny + 4) unsafeAtomicAdd(dst[t] + at(L.d, kk, jj, c), v);
-      *p = T(0);
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      T* const g = dst[t];
+      for (int e = tid; e < PER; e += NTH) {
+        const int rr = e / ADJ_KB, lev = e - rr * ADJ_KB;
+        T* p = &lds.ring[t][c & 3][rr][lev];
+        const T v = *p;
+        const int kk = k0 - 1 + lev, jj = j0 - 1 + rr;
+        if (v != T(0) && kk >= 1 && kk <= nz && jj >= 1 && jj <= ny + 4) unsafeAtomicAdd(g + at(L.d, kk, jj, c), v);
+        *p = T(0);
+      }
     }
   };
   for (int ic = i_begin; ic < i_end; ic++) {
@@ -891,9 +911,8 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L0, const AdjA
     __syncthreads();
   }
   // the last three columns the tile reached (distinct ring slots)
-  flush(i_end - 1);
-  flush(i_end);
-  flush(i_end + 1);
+#pragma nounroll
+  for (int c = i_end - 1; c <= i_end + 1; c++) flush(c);
   __syncthreads();
 }
 

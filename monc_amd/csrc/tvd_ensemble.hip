@@ -21,6 +21,9 @@
 // Adjoint step, halo fill and the adjoint's periodic fold: persistent
 // grid-stride launches over the blocks of every member (a prefix table of
 // per-member block counts maps a global block to its member).
+// max and min as compare and select in this translation unit (faster for the
+// persistent step kernel; see tvd_faces.hpp)
+#define TVD_MINMAX_SELECT 1
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

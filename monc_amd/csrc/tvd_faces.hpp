@@ -93,6 +93,16 @@ __device__ __forceinline__ S uniform(const S& x) {
   return __builtin_bit_cast(S, a);
 }
 
+// A pointer to global memory made uniform: its words through readfirstlane,
+// and the result typed as a global (address space 1) pointer again, so that
+// accesses through it stay global rather than generic (flat) instructions.
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  typedef __attribute__((address_space(1))) T GT;
+  const unsigned long long a = uniform((unsigned long long)p);
+  return (T*)(GT*)a;
+}
+
 // gfx9 raw-buffer descriptor word 3 (32-bit data, no swizzle)
 constexpr int BUFFER_RSRC_WORD3 = 0x00020000;
 
@@ -131,6 +141,41 @@ template <typename T>
 __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T>
 __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+// In the working precisions max and min are one instruction each
+// (v_max_f64, v_min_f32, ...) instead of a compare and a select per 32-bit
+// half.  For numbers they give the value of a > b ? a : b (a < b ? a : b);
+// at a tie of +0 and -0 the sign of the zero may differ, which no use here
+// can observe (each result is added to a non-zero term or passed to fabs, or
+// is a limiter ratio that multiplies a difference).  NaN operands aside.
+// Written as the instructions themselves: the fmax/fmin builtins also quiet
+// every operand first (IEEE mode), which costs an instruction per operand
+// and measured 36-40 more registers in the scalar-pair kernels (spilling).
+// A translation unit that defines TVD_MINMAX_SELECT keeps the compare and
+// select: the persistent ensemble step measured 7.33 ms per step so against
+// 8.33 ms with the instructions (the single-domain FP64 step 7.40 against
+// 8.03 ms the other way).
+#ifndef TVD_MINMAX_SELECT
+__device__ __forceinline__ double tmax(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ double tmin(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float tmax(float a, float b) {
+  float r;
+  asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float tmin(float a, float b) {
+  float r;
+  asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+#endif
 // Fortran SIGN(0.5,x): IEEE signed zero gives the negative branch for -0.0.
 template <typename T>
 __device__ __forceinline__ T half_sign(T x) { return copysign(T(0.5), x); }
