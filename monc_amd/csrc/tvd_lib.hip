@@ -279,7 +279,7 @@ void launch_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
 // spills.
 template <typename T, int KIND, int NS, int R>
 constexpr int stage_wpe() {
-  return sizeof(T) == 4 ? 4 : (KIND == KIND_SCALAR && NS == 2 ? 2 : 3);
+  return sizeof(T) == 4 ? 4 : (KIND == KIND_SCALAR && NS == 2 ? 2 : (R == 15 ? 4 : 3));
 }
 
 template <typename T, int KIND, int NS, int R, int MODE>
@@ -290,11 +290,15 @@ void launch_stage(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G,
                      tiles_y);
 }
 
-// Rows per block of the stage kernel (11 only for the FP64 flow kinds).
-constexpr int STAGE_ROWS[4] = {4, 6, 8, 11};
+// Rows per block of the stage kernel (11 and 15 only for the FP64 flow
+// kinds).  15 rows is 16 waves, 4 per SIMD, within 128 registers and 139 KB
+// of LDS, but measured slower than 11 at 512x512x256 (u 1.92 against 1.55 ms
+// per launch); the autotuner tries it.
+constexpr int STAGE_ROWS[5] = {4, 6, 8, 11, 15};
+constexpr int STAGE_DEFAULT_ROWS = 11;
 
 bool stage_shape_compiled(int precision, int kind, int rows) {
-  return rows != 11 || (precision == MONC_TVD_FP64 && kind != KIND_SCALAR);
+  return (rows != 11 && rows != 15) || (precision == MONC_TVD_FP64 && kind != KIND_SCALAR);
 }
 
 // Default kernel per field kind for the forward step, from timings at
@@ -307,13 +311,13 @@ bool stage_shape_compiled(int precision, int kind, int rows) {
 // strip, or 0 for the march kernel.
 int stage_default_strip(const monc_tvd_ctx* c, int kind) {
   if (c->precision != MONC_TVD_FP64 || kind == KIND_SCALAR) return 0;
-  const long tiles = (long)((c->nz + 63) / 64) * ((c->ny + 10) / 11);
+  const long tiles = (long)((c->nz + 63) / 64) * ((c->ny + STAGE_DEFAULT_ROWS - 1) / STAGE_DEFAULT_ROWS);
   for (int strip : {128, 64, 32})
     if (tiles * ((c->nx + strip - 1) / strip) >= 512) return strip;
   return 0;
 }
 
-bool stage_rows_ok(int rows) { return rows == 4 || rows == 6 || rows == 8 || rows == 11; }
+bool stage_rows_ok(int rows) { return rows == 4 || rows == 6 || rows == 8 || rows == 11 || rows == 15; }
 
 // A field group the stage kernel takes: a flow field advected by itself or
 // one or two scalars with the same timestep, fields under 4 GiB; MODE_STEP
@@ -343,6 +347,12 @@ void launch_stage_rows(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T
     case 11:
       if constexpr (sizeof(T) == 8 && KIND != KIND_SCALAR) {
         launch_stage<T, KIND, NS, 11, MODE>(c, L, G, strip);
+        break;
+      }
+      [[fallthrough]];
+    case 15:
+      if constexpr (sizeof(T) == 8 && KIND != KIND_SCALAR) {
+        launch_stage<T, KIND, NS, 15, MODE>(c, L, G, strip);
         break;
       }
       [[fallthrough]];
@@ -422,7 +432,7 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
       const bool dflt = c->kind_kernel[kk] == MONC_TVD_KERNEL_MARCH && !c->rows[kk] && !c->strip[kk] &&
                         c->kernel == MONC_TVD_KERNEL_MARCH && dstrip > 0;
       if ((c->kind_kernel[kk] == MONC_TVD_KERNEL_STAGE || dflt) && stage_fits<T, MODE>(c, L, G)) {
-        const int srows = dflt ? 11 : (c->rows[kk] > 0 ? c->rows[kk] : 6);
+        const int srows = dflt ? STAGE_DEFAULT_ROWS : (c->rows[kk] > 0 ? c->rows[kk] : 6);
         const int sstrip = dflt ? dstrip : (c->strip[kk] > 0 ? c->strip[kk] : 64);
         launch_stage_group<T, MODE>(c, L, G, kk, srows, sstrip);
         HIP_TRY(hipGetLastError());
@@ -1483,9 +1493,9 @@ int monc_tvd_set_tiling(monc_tvd_ctx* c, int kernel, int rows, int strip) {
   if (kernel != MONC_TVD_KERNEL_MARCH && kernel != MONC_TVD_KERNEL_CELL && kernel != MONC_TVD_KERNEL_STAGE)
     return fail(MONC_TVD_ERR_ARG, "kernel %d is not 0 (march), 1 (cell) or 2 (stage)", kernel);
   if (kernel == MONC_TVD_KERNEL_STAGE && rows != 0 && !stage_rows_ok(rows))
-    return fail(MONC_TVD_ERR_ARG, "no stage shape with %d rows (4, 6, 8 or 11)", rows);
-  if (kernel == MONC_TVD_KERNEL_STAGE && rows == 11 && c->precision != MONC_TVD_FP64)
-    return fail(MONC_TVD_ERR_ARG, "the 11-row stage shape exists in FP64 only");
+    return fail(MONC_TVD_ERR_ARG, "no stage shape with %d rows (4, 6, 8, 11 or 15)", rows);
+  if (kernel == MONC_TVD_KERNEL_STAGE && (rows == 11 || rows == 15) && c->precision != MONC_TVD_FP64)
+    return fail(MONC_TVD_ERR_ARG, "the %d-row stage shape exists in FP64 only", rows);
   if (strip < 0 || strip > 4096) return fail(MONC_TVD_ERR_ARG, "strip %d out of range [0,4096]", strip);
   const int nch = (c->nz + 63) / 64;
   if (kernel == MONC_TVD_KERNEL_MARCH && rows != 0 && !march_rows_ok(nch, rows))
@@ -1512,7 +1522,7 @@ int monc_tvd_get_tiling(monc_tvd_ctx* c, int* chosen) {
     const int dstrip = stage_default_strip(c, kind);
     if (kernel == MONC_TVD_KERNEL_MARCH && !rows && !strip && c->buffer_ok && dstrip > 0) {
       kernel = MONC_TVD_KERNEL_STAGE;
-      rows = 11;
+      rows = STAGE_DEFAULT_ROWS;
       strip = dstrip;
     } else if (kernel == MONC_TVD_KERNEL_STAGE) {
       rows = rows > 0 ? rows : 6;

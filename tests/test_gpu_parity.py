@@ -268,13 +268,13 @@ def test_forward_steps_with_tracers_and_inactive_q(gpu):
 # fill one to four 64-level bands partially and exactly, row counts that do
 # and do not divide ny, strips that do and do not divide nx, one and two
 # scalars, against the oracle's forward run.
-STAGE_CASES = [(nz, rows) for nz in (4, 5, 63, 64, 65, 70, 128, 150, 193, 256) for rows in (4, 6, 8, 11)]
+STAGE_CASES = [(nz, rows) for nz in (4, 5, 63, 64, 65, 70, 128, 150, 193, 256) for rows in (4, 6, 8, 11, 15)]
 
 
 @pytest.mark.parametrize("nz,rows", STAGE_CASES)
 @pytest.mark.parametrize("strip,n_q", [(5, 1), (16, 2)])
 def test_stage_kernel_matches_oracle(gpu, nz, rows, strip, n_q):
-    """rows = 11 is compiled for u, v and w only: the scalars then take 4 rows."""
+    """rows = 11 and 15 are compiled for u, v and w only: the scalars then take 4 rows."""
     st = ss.make_state(nz, 11, 13, 50.0, 25.0 * (nz - 1), seed=nz + rows + strip, n_q=n_q, with_zfields=False)
     dt = ss.cfl_dt(st, 0.3)
     ref = oracle.forward_run(st, dt, 3)
@@ -284,7 +284,7 @@ def test_stage_kernel_matches_oracle(gpu, nz, rows, strip, n_q):
         assert e <= TOL, (n, e)
 
 
-STAGE_ACCUM_CASES = [(nz, rows) for nz in (5, 64, 70, 150, 256) for rows in (4, 6, 11)]
+STAGE_ACCUM_CASES = [(nz, rows) for nz in (5, 64, 70, 150, 256) for rows in (4, 6, 11, 15)]
 
 
 @pytest.mark.parametrize("nz,rows", STAGE_ACCUM_CASES)
