@@ -295,23 +295,27 @@ void launch_stage(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G,
 // of LDS, but measured slower than 11 at 512x512x256 (u 1.92 against 1.55 ms
 // per launch); the autotuner tries it.
 constexpr int STAGE_ROWS[5] = {4, 6, 8, 11, 15};
-constexpr int STAGE_DEFAULT_ROWS = 11;
 
 bool stage_shape_compiled(int precision, int kind, int rows) {
   return (rows != 11 && rows != 15) || (precision == MONC_TVD_FP64 && kind != KIND_SCALAR);
 }
 
 // Default kernel per field kind for the forward step, from timings at
-// 512x512x256 on MI355X: FP64 u, v and w staged in 11-row tiles of 128
-// columns (1.67-1.71 ms per launch against 2.03-2.14 ms marching); the FP64
-// scalar pair and every FP32 kind march (their staged forms measured slower).
-// Small domains keep the march kernel: the staged form runs one block per CU,
-// so it needs at least two blocks per CU (512) to fill the chip; the strip is
-// the longest of 128, 64 and 32 columns that still gives them.  Returns the
-// strip, or 0 for the march kernel.
+// 512x512x256 on MI355X (the autotuner's pick there): in FP64, u, v and w
+// staged in 11-row tiles of 128 columns (1.53-1.57 ms per launch against
+// 2.0-2.1 ms marching) and the (th, q) pair in 8-row tiles of 128 columns
+// (2.54 against 2.71 ms for the march kernel's 2 x 64 shape, since max and
+// min became single instructions); every FP32 kind marches (its staged forms
+// measured slower).  Small domains keep the march kernel: the staged form runs
+// one block per CU, so it needs at least two blocks per CU (512) to fill the
+// chip; the strip is the longest of 128, 64 and 32 columns that still gives
+// them.  Returns the strip, or 0 for the march kernel.
+int stage_default_rows(int kind) { return kind == KIND_SCALAR ? 8 : 11; }
+
 int stage_default_strip(const monc_tvd_ctx* c, int kind) {
-  if (c->precision != MONC_TVD_FP64 || kind == KIND_SCALAR) return 0;
-  const long tiles = (long)((c->nz + 63) / 64) * ((c->ny + STAGE_DEFAULT_ROWS - 1) / STAGE_DEFAULT_ROWS);
+  if (c->precision != MONC_TVD_FP64) return 0;
+  const int rows = stage_default_rows(kind);
+  const long tiles = (long)((c->nz + 63) / 64) * ((c->ny + rows - 1) / rows);
   for (int strip : {128, 64, 32})
     if (tiles * ((c->nx + strip - 1) / strip) >= 512) return strip;
   return 0;
@@ -432,7 +436,7 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
       const bool dflt = c->kind_kernel[kk] == MONC_TVD_KERNEL_MARCH && !c->rows[kk] && !c->strip[kk] &&
                         c->kernel == MONC_TVD_KERNEL_MARCH && dstrip > 0;
       if ((c->kind_kernel[kk] == MONC_TVD_KERNEL_STAGE || dflt) && stage_fits<T, MODE>(c, L, G)) {
-        const int srows = dflt ? STAGE_DEFAULT_ROWS : (c->rows[kk] > 0 ? c->rows[kk] : 6);
+        const int srows = dflt ? stage_default_rows(S.kind[n]) : (c->rows[kk] > 0 ? c->rows[kk] : 6);
         const int sstrip = dflt ? dstrip : (c->strip[kk] > 0 ? c->strip[kk] : 64);
         launch_stage_group<T, MODE>(c, L, G, kk, srows, sstrip);
         HIP_TRY(hipGetLastError());
@@ -1522,7 +1526,7 @@ int monc_tvd_get_tiling(monc_tvd_ctx* c, int* chosen) {
     const int dstrip = stage_default_strip(c, kind);
     if (kernel == MONC_TVD_KERNEL_MARCH && !rows && !strip && c->buffer_ok && dstrip > 0) {
       kernel = MONC_TVD_KERNEL_STAGE;
-      rows = STAGE_DEFAULT_ROWS;
+      rows = stage_default_rows(kind);
       strip = dstrip;
     } else if (kernel == MONC_TVD_KERNEL_STAGE) {
       rows = rows > 0 ? rows : 6;
