@@ -663,7 +663,10 @@ march_kernel(LaunchArgs<T> L, FieldSet<T> S, int strip, int tiles_y) {
   const int t = bid < full ? (bid % 8) * (full / 8) + bid / 8 : bid;
   // The scalar pair reloads its x windows: two windows leave FP64 no
   // registers at 3 waves per SIMD and cost FP32 a wave per SIMD, which
-  // measured slower than the loads they save (pair_wide shapes excepted).  Vertical neighbours by wave
+  // measured slower than the loads they save (pair_wide shapes excepted).
+  // With max and min as single instructions the FP32 pair fits its windows
+  // in 76-77 registers at 6 waves per SIMD, but measured no faster (1.424
+  // against 1.429 ms per launch at 512x512x256), so it still reloads.  Vertical neighbours by wave
   // shifts pay for the w kernel (3% in FP64, 2% in FP32); the FP64 u, v and
   // scalar kernels, at 3 waves per SIMD and two shifts per value, run faster
   // loading them, and in FP32 the gain depends on the tile shape.
