@@ -42,11 +42,6 @@ template <typename T>
 struct LdsAcc {
   T* s;
   int k0, j0, rows;
-  // LDS atomic, for the adds whose offset is chosen at run time (dyn)
-  __device__ __forceinline__ void add(int k, int j, int i, T v) const {
-    if (v != T(0)) unsafeAtomicAdd(&s[((i & 3) * rows + (j - j0 + 1)) * ADJ_KB + (k - k0 + 1)], v);
-  }
-  __device__ __forceinline__ void dyn(int k, int j, int i, T v) const { add(k, j, i, v); }
   __device__ __forceinline__ T* ptr(int k, int j, int i) const {
     return &s[((i & 3) * rows + (j - j0 + 1)) * ADJ_KB + (k - k0 + 1)];
   }
@@ -58,12 +53,12 @@ struct LdsAcc {
 // against ~12 for a read and a write), and an owner adds ~110 times.  Slot
 // (t, dk, dj, di) holds the owner's total for target t at (k+dk, j+dj, i+di),
 // dk, dj, di in -1..2.  The offsets are compile-time constants at the call
-// sites of add(), so the slots no call reaches stay constant zero and vanish.
-// After each face the level offsets are folded onto each lane's own level
-// with wave shifts (lane L of a wave holds level k0+L), leaving one LDS
-// read-add-write per (t, dj, di) and the band's edge levels (flush_slots).
-// dyn() takes an offset chosen at run time (an upwind pick) straight to the
-// ring with an LDS atomic.
+// sites of add() (an offset an upwind pick chooses at run time becomes adds
+// to each candidate slot, the value selected per lane), so the slots no call
+// reaches stay constant zero and vanish.  After each face the level offsets
+// are folded onto each lane's own level with wave shifts (lane L of a wave
+// holds level k0+L), leaving one LDS read-add-write per (t, dj, di) and the
+// band's edge levels (flush_slots).  No LDS atomic remains.
 template <typename T, int NT>
 struct Slots {
   T v[NT][4][4][4];  // [target][dk+1][dj+1][di+1]
@@ -74,10 +69,7 @@ template <typename T, int NT>
 __device__ __forceinline__ void slot_add(Slots<T, NT>* s, const LdsAcc<T>* ring, int t, int k, int j, int i, int kk,
                                          int jj, int ii, T v) {
   const int dk = kk - k, dj = jj - j, di = ii - i;
-  if (dk < -1 || dk > 2 || dj < -1 || dj > 2 || di < -1 || di > 2) {  // none by construction
-    ring[t].add(kk, jj, ii, v);
-    return;
-  }
+  (void)ring;
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -94,7 +86,6 @@ struct SlotAcc {
   const LdsAcc<T>* ring;
   int k, j, i;
   __device__ __forceinline__ void add(int kk, int jj, int ii, T v) const { slot_add(s, ring, TGT, k, j, i, kk, jj, ii, v); }
-  __device__ __forceinline__ void dyn(int kk, int jj, int ii, T v) const { ring[TGT].add(kk, jj, ii, v); }
 };
 
 // The advected fields' accumulators: field n is target n (n a constant once
@@ -107,7 +98,6 @@ struct SlotAccF {
   __device__ __forceinline__ void add(int n, int kk, int jj, int ii, T v) const {
     slot_add(s, ring, n, k, j, i, kk, jj, ii, v);
   }
-  __device__ __forceinline__ void dyn(int n, int kk, int jj, int ii, T v) const { ring[n].add(kk, jj, ii, v); }
 };
 
 // A slot that some add() reaches (not the constant zero of one no call site
@@ -121,15 +111,14 @@ __device__ __forceinline__ bool live(T x) { return !(__builtin_constant_p(x) && 
 // go in phase dj+1: in one phase the wave of owner row r writes ring row
 // r+dj only, so no two waves write one row, and within a wave each lane
 // writes its own level (lane 0 and 63 alone the band's edges); a barrier
-// closes each phase.  The first barrier waits for every wave's face, whose
-// run-time-offset adds (dyn) go to the rings with LDS atomics.  Within a
+// closes each phase (the last one also orders the phases against the next
+// face's flush and the ring's column flush).  Within a
 // phase every ring value is read before any is written (the locations are
 // distinct), so the reads go out together instead of one LDS round trip per
 // read-add-write.
 template <typename T, int NT>
 __device__ __forceinline__ void flush_slots(const Slots<T, NT>& s, const LdsAcc<T> (&ring)[NT], int k, int j, int i,
                                             int lane) {
-  __syncthreads();
 #pragma unroll
   for (int b = 0; b < 4; b++) {
     const int jj = j + b - 1;
@@ -212,14 +201,8 @@ template <typename T, int DK, int DJ, int DI, class ACC>
 struct AccFlow {
   ACC a;
   int nz;
-  __device__ __forceinline__ void add(int k, int j, int i, T v) const { put<false>(k, j, i, v); }
-  __device__ __forceinline__ void dyn(int k, int j, int i, T v) const { put<true>(k, j, i, v); }
-  template <bool DYN>
-  __device__ __forceinline__ void put(int k, int j, int i, T v) const {
-    auto to = [&](int kk, int jj, int ii, T x) {
-      if constexpr (DYN) a.dyn(kk, jj, ii, x);
-      else a.add(kk, jj, ii, x);
-    };
+  __device__ __forceinline__ void add(int k, int j, int i, T v) const {
+    auto to = [&](int kk, int jj, int ii, T x) { a.add(kk, jj, ii, x); };
     if constexpr (DK == 0 && DJ == 0 && DI == 0) {
       to(k, j, i, v);
     } else if constexpr (DK == 1) {
@@ -282,6 +265,18 @@ __device__ __forceinline__ T arm_adjoint(int arm, T lam, T fu, T fc, T fd, T g1,
   return ac ? fc : (ad ? fd : (aq ? ffs : fr));
 }
 
+// The adds whose offset an upwind pick chooses at run time go to fixed-offset
+// slots with the value selected per lane (0 to the slots not chosen).  This
+// one: +x at level ksl and -x at level ksu of (jj, ii), the z-gradient pair of
+// the x and y faces, where (ksl, ksu) = (k, k-1) when pz, else (k, k+1) below
+// level nz and (k-1, k) at k = nz (knz).
+template <class AF, typename T>
+__device__ __forceinline__ void add_zpair(const AF& a, int n, bool pz, bool knz, int k, int jj, int ii, T x) {
+  a.add(n, k, jj, ii, pz || !knz ? x : -x);
+  a.add(n, k - 1, jj, ii, pz ? -x : (knz ? x : T(0.0)));
+  a.add(n, k + 1, jj, ii, pz || knz ? T(0.0) : -x);
+}
+
 // Right x face of column (j,i) at level k for the NS advected fields: values
 // (face[n], for the divergence adjoint of the flow coefficient) and reverse.
 // lam[n] is the adjoint of field n's face value.  The velocity-only
@@ -315,8 +310,10 @@ __device__ __forceinline__ void face_x_adj(int k, int j, int i, const Adv<T, NS>
   const T rzk = vm<T>(m.rdz, k + kdof);
   const T cz = fabs(sz) * rzk * dt;
   const T sy = T(0.125) * (v1 + v0 + v1m + v0m);
-  const int js = sign_pos(sy) ? j - 1 : j + 1;
+  const bool syp = sign_pos(sy);
+  const int js = syp ? j - 1 : j + 1;
   const T cyt = fabs(sy) * cy * dt;
+  const bool knz = k >= nz;  // (ksl, ksu) = (k-1, k) for W < 0 at the top level
   const HLim<T> lim(uk, cx, dt, s);
   // adjoints of cz, cyt, hrcfl, hcfl and cflcoef, summed over the fields
   T lcz = T(0.0), lcyt = T(0.0), lhr = T(0.0), lhc = T(0.0), lcc = T(0.0);
@@ -348,12 +345,13 @@ __device__ __forceinline__ void face_x_adj(int k, int j, int i, const Adv<T, NS>
     aF.add(n, k, j, i + 1, lfc * an + r.fd * ap);
     aF.add(n, k, j, i - 1, r.fu * ap);
     aF.add(n, k, j, i + 2, r.fu * an);
-    aF.dyn(n, ksl, j, i, ltmp * ap);
-    aF.dyn(n, ksu, j, i, -(ltmp * ap));
-    aF.dyn(n, ksl, j, i + 1, ltmp * an);
-    aF.dyn(n, ksu, j, i + 1, -(ltmp * an));
-    aF.dyn(n, k, js, i, lt * ap);
-    aF.dyn(n, k, js, i + 1, lt * an);
+    add_zpair(aF, n, pz, knz, k, j, i, ltmp * ap);
+    add_zpair(aF, n, pz, knz, k, j, i + 1, ltmp * an);
+    const T lta = lt * ap, ltb = lt * an;  // at row js = j-1 (syp) or j+1
+    aF.add(n, k, j - 1, i, syp ? lta : T(0.0));
+    aF.add(n, k, j + 1, i, syp ? T(0.0) : lta);
+    aF.add(n, k, j - 1, i + 1, syp ? ltb : T(0.0));
+    aF.add(n, k, j + 1, i + 1, syp ? T(0.0) : ltb);
   }
   // flows (hrcfl multiplies its adjoint first: hrcfl^2 may overflow where
   // the outflow sum is zero, and no face there has a non-zero adjoint)
@@ -419,6 +417,7 @@ __device__ __forceinline__ void face_y_adj(int k, int j, int i, const Adv<T, NS>
   const T sx = T(0.125) * (u0m + u0 + u1m + u1);
   const T posuf = T(0.5) + half_sign(sx);
   const T cxt = fabs(sx) * cx * dt;
+  const bool knz = k >= nz;  // (ksl, ksu) = (k-1, k) for W < 0 at the top level
   const HLim<T> lim(vk, cy, dt, s);
   T lcz = T(0.0), lcxt = T(0.0), lhr = T(0.0), lhc = T(0.0), lcc = T(0.0);
 #pragma unroll
@@ -446,10 +445,14 @@ __device__ __forceinline__ void face_y_adj(int k, int j, int i, const Adv<T, NS>
     aF.add(n, k, j + 1, i, lfc * an + r.fd * ap);
     aF.add(n, k, j - 1, i, r.fu * ap);
     aF.add(n, k, j + 2, i, r.fu * an);
-    aF.dyn(n, ksl, jj, i, ldiff);
-    aF.dyn(n, ksu, jj, i, -ldiff);
-    aF.dyn(n, k, jj, i + 1, ltx * (T(1.0) - posuf));
-    aF.dyn(n, k, jj, i - 1, ltx * posuf);
+    // row jj = j (vpos) or j+1
+    add_zpair(aF, n, pz, knz, k, j, i, vpos ? ldiff : T(0.0));
+    add_zpair(aF, n, pz, knz, k, j + 1, i, vpos ? T(0.0) : ldiff);
+    const T lte = ltx * (T(1.0) - posuf), ltw = ltx * posuf;
+    aF.add(n, k, j, i + 1, vpos ? lte : T(0.0));
+    aF.add(n, k, j + 1, i + 1, vpos ? T(0.0) : lte);
+    aF.add(n, k, j, i - 1, vpos ? ltw : T(0.0));
+    aF.add(n, k, j + 1, i - 1, vpos ? T(0.0) : ltw);
   }
   const T lS = -(lim.hrcfl * (lim.hrcfl * lhr));
   T lvk = courant_adj(vk, cy, dt, T(1.0), lhc, lcc);
@@ -559,8 +562,10 @@ __device__ __forceinline__ void face_z_adj(int k, int j, int i, const Adv<T, NS>
   // transverse weights (ZGrad), kept with their sums for the reverse
   const T v00 = V(k, j - 1, i), v01 = V(k, j, i), v10 = V(k + 1, j - 1, i), v11 = V(k + 1, j, i);
   const T sy = T(0.125) * (v00 + v01 + v10 + v11);
-  const int js = sign_pos(sy) ? j - 1 : j + 1;
+  const bool syp = sign_pos(sy);
+  const int js = syp ? j - 1 : j + 1;
   const T cyt = fabs(sy) * cy * dt;
+  const bool kup = kk == k;  // the upwind level kk is k (else k+1)
   const T u00 = U(k, j, i - 1), u01 = U(k, j, i), u10 = U(k + 1, j, i - 1), u11 = U(k + 1, j, i);
   const T sx = T(0.125) * (u00 + u01 + u10 + u11);
   const T posuf = T(0.5) + half_sign(sx);
@@ -600,26 +605,36 @@ __device__ __forceinline__ void face_z_adj(int k, int j, int i, const Adv<T, NS>
     lcc += r.cflcoef;
     // g1 = cyt*(fc - fsj), g2 = cxt*(fc - tx)
     const T lfc = r.fc + r.g1 * cyt + r.g2 * cxt;
-    aF.dyn(n, kk, js, i, -(r.g1 * cyt));
-    aF.dyn(n, kk, j, i + 1, -(r.g2 * cxt) * (T(1.0) - posuf));
-    aF.dyn(n, kk, j, i - 1, -(r.g2 * cxt) * posuf);
+    // level kk = k (kup) or k+1, row js = j-1 (syp) or j+1
+    const T ly1 = -(r.g1 * cyt);
+    aF.add(n, k, j - 1, i, kup && syp ? ly1 : T(0.0));
+    aF.add(n, k, j + 1, i, kup && !syp ? ly1 : T(0.0));
+    aF.add(n, k + 1, j - 1, i, !kup && syp ? ly1 : T(0.0));
+    aF.add(n, k + 1, j + 1, i, !kup && !syp ? ly1 : T(0.0));
+    const T lxe = -(r.g2 * cxt) * (T(1.0) - posuf), lxw = -(r.g2 * cxt) * posuf;
+    aF.add(n, k, j, i + 1, kup ? lxe : T(0.0));
+    aF.add(n, k + 1, j, i + 1, kup ? T(0.0) : lxe);
+    aF.add(n, k, j, i - 1, kup ? lxw : T(0.0));
+    aF.add(n, k + 1, j, i - 1, kup ? T(0.0) : lxw);
     ly += r.g1 * (fc - fsj);
     lx += r.g2 * (fc - tx);
     // stencil values
     if (mode == 0) {
       aF.add(n, k - 1, j, i, r.fu * ap);
       aF.add(n, k + 2, j, i, r.fu * an);
-      aF.dyn(n, kk, j, i, lfc);
+      aF.add(n, k, j, i, kup ? lfc : T(0.0));
+      aF.add(n, k + 1, j, i, kup ? T(0.0) : lfc);
       aF.add(n, k + 1, j, i, r.fd * ap);
       aF.add(n, k, j, i, r.fd * an);
     } else if (bottom) {
+      // k = 1 here
       if (mode == 1) {  // fu = fc = F(1), fd = F(2)
-        aF.dyn(n, 1, j, i, r.fu + lfc);
-        aF.dyn(n, 2, j, i, r.fd);
+        aF.add(n, k, j, i, r.fu + lfc);
+        aF.add(n, k + 1, j, i, r.fd);
       } else {  // fu = F(3), fc = F(2), fd = F(1)
-        aF.dyn(n, 3, j, i, r.fu);
-        aF.dyn(n, 2, j, i, lfc);
-        aF.dyn(n, 1, j, i, r.fd);
+        aF.add(n, k + 2, j, i, r.fu);
+        aF.add(n, k + 1, j, i, lfc);
+        aF.add(n, k, j, i, r.fd);
       }
     } else {
       if (mode == 1) {  // fu = F(k-1), fc = F(k), fd = F(k+1)
@@ -649,19 +664,30 @@ __device__ __forceinline__ void face_z_adj(int k, int j, int i, const Adv<T, NS>
   const T lS = -(lim.hrcfl * (lim.hrcfl * lhr));
   const T lS3 = lS * dt;
   const T lB = lS3 * cy, lC = lS3 * cx;
-  aV.dyn(kv, j, i, lB * dpos(vb0));
-  aV.dyn(kv, j - 1, i, lB * dnegabs(vb1));
-  aU.dyn(kv, j, i, lC * dpos(ub0));
-  aU.dyn(kv, j, i - 1, lC * dnegabs(ub1));
+  // level kv = k or k+1
+  const bool kvk = kv == k;
+  const T lv0 = lB * dpos(vb0), lv1 = lB * dnegabs(vb1), lu0 = lC * dpos(ub0), lu1 = lC * dnegabs(ub1);
+  aV.add(k, j, i, kvk ? lv0 : T(0.0));
+  aV.add(k + 1, j, i, kvk ? T(0.0) : lv0);
+  aV.add(k, j - 1, i, kvk ? lv1 : T(0.0));
+  aV.add(k + 1, j - 1, i, kvk ? T(0.0) : lv1);
+  aU.add(k, j, i, kvk ? lu0 : T(0.0));
+  aU.add(k + 1, j, i, kvk ? T(0.0) : lu0);
+  aU.add(k, j, i - 1, kvk ? lu1 : T(0.0));
+  aU.add(k + 1, j, i - 1, kvk ? T(0.0) : lu1);
   const T lW = lS3;  // adjoint of the W part a0 of S
   if (mode == 0) {
     const T wa = W(kw0, j, i), wb = W(kw1, j, i);
-    aW.dyn(kw0, j, i, lW * rzw * dpos(wa));
-    aW.dyn(kw1, j, i, lW * rzw * dnegabs(wb));
+    // kw0 = kk, kw1 = kk - 1
+    const T l0 = lW * rzw * dpos(wa), l1 = lW * rzw * dnegabs(wb);
+    aW.add(k, j, i, kup ? l0 : T(0.0));
+    aW.add(k + 1, j, i, kup ? T(0.0) : l0);
+    aW.add(k - 1, j, i, kup ? l1 : T(0.0));
+    aW.add(k, j, i, kup ? T(0.0) : l1);
   } else if (bottom && mode == 1) {  // a0 = wk * rzw
     lwk = lwk + lW * rzw;
   } else if (bottom) {  // a0 = rzw * (pos(W(2)) - wk)
-    aW.dyn(2, j, i, lW * rzw * dpos(W(2, j, i)));
+    aW.add(k + 1, j, i, lW * rzw * dpos(W(2, j, i)));  // k = 1
     lwk = lwk - lW * rzw;
   } else if (mode == 1) {  // a0 = rzw * (wk + negabs(W(k-1)))
     lwk = lwk + lW * rzw;
