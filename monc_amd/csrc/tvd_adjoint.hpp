@@ -942,10 +942,17 @@ __device__ __forceinline__ void adjoint_tile(const LaunchArgs<T>& L0, const AdjA
   __syncthreads();
 }
 
-// March adjoint: grid (64-level bands, owner-row tiles, x strips).
+// March adjoint: grid (64-level bands, owner-row tiles, x strips). The FP32
+// scalar-pair tile reading recorded arms is held to 128 registers per lane (4
+// waves per SIMD): left free it takes 131 (3 waves) and runs 5.4 against
+// 4.9 ms at 512x512x256. The hint slows the other FP32 tiles, which fit in 128
+// without it, so they go without.
 constexpr int ADJ_ROWS = 4;
+template <typename T, int NS, bool REC>
+constexpr int adj_waves_per_eu() { return sizeof(T) == 4 && NS == 2 && REC ? 4 : 1; }
 template <typename T, int KIND, int NS = 1, bool REC = false>
-__global__ void __launch_bounds__(64 * ADJ_ROWS) adjoint_march_kernel(LaunchArgs<T> L, AdjArgs<T, NS> A, int strip) {
+__global__ void __launch_bounds__(64 * ADJ_ROWS) __attribute__((amdgpu_waves_per_eu(adj_waves_per_eu<T, NS, REC>())))
+adjoint_march_kernel(LaunchArgs<T> L, AdjArgs<T, NS> A, int strip) {
   __shared__ AdjLds<T, ADJ_ROWS, NS + 3> lds;
   adjoint_tile<T, KIND, ADJ_ROWS, NS, REC>(L, A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, strip, lds);
 }
