@@ -125,14 +125,14 @@ int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **
  * MONC_TVD_KERNEL_STAGE: the forward step (monc_tvd_forward_steps,
  * monc_tvd_forward_step_into) and the drop-in call (monc_tvd_tendencies) of
  * each field kind with u, v, w (and the kind's scalars) staged in LDS;
- * rows = cell rows per block (4, 6, 8; 11 in FP64, where the scalar pair
- * takes 4, 6 or 8; 0: the default), strip as above.  A drop-in call whose
+ * rows = cell rows per block (4, 6, 8, 11 or 15; the scalars take 4 where 11
+ * or 15 is set; 0: the default), strip as above.  A drop-in call whose
  * momentum fields advect their z-fields (centred field stepping), or that
  * records z-face diagnostics, marches those kinds; the leapfrog step and the
  * other calls use the march kernel.  Without a tiling set (or after kernel 0
- * with rows = strip = 0) the forward step and the drop-in call stage FP64 u,
- * v and w where the domain gives at least 512 staged blocks, and march the
- * rest. */
+ * with rows = strip = 0) the forward step and the drop-in call stage u, v, w
+ * and the scalar pair in FP64, and u and v in FP32, where the domain gives at
+ * least 512 staged blocks, and march the rest. */
 #define MONC_TVD_KERNEL_MARCH 0
 #define MONC_TVD_KERNEL_CELL 1
 #define MONC_TVD_KERNEL_STAGE 2
@@ -140,7 +140,7 @@ int monc_tvd_set_tiling(monc_tvd_ctx *ctx, int kernel, int rows, int strip);
 /* Tile-shape autotuner (no counterpart in the reference).  For each kernel
  * of the forward step (u, v, w, the scalar group) it times every compiled
  * march shape and strip length (8, 16, 32, 64) and every stage shape (rows
- * 4, 6, 8 x strips 16, 32, 64) on the context's own fields, reps launches
+ * 6, 8, 11, 15 x strips 32, 64, 128) on the context's own fields, reps launches
  * each after one warm-up, writing into library scratch (the state is not
  * advanced), and keeps the fastest kernel and shape per field kind.  chosen
  * (12 ints, may be null) receives kernel, rows and strip per kind in the

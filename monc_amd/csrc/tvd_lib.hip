@@ -290,32 +290,43 @@ void launch_stage(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G,
                      tiles_y);
 }
 
-// Rows per block of the stage kernel (11 and 15 only for the FP64 flow
-// kinds).  15 rows is 16 waves, 4 per SIMD, within 128 registers and 139 KB
-// of LDS, but measured slower than 11 at 512x512x256 (u 1.92 against 1.55 ms
-// per launch); the autotuner tries it.
+// Rows per block of the stage kernel (11 and 15 only for the flow kinds).  In
+// FP64, 15 rows is 16 waves, 4 per SIMD, within 128 registers and 139 KB of
+// LDS; at 512x512x256 it runs u in 1.54 ms per launch in strips of 32 columns
+// (2240 blocks), 1.63 in strips of 64 (1120 blocks, a short last round of
+// one block per CU) and 1.92 in strips of 128, against 1.55 for 11 rows in
+// strips of 128; the autotuner tries them all.
 constexpr int STAGE_ROWS[5] = {4, 6, 8, 11, 15};
 
 bool stage_shape_compiled(int precision, int kind, int rows) {
-  return (rows != 11 && rows != 15) || (precision == MONC_TVD_FP64 && kind != KIND_SCALAR);
+  (void)precision;
+  return (rows != 11 && rows != 15) || kind != KIND_SCALAR;
 }
 
 // Default kernel per field kind for the forward step, from timings at
-// 512x512x256 on MI355X (the autotuner's pick there): in FP64, u, v and w
-// staged in 11-row tiles of 128 columns (1.53-1.57 ms per launch against
-// 2.0-2.1 ms marching) and the (th, q) pair in 8-row tiles of 128 columns
-// (2.54 against 2.71 ms for the march kernel's 2 x 64 shape, since max and
-// min became single instructions); every FP32 kind marches (its staged forms
-// measured slower).  Small domains keep the march kernel: the staged form runs
-// one block per CU, so it needs at least two blocks per CU (512) to fill the
-// chip; the strip is the longest of 128, 64 and 32 columns that still gives
-// them.  Returns the strip, or 0 for the march kernel.
-int stage_default_rows(int kind) { return kind == KIND_SCALAR ? 8 : 11; }
+// 512x512x256 on MI355X (the autotuner's pick there):
+//   * FP64: u, v and w staged in 11-row tiles of 128 columns (1.55-1.58 ms
+//     per launch against 2.0-2.1 ms marching) and the (th, q) pair in 8-row
+//     tiles of 128 columns (2.53 against 2.71 ms for the march kernel's 2 x 64
+//     shape);
+//   * FP32: u and v staged in 11-row tiles of 32 columns (0.75-0.77 ms per
+//     launch against 0.94 marching; 80 registers and 55 KB of LDS, two blocks
+//     per CU); w and the (th, q) pair march (their staged forms measured
+//     slower: w 0.91 against 0.88-0.91 ms, the pair 1.44 against 1.42).
+// Small domains keep the march kernel: the staged form runs one or two blocks
+// per CU, so it needs at least 512 blocks to fill the chip; in FP64 the strip
+// is the longest of 128, 64 and 32 columns that still gives them.  Returns the
+// strip, or 0 for the march kernel.
+int stage_default_rows(int precision, int kind) {
+  return precision == MONC_TVD_FP64 && kind == KIND_SCALAR ? 8 : 11;
+}
 
 int stage_default_strip(const monc_tvd_ctx* c, int kind) {
-  if (c->precision != MONC_TVD_FP64) return 0;
-  const int rows = stage_default_rows(kind);
+  const bool fp64 = c->precision == MONC_TVD_FP64;
+  if (!fp64 && kind != KIND_U && kind != KIND_V) return 0;
+  const int rows = stage_default_rows(c->precision, kind);
   const long tiles = (long)((c->nz + 63) / 64) * ((c->ny + rows - 1) / rows);
+  if (!fp64) return tiles * ((c->nx + 31) / 32) >= 512 ? 32 : 0;
   for (int strip : {128, 64, 32})
     if (tiles * ((c->nx + strip - 1) / strip) >= 512) return strip;
   return 0;
@@ -349,13 +360,13 @@ void launch_stage_rows(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T
     case 6: launch_stage<T, KIND, NS, 6, MODE>(c, L, G, strip); break;
     case 8: launch_stage<T, KIND, NS, 8, MODE>(c, L, G, strip); break;
     case 11:
-      if constexpr (sizeof(T) == 8 && KIND != KIND_SCALAR) {
+      if constexpr (KIND != KIND_SCALAR) {
         launch_stage<T, KIND, NS, 11, MODE>(c, L, G, strip);
         break;
       }
       [[fallthrough]];
     case 15:
-      if constexpr (sizeof(T) == 8 && KIND != KIND_SCALAR) {
+      if constexpr (KIND != KIND_SCALAR) {
         launch_stage<T, KIND, NS, 15, MODE>(c, L, G, strip);
         break;
       }
@@ -436,7 +447,7 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
       const bool dflt = c->kind_kernel[kk] == MONC_TVD_KERNEL_MARCH && !c->rows[kk] && !c->strip[kk] &&
                         c->kernel == MONC_TVD_KERNEL_MARCH && dstrip > 0;
       if ((c->kind_kernel[kk] == MONC_TVD_KERNEL_STAGE || dflt) && stage_fits<T, MODE>(c, L, G)) {
-        const int srows = dflt ? stage_default_rows(S.kind[n]) : (c->rows[kk] > 0 ? c->rows[kk] : 6);
+        const int srows = dflt ? stage_default_rows(c->precision, S.kind[n]) : (c->rows[kk] > 0 ? c->rows[kk] : 6);
         const int sstrip = dflt ? dstrip : (c->strip[kk] > 0 ? c->strip[kk] : 64);
         launch_stage_group<T, MODE>(c, L, G, kk, srows, sstrip);
         HIP_TRY(hipGetLastError());
@@ -1498,8 +1509,6 @@ int monc_tvd_set_tiling(monc_tvd_ctx* c, int kernel, int rows, int strip) {
     return fail(MONC_TVD_ERR_ARG, "kernel %d is not 0 (march), 1 (cell) or 2 (stage)", kernel);
   if (kernel == MONC_TVD_KERNEL_STAGE && rows != 0 && !stage_rows_ok(rows))
     return fail(MONC_TVD_ERR_ARG, "no stage shape with %d rows (4, 6, 8, 11 or 15)", rows);
-  if (kernel == MONC_TVD_KERNEL_STAGE && (rows == 11 || rows == 15) && c->precision != MONC_TVD_FP64)
-    return fail(MONC_TVD_ERR_ARG, "the %d-row stage shape exists in FP64 only", rows);
   if (strip < 0 || strip > 4096) return fail(MONC_TVD_ERR_ARG, "strip %d out of range [0,4096]", strip);
   const int nch = (c->nz + 63) / 64;
   if (kernel == MONC_TVD_KERNEL_MARCH && rows != 0 && !march_rows_ok(nch, rows))
@@ -1526,7 +1535,7 @@ int monc_tvd_get_tiling(monc_tvd_ctx* c, int* chosen) {
     const int dstrip = stage_default_strip(c, kind);
     if (kernel == MONC_TVD_KERNEL_MARCH && !rows && !strip && c->buffer_ok && dstrip > 0) {
       kernel = MONC_TVD_KERNEL_STAGE;
-      rows = stage_default_rows(kind);
+      rows = stage_default_rows(c->precision, kind);
       strip = dstrip;
     } else if (kernel == MONC_TVD_KERNEL_STAGE) {
       rows = rows > 0 ? rows : 6;

@@ -129,16 +129,47 @@ def test_fp32_error_on_monc_form_theta_under_nondivergent_flow(gpu):
         assert report[f"monc_form_after_{n}"] <= 1e-5, report
 
 
-def test_fp32_tendencies_vs_oracle(gpu):
+# the default shapes (march at this size), and u, v, w staged in 11- and
+# 15-row tiles (the FP32 default for u and v on large domains)
+@pytest.mark.parametrize("tiling", [None, (tvd.KERNEL_STAGE, 11, 32), (tvd.KERNEL_STAGE, 15, 16)],
+                         ids=["default", "stage11", "stage15"])
+def test_fp32_tendencies_vs_oracle(gpu, tiling):
     st = ss.make_state(64, 64, 64, 50.0, 2000.0, seed=0, n_q=1)
     ref, _ = oracle.state_tendencies(st, 1.0, 1, 1, 1)
-    got = gpu_tendencies(st, 1.0, 1, 1, 1, precision=tvd.FP32)
+    got = gpu_tendencies(st, 1.0, 1, 1, 1, precision=tvd.FP32, tiling=tiling)
     errs = {}
     for f in range(5):
         a, b = interior(got[f]), interior(ref[f])
         errs[NAMES[f]] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
     print("FP32_TENDENCY_REPORT " + json.dumps(errs))
     assert all(e <= 1e-4 for e in errs.values()), errs
+
+
+def test_fp32_default_stages_u_and_v_at_scale(gpu):
+    """512x512x256 FP32: the default step stages u and v (11-row tiles of 32
+    columns) and marches w and the scalars; two steps agree with the march
+    kernel forced for every kind to FP32 rounding."""
+    st = ss.make_state(256, 512, 512, 25.0, 5000.0, seed=0, n_q=1, with_zfields=False)
+    dt = ss.cfl_dt(st, 0.3)
+    out = []
+    for tiling in (None, (tvd.KERNEL_MARCH, 3, 32)):
+        t = tvd.TvdAdvection.initialise(st, precision=tvd.FP32)
+        try:
+            if tiling is None:
+                shapes = t.tiling()
+                assert shapes["u"] == (tvd.KERNEL_STAGE, 11, 32) and shapes["v"] == (tvd.KERNEL_STAGE, 11, 32), shapes
+                assert shapes["w"][0] == tvd.KERNEL_MARCH and shapes["scalars"][0] == tvd.KERNEL_MARCH, shapes
+            else:
+                t.set_tiling(*tiling)
+            t.halo_fill()
+            t.forward_steps(dt, 2)
+            out.append([t.get_field(f, tvd.PROG) for f in range(5)])
+        finally:
+            t.close()
+    for n in range(5):
+        a, b = interior(out[0][n]), interior(out[1][n])
+        e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        assert e <= 1e-6, (NAMES[n], e)
 
 
 def test_weighted_sum_matches_host(gpu):
