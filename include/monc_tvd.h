@@ -131,7 +131,7 @@ int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **
  * records z-face diagnostics, marches those kinds; the leapfrog step and the
  * other calls use the march kernel.  Without a tiling set (or after kernel 0
  * with rows = strip = 0) the forward step and the drop-in call stage u, v, w
- * and the scalar pair in FP64, and u and v in FP32, where the domain gives at
+ * and the scalar pair in FP64, and u, v and w in FP32, where the domain gives at
  * least 512 staged blocks, and march the rest. */
 #define MONC_TVD_KERNEL_MARCH 0
 #define MONC_TVD_KERNEL_CELL 1

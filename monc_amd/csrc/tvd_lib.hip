@@ -274,12 +274,14 @@ void launch_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
 // Stage kernel (MONC_TVD_KERNEL_STAGE): the step of one field kind with u, v,
 // w (and the kind's scalars) staged in LDS, over 64-level bands x R rows x x
 // strips.  The register budget (waves per SIMD the compiler plans for) is set
-// by the kind's demand, not by the LDS: FP32 fits 128 registers (4 waves),
-// FP64 168 (3 waves) and the FP64 scalar pair 256 (2 waves); a smaller budget
-// spills.
+// by the kind's demand, not by the LDS: the FP32 flow kinds fit 80 registers
+// (6 waves, two 11-row blocks per CU; w takes 90 unbounded and one block),
+// the FP32 scalars 128 (4 waves), FP64 168 (3 waves) and the FP64 scalar pair
+// 256 (2 waves); a smaller budget spills.
 template <typename T, int KIND, int NS, int R>
 constexpr int stage_wpe() {
-  return sizeof(T) == 4 ? 4 : (KIND == KIND_SCALAR && NS == 2 ? 2 : (R == 15 ? 4 : 3));
+  if (sizeof(T) == 4) return KIND == KIND_SCALAR ? 4 : 6;
+  return KIND == KIND_SCALAR && NS == 2 ? 2 : (R == 15 ? 4 : 3);
 }
 
 template <typename T, int KIND, int NS, int R, int MODE>
@@ -309,10 +311,10 @@ bool stage_shape_compiled(int precision, int kind, int rows) {
 //     per launch against 2.0-2.1 ms marching) and the (th, q) pair in 8-row
 //     tiles of 128 columns (2.53 against 2.71 ms for the march kernel's 2 x 64
 //     shape);
-//   * FP32: u and v staged in 11-row tiles of 32 columns (0.75-0.77 ms per
-//     launch against 0.94 marching; 80 registers and 55 KB of LDS, two blocks
-//     per CU); w and the (th, q) pair march (their staged forms measured
-//     slower: w 0.91 against 0.88-0.91 ms, the pair 1.44 against 1.42).
+//   * FP32: u, v and w staged in 11-row tiles of 32 columns (0.75-0.77 ms
+//     per launch against 0.88-0.94 marching; at most 80 registers and 55 KB
+//     of LDS, two blocks per CU); the (th, q) pair marches (staged, 1.44
+//     against 1.42 ms).
 // Small domains keep the march kernel: the staged form runs one or two blocks
 // per CU, so it needs at least 512 blocks to fill the chip; in FP64 the strip
 // is the longest of 128, 64 and 32 columns that still gives them.  Returns the
@@ -323,7 +325,7 @@ int stage_default_rows(int precision, int kind) {
 
 int stage_default_strip(const monc_tvd_ctx* c, int kind) {
   const bool fp64 = c->precision == MONC_TVD_FP64;
-  if (!fp64 && kind != KIND_U && kind != KIND_V) return 0;
+  if (!fp64 && kind == KIND_SCALAR) return 0;
   const int rows = stage_default_rows(c->precision, kind);
   const long tiles = (long)((c->nz + 63) / 64) * ((c->ny + rows - 1) / rows);
   if (!fp64) return tiles * ((c->nx + 31) / 32) >= 512 ? 32 : 0;

@@ -130,7 +130,7 @@ def test_fp32_error_on_monc_form_theta_under_nondivergent_flow(gpu):
 
 
 # the default shapes (march at this size), and u, v, w staged in 11- and
-# 15-row tiles (the FP32 default for u and v on large domains)
+# 15-row tiles (the FP32 default for u, v and w on large domains)
 @pytest.mark.parametrize("tiling", [None, (tvd.KERNEL_STAGE, 11, 32), (tvd.KERNEL_STAGE, 15, 16)],
                          ids=["default", "stage11", "stage15"])
 def test_fp32_tendencies_vs_oracle(gpu, tiling):
@@ -145,10 +145,10 @@ def test_fp32_tendencies_vs_oracle(gpu, tiling):
     assert all(e <= 1e-4 for e in errs.values()), errs
 
 
-def test_fp32_default_stages_u_and_v_at_scale(gpu):
-    """512x512x256 FP32: the default step stages u and v (11-row tiles of 32
-    columns) and marches w and the scalars; two steps agree with the march
-    kernel forced for every kind to FP32 rounding."""
+def test_fp32_default_stages_the_flows_at_scale(gpu):
+    """512x512x256 FP32: the default step stages u, v and w (11-row tiles of 32
+    columns) and marches the scalars; two steps agree with the march kernel
+    forced for every kind to FP32 rounding."""
     st = ss.make_state(256, 512, 512, 25.0, 5000.0, seed=0, n_q=1, with_zfields=False)
     dt = ss.cfl_dt(st, 0.3)
     out = []
@@ -157,8 +157,8 @@ def test_fp32_default_stages_u_and_v_at_scale(gpu):
         try:
             if tiling is None:
                 shapes = t.tiling()
-                assert shapes["u"] == (tvd.KERNEL_STAGE, 11, 32) and shapes["v"] == (tvd.KERNEL_STAGE, 11, 32), shapes
-                assert shapes["w"][0] == tvd.KERNEL_MARCH and shapes["scalars"][0] == tvd.KERNEL_MARCH, shapes
+                assert all(shapes[k] == (tvd.KERNEL_STAGE, 11, 32) for k in ("u", "v", "w")), shapes
+                assert shapes["scalars"][0] == tvd.KERNEL_MARCH, shapes
             else:
                 t.set_tiling(*tiling)
             t.halo_fill()
