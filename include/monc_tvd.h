@@ -276,8 +276,10 @@ int monc_tvd_forward_step_into(monc_tvd_ctx *ctx, double dtm, void *const *in, v
  * one byte per point of field f's layout (nz*(ny+4)*(nx+4)), receiving at
  * face owner (k, j, i) the arms of its right x, north y and top z faces in
  * bits 0-1, 2-3 and 4-5 (0: the face is fc, 1: fd, 2: the QUICKEST value,
- * 3: fu + (fc - fu)/S; ultimateflux.F90:556-614).  Runs the march kernel in
- * its default shape; columns of at most 256 levels.  Asynchronous.
+ * 3: fu + (fc - fu)/S; ultimateflux.F90:556-614).  Runs the stage kernel
+ * for the kinds monc_tvd_forward_step_into stages (same shapes, so the new
+ * state is the same bit for bit), the march kernel in its default shape for
+ * the others; columns of at most 256 levels.  Asynchronous.
  */
 int monc_tvd_forward_step_record(monc_tvd_ctx *ctx, double dtm, void *const *in, void *const *out,
                                  unsigned char *const *rec);

@@ -345,7 +345,7 @@ bool stage_fits(const monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>
   if (!c->buffer_ok || G.nf < 1 || G.nf > 2) return false;
   for (int n = 0; n < G.nf; n++) {
     if (G.dt[n] != G.dt[0] || G.kind[n] != G.kind[0]) return false;
-    if (MODE == MODE_STEP && G.f[n] != G.F[n]) return false;
+    if ((MODE == MODE_STEP || MODE == MODE_REC) && G.f[n] != G.F[n]) return false;
     if (MODE == MODE_ACCUM && G.zf[n]) return false;
   }
   switch (G.kind[0]) {
@@ -428,7 +428,9 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
       G.rec[m] = S.rec[n + m];
     }
     const int kk = pair || S.kind[n] == KIND_SCALAR ? 3 : S.kind[n];
-    // the leapfrog step and the recording step run the march kernel in its
+    // the leapfrog step runs the march kernel in its default shape; the
+    // recording step runs the stage kernel where the forward step does (so
+    // that the two write the same values), else the march kernel in its
     // default shape
     const bool leap = MODE == MODE_LEAP || MODE == MODE_REC;
     const int kernel = leap ? MONC_TVD_KERNEL_MARCH : c->kernel;
@@ -444,7 +446,7 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
     const bool wide = c->precision == MONC_TVD_FP64 && kk == 3 && nch == 4 && rows <= 2;
     const int strip = own && c->strip[kk] > 0 ? c->strip[kk] : default_strip(c->ny, c->nx, rows, wide);
     LaunchTimer timer(c, kk);
-    if constexpr (MODE == MODE_STEP || MODE == MODE_ACCUM) {
+    if constexpr (MODE == MODE_STEP || MODE == MODE_ACCUM || MODE == MODE_REC) {
       const int dstrip = stage_default_strip(c, S.kind[n]);
       const bool dflt = c->kind_kernel[kk] == MONC_TVD_KERNEL_MARCH && !c->rows[kk] && !c->strip[kk] &&
                         c->kernel == MONC_TVD_KERNEL_MARCH && dstrip > 0;

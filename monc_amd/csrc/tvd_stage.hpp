@@ -107,7 +107,7 @@ __device__ __forceinline__ SMet<T> smet(const LDS& lds, int li) {
 // NS fields a0 .. a0+NS-1 of kind KIND; kq = (min(k+1, nz)) - 1.
 template <typename T, int KIND, int NS, int R, int NA>
 __device__ __forceinline__ void sx_face(const SView<T, R, NA>& s, int a0, int k, int kq, int j, int i, T rdz_k,
-                                        T rdz_kd, T dt, T cx, T cy, T out[NS]) {
+                                        T rdz_kd, T dt, T cx, T cy, T out[NS], int cls[NS]) {
   const T u_m = sflow<T, KIND>(s, SA_U, k, j, i - 1), u_0 = sflow<T, KIND>(s, SA_U, k, j, i);
   const T u_p = sflow<T, KIND>(s, SA_U, k, j, i + 1);
   const T v_0 = sflow<T, KIND>(s, SA_V, k, j, i), v_s = sflow<T, KIND>(s, SA_V, k, j - 1, i);
@@ -140,14 +140,14 @@ __device__ __forceinline__ void sx_face(const SView<T, R, NA>& s, int a0, int k,
     const T g1 = cz * (ap ? d0 : d1);
     const T t = ap ? s(a, k, js, i) : s(a, k, js, i + 1);
     const T g2 = cyt * (fc - t);
-    out[n] = lim.face(fu, fc, fd, g1, g2);
+    out[n] = lim.face_cls(fu, fc, fd, g1, g2, cls[n]);
   }
 }
 
 // North y face of row j at column i, level k (ultimateflux.F90:189-233).
 template <typename T, int KIND, int NS, int R, int NA>
 __device__ __forceinline__ void sy_face(const SView<T, R, NA>& s, int a0, int k, int kq, int j, int i, T rdz_k,
-                                        T rdz_kd, T dt, T cx, T cy, T out[NS]) {
+                                        T rdz_kd, T dt, T cx, T cy, T out[NS], int cls[NS]) {
   const T vk = sflow<T, KIND>(s, SA_V, k, j, i), vn = sflow<T, KIND>(s, SA_V, k, j + 1, i);
   const T vs = sflow<T, KIND>(s, SA_V, k, j - 1, i);
   const T w0 = sflow<T, KIND>(s, SA_W, k, j, i), w0m = sflow<T, KIND>(s, SA_W, k - 1, j, i);
@@ -178,7 +178,7 @@ __device__ __forceinline__ void sy_face(const SView<T, R, NA>& s, int a0, int k,
     const T fd = vpos ? fb : fa;
     const T g1 = cz * (s(a, ksl, jj, i) - s(a, ksu, jj, i));
     const T g2 = cxt * (fc - s(a, k, jj, it));
-    out[n] = lim.face(fu, fc, fd, g1, g2);
+    out[n] = lim.face_cls(fu, fc, fd, g1, g2, cls[n]);
   }
 }
 
@@ -187,7 +187,7 @@ __device__ __forceinline__ void sy_face(const SView<T, R, NA>& s, int a0, int k,
 // k = nz-1, `bot` k = 1.
 template <typename T, int KIND, int NS, int R, int NA>
 __device__ __forceinline__ void sz_face(const SView<T, R, NA>& s, int a0, int k, int nz, int j, int i,
-                                        const SMet<T>& m, T dt, T cx, T cy, T out[NS]) {
+                                        const SMet<T>& m, T dt, T cx, T cy, T out[NS], int cls[NS]) {
   const bool edge = k == 1 || k == nz - 1, top = k == nz - 1, bot = k == 1;
   const T wk = sflow<T, KIND>(s, SA_W, k, j, i);
   const T wm_0 = sflow<T, KIND>(s, SA_W, k - 1, j, i), wk_0 = sflow<T, KIND>(s, SA_W, k + 1, j, i);
@@ -237,6 +237,7 @@ __device__ __forceinline__ void sz_face(const SView<T, R, NA>& s, int a0, int k,
     const T ftemp = tmax(fncs, tmin(tmin(fnfs, fnrefs), T(1.0)));
     const T lim = ftemp * fdels + fu;
     out[n] = fabs(fcurvs) >= fabs(fdels) ? fc : lim;
+    cls[n] = lim_class(fcurvs, fdels, fnfs, fncs, fnrefs);
   }
 }
 
@@ -245,25 +246,30 @@ constexpr int skind_first() { return KIND == KIND_SCALAR ? SA_S : KIND; }
 
 // Faces of one field kind for a row wave at (k, j), column i: right x, north
 // y and top z face of each field of the kind; the z face only where the kind
-// has one (scalars, u, v: 2 <= k <= nz-1; w: 1 <= k <= nz-1).
+// has one (scalars, u, v: 2 <= k <= nz-1; w: 1 <= k <= nz-1).  code[n]: the
+// limiter arms the three faces took, x | y << 2 | z << 4 (the recording
+// step's byte; unused otherwise).
 template <typename T, int KIND, int NSF, int R, int NA>
 __device__ __forceinline__ void skind_faces(const SView<T, R, NA>& s, const StageLds<T, R, NA, NSF>& lds, int k, int kq,
                                             int nz, int j, int i, int li, T dt, T cx, T cy, T fxr[NSF], T fyn[NSF],
-                                            T fzt[NSF]) {
+                                            T fzt[NSF], int code[NSF]) {
   constexpr int A0 = skind_first<KIND>();
   const SMet<T> m = smet<T>(lds, li);
-  sx_face<T, KIND, NSF>(s, A0, k, kq, j, i, m.rdz_k, m.rdz_kd, dt, cx, cy, fxr);
-  sy_face<T, KIND, NSF>(s, A0, k, kq, j, i, m.rdz_k, m.rdz_kd, dt, cx, cy, fyn);
+  int ax[NSF], ay[NSF], az[NSF];
+  sx_face<T, KIND, NSF>(s, A0, k, kq, j, i, m.rdz_k, m.rdz_kd, dt, cx, cy, fxr, ax);
+  sy_face<T, KIND, NSF>(s, A0, k, kq, j, i, m.rdz_k, m.rdz_kd, dt, cx, cy, fyn, ay);
   T zf[NSF];
-  sz_face<T, KIND, NSF>(s, A0, k, nz, j, i, m, dt, cx, cy, zf);
+  sz_face<T, KIND, NSF>(s, A0, k, nz, j, i, m, dt, cx, cy, zf, az);
+#pragma unroll
+  for (int n = 0; n < NSF; n++) code[n] = ax[n] | (ay[n] << 2) | (az[n] << 4);
   const bool has_z = KIND == KIND_W ? (k >= 1 && k <= nz - 1) : (k >= 2 && k <= nz - 1);
 #pragma unroll
   for (int n = 0; n < NSF; n++) fzt[n] = has_z ? zf[n] : T(0);
 }
 
 // Divergence of the fields of one kind at (k, j, i) (tvdadvection.F90:
-// 925-959).  MODE_STEP, the forward step (stepfields.F90:597-627): writes
-// f + dtm*s at the cell and its periodic halo images; levels the reference
+// 925-959).  MODE_STEP and MODE_REC, the forward step (stepfields.F90:
+// 597-627): writes f + dtm*s at the cell and its periodic halo images; levels the reference
 // leaves untouched (k = 1; k = nz for w) keep f.  MODE_ACCUM, the drop-in
 // call: adds s to the source field `out` at levels 2..nz (w: 2..nz-1),
 // overwriting it at k = nz for u, v and the scalars (:942-959).
@@ -403,9 +409,13 @@ __device__ __forceinline__ void stage_store(const StageFetch<T, R, NA>& f, LDS& 
 // The edge wave's faces of one field kind: the south y face of the block's
 // first row (the north face of row j0-1) for the 64 levels, and the z face
 // below the band (top face of level k0-1) of each row (lane r = row j0+r).
-template <typename T, int KIND, int NSF, int R, int NA>
-__device__ __forceinline__ void skind_edge(const SView<T, R, NA>& s, StageLds<T, R, NA, NSF>& lds, int lane, int k0,
-                                           int nz, int ny, int i, T dt, T cx, T cy) {
+// The recording step (MODE_REC) writes the arm of the south face of row 3 as
+// the y arm of owner (k, 2, i), the one face row 2 owns; the faces below the
+// band belong to the band below, which records them.
+template <typename T, int KIND, int NSF, int R, int NA, int MODE>
+__device__ __forceinline__ void skind_edge(const SView<T, R, NA>& s, StageLds<T, R, NA, NSF>& lds,
+                                           const FieldSet<T>& G, const Dims& d, int lane, int k0, int nz, int ny,
+                                           int i, T dt, T cx, T cy) {
   constexpr int A0 = skind_first<KIND>();
   const int k = k0 + lane, kq = k < nz ? k : nz - 1;
   const int j = s.j0 - 1;
@@ -413,15 +423,23 @@ __device__ __forceinline__ void skind_edge(const SView<T, R, NA>& s, StageLds<T,
   {
     const T rdz_k = lds.met[M_RDZ][lane + 1], rdz_kd = lds.met[M_RDZKD][lane + 1];
     T f[NSF];
-    sy_face<T, KIND, NSF>(se, A0, k, kq, j, i, rdz_k, rdz_kd, dt, cx, cy, f);
+    int ay[NSF];
+    sy_face<T, KIND, NSF>(se, A0, k, kq, j, i, rdz_k, rdz_kd, dt, cx, cy, f, ay);
 #pragma unroll
     for (int n = 0; n < NSF; n++) lds.yn[n][0][lane] = f[n];
+    if constexpr (MODE == MODE_REC) {
+      if (j == 2 && k <= nz) {
+#pragma unroll
+        for (int n = 0; n < NSF; n++) G.rec[n][at(d, k, 2, i)] = (unsigned char)(ay[n] << 2);
+      }
+    }
   }
   if (k0 > 1 && lane < R && s.j0 + lane <= ny + 2) {
     const SMet<T> m = smet<T>(lds, 0);
     const SView<T, R, NA> sz(&lds.ring[0][0][0][0], k0, s.j0, k0 - 1, s.j0 + lane);
     T f[NSF];
-    sz_face<T, KIND, NSF>(sz, A0, k0 - 1, nz, s.j0 + lane, i, m, dt, cx, cy, f);
+    int az[NSF];
+    sz_face<T, KIND, NSF>(sz, A0, k0 - 1, nz, s.j0 + lane, i, m, dt, cx, cy, f, az);
 #pragma unroll
     for (int n = 0; n < NSF; n++) lds.zb[n][lane] = f[n];
   }
@@ -431,9 +449,10 @@ __device__ __forceinline__ void skind_edge(const SView<T, R, NA>& s, StageLds<T,
 // by the same flows): a 1-D grid of (row tile, band, strip) tiles, row tiles
 // fastest so that consecutive tiles of one XCD share halo rows in its L2
 // (blocks are dealt to the 8 XCDs round-robin).  MODE_STEP with every advected
-// field its own prognostic, or MODE_ACCUM (the drop-in tendency call) with
-// the flow kinds advecting their own prognostic; G holds the kind's fields
-// (F, out) in order.
+// field its own prognostic, MODE_REC (the same step, also writing each
+// owner's limiter arms to G.rec), or MODE_ACCUM (the drop-in tendency call)
+// with the flow kinds advecting their own prognostic; G holds the kind's
+// fields (F, out) in order.
 template <typename T, int KIND, int NS, int R, int WPE, int MODE>
 __global__ void __launch_bounds__(64 * (R + 1)) __attribute__((amdgpu_waves_per_eu(WPE)))
 stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y) {
@@ -488,9 +507,17 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
   T fxl[NSF], fxr[NSF], fyn[NSF], fzt[NSF], ul = T(0);
 #pragma unroll
   for (int n = 0; n < NSF; n++) fxl[n] = fxr[n] = fyn[n] = fzt[n] = T(0);
-  if (active)  // the left x faces of column i_begin
+  if (active) {  // the left x faces of column i_begin
+    int ax[NSF];
     sx_face<T, KIND, NSF>(sv, A0, k, kq, j, i_begin - 1, lds.met[M_RDZ][li], lds.met[M_RDZKD][li], dt, cx, cy,
-                          fxl);
+                          fxl, ax);
+    if constexpr (MODE == MODE_REC) {  // the right x face of column 2 (owner (k, j, 2))
+      if (i_begin == 3 && k <= nz) {
+#pragma unroll
+        for (int n = 0; n < NSF; n++) G.rec[n][at(L.d, k, j, 2)] = (unsigned char)ax[n];
+      }
+    }
+  }
   __syncthreads();  // plane i_begin-2 is no longer read
   stage_fetch<T, R, NA>(fetch, src, g, wv, lane, k0, j0, nz, ny, i_begin + 2);
   stage_store<T, R, NA>(fetch, lds, wv, lane, i_begin + 2);
@@ -500,13 +527,20 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
     if (more) stage_fetch<T, R, NA>(fetch, src, g, wv, lane, k0, j0, nz, ny, i + 3);
     if (row_wave) {
       if (active) {
-        skind_faces<T, KIND, NSF>(sv, lds, k, kq, nz, j, i, li, dt, cx, cy, fxr, fyn, fzt);
+        int code[NSF];
+        skind_faces<T, KIND, NSF>(sv, lds, k, kq, nz, j, i, li, dt, cx, cy, fxr, fyn, fzt, code);
         ul = sflow<T, KIND>(sv, SA_U, k, j, i - 1);  // its plane is overwritten before the step
+        if constexpr (MODE == MODE_REC) {  // the arms of the faces cell (k, j, i) owns
+          if (k <= nz) {
+#pragma unroll
+            for (int n = 0; n < NSF; n++) G.rec[n][at(L.d, k, j, i)] = (unsigned char)code[n];
+          }
+        }
       }
 #pragma unroll
       for (int n = 0; n < NSF; n++) lds.yn[n][r + 1][lane] = fyn[n];
     } else {
-      skind_edge<T, KIND, NSF>(sv, lds, lane, k0, nz, ny, i, dt, cx, cy);
+      skind_edge<T, KIND, NSF, R, NA, MODE>(sv, lds, G, L.d, lane, k0, nz, ny, i, dt, cx, cy);
     }
     __syncthreads();
     if (active)
