@@ -125,8 +125,8 @@ int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **
  * MONC_TVD_KERNEL_STAGE: the forward step (monc_tvd_forward_steps,
  * monc_tvd_forward_step_into) and the drop-in call (monc_tvd_tendencies) of
  * each field kind with u, v, w (and the kind's scalars) staged in LDS;
- * rows = cell rows per block (4, 6, 8, 11 or 15; the scalars take 4 where 11
- * or 15 is set; 0: the default), strip as above.  A drop-in call whose
+ * rows = cell rows per block (4, 6, 8, 11 or 15; the scalars take 4 where 15
+ * is set; 0: the default), strip as above.  A drop-in call whose
  * momentum fields advect their z-fields (centred field stepping), or that
  * records z-face diagnostics, marches those kinds; the leapfrog step and the
  * other calls use the march kernel.  Without a tiling set (or after kernel 0

@@ -276,12 +276,13 @@ void launch_shape(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S,
 // strips.  The register budget (waves per SIMD the compiler plans for) is set
 // by the kind's demand, not by the LDS: the FP32 flow kinds fit 80 registers
 // (6 waves, two 11-row blocks per CU; w takes 90 unbounded and one block),
-// the FP32 scalars 128 (4 waves), FP64 168 (3 waves) and the FP64 scalar pair
-// 256 (2 waves); a smaller budget spills.
+// the FP32 scalars 128 (4 waves), FP64 168 (3 waves: the 12 waves of an
+// 11-row tile) and the FP64 scalar pair in tiles of up to 8 rows 256 (2
+// waves); a smaller budget spills.
 template <typename T, int KIND, int NS, int R>
 constexpr int stage_wpe() {
   if (sizeof(T) == 4) return KIND == KIND_SCALAR ? 4 : 6;
-  return KIND == KIND_SCALAR && NS == 2 ? 2 : (R == 15 ? 4 : 3);
+  return KIND == KIND_SCALAR && NS == 2 && R < 11 ? 2 : (R == 15 ? 4 : 3);
 }
 
 template <typename T, int KIND, int NS, int R, int MODE>
@@ -292,7 +293,8 @@ void launch_stage(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G,
                      tiles_y);
 }
 
-// Rows per block of the stage kernel (11 and 15 only for the flow kinds).  In
+// Rows per block of the stage kernel (15 only for the flow kinds: the FP64
+// scalar pair's ring of 16 planes of 19 rows would not fit the LDS).  In
 // FP64, 15 rows is 16 waves, 4 per SIMD, within 128 registers and 139 KB of
 // LDS; at 512x512x256 it runs u in 1.54 ms per launch in strips of 32 columns
 // (2240 blocks), 1.63 in strips of 64 (1120 blocks, a short last round of
@@ -302,15 +304,16 @@ constexpr int STAGE_ROWS[5] = {4, 6, 8, 11, 15};
 
 bool stage_shape_compiled(int precision, int kind, int rows) {
   (void)precision;
-  return (rows != 11 && rows != 15) || kind != KIND_SCALAR;
+  return rows != 15 || kind != KIND_SCALAR;
 }
 
 // Default kernel per field kind for the forward step, from timings at
 // 512x512x256 on MI355X (the autotuner's pick there):
-//   * FP64: u, v and w staged in 11-row tiles of 128 columns (1.55-1.58 ms
-//     per launch against 2.0-2.1 ms marching) and the (th, q) pair in 8-row
-//     tiles of 128 columns (2.53 against 2.71 ms for the march kernel's 2 x 64
-//     shape);
+//   * FP64: u, v and w staged in 11-row tiles of 128 columns (1.51-1.58 ms
+//     per launch against 2.0-2.1 ms marching) and the (th, q) pair in 11-row
+//     tiles of 128 columns (2.24 ms: 149 KB of LDS with v and w two planes
+//     deep, 12 waves at 160 registers; 2.53-2.56 in 8-row tiles of 9 waves,
+//     2.71 for the march kernel's 2 x 64 shape);
 //   * FP32: u, v and w staged in 11-row tiles of 32 columns (0.75-0.77 ms
 //     per launch against 0.88-0.94 marching; at most 80 registers and 55 KB
 //     of LDS, two blocks per CU); the (th, q) pair marches (staged, 1.44
@@ -320,7 +323,9 @@ bool stage_shape_compiled(int precision, int kind, int rows) {
 // is the longest of 128, 64 and 32 columns that still gives them.  Returns the
 // strip, or 0 for the march kernel.
 int stage_default_rows(int precision, int kind) {
-  return precision == MONC_TVD_FP64 && kind == KIND_SCALAR ? 8 : 11;
+  (void)precision;
+  (void)kind;
+  return 11;
 }
 
 int stage_default_strip(const monc_tvd_ctx* c, int kind) {
@@ -361,12 +366,7 @@ void launch_stage_rows(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T
   switch (rows) {
     case 6: launch_stage<T, KIND, NS, 6, MODE>(c, L, G, strip); break;
     case 8: launch_stage<T, KIND, NS, 8, MODE>(c, L, G, strip); break;
-    case 11:
-      if constexpr (KIND != KIND_SCALAR) {
-        launch_stage<T, KIND, NS, 11, MODE>(c, L, G, strip);
-        break;
-      }
-      [[fallthrough]];
+    case 11: launch_stage<T, KIND, NS, 11, MODE>(c, L, G, strip); break;
     case 15:
       if constexpr (KIND != KIND_SCALAR) {
         launch_stage<T, KIND, NS, 15, MODE>(c, L, G, strip);

@@ -9,7 +9,9 @@
 // is an LDS read.  A ring of four planes holds columns i-1 .. i+2 while
 // column i is computed; the plane of column i+3 is fetched into registers
 // during the faces of column i and written into the ring slot of column i-1
-// once the block has finished reading it.  The raw fields thus cross HBM
+// once the block has finished reading it.  (v and w of the scalar kinds are
+// read at columns i and i+1 only: two planes, column i+2 written into the slot
+// of column i, whose values the divergence takes before the barrier.)  The raw fields thus cross HBM
 // once per step (plus the 4/R row halo and 4/64 level halo) instead of once
 // per field kind.
 //
@@ -28,12 +30,36 @@
 namespace tvd {
 
 constexpr int ST_KB = 68;    // staged levels: k0-2 .. k0+65
-constexpr int ST_SLOTS = 4;  // x planes in the ring (column i in slot i & 3)
 constexpr int ST_NMET = 65;  // metric levels: k0-1 .. k0+63
 
 // Staged arrays: the flows, then the scalars (MODE_STEP advects each flow
 // field by itself, so u, v and w are also the advected fields of their kinds).
 enum { SA_U = 0, SA_V = 1, SA_W = 2, SA_S = 3 };
+
+// Ring depth of staged array a (NA arrays): four x planes (columns i-1 .. i+2
+// while column i is computed, column c in slot c & 3), except v and w of the
+// scalar kinds (NA > 3), whose faces and divergence read only columns i and
+// i+1: two planes (slot c & 1).  The planes of all arrays lie one after the
+// other, array by array.
+template <int NA>
+__host__ __device__ constexpr int st_depth(int a) { return NA > 3 && (a == SA_V || a == SA_W) ? 2 : 4; }
+template <int NA>
+__host__ __device__ constexpr int st_plane0(int a) {
+  return a <= SA_V ? 4 * a
+                   : (a == SA_W ? 4 + st_depth<NA>(SA_V)
+                                : 4 + st_depth<NA>(SA_V) + st_depth<NA>(SA_W) + 4 * (a - SA_S));
+}
+template <int NA>
+__host__ __device__ constexpr int st_planes() { return st_plane0<NA>(NA); }
+// The column array a stages when the arrays of depth 4 stage column c4 and
+// those of depth 2 column c2, and the ring plane that column goes to.
+template <int NA>
+__device__ __forceinline__ int st_col(int a, int c4, int c2) {
+  if constexpr (NA > 3) return st_depth<NA>(a) == 2 ? c2 : c4;
+  else return c4;
+}
+template <int NA>
+__device__ __forceinline__ int st_plane(int a, int c) { return st_plane0<NA>(a) + (c & (st_depth<NA>(a) - 1)); }
 
 // LDS of one block for a field kind with NSF fields.  The metric table comes
 // first: the views of the edge wave start a row or a level before the ring
@@ -41,7 +67,7 @@ enum { SA_U = 0, SA_V = 1, SA_W = 2, SA_S = 3 };
 template <typename T, int R, int NA, int NSF>
 struct StageLds {
   T met[M_COUNT][ST_NMET];  // the kind's metric roles (w: swapped), levels k0-1 .. k0+63
-  T ring[ST_SLOTS][NA][R + 4][ST_KB];
+  T ring[st_planes<NA>()][R + 4][ST_KB];
   T yn[NSF][R + 1][64];     // north y face of rows j0-1 .. j0+R-1
   T zb[NSF][R];             // top face of level k0-1 of each row
 };
@@ -53,8 +79,8 @@ struct StageLds {
 // added per access.
 template <typename T, int R, int NA>
 struct SView {
-  static constexpr int SLOT = NA * (R + 4) * ST_KB;  // elements per ring slot
-  const T* base;    // element (array 0, row j_at-2, level k_at-2) of slot 0
+  static constexpr int PLANE = (R + 4) * ST_KB;  // elements per x plane
+  const T* base;    // element (row j_at-2, level k_at-2) of plane 0
   int k_at, j_at;   // the lane's own level and row
   int j0;           // first row of the block
   __device__ __forceinline__ SView(const T* ring, int k0, int j0_, int k, int j)
@@ -62,7 +88,7 @@ struct SView {
   // every stencil offset is >= -2 in rows and levels, so the constant part
   // of an index is non-negative and folds into the LDS instruction's offset
   __device__ __forceinline__ T operator()(int a, int k, int j, int i) const {
-    return base[(i & (ST_SLOTS - 1)) * SLOT + a * ((R + 4) * ST_KB) + (j - j_at + 2) * ST_KB + (k - k_at + 2)];
+    return base[(st_plane0<NA>(a) + (i & (st_depth<NA>(a) - 1))) * PLANE + (j - j_at + 2) * ST_KB + (k - k_at + 2)];
   }
 };
 
@@ -273,18 +299,30 @@ __device__ __forceinline__ void skind_faces(const SView<T, R, NA>& s, const Stag
 // leaves untouched (k = 1; k = nz for w) keep f.  MODE_ACCUM, the drop-in
 // call: adds s to the source field `out` at levels 2..nz (w: 2..nz-1),
 // overwriting it at k = nz for u, v and the scalars (:942-959).
+// The flows the divergence of cell (k, j, i) reads at its own column (ur:
+// the right x face; vs, vn: south and north y faces; wb, wk: bottom and top z
+// faces).
+template <typename T>
+struct SDivFlows {
+  T ur, vs, vn, wb, wk;
+};
+
+template <typename T, int KIND, int R, int NA>
+__device__ __forceinline__ SDivFlows<T> sdiv_flows(const SView<T, R, NA>& s, int k, int j, int i) {
+  return {sflow<T, KIND>(s, SA_U, k, j, i), sflow<T, KIND>(s, SA_V, k, j - 1, i), sflow<T, KIND>(s, SA_V, k, j, i),
+          sflow<T, KIND>(s, SA_W, k - 1, j, i), sflow<T, KIND>(s, SA_W, k, j, i)};
+}
+
 template <typename T, int KIND, int NSF, int R, int NA, int MODE>
 __device__ __forceinline__ void skind_step(const SView<T, R, NA>& s, const StageLds<T, R, NA, NSF>& lds,
                                            const __amdgpu_buffer_rsrc_t (&out)[NSF], const Geo& g, const Dims& d,
                                            int k, int j, int i, int li, int lane, int r, T dtm, T cx, T cy, T ul,
-                                           const T fxl[NSF], const T fxr[NSF], const T fyn[NSF],
-                                           const T fzt[NSF]) {
+                                           const SDivFlows<T>& q, const T fxl[NSF], const T fxr[NSF],
+                                           const T fyn[NSF], const T fzt[NSF]) {
   constexpr int A0 = skind_first<KIND>();
   constexpr bool IS_W = KIND == KIND_W;
   const int nz = d.nz, ny = d.ny, nx = d.nx;
-  const T ur = sflow<T, KIND>(s, SA_U, k, j, i);
-  const T vs = sflow<T, KIND>(s, SA_V, k, j - 1, i), vn = sflow<T, KIND>(s, SA_V, k, j, i);
-  const T wb = sflow<T, KIND>(s, SA_W, k - 1, j, i), wk = sflow<T, KIND>(s, SA_W, k, j, i);
+  const T ur = q.ur, vs = q.vs, vn = q.vn, wb = q.wb, wk = q.wk;
   const T t1 = lds.met[M_T1][li], t2 = lds.met[M_T2][li];
   const T wt = k < nz ? wk : T(0);
   const int kmax = IS_W ? nz - 1 : nz;
@@ -366,22 +404,24 @@ struct StageFetch {
 // ring waits for them.  Pairs past the last are clamped to it (their loads are
 // redundant and not stored), rows past ny+4 to row ny+4 (no active cell's
 // stencil reaches them).  The four edge levels of the wave's q-th pair go to
-// lanes 4q .. 4q+3 of one register.
+// lanes 4q .. 4q+3 of one register.  The arrays of depth 4 stage column c4,
+// those of depth 2 column c2.
 template <typename T, int R, int NA>
 __device__ __forceinline__ void stage_fetch(StageFetch<T, R, NA>& f, const __amdgpu_buffer_rsrc_t (&src)[NA],
-                                            const Geo& g, int wv, int lane, int k0, int j0, int nz, int ny, int c) {
+                                            const Geo& g, int wv, int lane, int k0, int j0, int nz, int ny, int c4,
+                                            int c2) {
   using P = StagePlan<R, NA>;
   const unsigned es = sizeof(T);
   const unsigned lev = (unsigned)(min(k0 + lane, nz) - 1) * es;
   const int e = lane & 3, kk = e < 2 ? k0 - 2 + e : k0 + 62 + e;
   const unsigned elev = (unsigned)((kk < 1 ? 1 : (kk > nz ? nz : kk)) - 1) * es;
-  const unsigned sc = (unsigned)(c - 1) * g.sib;
   f.edge = T(0);
 #pragma unroll
   for (int q = 0; q < P::PER; q++) {
     // the pair is uniform over the wave: its array selects a buffer resource
     const int p = min((int)__builtin_amdgcn_readfirstlane(wv + q * P::WAVES), P::PAIRS - 1);
     const int a = p / (R + 4), row = p % (R + 4);
+    const unsigned sc = (unsigned)(st_col<NA>(a, c4, c2) - 1) * g.sib;
     const unsigned jo = (unsigned)(min(j0 - 2 + row, ny + 4) - 1) * g.sjb;
     __amdgpu_buffer_rsrc_t r = src[0];
 #pragma unroll
@@ -393,17 +433,22 @@ __device__ __forceinline__ void stage_fetch(StageFetch<T, R, NA>& f, const __amd
 
 template <typename T, int R, int NA, class LDS>
 __device__ __forceinline__ void stage_store(const StageFetch<T, R, NA>& f, LDS& lds, int wv,
-                                            int lane, int c) {
+                                            int lane, int c4, int c2) {
   using P = StagePlan<R, NA>;
-  const int slot = c & (ST_SLOTS - 1);
 #pragma unroll
   for (int q = 0; q < P::PER; q++) {
     const int p = wv + q * P::WAVES;
-    if (p < P::PAIRS) lds.ring[slot][p / (R + 4)][p % (R + 4)][2 + lane] = f.main[q];
+    if (p < P::PAIRS) {
+      const int a = p / (R + 4);
+      lds.ring[st_plane<NA>(a, st_col<NA>(a, c4, c2))][p % (R + 4)][2 + lane] = f.main[q];
+    }
   }
   const int q = lane >> 2, e = lane & 3;
   const int p = wv + q * P::WAVES;
-  if (q < P::PER && p < P::PAIRS) lds.ring[slot][p / (R + 4)][p % (R + 4)][e < 2 ? e : 64 + e] = f.edge;
+  if (q < P::PER && p < P::PAIRS) {
+    const int a = p / (R + 4);
+    lds.ring[st_plane<NA>(a, st_col<NA>(a, c4, c2))][p % (R + 4)][e < 2 ? e : 64 + e] = f.edge;
+  }
 }
 
 // The edge wave's faces of one field kind: the south y face of the block's
@@ -419,7 +464,7 @@ __device__ __forceinline__ void skind_edge(const SView<T, R, NA>& s, StageLds<T,
   constexpr int A0 = skind_first<KIND>();
   const int k = k0 + lane, kq = k < nz ? k : nz - 1;
   const int j = s.j0 - 1;
-  const SView<T, R, NA> se(&lds.ring[0][0][0][0], k0, s.j0, k, j);
+  const SView<T, R, NA> se(&lds.ring[0][0][0], k0, s.j0, k, j);
   {
     const T rdz_k = lds.met[M_RDZ][lane + 1], rdz_kd = lds.met[M_RDZKD][lane + 1];
     T f[NSF];
@@ -436,7 +481,7 @@ __device__ __forceinline__ void skind_edge(const SView<T, R, NA>& s, StageLds<T,
   }
   if (k0 > 1 && lane < R && s.j0 + lane <= ny + 2) {
     const SMet<T> m = smet<T>(lds, 0);
-    const SView<T, R, NA> sz(&lds.ring[0][0][0][0], k0, s.j0, k0 - 1, s.j0 + lane);
+    const SView<T, R, NA> sz(&lds.ring[0][0][0], k0, s.j0, k0 - 1, s.j0 + lane);
     T f[NSF];
     int az[NSF];
     sz_face<T, KIND, NSF>(sz, A0, k0 - 1, nz, s.j0 + lane, i, m, dt, cx, cy, f, az);
@@ -492,10 +537,11 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
     lds.met[mm][e % ST_NMET] = met_value<T>(mm, kk, nz, W, W ? L.mw : L.mc);
   }
   StageFetch<T, R, NA> fetch;
-  // the four planes the faces of column i_begin-1 read: i_begin-2 .. i_begin+1
+  // the planes the faces of column i_begin-1 read: i_begin-2 .. i_begin+1
+  // (arrays of depth 2: i_begin-1 and i_begin, the last staged twice)
   for (int c = i_begin - 2; c <= i_begin + 1; c++) {
-    stage_fetch<T, R, NA>(fetch, src, g, wv, lane, k0, j0, nz, ny, c);
-    stage_store<T, R, NA>(fetch, lds, wv, lane, c);
+    stage_fetch<T, R, NA>(fetch, src, g, wv, lane, k0, j0, nz, ny, c, min(c, i_begin));
+    stage_store<T, R, NA>(fetch, lds, wv, lane, c, min(c, i_begin));
   }
   __syncthreads();
   const bool row_wave = wv < R;
@@ -503,8 +549,9 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
   const bool active = row_wave && j <= ny + 2;  // wave-uniform
   const int k = k0 + lane, kq = k < nz ? k : nz - 1, li = lane + 1;
   // the row waves' view; the edge wave forms its own for its faces
-  const SView<T, R, NA> sv(&lds.ring[0][0][0][0], k0, j0, k, row_wave ? j : j0 - 1);
+  const SView<T, R, NA> sv(&lds.ring[0][0][0], k0, j0, k, row_wave ? j : j0 - 1);
   T fxl[NSF], fxr[NSF], fyn[NSF], fzt[NSF], ul = T(0);
+  SDivFlows<T> dq{};
 #pragma unroll
   for (int n = 0; n < NSF; n++) fxl[n] = fxr[n] = fyn[n] = fzt[n] = T(0);
   if (active) {  // the left x faces of column i_begin
@@ -518,18 +565,21 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
       }
     }
   }
-  __syncthreads();  // plane i_begin-2 is no longer read
-  stage_fetch<T, R, NA>(fetch, src, g, wv, lane, k0, j0, nz, ny, i_begin + 2);
-  stage_store<T, R, NA>(fetch, lds, wv, lane, i_begin + 2);
+  __syncthreads();  // planes i_begin-2 (depth 2: i_begin-1) are no longer read
+  stage_fetch<T, R, NA>(fetch, src, g, wv, lane, k0, j0, nz, ny, i_begin + 2, i_begin + 1);
+  stage_store<T, R, NA>(fetch, lds, wv, lane, i_begin + 2, i_begin + 1);
   __syncthreads();
   for (int i = i_begin; i < i_end; i++) {
-    const bool more = i + 3 <= i_end + 1;  // planes up to i_end+1 are read
-    if (more) stage_fetch<T, R, NA>(fetch, src, g, wv, lane, k0, j0, nz, ny, i + 3);
+    // planes up to i_end+1 are read (depth 2: up to i_end)
+    const bool more = i + 3 <= i_end + 1;
+    if (more) stage_fetch<T, R, NA>(fetch, src, g, wv, lane, k0, j0, nz, ny, i + 3, i + 2);
     if (row_wave) {
       if (active) {
         int code[NSF];
         skind_faces<T, KIND, NSF>(sv, lds, k, kq, nz, j, i, li, dt, cx, cy, fxr, fyn, fzt, code);
         ul = sflow<T, KIND>(sv, SA_U, k, j, i - 1);  // its plane is overwritten before the step
+        // so are the planes of v and w at column i where they are two deep
+        if constexpr (st_depth<NA>(SA_V) == 2) dq = sdiv_flows<T, KIND>(sv, k, j, i);
         if constexpr (MODE == MODE_REC) {  // the arms of the faces cell (k, j, i) owns
           if (k <= nz) {
 #pragma unroll
@@ -543,10 +593,12 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
       skind_edge<T, KIND, NSF, R, NA, MODE>(sv, lds, G, L.d, lane, k0, nz, ny, i, dt, cx, cy);
     }
     __syncthreads();
-    if (active)
-      skind_step<T, KIND, NSF, R, NA, MODE>(sv, lds, out, g, L.d, k, j, i, li, lane, r, dtm, cx, cy, ul, fxl, fxr,
-                                            fyn, fzt);
-    if (more) stage_store<T, R, NA>(fetch, lds, wv, lane, i + 3);
+    if (active) {
+      if constexpr (st_depth<NA>(SA_V) != 2) dq = sdiv_flows<T, KIND>(sv, k, j, i);
+      skind_step<T, KIND, NSF, R, NA, MODE>(sv, lds, out, g, L.d, k, j, i, li, lane, r, dtm, cx, cy, ul, dq, fxl,
+                                            fxr, fyn, fzt);
+    }
+    if (more) stage_store<T, R, NA>(fetch, lds, wv, lane, i + 3, i + 2);
 #pragma unroll
     for (int n = 0; n < NSF; n++) fxl[n] = fxr[n];
     __syncthreads();

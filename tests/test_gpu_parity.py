@@ -274,7 +274,7 @@ STAGE_CASES = [(nz, rows) for nz in (4, 5, 63, 64, 65, 70, 128, 150, 193, 256) f
 @pytest.mark.parametrize("nz,rows", STAGE_CASES)
 @pytest.mark.parametrize("strip,n_q", [(5, 1), (16, 2)])
 def test_stage_kernel_matches_oracle(gpu, nz, rows, strip, n_q):
-    """rows = 11 and 15 are compiled for u, v and w only: the scalars then take 4 rows."""
+    """rows = 15 is compiled for u, v and w only: the scalars then take 4 rows."""
     st = ss.make_state(nz, 11, 13, 50.0, 25.0 * (nz - 1), seed=nz + rows + strip, n_q=n_q, with_zfields=False)
     dt = ss.cfl_dt(st, 0.3)
     ref = oracle.forward_run(st, dt, 3)
