@@ -343,8 +343,9 @@ bool stage_rows_ok(int rows) { return rows == 4 || rows == 6 || rows == 8 || row
 
 // A field group the stage kernel takes: a flow field advected by itself or
 // one or two scalars with the same timestep, fields under 4 GiB; MODE_STEP
-// with each field stepped from the field it advects, MODE_ACCUM without
-// z-face diagnostics.
+// and MODE_REC with each field stepped from the field it advects, MODE_ACCUM
+// without z-face diagnostics, MODE_LEAP with any advected scalars (the flow
+// kinds only in a forward step, where they advect themselves).
 template <typename T, int MODE>
 bool stage_fits(const monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G) {
   if (!c->buffer_ok || G.nf < 1 || G.nf > 2) return false;
@@ -428,10 +429,11 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
       G.rec[m] = S.rec[n + m];
     }
     const int kk = pair || S.kind[n] == KIND_SCALAR ? 3 : S.kind[n];
-    // the leapfrog step runs the march kernel in its default shape; the
-    // recording step runs the stage kernel where the forward step does (so
+    // the recording step runs the stage kernel where the forward step does (so
     // that the two write the same values), else the march kernel in its
-    // default shape
+    // default shape; the leapfrog step too (a centred step advects the
+    // z-fields, so its flow kinds, whose advected field is then not the
+    // staged flow, march)
     const bool leap = MODE == MODE_LEAP || MODE == MODE_REC;
     const int kernel = leap ? MONC_TVD_KERNEL_MARCH : c->kernel;
     // the leapfrog step keeps the general shape: its FP64 pair measured
@@ -446,7 +448,7 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
     const bool wide = c->precision == MONC_TVD_FP64 && kk == 3 && nch == 4 && rows <= 2;
     const int strip = own && c->strip[kk] > 0 ? c->strip[kk] : default_strip(c->ny, c->nx, rows, wide);
     LaunchTimer timer(c, kk);
-    if constexpr (MODE == MODE_STEP || MODE == MODE_ACCUM || MODE == MODE_REC) {
+    if constexpr (MODE == MODE_STEP || MODE == MODE_ACCUM || MODE == MODE_REC || MODE == MODE_LEAP) {
       const int dstrip = stage_default_strip(c, S.kind[n]);
       const bool dflt = c->kind_kernel[kk] == MONC_TVD_KERNEL_MARCH && !c->rows[kk] && !c->strip[kk] &&
                         c->kernel == MONC_TVD_KERNEL_MARCH && dstrip > 0;

@@ -313,12 +313,22 @@ __device__ __forceinline__ SDivFlows<T> sdiv_flows(const SView<T, R, NA>& s, int
           sflow<T, KIND>(s, SA_W, k - 1, j, i), sflow<T, KIND>(s, SA_W, k, j, i)};
 }
 
+// MODE_LEAP: the current fields (f), their z-fields (z) and the buffers of
+// the next z-fields (o2), with each field's galilean shift.
+template <typename T, int NSF>
+struct SLeap {
+  __amdgpu_buffer_rsrc_t f[NSF], z[NSF], o2[NSF];
+  T gal[NSF];
+};
+
 template <typename T, int KIND, int NSF, int R, int NA, int MODE>
 __device__ __forceinline__ void skind_step(const SView<T, R, NA>& s, const StageLds<T, R, NA, NSF>& lds,
-                                           const __amdgpu_buffer_rsrc_t (&out)[NSF], const Geo& g, const Dims& d,
-                                           int k, int j, int i, int li, int lane, int r, T dtm, T cx, T cy, T ul,
-                                           const SDivFlows<T>& q, const T fxl[NSF], const T fxr[NSF],
-                                           const T fyn[NSF], const T fzt[NSF]) {
+                                           const __amdgpu_buffer_rsrc_t (&out)[NSF], const Geo& g,
+                                           const LaunchArgs<T>& L, const SLeap<T, NSF>& lp, int k, int j, int i,
+                                           int li, int lane, int r, T dtm, T cx, T cy, T ul, const SDivFlows<T>& q,
+                                           const T fxl[NSF], const T fxr[NSF], const T fyn[NSF],
+                                           const T fzt[NSF]) {
+  const Dims& d = L.d;
   constexpr int A0 = skind_first<KIND>();
   constexpr bool IS_W = KIND == KIND_W;
   const int nz = d.nz, ny = d.ny, nx = d.nx;
@@ -372,13 +382,40 @@ __device__ __forceinline__ void skind_step(const SView<T, R, NA>& s, const Stage
       sv = sv + T(4.0) * (wb * fzb * t1 - wt * fzt[n] * t2);
       sv = sv + (ul * fxl[n] - ur * fxr[n]) * cx;
     }
-    const T fcur = s(a, k, j, i);
-    const T val = cell ? fcur + dtm * sv : fcur;
-    if (k <= nz) {
-      stb<T>(out[n], val, vo, so);
-      if (jh) stb<T>(out[n], val, vo, soj);
-      if (ih) stb<T>(out[n], val, vo, soi);
-      if (jh && ih) stb<T>(out[n], val, vo, soji);
+    if constexpr (MODE == MODE_LEAP) {
+      // the fused MONC step (leap_update): the current field and its
+      // z-field from memory (the staged field is the advected one, the
+      // z-field in a centred step), reads clamped to the column
+      const unsigned vc = (unsigned)((k < nz ? k : nz) - 1) * (unsigned)sizeof(T);
+      const T fc = ldb<T>(lp.f[n], vc, so);
+      const T zc = L.forward ? T(0) : ldb<T>(lp.z[n], vc, so);
+      T fn, zn;
+      leap_update<T, KIND>(L, lp.gal[n], fc, zc, cell ? sv : T(0), fn, zn);
+      if (k <= nz) {
+        stb<T>(out[n], fn, vo, so);
+        stb<T>(lp.o2[n], zn, vo, so);
+        if (jh) {
+          stb<T>(out[n], fn, vo, soj);
+          stb<T>(lp.o2[n], zn, vo, soj);
+        }
+        if (ih) {
+          stb<T>(out[n], fn, vo, soi);
+          stb<T>(lp.o2[n], zn, vo, soi);
+        }
+        if (jh && ih) {
+          stb<T>(out[n], fn, vo, soji);
+          stb<T>(lp.o2[n], zn, vo, soji);
+        }
+      }
+    } else {
+      const T fcur = s(a, k, j, i);
+      const T val = cell ? fcur + dtm * sv : fcur;
+      if (k <= nz) {
+        stb<T>(out[n], val, vo, so);
+        if (jh) stb<T>(out[n], val, vo, soj);
+        if (ih) stb<T>(out[n], val, vo, soi);
+        if (jh && ih) stb<T>(out[n], val, vo, soji);
+      }
     }
   }
 }
@@ -495,9 +532,11 @@ __device__ __forceinline__ void skind_edge(const SView<T, R, NA>& s, StageLds<T,
 // fastest so that consecutive tiles of one XCD share halo rows in its L2
 // (blocks are dealt to the 8 XCDs round-robin).  MODE_STEP with every advected
 // field its own prognostic, MODE_REC (the same step, also writing each
-// owner's limiter arms to G.rec), or MODE_ACCUM (the drop-in tendency call)
-// with the flow kinds advecting their own prognostic; G holds the kind's
-// fields (F, out) in order.
+// owner's limiter arms to G.rec), MODE_ACCUM (the drop-in tendency call)
+// with the flow kinds advecting their own prognostic, or MODE_LEAP (the
+// fused MONC timestep: the advected fields F, the current fields f and
+// z-fields z, the next fields out and z-fields out2); G holds the kind's
+// fields in order.
 template <typename T, int KIND, int NS, int R, int WPE, int MODE>
 __global__ void __launch_bounds__(64 * (R + 1)) __attribute__((amdgpu_waves_per_eu(WPE)))
 stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y) {
@@ -531,6 +570,16 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
   }
 #pragma unroll
   for (int n = 0; n < NSF; n++) out[n] = rsrc(G.out[n]);
+  SLeap<T, NSF> lp{};
+  if constexpr (MODE == MODE_LEAP) {
+#pragma unroll
+    for (int n = 0; n < NSF; n++) {
+      lp.f[n] = rsrc(G.f[n]);
+      lp.z[n] = rsrc(G.z[n]);
+      lp.o2[n] = rsrc(G.out2[n]);
+      lp.gal[n] = G.gal[n];
+    }
+  }
   // metric table of the kind, levels k0-1 .. k0+63
   for (int e = lane + 64 * wv; e < M_COUNT * ST_NMET; e += 64 * (R + 1)) {
     const int mm = e / ST_NMET, kk = k0 - 1 + e % ST_NMET;
@@ -595,7 +644,7 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
     __syncthreads();
     if (active) {
       if constexpr (st_depth<NA>(SA_V) != 2) dq = sdiv_flows<T, KIND>(sv, k, j, i);
-      skind_step<T, KIND, NSF, R, NA, MODE>(sv, lds, out, g, L.d, k, j, i, li, lane, r, dtm, cx, cy, ul, dq, fxl,
+      skind_step<T, KIND, NSF, R, NA, MODE>(sv, lds, out, g, L, lp, k, j, i, li, lane, r, dtm, cx, cy, ul, dq, fxl,
                                             fxr, fyn, fzt);
     }
     if (more) stage_store<T, R, NA>(fetch, lds, wv, lane, i + 3, i + 2);

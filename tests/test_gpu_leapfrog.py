@@ -31,9 +31,11 @@ def _rel(a, b):
     return float(np.max(np.abs(ai - bi)) / max(np.max(np.abs(bi)), 1e-300))
 
 
-def _gpu_leapfrog(st, dt, tsmth, nsteps, ugal=0.0, vgal=0.0, first_forward=True, precision=tvd.FP64):
+def _gpu_leapfrog(st, dt, tsmth, nsteps, ugal=0.0, vgal=0.0, first_forward=True, precision=tvd.FP64, tiling=None):
     t = tvd.TvdAdvection.initialise(st, precision=precision)
     try:
+        if tiling is not None:
+            t.set_tiling(*tiling)
         t.halo_fill()
         t.leapfrog_steps(dt, tsmth, nsteps, ugal, vgal, first_forward)
         return ([t.get_field(n, tvd.PROG) for n in range(len(st.fields))],
@@ -42,10 +44,15 @@ def _gpu_leapfrog(st, dt, tsmth, nsteps, ugal=0.0, vgal=0.0, first_forward=True,
         t.close()
 
 
+# the march kernel (the default at these sizes), and the stage kernel in
+# 4-row tiles with strips of 5 columns and in 11-row tiles with strips of 16
+# (the scalars in every step, the flows in the forward first step)
+@pytest.mark.parametrize("tiling", [None, (tvd.KERNEL_STAGE, 4, 5), (tvd.KERNEL_STAGE, 11, 16)],
+                         ids=["march", "stage4", "stage11"])
 @pytest.mark.parametrize("case", GRIDS, ids=[f"{c[2]}x{c[1]}x{c[0]}" for c in GRIDS])
-def test_leapfrog_matches_oracle(gpu, case):
+def test_leapfrog_matches_oracle(gpu, case, tiling):
     st, dt = _state(case)
-    f, z = _gpu_leapfrog(st, dt, 0.06, 4, ugal=1.5, vgal=-0.7)
+    f, z = _gpu_leapfrog(st, dt, 0.06, 4, ugal=1.5, vgal=-0.7, tiling=tiling)
     rf, rz = oracle.leapfrog_run(st, dt, 0.06, 4, ugal=1.5, vgal=-0.7)
     for n in range(5):
         assert _rel(f[n], rf[n]) <= 1e-12, ("field", n, _rel(f[n], rf[n]))
