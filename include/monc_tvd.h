@@ -123,15 +123,15 @@ int monc_tvd_field_device_ptr(monc_tvd_ctx *ctx, int field, int variant, void **
  * MONC_TVD_KERNEL_CELL: one thread per cell, every face recomputed (any
  * column height; the fallback beyond 256 levels or 4 GiB per field).
  * MONC_TVD_KERNEL_STAGE: the forward step (monc_tvd_forward_steps,
- * monc_tvd_forward_step_into) and the drop-in call (monc_tvd_tendencies) of
- * each field kind with u, v, w (and the kind's scalars) staged in LDS;
- * rows = cell rows per block (4, 6, 8, 11 or 15; the scalars take 4 where 15
- * is set; 0: the default), strip as above.  A drop-in call whose
- * momentum fields advect their z-fields (centred field stepping), or that
- * records z-face diagnostics, marches those kinds; the leapfrog step and the
- * other calls use the march kernel.  Without a tiling set (or after kernel 0
- * with rows = strip = 0) the forward step and the drop-in call stage u, v, w
- * and the scalar pair in FP64, and u, v and w in FP32, where the domain gives at
+ * monc_tvd_forward_step_into, monc_tvd_forward_step_record), the drop-in call
+ * (monc_tvd_tendencies) and the leapfrog step (monc_tvd_leapfrog_steps) of
+ * each field kind with u, v, w (and the field the kind advects) staged in
+ * LDS; rows = cell rows per block (4, 6, 8, 11 or 15; the scalars, and a
+ * momentum field advecting its z-field, take 4 where 15 is set; 0: the
+ * default), strip as above.  A drop-in call that records z-face diagnostics
+ * marches; the other calls use the march kernel.  Without a tiling set (or
+ * after kernel 0 with rows = strip = 0) these calls stage u, v, w and the
+ * scalar pair in FP64, and u, v and w in FP32, where the domain gives at
  * least 512 staged blocks, and march the rest. */
 #define MONC_TVD_KERNEL_MARCH 0
 #define MONC_TVD_KERNEL_CELL 1

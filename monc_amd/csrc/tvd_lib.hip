@@ -341,11 +341,11 @@ int stage_default_strip(const monc_tvd_ctx* c, int kind) {
 
 bool stage_rows_ok(int rows) { return rows == 4 || rows == 6 || rows == 8 || rows == 11 || rows == 15; }
 
-// A field group the stage kernel takes: a flow field advected by itself or
-// one or two scalars with the same timestep, fields under 4 GiB; MODE_STEP
-// and MODE_REC with each field stepped from the field it advects, MODE_ACCUM
-// without z-face diagnostics, MODE_LEAP with any advected scalars (the flow
-// kinds only in a forward step, where they advect themselves).
+// A field group the stage kernel takes: a flow field, or one or two scalars
+// with the same timestep, fields under 4 GiB; MODE_STEP and MODE_REC with
+// each field stepped from the field it advects (a flow advecting itself),
+// MODE_ACCUM without z-face diagnostics, MODE_LEAP (in both, a flow kind may
+// advect another field, its z-field in a centred step).
 template <typename T, int MODE>
 bool stage_fits(const monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G) {
   if (!c->buffer_ok || G.nf < 1 || G.nf > 2) return false;
@@ -355,9 +355,9 @@ bool stage_fits(const monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>
     if (MODE == MODE_ACCUM && G.zf[n]) return false;
   }
   switch (G.kind[0]) {
-    case KIND_U: return G.nf == 1 && G.F[0] == L.u;
-    case KIND_V: return G.nf == 1 && G.F[0] == L.v;
-    case KIND_W: return G.nf == 1 && G.F[0] == L.w;
+    case KIND_U: return G.nf == 1 && (G.F[0] == L.u || MODE == MODE_ACCUM || MODE == MODE_LEAP);
+    case KIND_V: return G.nf == 1 && (G.F[0] == L.v || MODE == MODE_ACCUM || MODE == MODE_LEAP);
+    case KIND_W: return G.nf == 1 && (G.F[0] == L.w || MODE == MODE_ACCUM || MODE == MODE_LEAP);
     default: return true;
   }
 }
@@ -369,7 +369,7 @@ void launch_stage_rows(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T
     case 8: launch_stage<T, KIND, NS, 8, MODE>(c, L, G, strip); break;
     case 11: launch_stage<T, KIND, NS, 11, MODE>(c, L, G, strip); break;
     case 15:
-      if constexpr (KIND != KIND_SCALAR) {
+      if constexpr (KIND != KIND_SCALAR && NS == 0) {
         launch_stage<T, KIND, NS, 15, MODE>(c, L, G, strip);
         break;
       }
@@ -378,13 +378,24 @@ void launch_stage_rows(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T
   }
 }
 
+// A flow kind advecting its own staged flow (NS = 0), or, in the drop-in call
+// and the leapfrog step, another field (its z-field in a centred step),
+// staged as a fourth array (NS = 1; at most 11 rows: 16 planes of 19 rows
+// would not fit the LDS in FP64).
+template <typename T, int KIND, int MODE>
+void launch_stage_flow(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G, const T* flow, int rows,
+                       int strip) {
+  if (G.F[0] == flow) launch_stage_rows<T, KIND, 0, MODE>(c, L, G, rows, strip);
+  else if constexpr (MODE == MODE_ACCUM || MODE == MODE_LEAP) launch_stage_rows<T, KIND, 1, MODE>(c, L, G, rows, strip);
+}
+
 template <typename T, int MODE = MODE_STEP>
 void launch_stage_group(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& G, int kk, int rows,
                         int strip) {
   switch (G.kind[0]) {
-    case KIND_U: launch_stage_rows<T, KIND_U, 0, MODE>(c, L, G, rows, strip); break;
-    case KIND_V: launch_stage_rows<T, KIND_V, 0, MODE>(c, L, G, rows, strip); break;
-    case KIND_W: launch_stage_rows<T, KIND_W, 0, MODE>(c, L, G, rows, strip); break;
+    case KIND_U: launch_stage_flow<T, KIND_U, MODE>(c, L, G, L.u, rows, strip); break;
+    case KIND_V: launch_stage_flow<T, KIND_V, MODE>(c, L, G, L.v, rows, strip); break;
+    case KIND_W: launch_stage_flow<T, KIND_W, MODE>(c, L, G, L.w, rows, strip); break;
     default:
       if (G.nf == 2) launch_stage_rows<T, KIND_SCALAR, 2, MODE>(c, L, G, rows, strip);
       else launch_stage_rows<T, KIND_SCALAR, 1, MODE>(c, L, G, rows, strip);
@@ -431,16 +442,14 @@ int launch_set(monc_tvd_ctx* c, const LaunchArgs<T>& L, const FieldSet<T>& S) {
     const int kk = pair || S.kind[n] == KIND_SCALAR ? 3 : S.kind[n];
     // the recording step runs the stage kernel where the forward step does (so
     // that the two write the same values), else the march kernel in its
-    // default shape; the leapfrog step too (a centred step advects the
-    // z-fields, so its flow kinds, whose advected field is then not the
-    // staged flow, march)
+    // default shape; the leapfrog step too
     const bool leap = MODE == MODE_LEAP || MODE == MODE_REC;
     const int kernel = leap ? MONC_TVD_KERNEL_MARCH : c->kernel;
     // the leapfrog step keeps the general shape: its FP64 pair measured
     // slower in the wide one (6.26 against 6.42 Gcell/s)
     // a kind set to the stage kernel that does not fit it (a drop-in call
-    // whose momentum fields advect their z-fields, z-face diagnostics)
-    // marches in the default shape: its rows and strip are the stage's
+    // with z-face diagnostics) marches in the default shape: its rows and
+    // strip are the stage's
     const bool own = !leap && c->kind_kernel[kk] != MONC_TVD_KERNEL_STAGE;
     const int rows = own && c->rows[kk] > 0 && march_rows_ok(nch, c->rows[kk])
                          ? c->rows[kk]

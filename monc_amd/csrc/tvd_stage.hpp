@@ -37,37 +37,39 @@ constexpr int ST_NMET = 65;  // metric levels: k0-1 .. k0+63
 enum { SA_U = 0, SA_V = 1, SA_W = 2, SA_S = 3 };
 
 // Ring depth of staged array a (NA arrays): four x planes (columns i-1 .. i+2
-// while column i is computed, column c in slot c & 3), except v and w of the
-// scalar kinds (NA > 3), whose faces and divergence read only columns i and
-// i+1: two planes (slot c & 1).  The planes of all arrays lie one after the
-// other, array by array.
-template <int NA>
-__host__ __device__ constexpr int st_depth(int a) { return NA > 3 && (a == SA_V || a == SA_W) ? 2 : 4; }
-template <int NA>
+// while column i is computed, column c in slot c & 3), except, with V2 (the
+// scalar kinds), v and w, whose faces and divergence then read only columns i
+// and i+1: two planes (slot c & 1).  The planes of all arrays lie one after
+// the other, array by array.
+template <int NA, bool V2>
+__host__ __device__ constexpr int st_depth(int a) { return V2 && (a == SA_V || a == SA_W) ? 2 : 4; }
+template <int NA, bool V2>
 __host__ __device__ constexpr int st_plane0(int a) {
   return a <= SA_V ? 4 * a
-                   : (a == SA_W ? 4 + st_depth<NA>(SA_V)
-                                : 4 + st_depth<NA>(SA_V) + st_depth<NA>(SA_W) + 4 * (a - SA_S));
+                   : (a == SA_W ? 4 + st_depth<NA, V2>(SA_V)
+                                : 4 + st_depth<NA, V2>(SA_V) + st_depth<NA, V2>(SA_W) + 4 * (a - SA_S));
 }
-template <int NA>
-__host__ __device__ constexpr int st_planes() { return st_plane0<NA>(NA); }
+template <int NA, bool V2>
+__host__ __device__ constexpr int st_planes() { return st_plane0<NA, V2>(NA); }
 // The column array a stages when the arrays of depth 4 stage column c4 and
 // those of depth 2 column c2, and the ring plane that column goes to.
-template <int NA>
+template <int NA, bool V2>
 __device__ __forceinline__ int st_col(int a, int c4, int c2) {
-  if constexpr (NA > 3) return st_depth<NA>(a) == 2 ? c2 : c4;
+  if constexpr (V2) return st_depth<NA, V2>(a) == 2 ? c2 : c4;
   else return c4;
 }
-template <int NA>
-__device__ __forceinline__ int st_plane(int a, int c) { return st_plane0<NA>(a) + (c & (st_depth<NA>(a) - 1)); }
+template <int NA, bool V2>
+__device__ __forceinline__ int st_plane(int a, int c) {
+  return st_plane0<NA, V2>(a) + (c & (st_depth<NA, V2>(a) - 1));
+}
 
 // LDS of one block for a field kind with NSF fields.  The metric table comes
 // first: the views of the edge wave start a row or a level before the ring
 // (their stencils reach only later elements).
-template <typename T, int R, int NA, int NSF>
+template <typename T, int R, int NA, int NSF, bool V2>
 struct StageLds {
   T met[M_COUNT][ST_NMET];  // the kind's metric roles (w: swapped), levels k0-1 .. k0+63
-  T ring[st_planes<NA>()][R + 4][ST_KB];
+  T ring[st_planes<NA, V2>()][R + 4][ST_KB];
   T yn[NSF][R + 1][64];     // north y face of rows j0-1 .. j0+R-1
   T zb[NSF][R];             // top face of level k0-1 of each row
 };
@@ -77,7 +79,7 @@ struct StageLds {
 // position (k_at, j_at) so that the constant level and row shifts of the
 // stencils fold into the instruction's offset; only the column's ring slot is
 // added per access.
-template <typename T, int R, int NA>
+template <typename T, int R, int NA, bool V2>
 struct SView {
   static constexpr int PLANE = (R + 4) * ST_KB;  // elements per x plane
   const T* base;    // element (row j_at-2, level k_at-2) of plane 0
@@ -88,7 +90,8 @@ struct SView {
   // every stencil offset is >= -2 in rows and levels, so the constant part
   // of an index is non-negative and folds into the LDS instruction's offset
   __device__ __forceinline__ T operator()(int a, int k, int j, int i) const {
-    return base[(st_plane0<NA>(a) + (i & (st_depth<NA>(a) - 1))) * PLANE + (j - j_at + 2) * ST_KB + (k - k_at + 2)];
+    return base[(st_plane0<NA, V2>(a) + (i & (st_depth<NA, V2>(a) - 1))) * PLANE + (j - j_at + 2) * ST_KB +
+                (k - k_at + 2)];
   }
 };
 
@@ -96,8 +99,8 @@ struct SView {
 // the dual-grid interpolation 0.5*(f + f shifted along u's x, v's y or w's z)
 // for momentum.  At k = nz the level above replicates nz, so w's
 // interpolation there is 0.5*(f + f) = f, the reference's raw copy.
-template <typename T, int KIND, int R, int NA>
-__device__ __forceinline__ T sflow(const SView<T, R, NA>& s, int a, int k, int j, int i) {
+template <typename T, int KIND, int R, int NA, bool V2>
+__device__ __forceinline__ T sflow(const SView<T, R, NA, V2>& s, int a, int k, int j, int i) {
   const T x = s(a, k, j, i);
   if constexpr (KIND == KIND_U) return T(0.5) * (x + s(a, k, j, i + 1));
   else if constexpr (KIND == KIND_V) return T(0.5) * (x + s(a, k, j + 1, i));
@@ -131,8 +134,8 @@ __device__ __forceinline__ SMet<T> smet(const LDS& lds, int li) {
 
 // Right x face of column (j, i) at level k (ultimateflux.F90:102-165) for the
 // NS fields a0 .. a0+NS-1 of kind KIND; kq = (min(k+1, nz)) - 1.
-template <typename T, int KIND, int NS, int R, int NA>
-__device__ __forceinline__ void sx_face(const SView<T, R, NA>& s, int a0, int k, int kq, int j, int i, T rdz_k,
+template <typename T, int KIND, int NS, int R, int NA, bool V2>
+__device__ __forceinline__ void sx_face(const SView<T, R, NA, V2>& s, int a0, int k, int kq, int j, int i, T rdz_k,
                                         T rdz_kd, T dt, T cx, T cy, T out[NS], int cls[NS]) {
   const T u_m = sflow<T, KIND>(s, SA_U, k, j, i - 1), u_0 = sflow<T, KIND>(s, SA_U, k, j, i);
   const T u_p = sflow<T, KIND>(s, SA_U, k, j, i + 1);
@@ -171,8 +174,8 @@ __device__ __forceinline__ void sx_face(const SView<T, R, NA>& s, int a0, int k,
 }
 
 // North y face of row j at column i, level k (ultimateflux.F90:189-233).
-template <typename T, int KIND, int NS, int R, int NA>
-__device__ __forceinline__ void sy_face(const SView<T, R, NA>& s, int a0, int k, int kq, int j, int i, T rdz_k,
+template <typename T, int KIND, int NS, int R, int NA, bool V2>
+__device__ __forceinline__ void sy_face(const SView<T, R, NA, V2>& s, int a0, int k, int kq, int j, int i, T rdz_k,
                                         T rdz_kd, T dt, T cx, T cy, T out[NS], int cls[NS]) {
   const T vk = sflow<T, KIND>(s, SA_V, k, j, i), vn = sflow<T, KIND>(s, SA_V, k, j + 1, i);
   const T vs = sflow<T, KIND>(s, SA_V, k, j - 1, i);
@@ -211,8 +214,8 @@ __device__ __forceinline__ void sy_face(const SView<T, R, NA>& s, int a0, int k,
 // Top z face of level k (flux_z(k+1)), ultimateflux.F90:236-523, the three
 // branches in one as in mz_face: `edge` marks k = 1 and k = nz-1, `top`
 // k = nz-1, `bot` k = 1.
-template <typename T, int KIND, int NS, int R, int NA>
-__device__ __forceinline__ void sz_face(const SView<T, R, NA>& s, int a0, int k, int nz, int j, int i,
+template <typename T, int KIND, int NS, int R, int NA, bool V2>
+__device__ __forceinline__ void sz_face(const SView<T, R, NA, V2>& s, int a0, int k, int nz, int j, int i,
                                         const SMet<T>& m, T dt, T cx, T cy, T out[NS], int cls[NS]) {
   const bool edge = k == 1 || k == nz - 1, top = k == nz - 1, bot = k == 1;
   const T wk = sflow<T, KIND>(s, SA_W, k, j, i);
@@ -267,19 +270,21 @@ __device__ __forceinline__ void sz_face(const SView<T, R, NA>& s, int a0, int k,
   }
 }
 
-template <int KIND>
-constexpr int skind_first() { return KIND == KIND_SCALAR ? SA_S : KIND; }
+// The staged array a kind advects: its own flow, or the fourth array (the
+// scalars, or a flow kind's separate advected field).
+template <int KIND, int NA>
+constexpr int skind_first() { return KIND == KIND_SCALAR || NA > 3 ? SA_S : KIND; }
 
 // Faces of one field kind for a row wave at (k, j), column i: right x, north
 // y and top z face of each field of the kind; the z face only where the kind
 // has one (scalars, u, v: 2 <= k <= nz-1; w: 1 <= k <= nz-1).  code[n]: the
 // limiter arms the three faces took, x | y << 2 | z << 4 (the recording
 // step's byte; unused otherwise).
-template <typename T, int KIND, int NSF, int R, int NA>
-__device__ __forceinline__ void skind_faces(const SView<T, R, NA>& s, const StageLds<T, R, NA, NSF>& lds, int k, int kq,
+template <typename T, int KIND, int NSF, int R, int NA, bool V2>
+__device__ __forceinline__ void skind_faces(const SView<T, R, NA, V2>& s, const StageLds<T, R, NA, NSF, V2>& lds, int k, int kq,
                                             int nz, int j, int i, int li, T dt, T cx, T cy, T fxr[NSF], T fyn[NSF],
                                             T fzt[NSF], int code[NSF]) {
-  constexpr int A0 = skind_first<KIND>();
+  constexpr int A0 = skind_first<KIND, NA>();
   const SMet<T> m = smet<T>(lds, li);
   int ax[NSF], ay[NSF], az[NSF];
   sx_face<T, KIND, NSF>(s, A0, k, kq, j, i, m.rdz_k, m.rdz_kd, dt, cx, cy, fxr, ax);
@@ -307,8 +312,8 @@ struct SDivFlows {
   T ur, vs, vn, wb, wk;
 };
 
-template <typename T, int KIND, int R, int NA>
-__device__ __forceinline__ SDivFlows<T> sdiv_flows(const SView<T, R, NA>& s, int k, int j, int i) {
+template <typename T, int KIND, int R, int NA, bool V2>
+__device__ __forceinline__ SDivFlows<T> sdiv_flows(const SView<T, R, NA, V2>& s, int k, int j, int i) {
   return {sflow<T, KIND>(s, SA_U, k, j, i), sflow<T, KIND>(s, SA_V, k, j - 1, i), sflow<T, KIND>(s, SA_V, k, j, i),
           sflow<T, KIND>(s, SA_W, k - 1, j, i), sflow<T, KIND>(s, SA_W, k, j, i)};
 }
@@ -321,15 +326,15 @@ struct SLeap {
   T gal[NSF];
 };
 
-template <typename T, int KIND, int NSF, int R, int NA, int MODE>
-__device__ __forceinline__ void skind_step(const SView<T, R, NA>& s, const StageLds<T, R, NA, NSF>& lds,
+template <typename T, int KIND, int NSF, int R, int NA, bool V2, int MODE>
+__device__ __forceinline__ void skind_step(const SView<T, R, NA, V2>& s, const StageLds<T, R, NA, NSF, V2>& lds,
                                            const __amdgpu_buffer_rsrc_t (&out)[NSF], const Geo& g,
                                            const LaunchArgs<T>& L, const SLeap<T, NSF>& lp, int k, int j, int i,
                                            int li, int lane, int r, T dtm, T cx, T cy, T ul, const SDivFlows<T>& q,
                                            const T fxl[NSF], const T fxr[NSF], const T fyn[NSF],
                                            const T fzt[NSF]) {
   const Dims& d = L.d;
-  constexpr int A0 = skind_first<KIND>();
+  constexpr int A0 = skind_first<KIND, NA>();
   constexpr bool IS_W = KIND == KIND_W;
   const int nz = d.nz, ny = d.ny, nx = d.nx;
   const T ur = q.ur, vs = q.vs, vn = q.vn, wb = q.wb, wk = q.wk;
@@ -443,7 +448,7 @@ struct StageFetch {
 // stencil reaches them).  The four edge levels of the wave's q-th pair go to
 // lanes 4q .. 4q+3 of one register.  The arrays of depth 4 stage column c4,
 // those of depth 2 column c2.
-template <typename T, int R, int NA>
+template <typename T, int R, int NA, bool V2>
 __device__ __forceinline__ void stage_fetch(StageFetch<T, R, NA>& f, const __amdgpu_buffer_rsrc_t (&src)[NA],
                                             const Geo& g, int wv, int lane, int k0, int j0, int nz, int ny, int c4,
                                             int c2) {
@@ -458,7 +463,7 @@ __device__ __forceinline__ void stage_fetch(StageFetch<T, R, NA>& f, const __amd
     // the pair is uniform over the wave: its array selects a buffer resource
     const int p = min((int)__builtin_amdgcn_readfirstlane(wv + q * P::WAVES), P::PAIRS - 1);
     const int a = p / (R + 4), row = p % (R + 4);
-    const unsigned sc = (unsigned)(st_col<NA>(a, c4, c2) - 1) * g.sib;
+    const unsigned sc = (unsigned)(st_col<NA, V2>(a, c4, c2) - 1) * g.sib;
     const unsigned jo = (unsigned)(min(j0 - 2 + row, ny + 4) - 1) * g.sjb;
     __amdgpu_buffer_rsrc_t r = src[0];
 #pragma unroll
@@ -468,7 +473,7 @@ __device__ __forceinline__ void stage_fetch(StageFetch<T, R, NA>& f, const __amd
   }
 }
 
-template <typename T, int R, int NA, class LDS>
+template <typename T, int R, int NA, bool V2, class LDS>
 __device__ __forceinline__ void stage_store(const StageFetch<T, R, NA>& f, LDS& lds, int wv,
                                             int lane, int c4, int c2) {
   using P = StagePlan<R, NA>;
@@ -477,14 +482,14 @@ __device__ __forceinline__ void stage_store(const StageFetch<T, R, NA>& f, LDS& 
     const int p = wv + q * P::WAVES;
     if (p < P::PAIRS) {
       const int a = p / (R + 4);
-      lds.ring[st_plane<NA>(a, st_col<NA>(a, c4, c2))][p % (R + 4)][2 + lane] = f.main[q];
+      lds.ring[st_plane<NA, V2>(a, st_col<NA, V2>(a, c4, c2))][p % (R + 4)][2 + lane] = f.main[q];
     }
   }
   const int q = lane >> 2, e = lane & 3;
   const int p = wv + q * P::WAVES;
   if (q < P::PER && p < P::PAIRS) {
     const int a = p / (R + 4);
-    lds.ring[st_plane<NA>(a, st_col<NA>(a, c4, c2))][p % (R + 4)][e < 2 ? e : 64 + e] = f.edge;
+    lds.ring[st_plane<NA, V2>(a, st_col<NA, V2>(a, c4, c2))][p % (R + 4)][e < 2 ? e : 64 + e] = f.edge;
   }
 }
 
@@ -494,14 +499,14 @@ __device__ __forceinline__ void stage_store(const StageFetch<T, R, NA>& f, LDS& 
 // The recording step (MODE_REC) writes the arm of the south face of row 3 as
 // the y arm of owner (k, 2, i), the one face row 2 owns; the faces below the
 // band belong to the band below, which records them.
-template <typename T, int KIND, int NSF, int R, int NA, int MODE>
-__device__ __forceinline__ void skind_edge(const SView<T, R, NA>& s, StageLds<T, R, NA, NSF>& lds,
+template <typename T, int KIND, int NSF, int R, int NA, bool V2, int MODE>
+__device__ __forceinline__ void skind_edge(const SView<T, R, NA, V2>& s, StageLds<T, R, NA, NSF, V2>& lds,
                                            const FieldSet<T>& G, const Dims& d, int lane, int k0, int nz, int ny,
                                            int i, T dt, T cx, T cy) {
-  constexpr int A0 = skind_first<KIND>();
+  constexpr int A0 = skind_first<KIND, NA>();
   const int k = k0 + lane, kq = k < nz ? k : nz - 1;
   const int j = s.j0 - 1;
-  const SView<T, R, NA> se(&lds.ring[0][0][0], k0, s.j0, k, j);
+  const SView<T, R, NA, V2> se(&lds.ring[0][0][0], k0, s.j0, k, j);
   {
     const T rdz_k = lds.met[M_RDZ][lane + 1], rdz_kd = lds.met[M_RDZKD][lane + 1];
     T f[NSF];
@@ -518,7 +523,7 @@ __device__ __forceinline__ void skind_edge(const SView<T, R, NA>& s, StageLds<T,
   }
   if (k0 > 1 && lane < R && s.j0 + lane <= ny + 2) {
     const SMet<T> m = smet<T>(lds, 0);
-    const SView<T, R, NA> sz(&lds.ring[0][0][0], k0, s.j0, k0 - 1, s.j0 + lane);
+    const SView<T, R, NA, V2> sz(&lds.ring[0][0][0], k0, s.j0, k0 - 1, s.j0 + lane);
     T f[NSF];
     int az[NSF];
     sz_face<T, KIND, NSF>(sz, A0, k0 - 1, nz, s.j0 + lane, i, m, dt, cx, cy, f, az);
@@ -541,11 +546,14 @@ template <typename T, int KIND, int NS, int R, int WPE, int MODE>
 __global__ void __launch_bounds__(64 * (R + 1)) __attribute__((amdgpu_waves_per_eu(WPE)))
 stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y) {
   constexpr bool SC = KIND == KIND_SCALAR;
-  constexpr int NA = SC ? 3 + NS : 3;
+  // a flow kind with NS = 1 advects a field other than its staged flow (a
+  // z-field): staged as a fourth array
+  constexpr int NA = 3 + NS;
+  constexpr bool V2 = SC;
   constexpr int NSF = SC ? NS : 1;
-  constexpr int A0 = skind_first<KIND>();
+  constexpr int A0 = skind_first<KIND, NA>();
   constexpr int W = KIND == KIND_W ? 1 : 0;
-  __shared__ StageLds<T, R, NA, NSF> lds;
+  __shared__ StageLds<T, R, NA, NSF, V2> lds;
   const int nb = (int)gridDim.x, bid = (int)blockIdx.x;
   const int full = nb - nb % 8;
   const int t = bid < full ? (bid % 8) * (full / 8) + bid / 8 : bid;
@@ -564,10 +572,8 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
   src[SA_U] = rsrc(L.u);
   src[SA_V] = rsrc(L.v);
   src[SA_W] = rsrc(L.w);
-  if constexpr (SC) {
 #pragma unroll
-    for (int n = 0; n < NS; n++) src[SA_S + n] = rsrc(G.F[n]);
-  }
+  for (int n = 0; n < NA - 3; n++) src[SA_S + n] = rsrc(G.F[n]);
 #pragma unroll
   for (int n = 0; n < NSF; n++) out[n] = rsrc(G.out[n]);
   SLeap<T, NSF> lp{};
@@ -589,8 +595,8 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
   // the planes the faces of column i_begin-1 read: i_begin-2 .. i_begin+1
   // (arrays of depth 2: i_begin-1 and i_begin, the last staged twice)
   for (int c = i_begin - 2; c <= i_begin + 1; c++) {
-    stage_fetch<T, R, NA>(fetch, src, g, wv, lane, k0, j0, nz, ny, c, min(c, i_begin));
-    stage_store<T, R, NA>(fetch, lds, wv, lane, c, min(c, i_begin));
+    stage_fetch<T, R, NA, V2>(fetch, src, g, wv, lane, k0, j0, nz, ny, c, min(c, i_begin));
+    stage_store<T, R, NA, V2>(fetch, lds, wv, lane, c, min(c, i_begin));
   }
   __syncthreads();
   const bool row_wave = wv < R;
@@ -598,7 +604,7 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
   const bool active = row_wave && j <= ny + 2;  // wave-uniform
   const int k = k0 + lane, kq = k < nz ? k : nz - 1, li = lane + 1;
   // the row waves' view; the edge wave forms its own for its faces
-  const SView<T, R, NA> sv(&lds.ring[0][0][0], k0, j0, k, row_wave ? j : j0 - 1);
+  const SView<T, R, NA, V2> sv(&lds.ring[0][0][0], k0, j0, k, row_wave ? j : j0 - 1);
   T fxl[NSF], fxr[NSF], fyn[NSF], fzt[NSF], ul = T(0);
   SDivFlows<T> dq{};
 #pragma unroll
@@ -615,20 +621,20 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
     }
   }
   __syncthreads();  // planes i_begin-2 (depth 2: i_begin-1) are no longer read
-  stage_fetch<T, R, NA>(fetch, src, g, wv, lane, k0, j0, nz, ny, i_begin + 2, i_begin + 1);
-  stage_store<T, R, NA>(fetch, lds, wv, lane, i_begin + 2, i_begin + 1);
+  stage_fetch<T, R, NA, V2>(fetch, src, g, wv, lane, k0, j0, nz, ny, i_begin + 2, i_begin + 1);
+  stage_store<T, R, NA, V2>(fetch, lds, wv, lane, i_begin + 2, i_begin + 1);
   __syncthreads();
   for (int i = i_begin; i < i_end; i++) {
     // planes up to i_end+1 are read (depth 2: up to i_end)
     const bool more = i + 3 <= i_end + 1;
-    if (more) stage_fetch<T, R, NA>(fetch, src, g, wv, lane, k0, j0, nz, ny, i + 3, i + 2);
+    if (more) stage_fetch<T, R, NA, V2>(fetch, src, g, wv, lane, k0, j0, nz, ny, i + 3, i + 2);
     if (row_wave) {
       if (active) {
         int code[NSF];
         skind_faces<T, KIND, NSF>(sv, lds, k, kq, nz, j, i, li, dt, cx, cy, fxr, fyn, fzt, code);
         ul = sflow<T, KIND>(sv, SA_U, k, j, i - 1);  // its plane is overwritten before the step
         // so are the planes of v and w at column i where they are two deep
-        if constexpr (st_depth<NA>(SA_V) == 2) dq = sdiv_flows<T, KIND>(sv, k, j, i);
+        if constexpr (V2) dq = sdiv_flows<T, KIND>(sv, k, j, i);
         if constexpr (MODE == MODE_REC) {  // the arms of the faces cell (k, j, i) owns
           if (k <= nz) {
 #pragma unroll
@@ -639,15 +645,15 @@ stage_kernel(LaunchArgs<T> L, FieldSet<T> G, int strip, int nbands, int tiles_y)
 #pragma unroll
       for (int n = 0; n < NSF; n++) lds.yn[n][r + 1][lane] = fyn[n];
     } else {
-      skind_edge<T, KIND, NSF, R, NA, MODE>(sv, lds, G, L.d, lane, k0, nz, ny, i, dt, cx, cy);
+      skind_edge<T, KIND, NSF, R, NA, V2, MODE>(sv, lds, G, L.d, lane, k0, nz, ny, i, dt, cx, cy);
     }
     __syncthreads();
     if (active) {
-      if constexpr (st_depth<NA>(SA_V) != 2) dq = sdiv_flows<T, KIND>(sv, k, j, i);
-      skind_step<T, KIND, NSF, R, NA, MODE>(sv, lds, out, g, L, lp, k, j, i, li, lane, r, dtm, cx, cy, ul, dq, fxl,
+      if constexpr (!V2) dq = sdiv_flows<T, KIND>(sv, k, j, i);
+      skind_step<T, KIND, NSF, R, NA, V2, MODE>(sv, lds, out, g, L, lp, k, j, i, li, lane, r, dtm, cx, cy, ul, dq, fxl,
                                             fxr, fyn, fzt);
     }
-    if (more) stage_store<T, R, NA>(fetch, lds, wv, lane, i + 3, i + 2);
+    if (more) stage_store<T, R, NA, V2>(fetch, lds, wv, lane, i + 3, i + 2);
 #pragma unroll
     for (int n = 0; n < NSF; n++) fxl[n] = fxr[n];
     __syncthreads();

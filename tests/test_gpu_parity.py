@@ -292,7 +292,8 @@ STAGE_ACCUM_CASES = [(nz, rows) for nz in (5, 64, 70, 150, 256) for rows in (4, 
 def test_stage_kernel_drop_in_call(gpu, nz, rows, fs, ms, sc):
     """The drop-in call (s += tendency, overwritten at k = nz) through the
     stage kernel, into non-zero sources; with centred field stepping the
-    momentum fields advect their z-fields and march, the scalars stage."""
+    momentum fields advect their z-fields, staged as a fourth array beside
+    the flows (at 15 rows in 4-row tiles)."""
     st = ss.make_state(nz, 11, 13, 50.0, 25.0 * (nz - 1), seed=nz + 7 * rows + fs, n_q=2,
                        anelastic_scale=6000.0 if rows == 6 else None)
     rng = np.random.default_rng(nz + rows)
