@@ -1,29 +1,29 @@
-// Stage kernel: the fused forward step of every field (u, v, w and the
-// scalars) in one launch, with the raw fields staged in LDS.
+// Stage kernel: the step of one field kind (u, v, w, or the scalars advected
+// by the same flows) per launch, with the raw fields staged in LDS.
 //
 // A block owns a band of 64 levels (lane = level) of R cell rows and marches
-// along an x strip.  u, v, w and the scalars are staged in LDS as x planes:
-// each plane holds rows j0-2 .. j0+R+1 and levels k0-2 .. k0+65 of one
-// column, levels outside [1, nz] replicating the nearest level (the clamping
-// of ultflx's stencils), so every operand of every face of every field kind
+// along an x strip.  u, v, w and the field(s) the kind advects are staged in
+// LDS as x planes: each plane holds rows j0-2 .. j0+R+1 and levels
+// k0-2 .. k0+65 of one column, levels outside [1, nz] replicating the nearest
+// level (the clamping of ultflx's stencils), so every operand of every face
 // is an LDS read.  A ring of four planes holds columns i-1 .. i+2 while
 // column i is computed; the plane of column i+3 is fetched into registers
 // during the faces of column i and written into the ring slot of column i-1
 // once the block has finished reading it.  (v and w of the scalar kinds are
-// read at columns i and i+1 only: two planes, column i+2 written into the slot
-// of column i, whose values the divergence takes before the barrier.)  The raw fields thus cross HBM
-// once per step (plus the 4/R row halo and 4/64 level halo) instead of once
-// per field kind.
+// read at columns i and i+1 only: two planes, column i+2 written into the
+// slot of column i, whose values the divergence takes before the barrier.)
+// Each staged value thus crosses HBM once per launch (plus the 4/R row halo
+// and the 4/64 level halo).
 //
-// Each of the R row waves computes, per field kind, the right x face, the
-// north y face and the top z face of its cells (the face arithmetic of
-// tvd_faces.hpp / tvd_march.hpp, ultimateflux.F90:81-614, with the same
-// operand order); one more wave, the edge wave, computes the faces the row
-// waves do not own: the south y face of the first row and the z face below
-// the band (the top face of level k0-1) of every row.  Faces cross between
-// waves through LDS, between lanes by a wave shift.  After one barrier every
-// row wave forms the flux divergence (tvdadvection.F90:911-960) and writes
-// f + dtm * s with its periodic halo images (stepfields.F90:597-627).
+// Each of the R row waves computes the right x face, the north y face and the
+// top z face of its cells (the face arithmetic of tvd_faces.hpp /
+// tvd_march.hpp, ultimateflux.F90:81-614, with the same operand order); one
+// more wave, the edge wave, computes the faces the row waves do not own: the
+// south y face of the first row and the z face below the band (the top face
+// of level k0-1) of every row.  Faces cross between waves through LDS,
+// between lanes by a wave shift.  After one barrier every row wave forms the
+// flux divergence (tvdadvection.F90:911-960) and writes the step, the
+// tendency or the leapfrog update of its cells (the modes below).
 #pragma once
 #include "tvd_march.hpp"
 
