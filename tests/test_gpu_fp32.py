@@ -131,12 +131,16 @@ def test_fp32_error_on_monc_form_theta_under_nondivergent_flow(gpu):
 
 # the default shapes (march at this size), and u, v, w staged in 11- and
 # 15-row tiles (the FP32 default for u, v and w on large domains)
+# forward and centred stepping (centred: the momentum kinds advect their
+# z-fields, staged beside the flows)
+@pytest.mark.parametrize("stepping", [tvd.FORWARD_STEPPING, tvd.CENTRED_STEPPING], ids=["forward", "centred"])
 @pytest.mark.parametrize("tiling", [None, (tvd.KERNEL_STAGE, 11, 32), (tvd.KERNEL_STAGE, 15, 16)],
                          ids=["default", "stage11", "stage15"])
-def test_fp32_tendencies_vs_oracle(gpu, tiling):
+def test_fp32_tendencies_vs_oracle(gpu, tiling, stepping):
     st = ss.make_state(64, 64, 64, 50.0, 2000.0, seed=0, n_q=1)
-    ref, _ = oracle.state_tendencies(st, 1.0, 1, 1, 1)
-    got = gpu_tendencies(st, 1.0, 1, 1, 1, precision=tvd.FP32, tiling=tiling)
+    m = stepping
+    ref, _ = oracle.state_tendencies(st, 1.0, m, m, m)
+    got = gpu_tendencies(st, 1.0, m, m, m, precision=tvd.FP32, tiling=tiling)
     errs = {}
     for f in range(5):
         a, b = interior(got[f]), interior(ref[f])

@@ -91,10 +91,13 @@ def test_leapfrog_bound_buffers_odd_steps(gpu):
         t.close()
 
 
-def test_leapfrog_fp32_vs_fp64(gpu):
+# the FP32 step marching, and staged in 11-row tiles (momentum advecting its
+# z-fields as a fourth staged array)
+@pytest.mark.parametrize("tiling", [None, (tvd.KERNEL_STAGE, 11, 16)], ids=["march", "stage11"])
+def test_leapfrog_fp32_vs_fp64(gpu, tiling):
     st, dt = _state(GRIDS[1])
     f64, z64 = _gpu_leapfrog(st, dt, 0.06, 20)
-    f32, z32 = _gpu_leapfrog(st, dt, 0.06, 20, precision=tvd.FP32)
+    f32, z32 = _gpu_leapfrog(st, dt, 0.06, 20, precision=tvd.FP32, tiling=tiling)
     for n in range(5):
         ai, bi = interior(f32[n]), interior(f64[n])
         l2 = float(np.linalg.norm(ai - bi) / np.linalg.norm(bi))
